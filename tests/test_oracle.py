@@ -1,0 +1,216 @@
+"""Pins the CPU oracle before it is trusted as the parity checker.
+
+Checks: the reference's own known answers (tests/golden/reference_kats.json),
+an independent dense state-vector simulator (tests/dense.py) on random
+Clifford+T circuits, invariants that the mathematics fixes (<phi|phi> = 1
+exactly, unit norm, Hermiticity) and published ChaCha test vectors.  A
+planted mistake in the oracle (dropped term, sign, index) breaks the dense
+comparisons below.
+"""
+import json
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import dense as D
+from oracle import oracle as O
+
+KATS = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
+
+
+def _sv(entries):
+    return np.array([complex(a, b) for a, b in entries])
+
+
+def test_kat_h_t_cx():
+    k = KATS["h_t_cx"]
+    st = O.State.compile(k["n"], k["gates"])
+    assert st.chi == k["rank"]
+    assert np.abs(st.statevector() - _sv(k["statevector"])).max() < 1e-10
+    assert abs(st.norm() - k["norm"]) < 1e-10
+
+
+def test_kat_invalid_t_qubit_raises():
+    k = KATS["invalid_t_qubit"]
+    with pytest.raises(ValueError, match="Qubit index 3 is out of bounds for 3 qubits"):
+        O.State.compile(k["n"], k["gates"])
+
+
+def test_kat_toffoli():
+    k = KATS["toffoli"]
+    st = O.State.compile(k["n"], k["gates"])
+    assert abs(st.exp_value(("dense", [0, 0, 3])) - k["exp_ZII"]) < 1e-10
+    assert abs(st.inner_product(O.State.compile(k["n"], k["gates"])) - 1.0) < 1e-10
+    assert abs(st.norm() - 1.0) < 1e-10
+    assert np.abs(st.statevector() - _sv(k["statevector"])).max() < 1e-10
+    with pytest.raises(ValueError):
+        st.exp_value(("dense", [0, 0, 0, 0, 3, 3]))
+    with pytest.raises(ValueError):
+        st.exp_value(("sparse", [(3, 0), (2, 2), (1, 4)]))
+    with pytest.raises(ValueError):
+        st.exp_value(("dense", [0, 3]))
+
+
+@pytest.mark.parametrize("name", ["hadamard", "bell"])
+def test_kat_small_states(name):
+    k = KATS[name]
+    st = O.State.compile(k["n"], k["gates"])
+    assert np.abs(st.statevector() - _sv(k["statevector"])).max() < 1e-10
+
+
+@pytest.mark.parametrize("t", range(1, 12))
+def test_kat_t_tensor(t):
+    gates = [(O.H, q) for q in range(t)] + [(O.T, q) for q in range(t)]
+    st = O.State.compile(t, gates)
+    ref = _sv(KATS["t_tensor"][str(t)])
+    # reference builds kron(v_{t-1}, v_1): qubit order reversed vs little-endian,
+    # the product state is symmetric so both orders agree
+    assert np.abs(st.statevector() - ref).max() < 1e-10
+
+
+def test_chacha_known_answers():
+    assert O.chacha_block([0] * 8, 0, 0, 20) == KATS["chacha20_zero_block"]
+    assert O.chacha_block([0] * 8, 1, 0, 20) == KATS["chacha20_zero_block1"]
+    b = O.chacha_block([0] * 8, 0, 0, 12)
+    assert struct.pack("<16I", *b).hex() == KATS["chacha12_zero_bytes"]
+    # StdRng from an all-zero seed yields the ChaCha12 keystream in order
+    r = O.Rng(bytes(32))
+    assert [r.next_u32() for _ in range(16)] == b
+
+
+ALL_GATES = tuple(range(11))
+
+
+def _random(rng, n, nc, nt):
+    return D.random_circuit(rng, n, nc, nt, gate_set=ALL_GATES if n > 1 else tuple(range(8)), tdg=True)
+
+
+def test_random_circuits_vs_dense():
+    rng = np.random.default_rng(1234)
+    for _ in range(120):
+        n = int(rng.integers(1, 6))
+        gs = _random(rng, n, int(rng.integers(0, 40)), int(rng.integers(0, 7)))
+        st = O.State.compile(n, gs)
+        ref = D.run(n, gs)
+        assert np.abs(st.statevector() - ref).max() < 1e-9
+        assert abs(st.norm() - 1.0) < 1e-9
+
+
+def test_exp_value_inner_product_projection_vs_dense():
+    rng = np.random.default_rng(99)
+    for _ in range(60):
+        n = int(rng.integers(2, 6))
+        gs = _random(rng, n, 30, int(rng.integers(0, 6)))
+        st, ref = O.State.compile(n, gs), D.run(n, gs)
+        ops = [int(x) for x in rng.integers(0, 4, n)]
+        ev = st.exp_value(("dense", ops))
+        assert abs(ev - np.vdot(ref, D.apply_pauli(ref, n, ("dense", ops))).real) < 1e-9
+        qs = rng.choice(n, int(rng.integers(1, n + 1)), replace=False)
+        sp = [(int(rng.integers(1, 4)), int(q)) for q in qs]
+        ev = st.exp_value(("sparse", sp))
+        assert abs(ev - np.vdot(ref, D.apply_pauli(ref, n, ("sparse", sp))).real) < 1e-9
+        gs2 = _random(rng, n, 30, int(rng.integers(0, 6)))
+        st2, ref2 = O.State.compile(n, gs2), D.run(n, gs2)
+        assert abs(st.inner_product(st2) - np.vdot(ref, ref2)) < 1e-9
+        q, o = int(rng.integers(n)), int(rng.integers(2))
+        idx = np.arange(1 << n)
+        s3 = st.clone()
+        s3.project_unnormalized(q, o)
+        assert np.abs(s3.statevector() - np.where(((idx >> q) & 1) == o, ref, 0)).max() < 1e-9
+        pr0 = np.where(((idx >> q) & 1) == 0, ref, 0)
+        if np.linalg.norm(pr0) > 1e-6:
+            s4 = st.clone()
+            s4.project_normalized(q, 0)
+            s4.discard(q)
+            red = np.take(pr0.reshape((2,) * n), 0, axis=n - 1 - q).reshape(-1)
+            red = red / np.linalg.norm(red)
+            assert np.abs(s4.statevector() - red).max() < 1e-9
+
+
+def test_chform_inner_product_exact_invariants():
+    """<phi|phi> = (k=0, r=0) exactly; <a|b> = conj(<b|a>) as exact scalars;
+    |<a|b>| = 2^{-r/2} against the dense vectors."""
+    rng = np.random.default_rng(7)
+    for _ in range(40):
+        n = int(rng.integers(1, 6))
+        forms = []
+        for _ in range(2):
+            f = O.CHForm.new(n)
+            for g in _random(rng, n, 25, 0):
+                f.apply(*g)
+            forms.append(f)
+        a, b = forms
+        (z, k, r), val = a.inner_product(a)
+        assert (z, k, r) == (False, 0, 0)
+        (zab, kab, rab), vab = a.inner_product(b)
+        (zba, kba, rba), vba = b.inner_product(a)
+        assert zab == zba
+        if not zab:
+            assert rab == rba and kab == (8 - kba) % 8
+        assert abs(vab - np.vdot(a.statevector(), b.statevector())) < 1e-9
+
+
+def test_kron_and_discard_vs_dense():
+    rng = np.random.default_rng(11)
+    for _ in range(20):
+        f1, f2 = O.CHForm.new(3), O.CHForm.new(2)
+        for g in _random(rng, 3, 20, 0):
+            f1.apply(*g)
+        for g in _random(rng, 2, 10, 0):
+            f2.apply(*g)
+        k = f1.kron(f2)
+        sv = np.kron(f2.statevector(), f1.statevector())  # little-endian: f1 = low qubits
+        assert np.abs(k.statevector() - sv).max() < 1e-9
+
+
+def test_sampling_and_measure_statistics():
+    k = KATS["toffoli"]
+    st = O.State.compile(k["n"], k["gates"])
+    res = st.sample([0, 1, 2], 10000, 42)
+    counts = {"".join("1" if b else "0" for b in bits): c for bits, c in res}
+    assert set(counts) == set(k["sample_outcomes"])
+    assert sum(counts.values()) == 10000
+    for c in counts.values():
+        assert abs(c / 10000 - 0.25) < 0.05
+    rev = st.sample([2, 1, 0], 10000, 40)
+    assert {"".join("1" if b else "0" for b in bits) for bits, _ in rev} == set(k["sample_outcomes_reversed"])
+    assert st.sample([0, 1, 2], 1000, 123) == st.sample([0, 1, 2], 1000, 123)
+    assert st.sample([0, 1, 2], 1000, 123) != st.sample([0, 1, 2], 1000, 456)
+    for seed in range(10):
+        s = st.clone()
+        bits = s.measure([0, 1, 2], seed)
+        assert "".join("1" if b else "0" for b in bits) in k["sample_outcomes"]
+        assert s.measure([0, 1, 2], seed + 100) == bits
+
+
+def test_error_paths():
+    k = KATS["toffoli"]
+    st = O.State.compile(k["n"], k["gates"])
+    with pytest.raises(ValueError, match="Qubit index list is empty"):
+        st.sample([], 10, 1)
+    with pytest.raises(ValueError, match="Qubit index 3 is out of bounds for 3 qubits"):
+        st.sample([0, 1, 3], 10, 1)
+    with pytest.raises(ValueError, match="Duplicate qubit index found: 1"):
+        st.sample([0, 1, 1], 10, 1)
+    with pytest.raises(ValueError, match="Direct application of non-Clifford gate T"):
+        st.apply_gate(O.T, 0)
+    f = O.CHForm.new(2)
+    with pytest.raises(ValueError, match="Qubit index 2 is out of bounds for 2 qubits"):
+        f.apply(O.H, 2)
+    with pytest.raises(ValueError, match="different qubit indices, but got the same: 1"):
+        f.apply(O.CX, 1, 1)
+    s2 = O.State.compile(k["n"], k["gates"])
+    s2.project_normalized(0, True)
+    with pytest.raises(ValueError, match="Impossible projection on qubit 0"):
+        s2.project_normalized(0, False)
+
+
+def test_binomial_moments():
+    r = O.Rng(5)
+    for n, p in [(1000, 0.3), (20, 0.2), (50, 0.5), (3, 0.9)]:
+        xs = np.array([r.binomial(n, p) for _ in range(4000)], float)
+        assert abs(xs.mean() - n * p) < 5 * np.sqrt(n * p * (1 - p) / 4000) + 1e-9
+        assert xs.min() >= 0 and xs.max() <= n
+    assert r.binomial(10, 0.0) == 0 and r.binomial(10, 1.0) == 10
