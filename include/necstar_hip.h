@@ -1,0 +1,139 @@
+/*
+ * necstar_hip.h -- C-ABI of the MI355X (gfx950) stabilizer-decomposition
+ * hot path: the drop-in boundary under the Python module `necstar`.
+ *
+ * One nx_state is one QuantumState of the reference
+ * (crates/necstar/src/state/mod.rs:67-707): chi CH-form terms (device
+ * resident, bit-packed u64 rows), their coefficients and a global factor.
+ * Every function returns NX_OK (0) or an error code; the message of the last
+ * error on the calling thread is nx_last_error().  Messages reproduce the
+ * reference's Display strings (crates/necstar/src/error/mod.rs:10-91,
+ * crates/stabilizer-ch-form-rust/src/error/mod.rs:7-67) so the binding layer
+ * can raise ValueError(message) exactly as python/src/state.rs does.
+ *
+ * Gate codes (op): 0 H, 1 X, 2 Y, 3 Z, 4 S, 5 Sdg, 6 SqrtX, 7 SqrtXdg,
+ * 8 CX, 9 CZ, 10 Swap, 11 T, 12 Tdg, 13 CCX.  A gate list is int32
+ * [ngates][4] = (op, q0, q1, q2).  Bitstrings are little-endian (qubit j is
+ * bit j) packed into `words` u64 per string.  Pauli ops: 0 I, 1 X, 2 Y, 3 Z.
+ * Seeds are 32 bytes (python/src/utils.rs:4-39 packs ints into them); NULL
+ * means "from entropy".
+ */
+#ifndef NECSTAR_HIP_H
+#define NECSTAR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nx_state nx_state;
+
+enum {
+  NX_OK = 0,
+  NX_E_VALUE = 1,    /* reference Error -> PyValueError                     */
+  NX_E_TYPE = 2,     /* seed type error -> PyTypeError                      */
+  NX_E_DEVICE = 3,   /* HIP runtime failure / no usable gfx950 device       */
+  NX_E_INTERNAL = 4  /* a reference panic!/unreachable! state               */
+};
+
+/* Error message of the last failed call on this thread. */
+const char *nx_last_error(void);
+
+/* Selects the HIP device used by later calls on this thread (default 0)
+ * and checks it is a gfx950.  No reference counterpart. */
+int nx_init(int device);
+
+/* QuantumState::from_circuit -> StabDecompCompiler::compile
+ * (crates/necstar/src/state/mod.rs:98-102, state/compiler/mod.rs:44-140). */
+int nx_state_compile(int num_qubits, int64_t ngates, const int32_t *gates, nx_state **out);
+
+/* Clone / free (PyO3 value semantics, python/src/state.rs:12-15). */
+int nx_state_clone(const nx_state *s, nx_state **out);
+void nx_state_free(nx_state *s);
+
+/* num_qubits / stabilizer_rank getters (state/mod.rs:686-696). */
+int nx_state_num_qubits(const nx_state *s, int32_t *out);
+int nx_state_stabilizer_rank(const nx_state *s, int64_t *out);
+
+/* QuantumState::apply_gate / apply_gates (state/mod.rs:529-671,
+ * stabilizer_decomposed_state/gates.rs:7-37). */
+int nx_state_apply_gates(nx_state *s, int64_t ngates, const int32_t *gates);
+
+/* QuantumState::norm (state/mod.rs:702-706, norm.rs:8-35). */
+int nx_state_norm(const nx_state *s, double *out);
+int nx_state_norm_squared(const nx_state *s, double *out);
+
+/* QuantumState::exp_value (state/mod.rs:309-313, exp_value.rs:7-61).
+ * dense != 0: ops[len] little-endian (PauliString::Dense);
+ * dense == 0: sparse terms (ops[i], qubits[i]) (PauliString::Sparse). */
+int nx_state_exp_value(const nx_state *s, int dense, int32_t len, const int32_t *ops, const int32_t *qubits,
+                       double *out);
+
+/* QuantumState::inner_product <a|b> (state/mod.rs:165-172,
+ * stabilizer_decomposed_state/inner_product.rs:9-19). */
+int nx_state_inner_product(const nx_state *a, const nx_state *b, double *re, double *im);
+
+/* <x|psi> for nbits bitstrings (host buffers).  out: complex128 [nbits]
+ * interleaved (re, im).  per_term (optional, may be NULL): uint8
+ * [chi][nbits], 0xFF for a zero amplitude else the phase k (mod 8) of
+ * <x|phi_i> before omega; the exponent r of term i is per_term_r[i]
+ * (optional).  Same arithmetic as to_statevector (statevector.rs:12-24,
+ * form/amplitude.rs:28-44) per string.  An API addition: the reference
+ * reaches amplitudes only through to_statevector / inner_product. */
+int nx_state_amplitudes(const nx_state *s, int64_t nbits, const uint64_t *bits, int32_t words, double *out,
+                        uint8_t *per_term, int32_t *per_term_r);
+/* Same with device pointers (inputs already resident in HBM). */
+int nx_state_amplitudes_device(const nx_state *s, int64_t nbits, const uint64_t *d_bits, int32_t words,
+                               double *d_out);
+
+/* QuantumState::to_statevector (statevector.rs:12-24): out complex128 [2^n]. */
+int nx_state_statevector(const nx_state *s, double *out);
+
+/* project_normalized / project_unnormalized (state/mod.rs:319-351,
+ * projection.rs:10-57). */
+int nx_state_project(nx_state *s, int32_t qubit, int32_t outcome, int32_t normalized);
+
+/* QuantumState::discard (state/mod.rs:361-365, discard.rs:5-12). */
+int nx_state_discard(nx_state *s, int32_t qubit);
+
+/* measure / measure_all (state/mod.rs:190-227, measurement.rs:10-103). */
+int nx_state_measure(nx_state *s, int32_t nq, const int32_t *qargs, const uint8_t *seed32, uint8_t *outcomes);
+
+/* sample (state/mod.rs:247-284, sampling.rs:13-189): distinct outcomes in
+ * DFS order, outcome k as bits little-endian in 2 u64 words (bit i <->
+ * qargs[i]) and its count.  Capacity max_out; *nout receives the count. */
+int nx_state_sample(const nx_state *s, int32_t nq, const int32_t *qargs, uint64_t shots, const uint8_t *seed32,
+                    int64_t max_out, uint64_t *out_bits, uint64_t *out_counts, int64_t *nout);
+
+/* Parity export of term i in the reference's layout: G, F, M as n*n bytes
+ * (row-major), gamma n bytes (k mod 8), v, s n bytes, phase 1 byte
+ * (form/mod.rs:11-22).  No reference counterpart (tables are private). */
+int nx_state_export_term(const nx_state *s, int64_t term, uint8_t *G, uint8_t *F, uint8_t *M, uint8_t *gamma,
+                         uint8_t *v, uint8_t *sv, uint8_t *phase);
+
+/* Coefficients: exact Scalar (zero, k, r) arrays, or complex128 values.
+ * set_coefficients switches the state to Complex64 coefficients, the
+ * generic StabilizerDecomposedState<Complex64> path (coefficient.rs:6-42). */
+int nx_state_get_scalar_coefficients(const nx_state *s, uint8_t *zero, uint8_t *k, int64_t *r);
+int nx_state_get_coefficients(const nx_state *s, double *re_im);
+int nx_state_set_coefficients(nx_state *s, const double *re_im);
+int nx_state_global_factor(const nx_state *s, double *re, double *im);
+
+/* Exact pair scalars <phi_i|phi_j> for all i, j < chi (chi*chi uint32:
+ * 0xFFFFFFFF zero, else k | r << 8).  Parity aid; the reference computes
+ * the same values in StabilizerCHForm::inner_product (inner_product.rs:14-48). */
+int nx_state_pair_table(const nx_state *s, uint32_t *out);
+
+/* Coefficient backward of L = sum_k w_k exp_value(P_k) with respect to the
+ * decomposition coefficients: grad = dL/d(conj c) * 2 = 2 |g|^2 sum_k w_k G_k c,
+ * (G_k)_{ji} = <phi_j|P_k phi_i>.  paulis: npauli dense strings ops[npauli][n].
+ * grad: complex128 [chi].  No reference counterpart (an addition for
+ * differentiation; the forward is exp_value.rs:7-61). */
+int nx_state_grad_exp_values(const nx_state *s, int32_t npauli, const int32_t *ops, const double *weights,
+                             double *loss, double *grad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,84 @@
+// internal.hpp -- device-buffer descriptors and kernel launchers shared by
+// kernels.hip and runtime.hip.  Not part of the public C-ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nx {
+
+// Program op codes.  0..10 are the Clifford gates in the numbering of the
+// public header (H X Y Z S Sdg SqrtX SqrtXdg CX CZ Swap).
+enum : int {
+  OP_H = 0, OP_X = 1, OP_Y = 2, OP_Z = 3, OP_S = 4, OP_SDG = 5, OP_SQRTX = 6, OP_SQRTXDG = 7,
+  OP_CX = 8, OP_CZ = 9, OP_SWAP = 10,
+  OP_PROJ = 20,  // a = qubit, b = outcome
+  OP_DISC = 21,  // a = qubit
+};
+
+// per-term error codes reported by k_program (the reference panics or
+// returns an error at these points)
+enum ErrCode : int {
+  DEV_OK = 0,
+  DEV_ERR_SAME_VEC_MINUS = 1,  // resolve_superposition.rs:120-122 panic
+  DEV_ERR_CANNOT_DISCARD = 2,  // discard.rs:116 CannotDiscardQubit
+  DEV_ERR_G_ROW = 3,           // discard.rs:140 unreachable
+  DEV_ERR_G_COL = 4,           // discard.rs:155 unreachable
+};
+
+struct ProgOp {
+  int32_t code, a, b, c;
+};
+
+// Term tables of chi CH-forms on n qubits, W = ceil(n/64) words per row,
+// rows padded to R = 64*W.  G/F/M: [chi][R][W] u64; g: [chi][R] gamma/2;
+// v, s: [chi][W]; ph: [chi] phase_factor k mod 8.
+struct FormsDev {
+  uint64_t *G, *F, *M;
+  uint8_t* g;
+  uint64_t *v, *s;
+  uint8_t* ph;
+  int64_t chi;
+  int n, W;
+};
+
+// Quadratic-form rewrite of each term (see k_qform).
+struct QFormDev {
+  uint64_t *Q, *B, *C;       // [chi][R][W]
+  uint8_t* a;                // [chi][R]
+  uint64_t *alo, *ahi, *rhs; // [chi][W]
+  int32_t *nc, *k;           // [chi]
+  uint8_t* p0;               // [chi]
+  int64_t chi;
+  int n, W;
+};
+
+// Coefficients: exact Scalars (zero, k mod 8, r) when scalar != 0, else the
+// complex128 values c.
+struct CoefDev {
+  int scalar;
+  const uint8_t* cz;
+  const uint8_t* ck;
+  const int32_t* cr;
+  const double2* c;
+};
+
+// FP64 tables computed on the host with the same libm calls as the
+// reference (cos/sin of k*pi/4, powf(2, -r/2)).
+enum : int { TAB_COS = 0, TAB_SIN = 8, TAB_RMAX = 1024, TAB_POW = 16 + TAB_RMAX, TAB_SIZE = 16 + 2 * TAB_RMAX + 1 };
+
+hipError_t launch_init_zero(const FormsDev& out, hipStream_t st);
+hipError_t launch_program(const FormsDev& in, const FormsDev& out, const ProgOp* prog, int nprog, int4* status,
+                          hipStream_t st);
+hipError_t launch_kron(const FormsDev& A, const FormsDev& B, const FormsDev& O, hipStream_t st);
+hipError_t launch_gather(const FormsDev& in, const FormsDev& O, const int64_t* idx, hipStream_t st);
+hipError_t launch_qform(const FormsDev& in, const QFormDev& q, hipStream_t st);
+hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const uint64_t* bits, int64_t nb, int64_t t0,
+                            int64_t t1, const double* tabs, double2* out, uint8_t* per_term, int enumerate,
+                            double gre, double gim, hipStream_t st);
+hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, const int64_t* offsets, int64_t nfixed,
+                        int64_t nother, int triangle, int J, int mode, const CoefDev& cx, const CoefDev& cy,
+                        const double* tabs, double2* partial, int nblocks, uint32_t* exact_out, int64_t exact_ld,
+                        hipStream_t st);
+hipError_t launch_reduce(const double2* partial, int64_t n, double2* out, hipStream_t st);
+
+}  // namespace nx

@@ -1,0 +1,775 @@
+// kernels.hip -- HIP kernels of the CH-form hot path for gfx950.
+//
+//  k_program   one wavefront per term: applies a list of Clifford gates,
+//              Z-basis projections and qubit discards to every term
+//              (compile, chi-state gates, projection filter, Pauli images)
+//  k_kron      block-diagonal tensor product of two term sets
+//  k_gather    copies a subset of terms (clone / stream compaction)
+//  k_qform     one wavefront per term: rewrites the CH-form as an affine
+//              support {x : C x = rhs} and a Z8 quadratic phase
+//              p0 + 2 a.x + 4 sum_{l<m} Q_lm x_l x_m, magnitude 2^{-|v|/2}
+//  k_amplitude one lane per bitstring, loops over terms: <x|phi_i> exact
+//              (zero, phase, r) and the FP64 sum over coefficients
+//  k_pairs     one wavefront per (bra, ket) pair: the exponential sum
+//              <phi_a|phi_b> evaluated by GF(2) variable elimination (lane =
+//              variable row, pivots broadcast with readlane), FP64 partial
+//              sums per wavefront
+#include <hip/hip_runtime.h>
+
+#include "device_form.hpp"
+#include "internal.hpp"
+
+namespace nx {
+
+__device__ __forceinline__ size_t rowidx(int64_t t, int R, int r, int W, int w) {
+  return ((size_t)t * R + r) * W + w;
+}
+
+template <int W>
+__device__ __forceinline__ void load_form(Form<W>& f, const FormsDev& in, int64_t t, int lane) {
+  const int R = 64 * in.W;
+  f.lane = lane;
+  f.n = in.n;
+#pragma unroll
+  for (int sl = 0; sl < W; ++sl) {
+    const int r = 64 * sl + lane;
+    const bool okr = sl < in.W;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const bool ok = okr && w < in.W;
+      f.G[sl][w] = ok ? in.G[rowidx(t, R, r, in.W, w)] : 0ull;
+      f.F[sl][w] = ok ? in.F[rowidx(t, R, r, in.W, w)] : 0ull;
+      f.M[sl][w] = ok ? in.M[rowidx(t, R, r, in.W, w)] : 0ull;
+    }
+    f.g[sl] = okr ? (int)in.g[(size_t)t * R + r] : 0;
+  }
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    f.v[w] = w < in.W ? in.v[(size_t)t * in.W + w] : 0ull;
+    f.s[w] = w < in.W ? in.s[(size_t)t * in.W + w] : 0ull;
+  }
+  f.ph = in.ph[t];
+}
+
+template <int W>
+__device__ __forceinline__ void store_form(const Form<W>& f, const FormsDev& out, int64_t t) {
+  const int R = 64 * out.W;
+#pragma unroll
+  for (int sl = 0; sl < W; ++sl) {
+    if (sl >= out.W) continue;
+    const int r = 64 * sl + f.lane;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      if (w >= out.W) continue;
+      out.G[rowidx(t, R, r, out.W, w)] = f.G[sl][w];
+      out.F[rowidx(t, R, r, out.W, w)] = f.F[sl][w];
+      out.M[rowidx(t, R, r, out.W, w)] = f.M[sl][w];
+    }
+    out.g[(size_t)t * R + r] = (uint8_t)f.g[sl];
+  }
+  if (f.lane < out.W) {
+    out.v[(size_t)t * out.W + f.lane] = wsel<W>(f.v, f.lane);
+    out.s[(size_t)t * out.W + f.lane] = wsel<W>(f.s, f.lane);
+  }
+  if (f.lane == 0) out.ph[t] = (uint8_t)f.ph;
+}
+
+// ---------------------------------------------------------------------------
+template <int W>
+__global__ __launch_bounds__(256) void k_init_zero(FormsDev out) {
+  const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (t >= out.chi) return;
+  Form<W> f;
+  f.lane = lane;
+  f.n = out.n;
+#pragma unroll
+  for (int sl = 0; sl < W; ++sl) {
+    const int r = 64 * sl + lane;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const u64 diag = (r < out.n && (r >> 6) == w) ? (1ull << (r & 63)) : 0ull;
+      f.G[sl][w] = diag;
+      f.F[sl][w] = diag;
+      f.M[sl][w] = 0ull;
+    }
+    f.g[sl] = 0;
+  }
+#pragma unroll
+  for (int w = 0; w < W; ++w) { f.v[w] = 0ull; f.s[w] = 0ull; }
+  f.ph = 0;
+  store_form<W>(f, out, t);
+}
+
+// ---------------------------------------------------------------------------
+template <int W>
+__global__ __launch_bounds__(256) void k_program(FormsDev in, FormsDev out, const ProgOp* __restrict__ prog,
+                                                 int nprog, int4* __restrict__ status) {
+  const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (t >= in.chi) return;
+  Form<W> f;
+  load_form<W>(f, in, t, lane);
+  int dead = 0, ndet = 0, nrnd = 0, err = 0;
+  for (int i = 0; i < nprog; ++i) {
+    const int code = uni(prog[i].code), a = uni(prog[i].a), b = uni(prog[i].b);
+    switch (code) {
+      case OP_H: err = left_h<W>(f, a); break;
+      case OP_X: left_x<W>(f, a); break;
+      case OP_Y: left_y<W>(f, a); break;
+      case OP_Z: left_z<W>(f, a); break;
+      case OP_S: left_s_like<W>(f, a, 3); break;
+      case OP_SDG: left_s_like<W>(f, a, 1); break;
+      case OP_SQRTX:
+      case OP_SQRTXDG:
+        err = left_h<W>(f, a);
+        if (!err) {
+          left_s_like<W>(f, a, code == OP_SQRTX ? 3 : 1);
+          err = left_h<W>(f, a);
+        }
+        break;
+      case OP_CX: left_cx<W>(f, a, b); break;
+      case OP_CZ: left_cz<W>(f, a, b); break;
+      case OP_SWAP:
+        left_cx<W>(f, a, b);
+        left_cx<W>(f, b, a);
+        left_cx<W>(f, a, b);
+        break;
+      case OP_PROJ: {
+        int d = 0, dt = 0;
+        err = project<W>(f, a, b, &d, &dt);
+        if (d) dead = 1;
+        else if (dt) ++ndet;
+        else ++nrnd;
+        break;
+      }
+      case OP_DISC: err = discard<W>(f, a); break;
+      default: err = 99; break;
+    }
+    if (dead || err) break;
+  }
+  store_form<W>(f, out, t);
+  if (lane == 0) status[t] = make_int4(dead, ndet, nrnd, err);
+}
+
+// ---------------------------------------------------------------------------
+// one thread per (output term, row)
+__device__ __forceinline__ u64 shifted_word(const u64* row, int Win, int nshift, int w) {
+  const int ws = nshift >> 6, bs = nshift & 63;
+  const int i0 = w - ws, i1 = w - ws - 1;
+  u64 r = 0;
+  if (i0 >= 0 && i0 < Win) r |= row[i0] << bs;
+  if (bs && i1 >= 0 && i1 < Win) r |= row[i1] >> (64 - bs);
+  return r;
+}
+
+__global__ void k_kron(FormsDev A, FormsDev B, FormsDev O) {
+  const int RO = 64 * O.W, RA = 64 * A.W, RB = 64 * B.W;
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= O.chi * RO) return;
+  const int64_t o = gid / RO;
+  const int r = (int)(gid % RO);
+  const int64_t ia = o / B.chi, jb = o % B.chi;
+  const u64* srcs[3][2] = {{A.G, B.G}, {A.F, B.F}, {A.M, B.M}};
+  u64* dsts[3] = {O.G, O.F, O.M};
+  for (int m = 0; m < 3; ++m) {
+    for (int w = 0; w < O.W; ++w) {
+      u64 val = 0;
+      if (r < A.n) {
+        if (w < A.W) val = srcs[m][0][rowidx(ia, RA, r, A.W, w)];
+      } else if (r < A.n + B.n) {
+        val = shifted_word(srcs[m][1] + rowidx(jb, RB, r - A.n, B.W, 0), B.W, A.n, w);
+      }
+      dsts[m][rowidx(o, RO, r, O.W, w)] = val;
+    }
+  }
+  uint8_t gv = 0;
+  if (r < A.n) gv = A.g[(size_t)ia * RA + r];
+  else if (r < A.n + B.n) gv = B.g[(size_t)jb * RB + (r - A.n)];
+  O.g[(size_t)o * RO + r] = gv;
+  if (r < O.W) {
+    const int w = r;
+    u64 av = w < A.W ? A.v[(size_t)ia * A.W + w] : 0ull;
+    u64 as = w < A.W ? A.s[(size_t)ia * A.W + w] : 0ull;
+    av |= shifted_word(B.v + (size_t)jb * B.W, B.W, A.n, w);
+    as |= shifted_word(B.s + (size_t)jb * B.W, B.W, A.n, w);
+    O.v[(size_t)o * O.W + w] = av;
+    O.s[(size_t)o * O.W + w] = as;
+  }
+  if (r == 0) O.ph[o] = (uint8_t)((A.ph[ia] + B.ph[jb]) & 7);
+}
+
+// one thread per (output term, row): out[o] = in[idx[o]]
+__global__ void k_gather(FormsDev in, FormsDev O, const int64_t* __restrict__ idx) {
+  const int R = 64 * O.W;
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= O.chi * R) return;
+  const int64_t o = gid / R;
+  const int r = (int)(gid % R);
+  const int64_t src = idx ? idx[o] : o;
+  for (int w = 0; w < O.W; ++w) {
+    const size_t a = rowidx(src, R, r, O.W, w), b = rowidx(o, R, r, O.W, w);
+    O.G[b] = in.G[a];
+    O.F[b] = in.F[a];
+    O.M[b] = in.M[a];
+  }
+  O.g[(size_t)o * R + r] = in.g[(size_t)src * R + r];
+  if (r < O.W) {
+    O.v[(size_t)o * O.W + r] = in.v[(size_t)src * O.W + r];
+    O.s[(size_t)o * O.W + r] = in.s[(size_t)src * O.W + r];
+  }
+  if (r == 0) O.ph[o] = in.ph[src];
+}
+
+// ---------------------------------------------------------------------------
+// Quadratic-form rewrite of each term.  With u_j = (F_j & ~v) ^ (M_j & v),
+// w_j = (M_j & ~v) ^ (F_j & v), c_j = par(F_j & M_j & v), applying X to the
+// set bits of x in ascending order (amplitude.rs:36-43) leaves
+//   s' = s ^ XOR_{j in x} u_j
+//   phase' = phase + sum_{j in x} (gamma_j + 4 beta_j),
+//   beta_j = par(s_before_j & w_j) ^ c_j ,
+// so <x|phi> = [ (s' & ~v) == 0 ] zeta^{p0 + 2 a.x + 4 sum_{l<m} Q_lm x_l x_m}
+// 2^{-|v|/2} with a_j = gamma_j/2 + 2 (par(s & w_j) ^ c_j) mod 4 and
+// Q_lm = par(u_l & w_m) for l < m.  The support constraints are the columns
+// k of F with v_k = 0: par(F[:,k] & x) = s_k.
+template <int W>
+__global__ __launch_bounds__(256) void k_qform(FormsDev in, QFormDev q) {
+  const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (t >= in.chi) return;
+  Form<W> f;
+  load_form<W>(f, in, t, lane);
+  const int n = in.n, R = 64 * W;
+  u64 U[W][W], Wm[W][W], Q[W][W];
+  int a[W];
+#pragma unroll
+  for (int sl = 0; sl < W; ++sl) {
+    u64 fm[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      U[sl][w] = (f.F[sl][w] & ~f.v[w]) ^ (f.M[sl][w] & f.v[w]);
+      Wm[sl][w] = (f.M[sl][w] & ~f.v[w]) ^ (f.F[sl][w] & f.v[w]);
+      fm[w] = f.F[sl][w] & f.M[sl][w];
+      Q[sl][w] = 0ull;
+    }
+    const int c = parand<W>(fm, f.v);
+    const int L = parand<W>(f.s, Wm[sl]) ^ c;
+    a[sl] = (f.g[sl] + 2 * L) & 3;
+  }
+  for (int m = 0; m < n; ++m) {
+    u64 wm[W];
+    bcast<W>(Wm, m, wm);
+#pragma unroll
+    for (int sl = 0; sl < W; ++sl) {
+      const int row = 64 * sl + lane;
+      const int bit = (row < m) ? parand<W>(U[sl], wm) : 0;
+      flipbit<W>(Q[sl], m, bit);
+    }
+  }
+  // B = Q ^ Q^T
+  u64 Bm[W][W];
+#pragma unroll
+  for (int sl = 0; sl < W; ++sl)
+#pragma unroll
+    for (int w = 0; w < W; ++w) Bm[sl][w] = Q[sl][w];
+  for (int l = 0; l < n; ++l) {
+#pragma unroll
+    for (int sw = 0; sw < W; ++sw) {
+      const u64 col = __ballot(getbit<W>(Q[sw], l));
+#pragma unroll
+      for (int sl = 0; sl < W; ++sl)
+        if (owns<W>(f, l, sl)) Bm[sl][sw] |= col;
+    }
+  }
+  // constraints
+  u64 C[W][W];
+#pragma unroll
+  for (int sl = 0; sl < W; ++sl)
+#pragma unroll
+    for (int w = 0; w < W; ++w) C[sl][w] = 0ull;
+  u64 rhs[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) rhs[w] = 0ull;
+  int nc = 0;
+  for (int k = 0; k < n; ++k) {
+    if (getbit<W>(f.v, k)) continue;
+    u64 col[W];
+#pragma unroll
+    for (int sw = 0; sw < W; ++sw) col[sw] = __ballot(getbit<W>(f.F[sw], k));
+#pragma unroll
+    for (int sl = 0; sl < W; ++sl)
+      if (owns<W>(f, nc, sl)) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) C[sl][w] = col[w];
+      }
+    if (getbit<W>(f.s, k)) flipbit<W>(rhs, nc, 1);
+    ++nc;
+  }
+  // store
+#pragma unroll
+  for (int sl = 0; sl < W; ++sl) {
+    const int r = 64 * sl + lane;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      q.Q[rowidx(t, R, r, W, w)] = Q[sl][w];
+      q.B[rowidx(t, R, r, W, w)] = Bm[sl][w];
+      q.C[rowidx(t, R, r, W, w)] = C[sl][w];
+    }
+    q.a[(size_t)t * R + r] = (uint8_t)a[sl];
+  }
+#pragma unroll
+  for (int sl = 0; sl < W; ++sl) {
+    const u64 lo = __ballot(a[sl] & 1), hi = __ballot((a[sl] >> 1) & 1);
+    if (lane == 0) {
+      q.alo[(size_t)t * W + sl] = lo;
+      q.ahi[(size_t)t * W + sl] = hi;
+      q.rhs[(size_t)t * W + sl] = rhs[sl];
+    }
+  }
+  if (lane == 0) {
+    int k = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) k += __builtin_popcountll(f.v[w]);
+    q.nc[t] = nc;
+    q.k[t] = k;
+    q.p0[t] = (uint8_t)f.ph;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Amplitudes.  Thread = bitstring.  Terms [t0, t1) in order; the FP64
+// arithmetic per term is the reference's to_statevector arithmetic
+// (Scalar::to_complex, * omega, * coefficient, +=).
+template <int W>
+__global__ __launch_bounds__(256) void k_amplitude(QFormDev q, const double2* __restrict__ coef,
+                                                   const u64* __restrict__ bits, int64_t nb, int64_t t0,
+                                                   int64_t t1, const double* __restrict__ tabs,
+                                                   double2* __restrict__ out, uint8_t* __restrict__ per_term,
+                                                   int enumerate, double gre, double gim) {
+  __shared__ double s_cos[8], s_sin[8];
+  if (threadIdx.x < 8) {
+    s_cos[threadIdx.x] = tabs[TAB_COS + threadIdx.x];
+    s_sin[threadIdx.x] = tabs[TAB_SIN + threadIdx.x];
+  }
+  __syncthreads();
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = b < nb;
+  const int n = q.n, R = 64 * W;
+  u64 x[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    if (enumerate) x[w] = (w == 0 && active) ? (u64)b : 0ull;
+    else x[w] = active ? bits[(size_t)b * W + w] : 0ull;
+  }
+  double are = 0.0, aim = 0.0;
+  for (int64_t t = t0; t < t1; ++t) {
+    const int nc = uni(q.nc[t]);
+    const int k = uni(q.k[t]);
+    const int p0 = uni(q.p0[t]);
+    int bad = 0;
+    const u64* Cr = q.C + (size_t)t * R * W;
+    for (int c = 0; c < nc; ++c) {
+      int p = 0;
+#pragma unroll
+      for (int w = 0; w < W; ++w) p += __builtin_popcountll(Cr[(size_t)c * W + w] & x[w]);
+      bad |= (p & 1) ^ (int)((q.rhs[(size_t)t * W + (c >> 6)] >> (c & 63)) & 1ull);
+    }
+    int ax = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+      ax += __builtin_popcountll(x[w] & q.alo[(size_t)t * W + w]) +
+            2 * __builtin_popcountll(x[w] & q.ahi[(size_t)t * W + w]);
+    int quad = 0;
+    const u64* Qr = q.Q + (size_t)t * R * W;
+#pragma unroll
+    for (int wl = 0; wl < W; ++wl) {
+      for (int lb = 0; lb < 64; ++lb) {
+        const int l = 64 * wl + lb;
+        if (l >= n) break;
+        int p = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) p += __builtin_popcountll(Qr[(size_t)l * W + w] & x[w]);
+        quad ^= p & (int)(x[wl] >> lb);
+      }
+    }
+    const int ph = (p0 + 2 * ax + 4 * (quad & 1)) & 7;
+    if (per_term && active) per_term[(size_t)t * nb + b] = bad ? (uint8_t)0xFF : (uint8_t)ph;
+    // Scalar::to_complex then * omega (= 1 + 0i) then * coefficient
+    double zr = 0.0, zi = 0.0;
+    if (!bad) {
+      const double nrm = tabs[TAB_POW + k];
+      zr = s_cos[ph] * nrm;
+      zi = s_sin[ph] * nrm;
+    }
+    const double wr = zr * 1.0 - zi * 0.0, wi = zr * 0.0 + zi * 1.0;
+    const double2 c = coef[t];
+    const double tr = wr * c.x - wi * c.y, ti = wr * c.y + wi * c.x;
+    are = are + tr;
+    aim = aim + ti;
+  }
+  // * global_factor (statevector.rs:23)
+  if (active) out[b] = make_double2(are * gre - aim * gim, are * gim + aim * gre);
+}
+
+// ---------------------------------------------------------------------------
+// Pair kernel.
+template <int W>
+struct PairRegs {
+  u64 B[W][W];
+  int a[W];
+  u64 ca[W][W], cb[W][W];
+  u64 rhsa[W], rhsb[W];
+  u64 rem[W];
+  int c0, mag;
+  int lane;
+};
+
+template <int W>
+__device__ __forceinline__ int bcast_int(const int (&x)[W], int r) {
+  const int sl = r >> 6;
+  int val = x[0];
+#pragma unroll
+  for (int k = 1; k < W; ++k)
+    if (sl == k) val = x[k];
+  return rl32(val, r & 63);
+}
+
+// substitute x_p = c ^ <r, x> (r excludes p) into the form and the
+// remaining constraint rows
+template <int W>
+__device__ __forceinline__ void subst(PairRegs<W>& P, int p, const u64 (&r)[W], int c) {
+  const int ap = bcast_int<W>(P.a, p);
+  u64 Bp[W];
+  bcast<W>(P.B, p, Bp);
+  P.c0 = (P.c0 + 2 * ap * c) & 7;
+  const int sgn = 1 - 2 * c;
+  const bool aodd = ap & 1;
+#pragma unroll
+  for (int sl = 0; sl < W; ++sl) {
+    const int l = 64 * sl + P.lane;
+    const int rl = (int)((r[sl] >> P.lane) & 1ull);
+    const int bpl = getbit<W>(P.B[sl], p);
+    const bool live = ((P.rem[sl] >> P.lane) & 1ull) && l != p;
+    if (live) {
+      P.a[sl] = (P.a[sl] + ap * sgn * rl + 2 * (c * bpl + rl * bpl)) & 3;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        u64 d = 0;
+        if (rl) d ^= (aodd ? r[w] : 0ull) ^ Bp[w];
+        if (bpl) d ^= r[w];
+        P.B[sl][w] ^= d;
+      }
+      setbitv<W>(P.B[sl], l, 0);
+    }
+    setbitv<W>(P.B[sl], p, 0);
+    // constraint rows of both terms
+    const int ha = getbit<W>(P.ca[sl], p);
+    const int hb = getbit<W>(P.cb[sl], p);
+    if (ha) {
+#pragma unroll
+      for (int w = 0; w < W; ++w) P.ca[sl][w] ^= r[w];
+      setbitv<W>(P.ca[sl], p, 0);
+    }
+    if (hb) {
+#pragma unroll
+      for (int w = 0; w < W; ++w) P.cb[sl][w] ^= r[w];
+      setbitv<W>(P.cb[sl], p, 0);
+    }
+    const u64 ba = __ballot(ha), bb = __ballot(hb);
+    if (c) {
+      P.rhsa[sl] ^= ba;
+      P.rhsb[sl] ^= bb;
+    }
+  }
+  flipbit<W>(P.rem, p, getbit<W>(P.rem, p));
+}
+
+// returns 1 if a constraint is inconsistent (the sum is zero)
+template <int W>
+__device__ __forceinline__ int take_constraint(PairRegs<W>& P, int useb, int c) {
+  u64 row[W];
+  bcast<W>(useb ? P.cb : P.ca, c, row);
+  const int rhs = useb ? getbit<W>(P.rhsb, c) : getbit<W>(P.rhsa, c);
+  if (!anyv<W>(row)) return rhs;
+  const int p = ffsv<W>(row);
+  flipbit<W>(row, p, 1);
+  subst<W>(P, p, row, rhs);
+  return 0;
+}
+
+// exact <bra|ket>: returns zero flag, K (mod 8), r
+template <int W>
+__device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int* zero, int* K, int* R) {
+  for (int c = 0; c < nca; ++c)
+    if (take_constraint<W>(P, 0, c)) { *zero = 1; return; }
+  for (int c = 0; c < ncb; ++c)
+    if (take_constraint<W>(P, 1, c)) { *zero = 1; return; }
+  while (anyv<W>(P.rem)) {
+    const int p = uni(ffsv<W>(P.rem));
+    flipbit<W>(P.rem, p, 1);
+    u64 Bp[W];
+    bcast<W>(P.B, p, Bp);
+    const int ap = bcast_int<W>(P.a, p);
+    if (!anyv<W>(Bp)) {
+      if (ap == 2) { *zero = 1; return; }
+      if (ap == 0) P.mag -= 2;
+      else { P.mag -= 1; P.c0 = (P.c0 + (ap == 1 ? 1 : 7)) & 7; }
+    } else if (ap & 1) {
+      const int add = (ap == 1) ? 3 : 1;  // a_l -= sigma (sigma = +1 for a_p = 1, -1 for a_p = 3)
+      P.mag -= 1;
+      P.c0 = (P.c0 + (ap == 1 ? 1 : 7)) & 7;
+#pragma unroll
+      for (int sl = 0; sl < W; ++sl) {
+        const int l = 64 * sl + P.lane;
+        const int bpl = getbit<W>(P.B[sl], p);
+        const bool live = ((P.rem[sl] >> P.lane) & 1ull);
+        if (live && bpl) {
+          P.a[sl] = (P.a[sl] + add) & 3;
+#pragma unroll
+          for (int w = 0; w < W; ++w) P.B[sl][w] ^= Bp[w];
+          setbitv<W>(P.B[sl], l, 0);
+        }
+        setbitv<W>(P.B[sl], p, 0);
+      }
+    } else {
+      P.mag -= 2;
+#pragma unroll
+      for (int sl = 0; sl < W; ++sl) setbitv<W>(P.B[sl], p, 0);
+      const int qv = ffsv<W>(Bp);
+      flipbit<W>(Bp, qv, 1);
+      subst<W>(P, qv, Bp, ap >> 1);
+    }
+  }
+  *zero = 0;
+  *K = P.c0;
+  *R = P.mag;
+}
+
+template <int W>
+__device__ __forceinline__ void load_qrows(const QFormDev& q, int64_t t, int lane, u64 (&B)[W][W], int (&a)[W],
+                                           u64 (&C)[W][W], u64 (&rhs)[W]) {
+  const int R = 64 * W;
+#pragma unroll
+  for (int sl = 0; sl < W; ++sl) {
+    const int r = 64 * sl + lane;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      B[sl][w] = q.B[rowidx(t, R, r, W, w)];
+      C[sl][w] = q.C[rowidx(t, R, r, W, w)];
+    }
+    a[sl] = q.a[(size_t)t * R + r];
+  }
+#pragma unroll
+  for (int w = 0; w < W; ++w) rhs[w] = q.rhs[(size_t)t * W + w];
+}
+
+// Work decomposition: fixed index f in [0, nfixed) pairs with the other
+// indices o in [lo_f, nother), lo_f = f (triangle) or 0 (full), cut into
+// units of J; offsets[f] is the first unit of row f (prefix sum on the
+// host).  mode bit 0: fixed index is the ket (else the bra); mode bit 1:
+// Hermitian real sum (diagonal once, off-diagonal twice, norm.rs:16-27 and
+// exp_value.rs:38-58) else the complex sum (inner_product.rs:12-17).
+// Coefficients: exact Scalars (ck, cr, cz) when scalar != 0, else c.
+template <int W>
+__global__ __launch_bounds__(256) void k_pairs(QFormDev X, QFormDev Y, const int64_t* __restrict__ offsets,
+                                               int64_t nfixed, int64_t nother, int triangle, int J, int mode,
+                                               CoefDev cx, CoefDev cy, const double* __restrict__ tabs,
+                                               double2* __restrict__ partial, uint32_t* __restrict__ exact_out,
+                                               int64_t exact_ld) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t nwaves = (int64_t)gridDim.x * 4;
+  const int64_t nunits = offsets[nfixed];
+  const int n = X.n;
+  u64 valid[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const int hi = n - 64 * w;
+    valid[w] = hi >= 64 ? ~0ull : (hi <= 0 ? 0ull : ((1ull << hi) - 1ull));
+  }
+  const bool fixed_is_ket = mode & 1;
+  const bool herm = (mode >> 1) & 1;
+  double acc_re = 0.0, acc_im = 0.0;
+  for (int64_t u = wave; u < nunits; u += nwaves) {
+    // row f with offsets[f] <= u < offsets[f+1]
+    int64_t lo_r = 0, hi_r = nfixed;
+    while (hi_r - lo_r > 1) {
+      const int64_t mid = (lo_r + hi_r) >> 1;
+      if (offsets[mid] <= u) lo_r = mid;
+      else hi_r = mid;
+    }
+    const int64_t fx = lo_r;
+    const int64_t base = triangle ? fx : 0;
+    const int64_t o_lo = base + (u - offsets[fx]) * J;
+    const int64_t o_hi = (o_lo + J < nother) ? o_lo + J : nother;
+    const QFormDev& qf = fixed_is_ket ? Y : X;
+    const QFormDev& qo = fixed_is_ket ? X : Y;
+    u64 Bf[W][W], Cf[W][W], rf[W];
+    int af[W];
+    load_qrows<W>(qf, fx, lane, Bf, af, Cf, rf);
+    const int ncf = uni(qf.nc[fx]), kf = uni(qf.k[fx]), pf = uni(qf.p0[fx]);
+    for (int64_t o = o_lo; o < o_hi; ++o) {
+      u64 Bo[W][W], Co[W][W], ro[W];
+      int ao[W];
+      load_qrows<W>(qo, o, lane, Bo, ao, Co, ro);
+      const int nco = uni(qo.nc[o]), ko = uni(qo.k[o]), po = uni(qo.p0[o]);
+      const int64_t bra = fixed_is_ket ? o : fx;
+      const int64_t ket = fixed_is_ket ? fx : o;
+      PairRegs<W> P;
+      P.lane = lane;
+#pragma unroll
+      for (int sl = 0; sl < W; ++sl) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) P.B[sl][w] = Bf[sl][w] ^ Bo[sl][w];
+        // a = a_ket - a_bra
+        P.a[sl] = fixed_is_ket ? ((af[sl] - ao[sl]) & 3) : ((ao[sl] - af[sl]) & 3);
+      }
+      // constraints: bra rows in ca, ket rows in cb
+#pragma unroll
+      for (int sl = 0; sl < W; ++sl)
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          P.ca[sl][w] = fixed_is_ket ? Co[sl][w] : Cf[sl][w];
+          P.cb[sl][w] = fixed_is_ket ? Cf[sl][w] : Co[sl][w];
+        }
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        P.rhsa[w] = fixed_is_ket ? ro[w] : rf[w];
+        P.rhsb[w] = fixed_is_ket ? rf[w] : ro[w];
+        P.rem[w] = valid[w];
+      }
+      const int pbra = fixed_is_ket ? po : pf, pket = fixed_is_ket ? pf : po;
+      P.c0 = (pket - pbra) & 7;
+      P.mag = kf + ko;
+      const int nca = fixed_is_ket ? nco : ncf, ncb = fixed_is_ket ? ncf : nco;
+      int zero = 0, K = 0, Rr = 0;
+      pair_eval<W>(P, nca, ncb, &zero, &K, &Rr);
+      if (exact_out && lane == 0)
+        exact_out[(size_t)bra * exact_ld + ket] = zero ? 0xFFFFFFFFu : ((uint32_t)K | ((uint32_t)Rr << 8));
+      if (!zero) {
+        // ip = omega_a * omega_b * Scalar::to_complex (inner_product.rs:44-47)
+        const double nrm = tabs[TAB_POW + Rr];
+        const double ir = tabs[TAB_COS + K] * nrm, ii = tabs[TAB_SIN + K] * nrm;
+        // (c_bra.conj() * c_ket).into()
+        double cr_, ci_;
+        if (cx.scalar) {
+          const int zb = cx.cz[bra], zk = cy.cz[ket];
+          if (zb || zk) { cr_ = 0.0; ci_ = 0.0; }
+          else {
+            const int kk = ((8 - cx.ck[bra]) % 8 + cy.ck[ket]) % 8;
+            const int rr = cx.cr[bra] + cy.cr[ket];
+            const double nn = tabs[TAB_POW + rr];
+            cr_ = tabs[TAB_COS + kk] * nn;
+            ci_ = tabs[TAB_SIN + kk] * nn;
+          }
+        } else {
+          const double2 b0 = cx.c[bra], k0 = cy.c[ket];
+          const double br = b0.x, bi = -b0.y;
+          cr_ = br * k0.x - bi * k0.y;
+          ci_ = br * k0.y + bi * k0.x;
+        }
+        const double tr = cr_ * ir - ci_ * ii, ti = cr_ * ii + ci_ * ir;
+        if (herm) {
+          acc_re = (bra == ket) ? acc_re + tr : acc_re + (tr + tr);
+        } else {
+          acc_re = acc_re + tr;
+          acc_im = acc_im + ti;
+        }
+      }
+    }
+  }
+  if (lane == 0) partial[wave] = make_double2(acc_re, acc_im);
+}
+
+// single-block ordered reduction of the per-wave partial sums
+__global__ void k_reduce(const double2* __restrict__ partial, int64_t n, double2* __restrict__ out) {
+  __shared__ double sr[256], si[256];
+  double r = 0.0, i = 0.0;
+  for (int64_t k = threadIdx.x; k < n; k += blockDim.x) {
+    r += partial[k].x;
+    i += partial[k].y;
+  }
+  sr[threadIdx.x] = r;
+  si[threadIdx.x] = i;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sr[threadIdx.x] += sr[threadIdx.x + s];
+      si[threadIdx.x] += si[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = make_double2(sr[0], si[0]);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+
+#define NX_DISPATCH_W(W_, ...)                             \
+  switch (W_) {                                            \
+    case 1: { constexpr int WW = 1; __VA_ARGS__; } break;  \
+    case 2: { constexpr int WW = 2; __VA_ARGS__; } break;  \
+    case 3: { constexpr int WW = 3; __VA_ARGS__; } break;  \
+    case 4: { constexpr int WW = 4; __VA_ARGS__; } break;  \
+    default: return hipErrorInvalidValue;                  \
+  }
+
+static inline unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
+
+hipError_t launch_init_zero(const FormsDev& out, hipStream_t st) {
+  if (out.chi == 0) return hipSuccess;
+  NX_DISPATCH_W(out.W, k_init_zero<WW><<<dim3(blocks_for(out.chi, 4)), dim3(256), 0, st>>>(out));
+  return hipGetLastError();
+}
+
+hipError_t launch_program(const FormsDev& in, const FormsDev& out, const ProgOp* prog, int nprog, int4* status,
+                          hipStream_t st) {
+  if (in.chi == 0) return hipSuccess;
+  NX_DISPATCH_W(in.W, k_program<WW><<<dim3(blocks_for(in.chi, 4)), dim3(256), 0, st>>>(in, out, prog, nprog, status));
+  return hipGetLastError();
+}
+
+hipError_t launch_kron(const FormsDev& A, const FormsDev& B, const FormsDev& O, hipStream_t st) {
+  const int64_t total = O.chi * 64 * O.W;
+  if (total == 0) return hipSuccess;
+  k_kron<<<dim3(blocks_for(total, 256)), dim3(256), 0, st>>>(A, B, O);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const FormsDev& in, const FormsDev& O, const int64_t* idx, hipStream_t st) {
+  const int64_t total = O.chi * 64 * O.W;
+  if (total == 0) return hipSuccess;
+  k_gather<<<dim3(blocks_for(total, 256)), dim3(256), 0, st>>>(in, O, idx);
+  return hipGetLastError();
+}
+
+hipError_t launch_qform(const FormsDev& in, const QFormDev& q, hipStream_t st) {
+  if (in.chi == 0) return hipSuccess;
+  NX_DISPATCH_W(in.W, k_qform<WW><<<dim3(blocks_for(in.chi, 4)), dim3(256), 0, st>>>(in, q));
+  return hipGetLastError();
+}
+
+hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const u64* bits, int64_t nb, int64_t t0,
+                            int64_t t1, const double* tabs, double2* out, uint8_t* per_term, int enumerate,
+                            double gre, double gim, hipStream_t st) {
+  if (nb == 0) return hipSuccess;
+  NX_DISPATCH_W(q.W, k_amplitude<WW><<<dim3(blocks_for(nb, 256)), dim3(256), 0, st>>>(q, coef, bits, nb, t0, t1, tabs, out, per_term, enumerate, gre, gim));
+  return hipGetLastError();
+}
+
+hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, const int64_t* offsets, int64_t nfixed,
+                        int64_t nother, int triangle, int J, int mode, const CoefDev& cx, const CoefDev& cy,
+                        const double* tabs, double2* partial, int nblocks, uint32_t* exact_out, int64_t exact_ld,
+                        hipStream_t st) {
+  NX_DISPATCH_W(X.W, k_pairs<WW><<<dim3(nblocks), dim3(256), 0, st>>>(X, Y, offsets, nfixed, nother, triangle, J,
+                                                                      mode, cx, cy, tabs, partial, exact_out,
+                                                                      exact_ld));
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce(const double2* partial, int64_t n, double2* out, hipStream_t st) {
+  k_reduce<<<dim3(1), dim3(256), 0, st>>>(partial, n, out);
+  return hipGetLastError();
+}
+
+}  // namespace nx

@@ -1,0 +1,1151 @@
+// runtime.hip -- host side of the MI355X hot path behind the C-ABI in
+// include/necstar_hip.h.  It mirrors the reference's QuantumState facade
+// (crates/necstar/src/state/mod.rs) and compiler
+// (crates/necstar/src/state/compiler/mod.rs) with the per-term arithmetic
+// on the GPU: CH-form tables stay device resident; the host only keeps exact
+// coefficient scalars, gate lists, and the RNG draws of sampling.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+#include "necstar_hip.h"
+#include "rng.hpp"
+
+using cplx = std::complex<double>;
+
+namespace nx {
+
+// ---------------------------------------------------------------------------
+// errors
+thread_local std::string g_err;
+
+struct NxError {
+  int code;
+};
+
+[[noreturn]] static void fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  throw NxError{code};
+}
+
+static void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) fail(NX_E_DEVICE, "HIP error in %s: %s", what, hipGetErrorString(e));
+}
+
+// reference Display strings (crates/necstar/src/error/mod.rs,
+// crates/stabilizer-ch-form-rust/src/error/mod.rs)
+[[noreturn]] static void err_oob(int q, int n) {
+  fail(NX_E_VALUE, "Qubit index %d is out of bounds for %d qubits.", q, n);
+}
+[[noreturn]] static void err_dup_gate(int q) {
+  fail(NX_E_VALUE, "Two-qubit gate requires two different qubit indices, but got the same: %d.", q);
+}
+static const char* gate_name(int op) {
+  static const char* names[] = {"H", "X", "Y", "Z", "S", "Sdg", "SqrtX", "SqrtXdg", "CX", "CZ", "Swap",
+                                "T", "Tdg", "CCX"};
+  return (op >= 0 && op <= 13) ? names[op] : "?";
+}
+
+// ---------------------------------------------------------------------------
+// device context: stream, FP64 tables, caching allocator
+struct DeviceCtx {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  double* tabs = nullptr;
+  std::mutex mu;
+  std::map<size_t, std::vector<void*>> free_lists;
+};
+
+static std::mutex g_ctx_mu;
+static std::map<int, DeviceCtx*> g_ctx;
+static thread_local int t_dev = 0;
+
+static DeviceCtx* ctx() {
+  std::lock_guard<std::mutex> lk(g_ctx_mu);
+  auto it = g_ctx.find(t_dev);
+  if (it != g_ctx.end()) {
+    hip_check(hipSetDevice(t_dev), "hipSetDevice");
+    return it->second;
+  }
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= t_dev)
+    fail(NX_E_DEVICE, "no HIP device %d available (the necstar HIP path needs an MI355X / gfx950)", t_dev);
+  hip_check(hipSetDevice(t_dev), "hipSetDevice");
+  hipDeviceProp_t prop;
+  hip_check(hipGetDeviceProperties(&prop, t_dev), "hipGetDeviceProperties");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    fail(NX_E_DEVICE, "device %d is %s; this library is built for gfx950 only", t_dev, prop.gcnArchName);
+  DeviceCtx* c = new DeviceCtx();
+  c->dev = t_dev;
+  hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+  // PhaseFactor::to_complex (phase_factor.rs:19-22) and powf(2, -r/2)
+  // (scalar.rs:30-38): computed with the same libm calls on the host
+  std::vector<double> t(TAB_SIZE);
+  for (int k = 0; k < 8; ++k) {
+    const double angle = (double)k * M_PI_4;
+    t[TAB_COS + k] = std::cos(angle);
+    t[TAB_SIN + k] = std::sin(angle);
+  }
+  for (int r = -TAB_RMAX; r <= TAB_RMAX; ++r) t[TAB_POW + r] = std::pow(2.0, -((double)r) / 2.0);
+  hip_check(hipMalloc(&c->tabs, sizeof(double) * TAB_SIZE), "hipMalloc");
+  hip_check(hipMemcpy(c->tabs, t.data(), sizeof(double) * TAB_SIZE, hipMemcpyHostToDevice), "hipMemcpy");
+  g_ctx[t_dev] = c;
+  return c;
+}
+
+static size_t size_class(size_t b) {
+  size_t c = 256;
+  while (c < b) c <<= 1;
+  return c;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  DeviceCtx* c = nullptr;
+  DevBuf() = default;
+  explicit DevBuf(size_t bytes) { alloc(bytes); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), cap(o.cap), c(o.c) { o.p = nullptr; o.cap = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p; cap = o.cap; c = o.c;
+      o.p = nullptr; o.cap = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t bytes) {
+    release();
+    if (bytes == 0) return;
+    c = ctx();
+    cap = size_class(bytes);
+    {
+      std::lock_guard<std::mutex> lk(c->mu);
+      auto& fl = c->free_lists[cap];
+      if (!fl.empty()) {
+        p = fl.back();
+        fl.pop_back();
+        return;
+      }
+    }
+    if (hipMalloc(&p, cap) != hipSuccess) {
+      p = nullptr;
+      cap = 0;
+      fail(NX_E_DEVICE, "out of device memory (%zu bytes)", bytes);
+    }
+  }
+  void release() {
+    if (p && c) {
+      std::lock_guard<std::mutex> lk(c->mu);
+      c->free_lists[cap].push_back(p);
+    }
+    p = nullptr;
+    cap = 0;
+  }
+  template <class T>
+  T* as() const { return (T*)p; }
+};
+
+static hipStream_t stream() { return ctx()->stream; }
+static void sync() { hip_check(hipStreamSynchronize(stream()), "hipStreamSynchronize"); }
+static void h2d(void* d, const void* h, size_t b) {
+  if (b) hip_check(hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, stream()), "hipMemcpyAsync H2D");
+}
+static void d2h(void* h, const void* d, size_t b) {
+  if (b) hip_check(hipMemcpyAsync(h, d, b, hipMemcpyDeviceToHost, stream()), "hipMemcpyAsync D2H");
+}
+static void d2d(void* d, const void* s, size_t b) {
+  if (b) hip_check(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, stream()), "hipMemcpyAsync D2D");
+}
+
+static int words_for(int n) { return n <= 64 ? 1 : (n + 63) / 64; }
+static const int MAX_QUBITS = 256;
+
+// ---------------------------------------------------------------------------
+// term tables
+struct Forms {
+  DevBuf G, F, M, g, v, s, ph;
+  int64_t chi = 0;
+  int n = 0, W = 1;
+  void alloc(int64_t chi_, int n_) {
+    chi = chi_;
+    n = n_;
+    W = words_for(n_);
+    const size_t R = 64 * (size_t)W;
+    G.alloc(chi * R * W * 8);
+    F.alloc(chi * R * W * 8);
+    M.alloc(chi * R * W * 8);
+    g.alloc(chi * R);
+    v.alloc(chi * W * 8);
+    s.alloc(chi * W * 8);
+    ph.alloc(chi);
+  }
+  FormsDev desc() const {
+    FormsDev d;
+    d.G = G.as<uint64_t>(); d.F = F.as<uint64_t>(); d.M = M.as<uint64_t>();
+    d.g = g.as<uint8_t>(); d.v = v.as<uint64_t>(); d.s = s.as<uint64_t>(); d.ph = ph.as<uint8_t>();
+    d.chi = chi; d.n = n; d.W = W;
+    return d;
+  }
+};
+
+struct QForms {
+  DevBuf Q, B, C, a, alo, ahi, rhs, nc, k, p0;
+  int64_t chi = 0;
+  int n = 0, W = 1;
+  void alloc(int64_t chi_, int n_) {
+    chi = chi_;
+    n = n_;
+    W = words_for(n_);
+    const size_t R = 64 * (size_t)W;
+    Q.alloc(chi * R * W * 8);
+    B.alloc(chi * R * W * 8);
+    C.alloc(chi * R * W * 8);
+    a.alloc(chi * R);
+    alo.alloc(chi * W * 8);
+    ahi.alloc(chi * W * 8);
+    rhs.alloc(chi * W * 8);
+    nc.alloc(chi * 4);
+    k.alloc(chi * 4);
+    p0.alloc(chi);
+  }
+  QFormDev desc() const {
+    QFormDev d;
+    d.Q = Q.as<uint64_t>(); d.B = B.as<uint64_t>(); d.C = C.as<uint64_t>(); d.a = a.as<uint8_t>();
+    d.alo = alo.as<uint64_t>(); d.ahi = ahi.as<uint64_t>(); d.rhs = rhs.as<uint64_t>();
+    d.nc = nc.as<int32_t>(); d.k = k.as<int32_t>(); d.p0 = p0.as<uint8_t>();
+    d.chi = chi; d.n = n; d.W = W;
+    return d;
+  }
+};
+
+static Forms init_zero(int64_t chi, int n) {
+  Forms f;
+  f.alloc(chi, n);
+  hip_check(launch_init_zero(f.desc(), stream()), "k_init_zero");
+  return f;
+}
+
+// Runs a program on every term.  n_out: qubit count after the program.
+static Forms run_program(Forms& in, const std::vector<ProgOp>& prog, int n_out, std::vector<int4>* status) {
+  Forms out;
+  const bool inplace = words_for(n_out) == in.W;
+  if (!inplace) out.alloc(in.chi, n_out);
+  DevBuf dprog(prog.size() * sizeof(ProgOp));
+  DevBuf dstat(in.chi * sizeof(int4));
+  h2d(dprog.p, prog.data(), prog.size() * sizeof(ProgOp));
+  FormsDev od = inplace ? in.desc() : out.desc();
+  hip_check(launch_program(in.desc(), od, dprog.as<ProgOp>(), (int)prog.size(), dstat.as<int4>(), stream()),
+            "k_program");
+  std::vector<int4> st(in.chi);
+  d2h(st.data(), dstat.p, in.chi * sizeof(int4));
+  sync();
+  for (int64_t i = 0; i < in.chi; ++i) {
+    if (st[i].w == DEV_ERR_SAME_VEC_MINUS) fail(NX_E_INTERNAL, "Inconsistent state: superposition with -1 coefficient.");
+    if (st[i].w == DEV_ERR_G_ROW) fail(NX_E_INTERNAL, "Cannot zero G row due to G matrix state.");
+    if (st[i].w == DEV_ERR_G_COL) fail(NX_E_INTERNAL, "Cannot zero G column due to v vector state.");
+    if (st[i].w != 0 && st[i].w != DEV_ERR_CANNOT_DISCARD) fail(NX_E_INTERNAL, "device program error %d", st[i].w);
+  }
+  if (status) *status = std::move(st);
+  if (inplace) {
+    in.n = n_out;
+    return std::move(in);
+  }
+  out.n = n_out;
+  return out;
+}
+
+static Forms gather(const Forms& in, const std::vector<int64_t>& idx) {
+  Forms out;
+  out.alloc((int64_t)idx.size(), in.n);
+  if (idx.empty()) return out;
+  DevBuf didx(idx.size() * 8);
+  h2d(didx.p, idx.data(), idx.size() * 8);
+  hip_check(launch_gather(in.desc(), out.desc(), didx.as<int64_t>(), stream()), "k_gather");
+  sync();
+  return out;
+}
+
+static Forms clone_forms(const Forms& in) {
+  Forms out;
+  out.alloc(in.chi, in.n);
+  const size_t R = 64 * (size_t)in.W;
+  d2d(out.G.p, in.G.p, in.chi * R * in.W * 8);
+  d2d(out.F.p, in.F.p, in.chi * R * in.W * 8);
+  d2d(out.M.p, in.M.p, in.chi * R * in.W * 8);
+  d2d(out.g.p, in.g.p, in.chi * R);
+  d2d(out.v.p, in.v.p, in.chi * in.W * 8);
+  d2d(out.s.p, in.s.p, in.chi * in.W * 8);
+  d2d(out.ph.p, in.ph.p, in.chi);
+  return out;
+}
+
+static Forms concat(const std::vector<const Forms*>& parts) {
+  Forms out;
+  int64_t chi = 0;
+  for (auto* p : parts) chi += p->chi;
+  out.alloc(chi, parts[0]->n);
+  const size_t R = 64 * (size_t)out.W, W = out.W;
+  int64_t off = 0;
+  for (auto* p : parts) {
+    d2d(out.G.as<uint8_t>() + off * R * W * 8, p->G.p, p->chi * R * W * 8);
+    d2d(out.F.as<uint8_t>() + off * R * W * 8, p->F.p, p->chi * R * W * 8);
+    d2d(out.M.as<uint8_t>() + off * R * W * 8, p->M.p, p->chi * R * W * 8);
+    d2d(out.g.as<uint8_t>() + off * R, p->g.p, p->chi * R);
+    d2d(out.v.as<uint8_t>() + off * W * 8, p->v.p, p->chi * W * 8);
+    d2d(out.s.as<uint8_t>() + off * W * 8, p->s.p, p->chi * W * 8);
+    d2d(out.ph.as<uint8_t>() + off, p->ph.p, p->chi);
+    off += p->chi;
+  }
+  return out;
+}
+
+static Forms kron(const Forms& A, const Forms& B) {
+  Forms out;
+  out.alloc(A.chi * B.chi, A.n + B.n);
+  hip_check(launch_kron(A.desc(), B.desc(), out.desc(), stream()), "k_kron");
+  return out;
+}
+
+static QForms make_qform(const Forms& f) {
+  QForms q;
+  q.alloc(f.chi, f.n);
+  hip_check(launch_qform(f.desc(), q.desc(), stream()), "k_qform");
+  return q;
+}
+
+// ---------------------------------------------------------------------------
+// exact coefficients (crates/necstar/src/state/types/scalar.rs)
+struct Scal {
+  uint8_t z = 0, k = 0;
+  int64_t r = 0;
+};
+static Scal smul(Scal a, Scal b) {
+  Scal o;
+  o.z = a.z || b.z;
+  o.k = o.z ? 0 : (uint8_t)((a.k + b.k) % 8);
+  o.r = o.z ? 0 : a.r + b.r;
+  return o;
+}
+// Scalar::to_complex (scalar.rs:30-38)
+static cplx scal_value(Scal s) {
+  if (s.z) return cplx(0.0, 0.0);
+  const double nrm = std::pow(2.0, -((double)s.r) / 2.0);
+  const double angle = (double)s.k * M_PI_4;
+  return cplx(std::cos(angle) * nrm, std::sin(angle) * nrm);
+}
+
+// num-complex products (no FMA, same operand order)
+static cplx cmul(cplx a, cplx b) {
+  return cplx(a.real() * b.real() - a.imag() * b.imag(), a.real() * b.imag() + a.imag() * b.real());
+}
+
+}  // namespace nx
+
+// ---------------------------------------------------------------------------
+// the state object
+struct nx_state {
+  int n = 0;
+  nx::Forms forms;
+  bool is_complex = false;
+  std::vector<nx::Scal> sc;
+  std::vector<cplx> cc;
+  cplx g{1.0, 0.0};
+  // caches (device)
+  mutable bool q_ok = false;
+  mutable nx::QForms q;
+  mutable bool c_ok = false;
+  mutable nx::DevBuf d_c, d_cz, d_ck, d_cr;
+
+  int64_t chi() const { return forms.chi; }
+  void invalidate() { q_ok = false; c_ok = false; }
+  cplx coeff_value(int64_t i) const { return is_complex ? cc[i] : nx::scal_value(sc[i]); }
+};
+
+namespace nx {
+
+static const QForms& qform_of(const nx_state* s) {
+  if (!s->q_ok) {
+    s->q = make_qform(s->forms);
+    s->q_ok = true;
+  }
+  return s->q;
+}
+
+static CoefDev coef_of(const nx_state* s) {
+  const int64_t chi = s->chi();
+  if (!s->c_ok) {
+    std::vector<double> c(2 * chi);
+    for (int64_t i = 0; i < chi; ++i) {
+      cplx v = s->coeff_value(i);
+      c[2 * i] = v.real();
+      c[2 * i + 1] = v.imag();
+    }
+    s->d_c.alloc(chi * 16);
+    h2d(s->d_c.p, c.data(), chi * 16);
+    if (!s->is_complex) {
+      std::vector<uint8_t> z(chi), k(chi);
+      std::vector<int32_t> r(chi);
+      for (int64_t i = 0; i < chi; ++i) {
+        z[i] = s->sc[i].z;
+        k[i] = s->sc[i].k;
+        if (s->sc[i].r < -TAB_RMAX / 2 || s->sc[i].r > TAB_RMAX / 2)
+          fail(NX_E_INTERNAL, "coefficient exponent %lld outside the supported range", (long long)s->sc[i].r);
+        r[i] = (int32_t)s->sc[i].r;
+      }
+      s->d_cz.alloc(chi);
+      s->d_ck.alloc(chi);
+      s->d_cr.alloc(chi * 4);
+      h2d(s->d_cz.p, z.data(), chi);
+      h2d(s->d_ck.p, k.data(), chi);
+      h2d(s->d_cr.p, r.data(), chi * 4);
+      sync();
+    }
+    sync();
+    s->c_ok = true;
+  }
+  CoefDev d;
+  d.scalar = s->is_complex ? 0 : 1;
+  d.cz = s->d_cz.as<uint8_t>();
+  d.ck = s->d_ck.as<uint8_t>();
+  d.cr = s->d_cr.as<int32_t>();
+  d.c = s->d_c.as<double2>();
+  return d;
+}
+
+// sum over pairs; triangle: o in [f, nother) else [0, nother)
+static cplx run_pairs(const QForms& X, const QForms& Y, int64_t nfixed, int64_t nother, bool triangle, int mode,
+                      const CoefDev& cx, const CoefDev& cy, uint32_t* d_exact = nullptr, int64_t ld = 0) {
+  if (nfixed == 0 || nother == 0) return cplx(0.0, 0.0);
+  const int J = 16;
+  std::vector<int64_t> off(nfixed + 1);
+  off[0] = 0;
+  for (int64_t f = 0; f < nfixed; ++f) {
+    const int64_t cnt = triangle ? nother - f : nother;
+    off[f + 1] = off[f] + (cnt + J - 1) / J;
+  }
+  const int64_t units = off[nfixed];
+  DevBuf doff(off.size() * 8);
+  h2d(doff.p, off.data(), off.size() * 8);
+  int nblocks = (int)std::min<int64_t>((units + 3) / 4, 4096);
+  if (nblocks < 1) nblocks = 1;
+  DevBuf partial((size_t)nblocks * 4 * 16), out(16);
+  hip_check(launch_pairs(X.desc(), Y.desc(), doff.as<int64_t>(), nfixed, nother, triangle ? 1 : 0, J, mode, cx, cy,
+                         ctx()->tabs, partial.as<double2>(), nblocks, d_exact, ld, stream()),
+            "k_pairs");
+  hip_check(launch_reduce(partial.as<double2>(), (int64_t)nblocks * 4, out.as<double2>(), stream()), "k_reduce");
+  double h[2];
+  d2h(h, out.p, 16);
+  sync();
+  return cplx(h[0], h[1]);
+}
+
+// norm.rs:8-30
+static double norm_squared(const nx_state* s) {
+  if (s->chi() == 0) return 0.0 * std::norm(s->g);
+  const QForms& q = qform_of(s);
+  CoefDev c = coef_of(s);
+  cplx sum = run_pairs(q, q, s->chi(), s->chi(), true, /*fixed bra, herm*/ 2, c, c);
+  return sum.real() * (s->g.real() * s->g.real() + s->g.imag() * s->g.imag());
+}
+
+// ---------------------------------------------------------------------------
+// gate validation in the reference's error order (left_multiplication/*.rs)
+static void validate_gate(int op, int q0, int q1, int n) {
+  switch (op) {
+    case OP_CX:
+    case OP_CZ:
+    case OP_SWAP:
+      if (q0 < 0 || q0 >= n) err_oob(q0, n);
+      if (q1 < 0 || q1 >= n) err_oob(q1, n);
+      if (q0 == q1) err_dup_gate(q0);
+      return;
+    default:
+      if (q0 < 0 || q0 >= n) err_oob(q0, n);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// |T>^{(x)t} decomposition (magic_states/cat_state.rs, t_state.rs)
+struct Decomp {
+  Forms f;
+  std::vector<Scal> c;
+};
+
+static Forms single(int m, const std::vector<ProgOp>& prog) {
+  Forms f = init_zero(1, m);
+  if (prog.empty()) return f;
+  return run_program(f, prog, m, nullptr);
+}
+
+static std::vector<ProgOp> zmio_prog(int m) {  // zero_minus_i_one_state
+  std::vector<ProgOp> p{{OP_H, 0, 0, 0}};
+  for (int i = 1; i < m; ++i) p.push_back({OP_CX, 0, i, 0});
+  p.push_back({OP_SDG, 0, 0, 0});
+  return p;
+}
+static std::vector<ProgOp> even_parity_prog(int m, bool flipped) {
+  std::vector<ProgOp> p;
+  for (int i = 0; i < m - 1; ++i) p.push_back({OP_H, i, 0, 0});
+  for (int i = 0; i < m - 1; ++i) p.push_back({OP_CX, i, m - 1, 0});
+  if (flipped)
+    for (int i = 0; i < m; ++i)
+      for (int j = i + 1; j < m; ++j) p.push_back({OP_CZ, i, j, 0});
+  return p;
+}
+
+static void check_alive(const std::vector<int4>& st) {
+  for (auto& x : st)
+    if (x.x || x.w) fail(NX_E_INTERNAL, "magic-state construction hit an impossible projection");
+}
+
+static Decomp cat_state(int m) {
+  Decomp d;
+  switch (m) {
+    case 1:
+      d.f = init_zero(1, 1);
+      d.c = {Scal{0, 0, 0}};
+      return d;
+    case 2:
+      d.f = single(2, {{OP_H, 0, 0, 0}, {OP_CX, 0, 1, 0}, {OP_S, 1, 0, 0}});
+      d.c = {Scal{0, 0, 0}};
+      return d;
+    case 4: {
+      Forms a = single(4, zmio_prog(4)), b = single(4, even_parity_prog(4, false));
+      d.f = concat({&a, &b});
+      d.c = {Scal{0, 7, 1}, Scal{0, 2, 0}};
+      return d;
+    }
+    case 6: {
+      Forms a = single(6, zmio_prog(6)), b = single(6, even_parity_prog(6, false)),
+            c = single(6, even_parity_prog(6, true));
+      d.f = concat({&a, &b, &c});
+      d.c = {Scal{0, 0, 2}, Scal{0, 3, 1}, Scal{0, 5, 1}};
+      return d;
+    }
+    case 3:
+    case 5: {  // reduce_cat_state
+      Decomp big = cat_state(m + 1);
+      std::vector<int4> st;
+      d.f = run_program(big.f, {{OP_PROJ, m, 0, 0}, {OP_DISC, m, 0, 0}}, m, &st);
+      check_alive(st);
+      d.c = big.c;
+      return d;
+    }
+    default: {
+      Decomp a = cat_state(m - 4), b = cat_state(6);
+      Forms k = kron(a.f, b.f);
+      for (auto& ca : a.c)
+        for (auto& cb : b.c) d.c.push_back(smul(ca, cb));
+      const int q0 = m - 5, q1 = m - 4;
+      std::vector<int4> st;
+      d.f = run_program(k,
+                        {{OP_SDG, q0, 0, 0}, {OP_CX, q0, q1, 0}, {OP_H, q0, 0, 0}, {OP_PROJ, q0, 0, 0},
+                         {OP_PROJ, q1, 0, 0}, {OP_DISC, q1, 0, 0}, {OP_DISC, q0, 0, 0}},
+                        m, &st);
+      check_alive(st);
+      return d;
+    }
+  }
+}
+
+static Decomp t_tensor_state(int t) {
+  Decomp cat = cat_state(t);
+  Forms xs = clone_forms(cat.f);
+  xs = run_program(xs, {{OP_X, 0, 0, 0}, {OP_S, 0, 0, 0}}, t, nullptr);
+  Decomp d;
+  d.f = concat({&cat.f, &xs});
+  for (auto& c : cat.c) d.c.push_back(smul(c, Scal{0, 0, 1}));            // * ONE_OVER_SQRT_2
+  for (auto& c : cat.c) d.c.push_back(smul(smul(c, Scal{0, 0, 1}), Scal{0, 7, 0}));  // * e^{-i pi/4}
+  return d;
+}
+
+// ---------------------------------------------------------------------------
+// compile (compiler/mod.rs:44-140)
+static nx_state* compile(int n, int64_t ngates, const int32_t* gates) {
+  std::vector<ProgOp> ops;
+  int t = 0;
+  for (int64_t i = 0; i < ngates; ++i) {
+    const int op = gates[4 * i], q0 = gates[4 * i + 1], q1 = gates[4 * i + 2];
+    if (op >= OP_H && op <= OP_SWAP) {
+      ops.push_back({op, q0, q1, 0});
+    } else if (op == 11 || op == 12) {
+      const int anc = n + t;
+      ops.push_back({OP_CX, q0, anc, 0});
+      if (op == 12) ops.push_back({OP_SDG, q0, 0, 0});
+      ++t;
+    } else {
+      fail(NX_E_VALUE, "Gate %s is not supported.", gate_name(op));
+    }
+  }
+  if (n <= 0) fail(NX_E_VALUE, "Number of qubits must be greater than zero, got %d.", n);
+  const int ntot = n + t;
+  if (ntot > MAX_QUBITS) fail(NX_E_VALUE, "circuits above %d qubits (including one ancilla per T gate) are not supported", MAX_QUBITS);
+  for (auto& o : ops) validate_gate(o.code, o.a, o.b, ntot);
+  nx_state* s = new nx_state();
+  try {
+    s->n = n;
+    if (t == 0) {
+      s->forms = single(n, ops);
+      s->sc = {Scal{0, 0, 0}};
+      return s;
+    }
+    Decomp ts = t_tensor_state(t);
+    Forms zero = init_zero(1, n);
+    Forms full = kron(zero, ts.f);
+    std::vector<ProgOp> prog = ops;
+    for (int q = ntot - 1; q >= n; --q) prog.push_back({OP_PROJ, q, 0, 0});
+    for (int q = ntot - 1; q >= n; --q) prog.push_back({OP_DISC, q, 0, 0});
+    std::vector<int4> st;
+    Forms out = run_program(full, prog, n, &st);
+    std::vector<int64_t> keep;
+    for (int64_t i = 0; i < out.chi; ++i)
+      if (!st[i].x) {
+        if (st[i].w) fail(NX_E_INTERNAL, "Cannot discard qubit after successful projection");
+        keep.push_back(i);
+        Scal c = ts.c[i];
+        if (!c.z) c.r -= st[i].y;  // amplify(#deterministic)
+        s->sc.push_back(c);
+      }
+    s->forms = gather(out, keep);
+    s->forms.n = n;
+    return s;
+  } catch (...) {
+    delete s;
+    throw;
+  }
+}
+
+// ---------------------------------------------------------------------------
+static void apply_program_all(nx_state* s, const std::vector<ProgOp>& prog) {
+  if (prog.empty() || s->chi() == 0) return;
+  s->forms = run_program(s->forms, prog, s->n, nullptr);
+  s->invalidate();
+}
+
+// projection.rs:27-57
+static void project_unnormalized(nx_state* s, int q, int outcome) {
+  if (q < 0 || q >= s->n) err_oob(q, s->n);
+  if (s->chi() == 0) return;
+  std::vector<int4> st;
+  Forms f = run_program(s->forms, {{OP_PROJ, q, outcome ? 1 : 0, 0}}, s->n, &st);
+  std::vector<int64_t> keep;
+  std::vector<Scal> sc;
+  std::vector<cplx> cc;
+  for (int64_t i = 0; i < f.chi; ++i) {
+    if (st[i].x) continue;
+    keep.push_back(i);
+    const bool rnd = st[i].z != 0;
+    if (s->is_complex) {
+      cplx c = s->cc[i];
+      if (rnd) {  // Amplify for Complex64 (coefficient.rs:28-34)
+        const double scale = std::pow(2.0, -1.0 / 2.0);
+        c = cplx(c.real() * scale, c.imag() * scale);
+      }
+      cc.push_back(c);
+    } else {
+      Scal c = s->sc[i];
+      if (rnd && !c.z) c.r += 1;  // amplify(-1)
+      sc.push_back(c);
+    }
+  }
+  if ((int64_t)keep.size() == f.chi) s->forms = std::move(f);
+  else s->forms = gather(f, keep);
+  s->sc = std::move(sc);
+  s->cc = std::move(cc);
+  s->invalidate();
+}
+
+// stabilizer_decomposed_state/discard.rs:5-12
+static void discard(nx_state* s, int q) {
+  if (s->chi() > 0) {
+    if (q < 0 || q >= s->n) err_oob(q, s->n);
+    std::vector<int4> st;
+    Forms f = clone_forms(s->forms);
+    f = run_program(f, {{OP_DISC, q, 0, 0}}, s->n - 1, &st);
+    for (auto& x : st)
+      if (x.w == DEV_ERR_CANNOT_DISCARD) fail(NX_E_VALUE, "Cannot discard qubit %d because it is not in a proper state", q);
+    s->forms = std::move(f);
+  } else {
+    s->forms.n = s->n - 1;
+  }
+  s->n -= 1;
+  s->invalidate();
+}
+
+static nx_state* clone_state(const nx_state* s) {
+  nx_state* c = new nx_state();
+  c->n = s->n;
+  c->forms = clone_forms(s->forms);
+  c->forms.n = s->forms.n;
+  c->is_complex = s->is_complex;
+  c->sc = s->sc;
+  c->cc = s->cc;
+  c->g = s->g;
+  return c;
+}
+
+struct StateGuard {
+  nx_state* p;
+  explicit StateGuard(nx_state* s) : p(s) {}
+  ~StateGuard() { delete p; }
+  StateGuard(const StateGuard&) = delete;
+  nx_state* release() { nx_state* r = p; p = nullptr; return r; }
+};
+
+// stabilizer_decomposed_state/mod.rs:44-67
+static void validate_qargs(const nx_state* s, int nq, const int32_t* qargs) {
+  if (nq == 0) fail(NX_E_VALUE, "Qubit index list is empty.");
+  for (int i = 0; i < nq; ++i)
+    if (qargs[i] < 0 || qargs[i] >= s->n) err_oob(qargs[i], s->n);
+  std::vector<int> v(qargs, qargs + nq);
+  std::sort(v.begin(), v.end());
+  for (size_t i = 0; i + 1 < v.size(); ++i)
+    if (v[i] == v[i + 1]) fail(NX_E_VALUE, "Duplicate qubit index found: %d.", v[i]);
+}
+
+static void seed_or_entropy(const uint8_t* seed32, uint8_t out[32]) {
+  if (seed32) {
+    std::memcpy(out, seed32, 32);
+    return;
+  }
+  std::random_device rd;
+  for (int i = 0; i < 32; i += 4) {
+    uint32_t x = rd();
+    std::memcpy(out + i, &x, 4);
+  }
+}
+
+// measurement.rs:34-103
+static int measure_single(nx_state* s, int q, ChaCha12Rng& rng) {
+  StateGuard zero(clone_state(s)), one(clone_state(s));
+  project_unnormalized(zero.p, q, 0);
+  project_unnormalized(one.p, q, 1);
+  const double p0 = norm_squared(zero.p), p1 = norm_squared(one.p);
+  const double total = p0 + p1;
+  if (std::fabs(total) < 1e-12) fail(NX_E_VALUE, "Not implemented: Measurement on a zero-norm state is not possible.");
+  const double pz = p0 / total;
+  const double sample = rng.gen_f64();
+  const int res = sample < pz ? 0 : 1;
+  const double nsq = res ? p1 : p0;
+  nx_state* chosen = res ? one.release() : zero.release();
+  s->forms = std::move(chosen->forms);
+  s->sc = std::move(chosen->sc);
+  s->cc = std::move(chosen->cc);
+  s->g = chosen->g;
+  delete chosen;
+  s->invalidate();
+  const double nrm = std::sqrt(nsq);
+  if (std::fabs(nrm) < 1e-12)
+    fail(NX_E_VALUE, "Not implemented: Measurement resulted in a zero-norm state, which is not physical.");
+  s->g = cmul(s->g, cplx(1.0 / nrm, 0.0));
+  return res;
+}
+
+struct SampleCtx {
+  std::vector<int> q, bit;
+  std::vector<uint64_t> bits, counts;
+  ChaCha12Rng* rng;
+};
+
+// sampling.rs:80-189
+static void recursive_sample(const nx_state* s, SampleCtx& cx, size_t idx, uint64_t shots, uint64_t lo,
+                             uint64_t hi) {
+  if (shots == 0) return;
+  if (idx == cx.q.size()) {
+    cx.bits.push_back(lo);
+    cx.bits.push_back(hi);
+    cx.counts.push_back(shots);
+    return;
+  }
+  const int q = cx.q[idx], b = cx.bit[idx];
+  StateGuard zero(clone_state(s)), one(clone_state(s));
+  project_unnormalized(zero.p, q, 0);
+  project_unnormalized(one.p, q, 1);
+  // the reference evaluates state_zero.norm_squared() twice; it is a pure
+  // function of the state, so one evaluation gives the same value
+  const double nz = norm_squared(zero.p), no = norm_squared(one.p);
+  double pz = nz / (nz + no);
+  if (!std::isnan(pz)) pz = pz < 0.0 ? 0.0 : (pz > 1.0 ? 1.0 : pz);
+  if (!(pz >= 0.0)) fail(NX_E_VALUE, "p < 0 or is NaN in binomial distribution");
+  const uint64_t nzeros = binomial(*cx.rng, shots, pz);
+  const uint64_t nones = shots - nzeros;
+  if (nzeros > 0) {
+    discard(zero.p, q);
+    recursive_sample(zero.p, cx, idx + 1, nzeros, lo, hi);
+  }
+  if (nones > 0) {
+    uint64_t nlo = lo, nhi = hi;
+    if (b < 64) nlo |= 1ull << b;
+    else nhi |= 1ull << (b - 64);
+    apply_program_all(one.p, {{OP_X, q, 0, 0}});
+    discard(one.p, q);
+    recursive_sample(one.p, cx, idx + 1, nones, nlo, nhi);
+  }
+}
+
+// Pauli program (gate_application.rs:186-210) with the length checks of
+// exp_value.rs:9-27
+static std::vector<ProgOp> pauli_prog(const nx_state* s, int dense, int32_t len, const int32_t* ops,
+                                      const int32_t* qubits) {
+  std::vector<ProgOp> p;
+  if (dense) {
+    if (len != s->n) fail(NX_E_VALUE, "Invalid Pauli string length: expected %d, found %d.", s->n, len);
+    for (int i = 0; i < len; ++i)
+      if (ops[i] >= 1 && ops[i] <= 3) p.push_back({ops[i] == 1 ? OP_X : (ops[i] == 2 ? OP_Y : OP_Z), i, 0, 0});
+  } else {
+    int mx = 0;
+    for (int i = 0; i < len; ++i) mx = std::max(mx, (int)qubits[i]);
+    if (mx >= s->n) fail(NX_E_VALUE, "Invalid Pauli string length: expected %d, found %d.", s->n, mx + 1);
+    for (int i = 0; i < len; ++i)
+      if (ops[i] >= 1 && ops[i] <= 3) p.push_back({ops[i] == 1 ? OP_X : (ops[i] == 2 ? OP_Y : OP_Z), qubits[i], 0, 0});
+  }
+  return p;
+}
+
+// exp_value.rs:7-61: sum over i <= j of c_j* c_i <phi_j|P phi_i>
+static double exp_value(const nx_state* s, const std::vector<ProgOp>& prog) {
+  if (s->chi() == 0) return 0.0;
+  const QForms& q = qform_of(s);
+  Forms pf = clone_forms(s->forms);
+  if (!prog.empty()) pf = run_program(pf, prog, s->n, nullptr);
+  QForms qp = make_qform(pf);
+  CoefDev c = coef_of(s);
+  // fixed index = ket i (the Pauli image), bras j in [i, chi)
+  cplx sum = run_pairs(q, qp, s->chi(), s->chi(), true, /*fixed ket | herm*/ 3, c, c);
+  return sum.real() * (s->g.real() * s->g.real() + s->g.imag() * s->g.imag());
+}
+
+}  // namespace nx
+
+// ---------------------------------------------------------------------------
+// C-ABI
+using namespace nx;
+
+#define NX_TRY(...)                                   \
+  try {                                               \
+    __VA_ARGS__;                                      \
+    return NX_OK;                                     \
+  } catch (const NxError& e) {                        \
+    return e.code;                                    \
+  } catch (const std::bad_alloc&) {                   \
+    g_err = "host out of memory";                     \
+    return NX_E_INTERNAL;                             \
+  } catch (const std::exception& e) {                 \
+    g_err = e.what();                                 \
+    return NX_E_INTERNAL;                             \
+  }
+
+extern "C" {
+
+const char* nx_last_error(void) { return g_err.c_str(); }
+
+int nx_init(int device) { NX_TRY(t_dev = device; (void)ctx()) }
+
+int nx_state_compile(int num_qubits, int64_t ngates, const int32_t* gates, nx_state** out) {
+  NX_TRY(*out = compile(num_qubits, ngates, gates))
+}
+
+int nx_state_clone(const nx_state* s, nx_state** out) { NX_TRY(*out = clone_state(s)) }
+
+void nx_state_free(nx_state* s) { delete s; }
+
+int nx_state_num_qubits(const nx_state* s, int32_t* out) { NX_TRY(*out = s->n) }
+
+int nx_state_stabilizer_rank(const nx_state* s, int64_t* out) { NX_TRY(*out = s->chi()) }
+
+int nx_state_apply_gates(nx_state* s, int64_t ngates, const int32_t* gates) {
+  NX_TRY({
+    std::vector<ProgOp> prog;
+    for (int64_t i = 0; i < ngates; ++i) {
+      const int op = gates[4 * i], q0 = gates[4 * i + 1], q1 = gates[4 * i + 2];
+      bool bad = false;
+      std::string msg;
+      int code = NX_OK;
+      if (op == 11 || op == 12 || op == 13) {
+        bad = true;
+        char buf[160];
+        snprintf(buf, sizeof buf, "Direct application of non-Clifford gate %s is not supported.", gate_name(op));
+        msg = buf;
+        code = NX_E_VALUE;
+      } else if (op < 0 || op > 10) {
+        bad = true;
+        msg = "unknown gate code";
+        code = NX_E_VALUE;
+      } else if (s->chi() > 0) {
+        try {
+          validate_gate(op, q0, q1, s->n);
+        } catch (const NxError& e) {
+          bad = true;
+          msg = g_err;
+          code = e.code;
+        }
+      }
+      if (bad) {
+        apply_program_all(s, prog);  // gates before the failing one are applied
+        g_err = msg;
+        throw NxError{code};
+      }
+      prog.push_back({op, q0, q1, 0});
+    }
+    apply_program_all(s, prog);
+  })
+}
+
+int nx_state_norm_squared(const nx_state* s, double* out) { NX_TRY(*out = norm_squared(s)) }
+
+int nx_state_norm(const nx_state* s, double* out) { NX_TRY(*out = std::sqrt(norm_squared(s))) }
+
+int nx_state_exp_value(const nx_state* s, int dense, int32_t len, const int32_t* ops, const int32_t* qubits,
+                       double* out) {
+  NX_TRY(*out = exp_value(s, pauli_prog(s, dense, len, ops, qubits)))
+}
+
+int nx_state_inner_product(const nx_state* a, const nx_state* b, double* re, double* im) {
+  NX_TRY({
+    cplx res(0.0, 0.0);
+    if (a->chi() > 0 && b->chi() > 0) {
+      if (a->n != b->n)
+        fail(NX_E_VALUE, "The qubit counts of the two states must match for calculating inner product, got %d and %d.",
+             a->n, b->n);
+      const QForms& qa = qform_of(a);
+      const QForms& qb = qform_of(b);
+      CoefDev ca = coef_of(a), cb = coef_of(b);
+      res = run_pairs(qa, qb, a->chi(), b->chi(), false, /*fixed bra, complex*/ 0, ca, cb);
+    }
+    res = cmul(cmul(res, std::conj(a->g)), b->g);
+    *re = res.real();
+    *im = res.imag();
+  })
+}
+
+static void amplitudes_impl(const nx_state* s, int64_t nbits, const uint64_t* d_bits, double* d_out,
+                            uint8_t* d_per_term, int enumerate) {
+  const QForms& q = qform_of(s);
+  CoefDev c = coef_of(s);
+  hip_check(launch_amplitude(q.desc(), c.c, d_bits, nbits, 0, s->chi(), ctx()->tabs, (double2*)d_out, d_per_term,
+                             enumerate, s->g.real(), s->g.imag(), stream()),
+            "k_amplitude");
+}
+
+int nx_state_amplitudes(const nx_state* s, int64_t nbits, const uint64_t* bits, int32_t words, double* out,
+                        uint8_t* per_term, int32_t* per_term_r) {
+  NX_TRY({
+    const int W = words_for(s->n);
+    if (words < W) fail(NX_E_VALUE, "bitstrings need %d words for %d qubits", W, s->n);
+    std::vector<uint64_t> packed((size_t)nbits * W);
+    for (int64_t b = 0; b < nbits; ++b)
+      for (int w = 0; w < W; ++w) {
+        uint64_t x = bits[(size_t)b * words + w];
+        const int hi = s->n - 64 * w;
+        if (hi < 64) x &= hi <= 0 ? 0ull : ((1ull << hi) - 1ull);
+        packed[(size_t)b * W + w] = x;
+      }
+    DevBuf dbits(packed.size() * 8), dout((size_t)nbits * 16);
+    DevBuf dpt(per_term ? (size_t)s->chi() * nbits : 0);
+    h2d(dbits.p, packed.data(), packed.size() * 8);
+    if (s->chi() == 0) {
+      std::memset(out, 0, (size_t)nbits * 16);
+    } else {
+      amplitudes_impl(s, nbits, dbits.as<uint64_t>(), dout.as<double>(), per_term ? dpt.as<uint8_t>() : nullptr, 0);
+      d2h(out, dout.p, (size_t)nbits * 16);
+      if (per_term) d2h(per_term, dpt.p, (size_t)s->chi() * nbits);
+      if (per_term_r) {
+        std::vector<int32_t> k(s->chi());
+        d2h(k.data(), qform_of(s).k.p, s->chi() * 4);
+        sync();
+        std::memcpy(per_term_r, k.data(), s->chi() * 4);
+      }
+    }
+    sync();
+  })
+}
+
+int nx_state_amplitudes_device(const nx_state* s, int64_t nbits, const uint64_t* d_bits, int32_t words,
+                               double* d_out) {
+  NX_TRY({
+    if (words != words_for(s->n)) fail(NX_E_VALUE, "device bitstrings must use %d words", words_for(s->n));
+    if (s->chi() == 0) {
+      hip_check(hipMemsetAsync(d_out, 0, (size_t)nbits * 16, stream()), "hipMemsetAsync");
+    } else {
+      amplitudes_impl(s, nbits, d_bits, d_out, nullptr, 0);
+    }
+    sync();
+  })
+}
+
+int nx_state_statevector(const nx_state* s, double* out) {
+  NX_TRY({
+    if (s->n > 28) fail(NX_E_VALUE, "Calculating the statevector for a state with %d qubits is not feasible.", s->n);
+    const int64_t dim = (int64_t)1 << s->n;
+    if (s->chi() == 0) {
+      std::memset(out, 0, (size_t)dim * 16);
+    } else {
+      DevBuf dout((size_t)dim * 16);
+      amplitudes_impl(s, dim, nullptr, dout.as<double>(), nullptr, 1);
+      d2h(out, dout.p, (size_t)dim * 16);
+      sync();
+    }
+  })
+}
+
+int nx_state_project(nx_state* s, int32_t qubit, int32_t outcome, int32_t normalized) {
+  NX_TRY({
+    project_unnormalized(s, qubit, outcome);
+    if (normalized) {  // projection.rs:10-21
+      const double nrm = std::sqrt(norm_squared(s));
+      if (std::fabs(nrm) < 1e-12 || std::isnan(nrm))
+        fail(NX_E_VALUE, "Impossible projection on qubit %d: cannot project determined state |%d> onto |%d>.", qubit,
+             outcome ? 0 : 1, outcome ? 1 : 0);
+      s->g = cplx(s->g.real() / nrm, s->g.imag() / nrm);
+    }
+  })
+}
+
+int nx_state_discard(nx_state* s, int32_t qubit) { NX_TRY(discard(s, qubit)) }
+
+int nx_state_measure(nx_state* s, int32_t nq, const int32_t* qargs, const uint8_t* seed32, uint8_t* outcomes) {
+  NX_TRY({
+    validate_qargs(s, nq, qargs);
+    uint8_t seed[32];
+    seed_or_entropy(seed32, seed);
+    ChaCha12Rng rng(seed);
+    for (int i = 0; i < nq; ++i) outcomes[i] = (uint8_t)measure_single(s, qargs[i], rng);
+  })
+}
+
+int nx_state_sample(const nx_state* s, int32_t nq, const int32_t* qargs, uint64_t shots, const uint8_t* seed32,
+                    int64_t max_out, uint64_t* out_bits, uint64_t* out_counts, int64_t* nout) {
+  NX_TRY({
+    validate_qargs(s, nq, qargs);
+    uint8_t seed[32];
+    seed_or_entropy(seed32, seed);
+    if (nq > 128) fail(NX_E_VALUE, "Sampling more than 128 qubits is not supported.");
+    ChaCha12Rng rng(seed);
+    SampleCtx cx;
+    cx.rng = &rng;
+    std::vector<std::pair<int, int>> sorted;
+    for (int i = 0; i < nq; ++i) sorted.push_back({qargs[i], i});
+    std::stable_sort(sorted.begin(), sorted.end(), [](auto& a, auto& b) { return a.first > b.first; });
+    for (auto& p : sorted) {
+      cx.q.push_back(p.first);
+      cx.bit.push_back(p.second);
+    }
+    recursive_sample(s, cx, 0, shots, 0, 0);
+    const int64_t k = (int64_t)cx.counts.size();
+    if (k > max_out) fail(NX_E_VALUE, "sample output buffer too small (%lld outcomes)", (long long)k);
+    std::memcpy(out_bits, cx.bits.data(), k * 16);
+    std::memcpy(out_counts, cx.counts.data(), k * 8);
+    *nout = k;
+  })
+}
+
+int nx_state_export_term(const nx_state* s, int64_t term, uint8_t* G, uint8_t* F, uint8_t* M, uint8_t* gamma,
+                         uint8_t* v, uint8_t* sv, uint8_t* phase) {
+  NX_TRY({
+    if (term < 0 || term >= s->chi()) fail(NX_E_VALUE, "term index out of range");
+    const int n = s->n, W = s->forms.W, R = 64 * W;
+    std::vector<uint64_t> g(R * W), f(R * W), m(R * W), vv(W), ss(W);
+    std::vector<uint8_t> gm(R);
+    uint8_t ph;
+    const size_t mo = (size_t)term * R * W * 8;
+    d2h(g.data(), s->forms.G.as<uint8_t>() + mo, R * W * 8);
+    d2h(f.data(), s->forms.F.as<uint8_t>() + mo, R * W * 8);
+    d2h(m.data(), s->forms.M.as<uint8_t>() + mo, R * W * 8);
+    d2h(gm.data(), s->forms.g.as<uint8_t>() + (size_t)term * R, R);
+    d2h(vv.data(), s->forms.v.as<uint8_t>() + (size_t)term * W * 8, W * 8);
+    d2h(ss.data(), s->forms.s.as<uint8_t>() + (size_t)term * W * 8, W * 8);
+    d2h(&ph, s->forms.ph.as<uint8_t>() + term, 1);
+    sync();
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        const size_t wi = (size_t)i * W + (j >> 6);
+        G[(size_t)i * n + j] = (uint8_t)((g[wi] >> (j & 63)) & 1);
+        F[(size_t)i * n + j] = (uint8_t)((f[wi] >> (j & 63)) & 1);
+        M[(size_t)i * n + j] = (uint8_t)((m[wi] >> (j & 63)) & 1);
+      }
+    for (int i = 0; i < n; ++i) {
+      gamma[i] = (uint8_t)(2 * gm[i]);
+      v[i] = (uint8_t)((vv[i >> 6] >> (i & 63)) & 1);
+      sv[i] = (uint8_t)((ss[i >> 6] >> (i & 63)) & 1);
+    }
+    *phase = ph;
+  })
+}
+
+int nx_state_get_scalar_coefficients(const nx_state* s, uint8_t* zero, uint8_t* k, int64_t* r) {
+  NX_TRY({
+    if (s->is_complex) fail(NX_E_VALUE, "state holds Complex64 coefficients");
+    for (int64_t i = 0; i < s->chi(); ++i) {
+      zero[i] = s->sc[i].z;
+      k[i] = s->sc[i].k;
+      r[i] = s->sc[i].r;
+    }
+  })
+}
+
+int nx_state_get_coefficients(const nx_state* s, double* re_im) {
+  NX_TRY(for (int64_t i = 0; i < s->chi(); ++i) {
+    cplx c = s->coeff_value(i);
+    re_im[2 * i] = c.real();
+    re_im[2 * i + 1] = c.imag();
+  })
+}
+
+int nx_state_set_coefficients(nx_state* s, const double* re_im) {
+  NX_TRY({
+    s->cc.resize(s->chi());
+    for (int64_t i = 0; i < s->chi(); ++i) s->cc[i] = cplx(re_im[2 * i], re_im[2 * i + 1]);
+    s->is_complex = true;
+    s->sc.clear();
+    s->c_ok = false;
+  })
+}
+
+int nx_state_global_factor(const nx_state* s, double* re, double* im) {
+  NX_TRY(*re = s->g.real(); *im = s->g.imag())
+}
+
+int nx_state_pair_table(const nx_state* s, uint32_t* out) {
+  NX_TRY({
+    const int64_t chi = s->chi();
+    if (chi == 0) return NX_OK;
+    const QForms& q = qform_of(s);
+    CoefDev c = coef_of(s);
+    DevBuf ex((size_t)chi * chi * 4);
+    run_pairs(q, q, chi, chi, false, 0, c, c, ex.as<uint32_t>(), chi);
+    d2h(out, ex.p, (size_t)chi * chi * 4);
+    sync();
+  })
+}
+
+int nx_state_grad_exp_values(const nx_state* s, int32_t npauli, const int32_t* ops, const double* weights,
+                             double* loss, double* grad) {
+  NX_TRY({
+    (void)s; (void)npauli; (void)ops; (void)weights; (void)loss; (void)grad;
+    fail(NX_E_INTERNAL, "nx_state_grad_exp_values: not implemented yet");
+  })
+}
+
+}  // extern "C"

@@ -1,0 +1,272 @@
+"""QuantumState (python/src/state.rs:12-222) over the HIP C-ABI."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import check, lib
+from .gate import QuantumGate, _usize
+from .pauli_string import PauliString
+
+
+def parse_seed(seed):
+    """python/src/utils.rs:4-39: None, an int (< 2^64 packed into 8
+    little-endian bytes, larger ints into 32), or exactly 32 bytes."""
+    if seed is None:
+        return None
+    if isinstance(seed, int):
+        if 0 <= seed < 2 ** 64:
+            return seed.to_bytes(8, "little") + bytes(24)
+        return seed.to_bytes(32, "little")
+    if isinstance(seed, (bytes,)):
+        if len(seed) != 32:
+            raise ValueError(f"Seed must be 32 bytes, got {len(seed)}")
+        return bytes(seed)
+    raise TypeError("Seed must be an integer or 32 bytes")
+
+
+def _seed_ptr(seed):
+    b = parse_seed(seed)
+    if b is None:
+        return None, None
+    buf = (C.c_uint8 * 32).from_buffer_copy(b)
+    return buf, C.cast(buf, C.POINTER(C.c_uint8))
+
+
+def _i32_array(values):
+    arr = np.ascontiguousarray(np.asarray(values, dtype=np.int64))
+    if arr.size and (arr.min() < -(2 ** 31) or arr.max() >= 2 ** 31):
+        raise OverflowError("qubit index out of range")
+    arr = arr.astype(np.int32)
+    if arr.size == 0:
+        arr = np.zeros(1, np.int32)
+    return arr, arr.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _gate_rows(gates):
+    rows = np.zeros((max(len(gates), 1), 4), np.int32)
+    for i, g in enumerate(gates):
+        rows[i] = g._row()
+    return rows
+
+
+class QuantumState:
+    __slots__ = ("_h",)
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().nx_state_free(h)
+            self._h = None
+
+    # -- construction --------------------------------------------------------
+    @staticmethod
+    def from_circuit(circuit) -> "QuantumState":
+        rows = _gate_rows(circuit.gates)
+        out = C.c_void_p()
+        check(lib().nx_state_compile(circuit.num_qubits, circuit.num_gates,
+                                     rows.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(out)))
+        return QuantumState(out.value)
+
+    def clone(self) -> "QuantumState":
+        out = C.c_void_p()
+        check(lib().nx_state_clone(self._h, C.byref(out)))
+        return QuantumState(out.value)
+
+    # -- getters -------------------------------------------------------------
+    @property
+    def num_qubits(self) -> int:
+        v = C.c_int32()
+        check(lib().nx_state_num_qubits(self._h, C.byref(v)))
+        return v.value
+
+    @property
+    def stabilizer_rank(self) -> int:
+        v = C.c_int64()
+        check(lib().nx_state_stabilizer_rank(self._h, C.byref(v)))
+        return v.value
+
+    def __str__(self):
+        return f"QuantumState(num_qubits={self.num_qubits}, stabilizer_rank={self.stabilizer_rank})"
+
+    # -- read-only computations ----------------------------------------------
+    def to_statevector(self):
+        n = self.num_qubits
+        if n > 28:
+            out = np.zeros(2, np.float64)
+        else:
+            out = np.zeros(2 << n, np.float64)
+        check(lib().nx_state_statevector(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return [complex(a, b) for a, b in zip(out[0::2], out[1::2])]
+
+    def statevector_array(self) -> np.ndarray:
+        """to_statevector as a numpy complex128 array (no list conversion)."""
+        n = self.num_qubits
+        out = np.zeros(2 << min(n, 28), np.float64)
+        check(lib().nx_state_statevector(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out.view(np.complex128)
+
+    def inner_product(self, other: "QuantumState") -> complex:
+        re, im = C.c_double(), C.c_double()
+        check(lib().nx_state_inner_product(self._h, other._h, C.byref(re), C.byref(im)))
+        return complex(re.value, im.value)
+
+    def norm(self) -> float:
+        v = C.c_double()
+        check(lib().nx_state_norm(self._h, C.byref(v)))
+        return v.value
+
+    def exp_value(self, pauli_string: PauliString) -> float:
+        ops, opp = _i32_array(pauli_string._ops)
+        qs, qsp = _i32_array(pauli_string._qubits if not pauli_string._dense else [])
+        v = C.c_double()
+        check(lib().nx_state_exp_value(self._h, 1 if pauli_string._dense else 0, len(pauli_string._ops),
+                                       opp, qsp, C.byref(v)))
+        return v.value
+
+    def amplitudes(self, bitstrings, per_term: bool = False):
+        """<x|psi> for a batch of bitstrings (API addition).  bitstrings:
+        uint64 array [B] (n <= 64) or [B, words], little-endian (qubit j =
+        bit j).  Returns complex128 [B]; with per_term, also the exact
+        per-term phases (uint8 [chi, B], 255 = zero) and exponents r [chi]."""
+        n = self.num_qubits
+        words = max(1, (n + 63) // 64)
+        b = np.ascontiguousarray(np.asarray(bitstrings, dtype=np.uint64))
+        if b.ndim == 1:
+            b = b.reshape(-1, 1)
+        if b.shape[1] < words:
+            b = np.concatenate([b, np.zeros((b.shape[0], words - b.shape[1]), np.uint64)], axis=1)
+        b = np.ascontiguousarray(b)
+        nb = b.shape[0]
+        out = np.zeros(2 * max(nb, 1), np.float64)
+        chi = self.stabilizer_rank
+        pt = np.zeros((max(chi, 1), max(nb, 1)), np.uint8) if per_term else None
+        rr = np.zeros(max(chi, 1), np.int32) if per_term else None
+        check(lib().nx_state_amplitudes(
+            self._h, nb, b.ctypes.data_as(C.POINTER(C.c_uint64)), b.shape[1],
+            out.ctypes.data_as(C.POINTER(C.c_double)),
+            pt.ctypes.data_as(C.POINTER(C.c_uint8)) if per_term else None,
+            rr.ctypes.data_as(C.POINTER(C.c_int32)) if per_term else None))
+        vals = out[: 2 * nb].view(np.complex128)
+        if per_term:
+            return vals, pt[:chi, :nb], rr[:chi]
+        return vals
+
+    # -- measurement ---------------------------------------------------------
+    def measure(self, qargs, seed=None):
+        qargs = [_usize(q) for q in qargs]
+        arr, p = _i32_array(qargs)
+        _buf, sp = _seed_ptr(seed)
+        out = (C.c_uint8 * max(len(qargs), 1))()
+        check(lib().nx_state_measure(self._h, len(qargs), p, sp, out))
+        return [bool(out[i]) for i in range(len(qargs))]
+
+    def measure_all(self, seed=None):
+        return self.measure(list(range(self.num_qubits)), seed)
+
+    def sample(self, qargs, shots, seed=None):
+        qargs = [_usize(q) for q in qargs]
+        shots = _usize(shots)
+        arr, p = _i32_array(qargs)
+        _buf, sp = _seed_ptr(seed)
+        cap = min(shots, 1 << min(len(qargs), 24)) + 1
+        bits = np.zeros(2 * cap, np.uint64)
+        counts = np.zeros(cap, np.uint64)
+        nout = C.c_int64()
+        check(lib().nx_state_sample(self._h, len(qargs), p, shots, sp, cap,
+                                    bits.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                    counts.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(nout)))
+        res = {}
+        k = len(qargs)
+        for i in range(nout.value):
+            word = int(bits[2 * i]) | (int(bits[2 * i + 1]) << 64)
+            key = "".join("1" if (word >> j) & 1 else "0" for j in range(k))
+            res[key] = int(counts[i])
+        return res
+
+    # -- state updates -------------------------------------------------------
+    def project_normalized(self, qubit: int, outcome: bool) -> None:
+        check(lib().nx_state_project(self._h, _usize(qubit), int(bool(outcome)), 1))
+
+    def project_unnormalized(self, qubit: int, outcome: bool) -> None:
+        check(lib().nx_state_project(self._h, _usize(qubit), int(bool(outcome)), 0))
+
+    def discard(self, qubit: int) -> None:
+        check(lib().nx_state_discard(self._h, _usize(qubit)))
+
+    def apply_gate(self, gate: QuantumGate) -> None:
+        self.apply_gates([gate])
+
+    def apply_gates(self, gates) -> None:
+        gates = list(gates)
+        rows = _gate_rows(gates)
+        check(lib().nx_state_apply_gates(self._h, len(gates), rows.ctypes.data_as(C.POINTER(C.c_int32))))
+
+    def apply_x(self, qubit): self.apply_gate(QuantumGate.x(qubit))
+
+    def apply_y(self, qubit): self.apply_gate(QuantumGate.y(qubit))
+
+    def apply_z(self, qubit): self.apply_gate(QuantumGate.z(qubit))
+
+    def apply_h(self, qubit): self.apply_gate(QuantumGate.h(qubit))
+
+    def apply_s(self, qubit): self.apply_gate(QuantumGate.s(qubit))
+
+    def apply_sdg(self, qubit): self.apply_gate(QuantumGate.sdg(qubit))
+
+    def apply_sqrt_x(self, qubit): self.apply_gate(QuantumGate.sqrt_x(qubit))
+
+    def apply_sqrt_xdg(self, qubit): self.apply_gate(QuantumGate.sqrt_xdg(qubit))
+
+    def apply_cx(self, control, target): self.apply_gate(QuantumGate.cx(control, target))
+
+    def apply_cz(self, qarg1, qarg2): self.apply_gate(QuantumGate.cz(qarg1, qarg2))
+
+    def apply_swap(self, qarg1, qarg2): self.apply_gate(QuantumGate.swap(qarg1, qarg2))
+
+    # -- parity / coefficient access (additions) -------------------------------
+    def export_term(self, i: int):
+        n = self.num_qubits
+        G = np.zeros((n, n), np.uint8); F = np.zeros((n, n), np.uint8); M = np.zeros((n, n), np.uint8)
+        g = np.zeros(max(n, 1), np.uint8); v = np.zeros(max(n, 1), np.uint8); s = np.zeros(max(n, 1), np.uint8)
+        ph = np.zeros(1, np.uint8)
+        P = C.POINTER(C.c_uint8)
+        check(lib().nx_state_export_term(self._h, i, *(a.ctypes.data_as(P) for a in (G, F, M, g, v, s, ph))))
+        return dict(G=G, F=F, M=M, gamma=g[:n], v=v[:n], s=s[:n], phase=int(ph[0]))
+
+    def scalar_coefficients(self):
+        chi = self.stabilizer_rank
+        z = np.zeros(max(chi, 1), np.uint8); k = np.zeros(max(chi, 1), np.uint8); r = np.zeros(max(chi, 1), np.int64)
+        check(lib().nx_state_get_scalar_coefficients(self._h, z.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                                     k.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                                     r.ctypes.data_as(C.POINTER(C.c_int64))))
+        return z[:chi].astype(bool), k[:chi], r[:chi]
+
+    def coefficients(self) -> np.ndarray:
+        chi = self.stabilizer_rank
+        out = np.zeros(2 * max(chi, 1), np.float64)
+        check(lib().nx_state_get_coefficients(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out[: 2 * chi].view(np.complex128).copy()
+
+    def set_coefficients(self, c) -> None:
+        c = np.ascontiguousarray(np.asarray(c, np.complex128))
+        if c.shape != (self.stabilizer_rank,):
+            raise ValueError("one coefficient per stabilizer term is required")
+        buf = c.view(np.float64)
+        check(lib().nx_state_set_coefficients(self._h, buf.ctypes.data_as(C.POINTER(C.c_double))))
+
+    @property
+    def global_factor(self) -> complex:
+        re, im = C.c_double(), C.c_double()
+        check(lib().nx_state_global_factor(self._h, C.byref(re), C.byref(im)))
+        return complex(re.value, im.value)
+
+    def pair_table(self) -> np.ndarray:
+        chi = self.stabilizer_rank
+        out = np.zeros((max(chi, 1), max(chi, 1)), np.uint32)
+        check(lib().nx_state_pair_table(self._h, out.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return out[:chi, :chi]

@@ -1,0 +1,240 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU oracle.
+
+Integer results are compared bit for bit: per-term CH-form tables (G, F,
+M, gamma, v, s, phase), exact coefficient scalars, exact pair scalars
+<phi_i|phi_j>, exact per-term amplitude phases.  FP64 sums are compared with
+|gpu - oracle| <= 1e-12 * max(1, |oracle|) (the states here have unit norm,
+so this is the 1e-12 relative bound).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import dense as D
+import workloads as WL
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def nx():
+    import necstar
+    from necstar import _lib
+    _lib.check(_lib.lib().nx_init(0))
+    return necstar
+
+
+def compile_both(n, gates):
+    st = WL.to_circuit(n, gates)
+    from necstar import QuantumState
+    return QuantumState.from_circuit(st), O.State.compile(n, gates)
+
+
+def assert_tables_equal(g, o):
+    assert g.stabilizer_rank == o.chi
+    assert g.num_qubits == o.n
+    z, k, r = g.scalar_coefficients()
+    for i in range(o.chi):
+        tg, to = g.export_term(i), o.term(i).tables()
+        for key in ("G", "F", "M", "gamma", "v", "s"):
+            assert np.array_equal(tg[key], to[key]), (i, key)
+        assert tg["phase"] == to["phase"], i
+        oz, ok, orr = o.coeff_scalar(i)
+        assert (bool(z[i]), int(k[i]), int(r[i])) == (oz, ok, orr), i
+
+
+CASES = [  # (seed, n, cliffords, t, gate set)
+    (1, 3, 30, 3, tuple(range(11))),
+    (2, 5, 60, 5, tuple(range(11))),
+    (3, 8, 100, 7, tuple(range(11))),
+    (4, 12, 200, 8, WL.CLIFFORD_SET),
+    (5, 30, 300, 10, tuple(range(11))),
+    (6, 63, 400, 6, WL.CLIFFORD_SET),
+    (7, 64, 500, 8, tuple(range(11))),
+    (8, 70, 500, 9, tuple(range(11))),
+    (9, 130, 700, 5, WL.CLIFFORD_SET),
+]
+
+
+@pytest.mark.parametrize("seed,n,nc,nt,gs", CASES)
+def test_compile_tables_bit_exact(nx, seed, n, nc, nt, gs):
+    rng = np.random.default_rng(seed)
+    gates = WL.random_circuit(rng, n, nc, nt, gate_set=gs)
+    say(f"compile seed={seed} n={n} cliffords={nc} t={nt}")
+    g, o = compile_both(n, gates)
+    assert_tables_equal(g, o)
+
+
+@pytest.mark.parametrize("seed,n,nc,nt,gs", CASES[:7])
+def test_pair_scalars_exact(nx, seed, n, nc, nt, gs):
+    rng = np.random.default_rng(100 + seed)
+    gates = WL.random_circuit(rng, n, nc, nt, gate_set=gs)
+    g, o = compile_both(n, gates)
+    say(f"pairs seed={seed} n={n} chi={o.chi}")
+    tab = g.pair_table()
+    chi = o.chi
+    idx = [(i, j) for i in range(chi) for j in range(chi)]
+    if len(idx) > 400:
+        pick = rng.choice(len(idx), 400, replace=False)
+        idx = [idx[p] for p in pick]
+    for i, j in idx:
+        (z, k, r), _ = o.term(i).inner_product(o.term(j))
+        v = int(tab[i, j])
+        if z:
+            assert v == 0xFFFFFFFF, (i, j, v)
+        else:
+            assert v == (k | (r << 8)), (i, j, v, k, r)
+
+
+@pytest.mark.parametrize("seed,n,nc,nt,gs", CASES[:7])
+def test_norm_exp_value_inner_product(nx, seed, n, nc, nt, gs):
+    rng = np.random.default_rng(200 + seed)
+    gates = WL.random_circuit(rng, n, nc, nt, gate_set=gs)
+    g, o = compile_both(n, gates)
+    say(f"norm/exp seed={seed} n={n} chi={o.chi}")
+    assert abs(g.norm() - o.norm()) <= TOL
+    from necstar import PauliString
+    for ops in WL.random_paulis(rng, n, 4, 1, min(4, n)):
+        s = "".join("IXYZ"[x] for x in reversed(ops))
+        ev_g = g.exp_value(PauliString.from_str(s))
+        ev_o = o.exp_value(("dense", ops))
+        assert abs(ev_g - ev_o) <= TOL, (s, ev_g, ev_o)
+    gates2 = WL.random_circuit(rng, n, nc, nt, gate_set=gs)
+    g2, o2 = compile_both(n, gates2)
+    ip_g, ip_o = g.inner_product(g2), o.inner_product(o2)
+    assert abs(ip_g - ip_o) <= TOL, (ip_g, ip_o)
+
+
+@pytest.mark.parametrize("seed,n,nc,nt,gs", CASES)
+def test_amplitudes_exact(nx, seed, n, nc, nt, gs):
+    rng = np.random.default_rng(300 + seed)
+    gates = WL.random_circuit(rng, n, nc, nt, gate_set=gs)
+    g, o = compile_both(n, gates)
+    bits = WL.random_bitstrings(rng, n, 48)
+    # a few strings inside the support: basis states of nonzero amplitude
+    say(f"amplitudes seed={seed} n={n} chi={o.chi}")
+    vals, pt, rr = g.amplitudes(bits, per_term=True)
+    for b in range(len(bits)):
+        x = np.array([(int(bits[b, q // 64]) >> (q % 64)) & 1 for q in range(n)], np.uint8)
+        val_o, (zo, ko, ro) = o.amplitude(x, per_term=True)
+        for i in range(o.chi):
+            if zo[i]:
+                assert pt[i, b] == 255, (b, i)
+            else:
+                assert pt[i, b] == ko[i] and rr[i] == ro[i], (b, i, pt[i, b], ko[i], rr[i], ro[i])
+        assert vals[b] == val_o or abs(vals[b] - val_o) <= TOL
+
+
+def test_statevector_vs_dense(nx):
+    rng = np.random.default_rng(77)
+    for trial in range(6):
+        n = int(rng.integers(1, 9))
+        gates = D.random_circuit(rng, n, 40, int(rng.integers(0, 7)),
+                                 gate_set=tuple(range(11)) if n > 1 else tuple(range(8)), tdg=True)
+        say(f"statevector trial={trial} n={n}")
+        from necstar import QuantumState
+        g = QuantumState.from_circuit(WL.to_circuit(n, gates))
+        o = O.State.compile(n, gates)
+        sv = g.statevector_array()
+        assert np.array_equal(sv, o.statevector()), "sequential term order must give identical FP64"
+        assert np.abs(sv - D.run(n, gates)).max() < 1e-9
+
+
+def test_projection_discard_gates_parity(nx):
+    rng = np.random.default_rng(404)
+    for trial in range(8):
+        n = int(rng.integers(3, 10))
+        gates = WL.random_circuit(rng, n, 60, int(rng.integers(2, 7)), gate_set=tuple(range(11)))
+        say(f"project/discard trial={trial} n={n}")
+        g, o = compile_both(n, gates)
+        cl = WL.random_circuit(rng, n, 20, 0, gate_set=tuple(range(11)))
+        from necstar.gate import QuantumGate
+        g.apply_gates([QuantumGate(x[0], x[1:]) for x in cl])
+        for x in cl:
+            o.apply_gate(*x)
+        assert_tables_equal(g, o)
+        q, outc = int(rng.integers(n)), bool(rng.integers(2))
+        g.project_unnormalized(q, outc)
+        o.project_unnormalized(q, outc)
+        assert_tables_equal(g, o)
+        if o.chi:
+            assert abs(g.norm() - o.norm()) <= TOL
+        g.project_unnormalized(q, False)
+        o.project_unnormalized(q, False)
+        if o.chi:
+            g.discard(q)
+            o.discard(q)
+            assert_tables_equal(g, o)
+
+
+def test_measure_and_sample_same_stream_as_oracle(nx):
+    """Same seed -> same outcomes as the oracle (both restate rand 0.8's
+    StdRng; the device norms agree with the oracle's to 1e-16)."""
+    rng = np.random.default_rng(505)
+    for trial in range(4):
+        n = int(rng.integers(3, 8))
+        gates = WL.random_circuit(rng, n, 50, 4, gate_set=tuple(range(11)))
+        g, o = compile_both(n, gates)
+        say(f"sample trial={trial} n={n} chi={o.chi}")
+        qargs = [int(q) for q in rng.permutation(n)]
+        res_g = g.sample(qargs, 500, seed=trial)
+        res_o = {"".join("1" if b else "0" for b in bits): c for bits, c in o.sample(qargs, 500, trial)}
+        assert res_g == res_o
+        m_g = g.clone().measure(qargs, seed=trial + 7)
+        m_o = o.clone().measure(qargs, trial + 7)
+        assert m_g == m_o
+
+
+def test_errors_match_oracle(nx):
+    from necstar import PauliString, QuantumCircuit, QuantumState
+    k = [(0, 0), (11, 0), (8, 0, 1)]
+    with pytest.raises(ValueError, match="Qubit index 3 is out of bounds for 3 qubits"):
+        QuantumState.from_circuit(WL.to_circuit(2, [(0, 0), (11, 3), (8, 0, 1)]))
+    st = QuantumState.from_circuit(WL.to_circuit(2, k))
+    with pytest.raises(ValueError, match="Direct application of non-Clifford gate T is not supported."):
+        st.apply_gates([__import__("necstar").QuantumGate.t(0)])
+    with pytest.raises(ValueError, match="Qubit index 5 is out of bounds for 2 qubits."):
+        st.apply_h(5)
+    with pytest.raises(ValueError, match="Two-qubit gate requires two different qubit indices"):
+        st.apply_cx(1, 1)
+    with pytest.raises(ValueError, match="Invalid Pauli string length: expected 2, found 3."):
+        st.exp_value(PauliString.from_str("ZZZ"))
+    with pytest.raises(ValueError, match="Invalid Pauli string length: expected 2, found 5."):
+        st.exp_value(PauliString.from_str("Z4"))
+    with pytest.raises(ValueError, match="Qubit index list is empty."):
+        st.sample([], 10)
+    with pytest.raises(ValueError, match="Duplicate qubit index found: 0."):
+        st.measure([0, 0])
+    with pytest.raises(ValueError, match="Gate CCX is not supported."):
+        c = QuantumCircuit(3)
+        c.apply_ccx(0, 1, 2)
+        QuantumState.from_circuit(c)
+    big = QuantumState.from_circuit(QuantumCircuit(3))
+    with pytest.raises(ValueError, match="The qubit counts of the two states must match"):
+        st.inner_product(big)
+
+
+def test_config0_workload(nx):
+    cfg = WL.config(0)
+    n = cfg["n"]
+    g, o = compile_both(n, cfg["gates"])
+    say(f"cfg0 chi={o.chi}")
+    assert_tables_equal(g, o)
+    assert abs(g.norm() - o.norm()) <= TOL and abs(g.norm() - 1.0) <= 1e-12
+    vals = g.amplitudes(cfg["bitstrings"])
+    for b in range(len(cfg["bitstrings"])):
+        x = np.array([(int(cfg["bitstrings"][b, 0]) >> q) & 1 for q in range(n)], np.uint8)
+        assert vals[b] == o.amplitude(x)
+    from necstar import PauliString
+    for ops in cfg["paulis"]:
+        s = "".join("IXYZ"[x] for x in reversed(ops))
+        assert abs(g.exp_value(PauliString.from_str(s)) - o.exp_value(("dense", ops))) <= TOL
