@@ -1,0 +1,194 @@
+"""Benchmark of the CH-form hot path on one MI355X.
+
+Workload (cfg1): n = 64 qubits, 2000 random Clifford gates from {H, S, CX,
+CZ} plus 24 T gates (seeded), compiled to chi stabilizer terms.  One step is
+one pass of the hot path over device-resident inputs:
+  * the per-term quadratic-form rewrite (k_qform, chi terms),
+  * <x|psi> for 65,536 random 64-bit strings (k_amplitude, chi*65536
+    CH-form amplitude evaluations),
+  * the norm via all chi(chi+1)/2 pairwise CH-form inner products (k_pairs).
+value = (amplitude evaluations + inner products) per second of step time.
+
+Prints one JSON line.  The CPU baseline is the oracle (a C restatement of
+the reference algorithm, one core) on a bounded sample of the same units.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "stab-decomp-simulator-rust_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "CH-form amplitude evals/s + inner products/s, n=64; batched vs plain, 1 GPU"
+# MI355X: 256 CUs x 64 lanes x 2.4 GHz peak engine clock = int32 lane-ops/s
+PEAK_VALU_TOPS = 256 * 64 * 2.4e9 / 1e12
+
+
+def ops_amp(n):  # int32 lane-ops per amplitude evaluation
+    w32 = (n + 31) // 32
+    return 12 * ((n + 1) // 2) * w32 + 4 * w32
+
+
+def ops_ip(n):  # int32 lane-ops per CH-form inner product
+    return 6 * n * n * ((n + 31) // 32)
+
+
+def cpu_baseline(cfg, seconds):
+    """Oracle (C restatement of the reference, 1 core) on a bounded sample."""
+    import numpy as np
+    from oracle import oracle as O
+    n = cfg["n"]
+    t0 = time.perf_counter()
+    st = O.State.compile(n, cfg["gates"])
+    compile_s = time.perf_counter() - t0
+    chi = st.chi
+    bits = cfg["bitstrings"]
+    # amplitude evaluations: whole strings (all chi terms each)
+    t0 = time.perf_counter()
+    na = 0
+    while time.perf_counter() - t0 < seconds / 2 and na < len(bits):
+        x = np.array([(int(bits[na, 0]) >> q) & 1 for q in range(n)], np.uint8)
+        st.amplitude(x)
+        na += 1
+    t_amp = (time.perf_counter() - t0) / (na * chi)
+    # pairwise inner products
+    t0 = time.perf_counter()
+    npairs, i, j = 0, 0, 0
+    while time.perf_counter() - t0 < seconds / 2:
+        st.term(i).inner_product(st.term(j))
+        npairs += 1
+        j += 1
+        if j >= chi:
+            i, j = i + 1, i + 1
+    t_ip = (time.perf_counter() - t0) / npairs
+    return dict(chi=chi, compile_s=compile_s, t_amp=t_amp, t_ip=t_ip,
+                sample=f"{na} strings x {chi} terms amplitudes + {npairs} pairs, cfg1 state")
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_cfg1_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=16.0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a plain 1 GiB device copy (PMC calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import workloads as WL
+    from necstar import QuantumState
+    from necstar import _lib
+
+    L = _lib.lib()
+    _lib.check(L.nx_init(0))
+    cfg = WL.config(1)
+    n = cfg["n"]
+    t0 = time.perf_counter()
+    st = QuantumState.from_circuit(WL.to_circuit(n, cfg["gates"]))
+    compile_s = time.perf_counter() - t0
+    chi = st.stabilizer_rank
+    bits = cfg["bitstrings"]
+    B = bits.shape[0]
+    dev = torch.device("cuda:0")
+    d_bits = torch.from_numpy(bits.view(np.int64).copy()).to(dev)
+    d_out = torch.empty(2 * B, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    bits_p = C.cast(d_bits.data_ptr(), C.POINTER(C.c_uint64))
+    out_p = C.cast(d_out.data_ptr(), C.POINTER(C.c_double))
+    norm = C.c_double()
+
+    def step():
+        _lib.check(L.nx_state_drop_cache(st._h))
+        _lib.check(L.nx_state_amplitudes_device(st._h, B, bits_p, 1, out_p))
+        _lib.check(L.nx_state_norm(st._h, C.byref(norm)))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    _lib.check(L.nx_profile_enable(1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = {}
+    for k in ("k_amplitude", "k_pairs", "k_qform"):
+        ms, cnt = C.c_double(), C.c_int64()
+        _lib.check(L.nx_profile_get(k.encode(), C.byref(ms), C.byref(cnt)))
+        prof[k] = (ms.value / max(cnt.value, 1), cnt.value)
+    _lib.check(L.nx_profile_enable(0))
+    if args.copy_probe:
+        src = torch.empty(1 << 27, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    amp_units = chi * B
+    pair_units = chi * (chi + 1) // 2
+    value = (amp_units + pair_units) / (ms_step / 1e3)
+    amp_ms, pair_ms = prof["k_amplitude"][0], prof["k_pairs"][0]
+    amp_rate = amp_units / (amp_ms / 1e3) if amp_ms > 0 else None
+    pair_rate = pair_units / (pair_ms / 1e3) if pair_ms > 0 else None
+    if pair_ms >= amp_ms:
+        kern, per_unit, units, kms = "k_pairs", ops_ip(n), pair_units, pair_ms
+    else:
+        kern, per_unit, units, kms = "k_amplitude", ops_amp(n), amp_units, amp_ms
+    achieved = per_unit * units / (kms / 1e3) / 1e12
+    roofline = {"bound": "valu", "kernel": kern, "achieved": achieved, "peak": PEAK_VALU_TOPS,
+                "unit": "TOP/s (int32 lane-ops)", "frac": achieved / PEAK_VALU_TOPS,
+                "ops_per_unit": per_unit, "units_per_launch": units, "avg_launch_ms": kms,
+                "traffic": read_traffic(kern)}
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        cb = cpu_baseline(cfg, args.cpu_seconds)
+        cpu_step_s = amp_units * cb["t_amp"] + pair_units * cb["t_ip"]
+        cpu = {"value": (amp_units + pair_units) / cpu_step_s, "unit": "CH-form evals/s", "cores": 1,
+               "kind": "port", "sample": cb["sample"],
+               "amplitude_evals_per_s": 1.0 / cb["t_amp"], "inner_products_per_s": 1.0 / cb["t_ip"],
+               "compile_s": cb["compile_s"]}
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "CH-form evals/s (amplitude evals + inner products)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "cfg1: n=64, 2000 Clifford + 24 T gates, 65536 amplitude strings + full norm",
+                   "n": n, "chi": chi, "bitstrings": B, "pairs": pair_units, "cliffords": 2000, "t": 24},
+        "amplitude_evals_per_s": amp_rate, "inner_products_per_s": pair_rate,
+        "kernel_ms": {k: v[0] for k, v in prof.items()},
+        "norm": norm.value, "compile_s": compile_s,
+        "batched_vs_plain": None,
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

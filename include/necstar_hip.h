@@ -133,6 +133,17 @@ int nx_state_pair_table(const nx_state *s, uint32_t *out);
 int nx_state_grad_exp_values(const nx_state *s, int32_t npauli, const int32_t *ops, const double *weights,
                              double *loss, double *grad);
 
+/* Kernel timing on the library's own stream (HIP events around each
+ * amplitude / pair / qform / program launch; callers' events on other
+ * streams cannot see these launches).  kernel: "k_amplitude", "k_pairs",
+ * "k_qform", "k_program".  No reference counterpart (measurement aid). */
+int nx_profile_enable(int on);
+int nx_profile_get(const char *kernel, double *total_ms, int64_t *count);
+
+/* Drops the derived per-term device data (quadratic forms, coefficient
+ * copies) so the next call recomputes it.  No reference counterpart. */
+int nx_state_drop_cache(nx_state *s);
+
 #ifdef __cplusplus
 }
 #endif

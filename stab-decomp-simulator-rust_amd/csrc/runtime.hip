@@ -179,6 +179,47 @@ static void d2d(void* d, const void* s, size_t b) {
   if (b) hip_check(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, stream()), "hipMemcpyAsync D2D");
 }
 
+// kernel timing (HIP events on the library stream)
+struct Prof {
+  bool on = false;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  std::map<std::string, std::pair<double, int64_t>> acc;
+};
+static std::mutex g_prof_mu;
+static Prof g_prof;
+
+struct ProfScope {
+  const char* name;
+  bool on;
+  explicit ProfScope(const char* nm) : name(nm), on(false) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    if (!g_prof.on) return;
+    on = true;
+    if (!g_prof.e0) {
+      hip_check(hipEventCreate(&g_prof.e0), "hipEventCreate");
+      hip_check(hipEventCreate(&g_prof.e1), "hipEventCreate");
+    }
+    hip_check(hipEventRecord(g_prof.e0, stream()), "hipEventRecord");
+  }
+  void end() {
+    if (!on) return;
+    on = false;
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    hip_check(hipEventRecord(g_prof.e1, stream()), "hipEventRecord");
+    hip_check(hipEventSynchronize(g_prof.e1), "hipEventSynchronize");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, g_prof.e0, g_prof.e1), "hipEventElapsedTime");
+    auto& a = g_prof.acc[name];
+    a.first += ms;
+    a.second += 1;
+  }
+  ~ProfScope() {
+    if (on) {
+      try { end(); } catch (...) {}
+    }
+  }
+};
+
 static int words_for(int n) { return n <= 64 ? 1 : (n + 63) / 64; }
 static const int MAX_QUBITS = 256;
 
@@ -256,8 +297,12 @@ static Forms run_program(Forms& in, const std::vector<ProgOp>& prog, int n_out, 
   DevBuf dstat(in.chi * sizeof(int4));
   h2d(dprog.p, prog.data(), prog.size() * sizeof(ProgOp));
   FormsDev od = inplace ? in.desc() : out.desc();
-  hip_check(launch_program(in.desc(), od, dprog.as<ProgOp>(), (int)prog.size(), dstat.as<int4>(), stream()),
-            "k_program");
+  {
+    ProfScope ps("k_program");
+    hip_check(launch_program(in.desc(), od, dprog.as<ProgOp>(), (int)prog.size(), dstat.as<int4>(), stream()),
+              "k_program");
+    ps.end();
+  }
   std::vector<int4> st(in.chi);
   d2h(st.data(), dstat.p, in.chi * sizeof(int4));
   sync();
@@ -331,7 +376,9 @@ static Forms kron(const Forms& A, const Forms& B) {
 static QForms make_qform(const Forms& f) {
   QForms q;
   q.alloc(f.chi, f.n);
+  ProfScope ps("k_qform");
   hip_check(launch_qform(f.desc(), q.desc(), stream()), "k_qform");
+  ps.end();
   return q;
 }
 
@@ -451,9 +498,11 @@ static cplx run_pairs(const QForms& X, const QForms& Y, int64_t nfixed, int64_t 
   int nblocks = (int)std::min<int64_t>((units + 3) / 4, 4096);
   if (nblocks < 1) nblocks = 1;
   DevBuf partial((size_t)nblocks * 4 * 16), out(16);
+  ProfScope ps("k_pairs");
   hip_check(launch_pairs(X.desc(), Y.desc(), doff.as<int64_t>(), nfixed, nother, triangle ? 1 : 0, J, mode, cx, cy,
                          ctx()->tabs, partial.as<double2>(), nblocks, d_exact, ld, stream()),
             "k_pairs");
+  ps.end();
   hip_check(launch_reduce(partial.as<double2>(), (int64_t)nblocks * 4, out.as<double2>(), stream()), "k_reduce");
   double h[2];
   d2h(h, out.p, 16);
@@ -944,9 +993,11 @@ static void amplitudes_impl(const nx_state* s, int64_t nbits, const uint64_t* d_
                             uint8_t* d_per_term, int enumerate) {
   const QForms& q = qform_of(s);
   CoefDev c = coef_of(s);
+  ProfScope ps("k_amplitude");
   hip_check(launch_amplitude(q.desc(), c.c, d_bits, nbits, 0, s->chi(), ctx()->tabs, (double2*)d_out, d_per_term,
                              enumerate, s->g.real(), s->g.imag(), stream()),
             "k_amplitude");
+  ps.end();
 }
 
 int nx_state_amplitudes(const nx_state* s, int64_t nbits, const uint64_t* bits, int32_t words, double* out,
@@ -1145,6 +1196,32 @@ int nx_state_grad_exp_values(const nx_state* s, int32_t npauli, const int32_t* o
   NX_TRY({
     (void)s; (void)npauli; (void)ops; (void)weights; (void)loss; (void)grad;
     fail(NX_E_INTERNAL, "nx_state_grad_exp_values: not implemented yet");
+  })
+}
+
+int nx_profile_enable(int on) {
+  NX_TRY({
+    (void)ctx();
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof.on = on != 0;
+    g_prof.acc.clear();
+  })
+}
+
+int nx_profile_get(const char* kernel, double* total_ms, int64_t* count) {
+  NX_TRY({
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    auto it = g_prof.acc.find(kernel ? kernel : "");
+    *total_ms = it == g_prof.acc.end() ? 0.0 : it->second.first;
+    *count = it == g_prof.acc.end() ? 0 : it->second.second;
+  })
+}
+
+int nx_state_drop_cache(nx_state* s) {
+  NX_TRY({
+    s->q = QForms();
+    s->q_ok = false;
+    s->c_ok = false;
   })
 }
 
