@@ -72,9 +72,12 @@ hipError_t launch_program(const FormsDev& in, const FormsDev& out, const ProgOp*
 hipError_t launch_kron(const FormsDev& A, const FormsDev& B, const FormsDev& O, hipStream_t st);
 hipError_t launch_gather(const FormsDev& in, const FormsDev& O, const int64_t* idx, hipStream_t st);
 hipError_t launch_qform(const FormsDev& in, const QFormDev& q, hipStream_t st);
-hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const uint64_t* bits, int64_t nb, int64_t t0,
-                            int64_t t1, const double* tabs, double2* out, uint8_t* per_term, int enumerate,
-                            double gre, double gim, hipStream_t st);
+// nchunks > 1: partial must hold nchunks * nb values
+hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const uint64_t* bits, int64_t nb,
+                            int64_t chunk_terms, int nchunks, const double* tabs, double2* out, double2* partial,
+                            uint8_t* per_term, int enumerate, double gre, double gim, hipStream_t st);
+hipError_t launch_amp_reduce(const double2* partial, int64_t nb, int nchunks, double2* out, double gre, double gim,
+                             hipStream_t st);
 hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, const int64_t* offsets, int64_t nfixed,
                         int64_t nother, int triangle, int J, int mode, const CoefDev& cx, const CoefDev& cy,
                         const double* tabs, double2* partial, int nblocks, uint32_t* exact_out, int64_t exact_ld,

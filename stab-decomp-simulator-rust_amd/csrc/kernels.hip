@@ -337,78 +337,129 @@ __global__ __launch_bounds__(256) void k_qform(FormsDev in, QFormDev q) {
 }
 
 // ---------------------------------------------------------------------------
-// Amplitudes.  Thread = bitstring.  Terms [t0, t1) in order; the FP64
-// arithmetic per term is the reference's to_statevector arithmetic
-// (Scalar::to_complex, * omega, * coefficient, +=).
-template <int W>
+// Amplitudes.  Each thread evaluates XB bitstrings; blockIdx.y selects a
+// chunk of terms [t0, t1).  Per term and string: the support test (C x =
+// rhs), the linear part a.x mod 4 from two bit planes, and the quadratic
+// part sum_{l<m} Q_lm x_l x_m = par(x & XOR_{l in x} Q_l) (Q_l holds only
+// bits m > l).  The FP64 arithmetic per term is the reference's
+// to_statevector arithmetic (Scalar::to_complex, * omega, * coefficient,
+// +=) in term order within the chunk; chunks are summed in chunk order by
+// k_amp_reduce, which also applies the global factor.
+template <int W, int XB>
 __global__ __launch_bounds__(256) void k_amplitude(QFormDev q, const double2* __restrict__ coef,
-                                                   const u64* __restrict__ bits, int64_t nb, int64_t t0,
-                                                   int64_t t1, const double* __restrict__ tabs,
-                                                   double2* __restrict__ out, uint8_t* __restrict__ per_term,
-                                                   int enumerate, double gre, double gim) {
+                                                   const u64* __restrict__ bits, int64_t nb, int64_t chunk_terms,
+                                                   const double* __restrict__ tabs, double2* __restrict__ out,
+                                                   uint8_t* __restrict__ per_term, int enumerate, int final_g,
+                                                   double gre, double gim) {
   __shared__ double s_cos[8], s_sin[8];
   if (threadIdx.x < 8) {
     s_cos[threadIdx.x] = tabs[TAB_COS + threadIdx.x];
     s_sin[threadIdx.x] = tabs[TAB_SIN + threadIdx.x];
   }
   __syncthreads();
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = b < nb;
+  const int64_t t0 = (int64_t)blockIdx.y * chunk_terms;
+  const int64_t t1 = (t0 + chunk_terms < q.chi) ? t0 + chunk_terms : q.chi;
   const int n = q.n, R = 64 * W;
-  u64 x[W];
+  int64_t bidx[XB];
+  bool active[XB];
+  u64 x[XB][W];
 #pragma unroll
-  for (int w = 0; w < W; ++w) {
-    if (enumerate) x[w] = (w == 0 && active) ? (u64)b : 0ull;
-    else x[w] = active ? bits[(size_t)b * W + w] : 0ull;
+  for (int j = 0; j < XB; ++j) {
+    bidx[j] = ((int64_t)blockIdx.x * XB + j) * blockDim.x + threadIdx.x;
+    active[j] = bidx[j] < nb;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      if (enumerate) x[j][w] = (w == 0 && active[j]) ? (u64)bidx[j] : 0ull;
+      else x[j][w] = active[j] ? bits[(size_t)bidx[j] * W + w] : 0ull;
+    }
   }
-  double are = 0.0, aim = 0.0;
+  double are[XB], aim[XB];
+#pragma unroll
+  for (int j = 0; j < XB; ++j) { are[j] = 0.0; aim[j] = 0.0; }
   for (int64_t t = t0; t < t1; ++t) {
     const int nc = uni(q.nc[t]);
     const int k = uni(q.k[t]);
     const int p0 = uni(q.p0[t]);
-    int bad = 0;
+    int bad[XB];
+#pragma unroll
+    for (int j = 0; j < XB; ++j) bad[j] = 0;
     const u64* Cr = q.C + (size_t)t * R * W;
     for (int c = 0; c < nc; ++c) {
-      int p = 0;
+      const int rhs = (int)((q.rhs[(size_t)t * W + (c >> 6)] >> (c & 63)) & 1ull);
 #pragma unroll
-      for (int w = 0; w < W; ++w) p += __builtin_popcountll(Cr[(size_t)c * W + w] & x[w]);
-      bad |= (p & 1) ^ (int)((q.rhs[(size_t)t * W + (c >> 6)] >> (c & 63)) & 1ull);
+      for (int j = 0; j < XB; ++j) {
+        int p = rhs;
+#pragma unroll
+        for (int w = 0; w < W; ++w) p += __builtin_popcountll(Cr[(size_t)c * W + w] & x[j][w]);
+        bad[j] |= p & 1;
+      }
     }
-    int ax = 0;
+    u64 acc[XB][W];
 #pragma unroll
-    for (int w = 0; w < W; ++w)
-      ax += __builtin_popcountll(x[w] & q.alo[(size_t)t * W + w]) +
-            2 * __builtin_popcountll(x[w] & q.ahi[(size_t)t * W + w]);
-    int quad = 0;
+    for (int j = 0; j < XB; ++j)
+#pragma unroll
+      for (int w = 0; w < W; ++w) acc[j][w] = 0ull;
     const u64* Qr = q.Q + (size_t)t * R * W;
 #pragma unroll
     for (int wl = 0; wl < W; ++wl) {
-      for (int lb = 0; lb < 64; ++lb) {
-        const int l = 64 * wl + lb;
-        if (l >= n) break;
-        int p = 0;
+      const int lmax = (n - 64 * wl) < 64 ? (n - 64 * wl) : 64;
+      for (int lb = 0; lb < lmax; ++lb) {
+        u64 Ql[W];
 #pragma unroll
-        for (int w = 0; w < W; ++w) p += __builtin_popcountll(Qr[(size_t)l * W + w] & x[w]);
-        quad ^= p & (int)(x[wl] >> lb);
+        for (int w = 0; w < W; ++w) Ql[w] = Qr[(size_t)(64 * wl + lb) * W + w];
+#pragma unroll
+        for (int j = 0; j < XB; ++j) {
+          const u64 m = 0ull - ((x[j][wl] >> lb) & 1ull);
+#pragma unroll
+          for (int w = 0; w < W; ++w) acc[j][w] ^= Ql[w] & m;
+        }
       }
     }
-    const int ph = (p0 + 2 * ax + 4 * (quad & 1)) & 7;
-    if (per_term && active) per_term[(size_t)t * nb + b] = bad ? (uint8_t)0xFF : (uint8_t)ph;
-    // Scalar::to_complex then * omega (= 1 + 0i) then * coefficient
-    double zr = 0.0, zi = 0.0;
-    if (!bad) {
-      const double nrm = tabs[TAB_POW + k];
-      zr = s_cos[ph] * nrm;
-      zi = s_sin[ph] * nrm;
-    }
-    const double wr = zr * 1.0 - zi * 0.0, wi = zr * 0.0 + zi * 1.0;
+    const double nrm = tabs[TAB_POW + k];
     const double2 c = coef[t];
-    const double tr = wr * c.x - wi * c.y, ti = wr * c.y + wi * c.x;
-    are = are + tr;
-    aim = aim + ti;
+#pragma unroll
+    for (int j = 0; j < XB; ++j) {
+      int ax = 0;
+#pragma unroll
+      for (int w = 0; w < W; ++w)
+        ax += __builtin_popcountll(x[j][w] & q.alo[(size_t)t * W + w]) +
+              2 * __builtin_popcountll(x[j][w] & q.ahi[(size_t)t * W + w]);
+      const int quad = parand<W>(acc[j], x[j]);
+      const int ph = (p0 + 2 * ax + 4 * quad) & 7;
+      if (per_term && active[j]) per_term[(size_t)t * nb + bidx[j]] = bad[j] ? (uint8_t)0xFF : (uint8_t)ph;
+      double zr = 0.0, zi = 0.0;
+      if (!bad[j]) {
+        zr = s_cos[ph] * nrm;
+        zi = s_sin[ph] * nrm;
+      }
+      const double wr = zr * 1.0 - zi * 0.0, wi = zr * 0.0 + zi * 1.0;
+      const double tr = wr * c.x - wi * c.y, ti = wr * c.y + wi * c.x;
+      are[j] = are[j] + tr;
+      aim[j] = aim[j] + ti;
+    }
   }
-  // * global_factor (statevector.rs:23)
-  if (active) out[b] = make_double2(are * gre - aim * gim, are * gim + aim * gre);
+#pragma unroll
+  for (int j = 0; j < XB; ++j) {
+    if (!active[j]) continue;
+    if (final_g)  // * global_factor (statevector.rs:23)
+      out[bidx[j]] = make_double2(are[j] * gre - aim[j] * gim, are[j] * gim + aim[j] * gre);
+    else
+      out[(size_t)blockIdx.y * nb + bidx[j]] = make_double2(are[j], aim[j]);
+  }
+}
+
+// sum of the per-chunk partial amplitudes in chunk order, then * global_factor
+__global__ void k_amp_reduce(const double2* __restrict__ part, int64_t nb, int nchunks, double2* __restrict__ out,
+                             double gre, double gim) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double r = 0.0, i = 0.0;
+  for (int c = 0; c < nchunks; ++c) {
+    const double2 v = part[(size_t)c * nb + b];
+    r = r + v.x;
+    i = i + v.y;
+  }
+  out[b] = make_double2(r * gre - i * gim, r * gim + i * gre);
 }
 
 // ---------------------------------------------------------------------------
@@ -749,11 +800,23 @@ hipError_t launch_qform(const FormsDev& in, const QFormDev& q, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const u64* bits, int64_t nb, int64_t t0,
-                            int64_t t1, const double* tabs, double2* out, uint8_t* per_term, int enumerate,
-                            double gre, double gim, hipStream_t st) {
+hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const u64* bits, int64_t nb, int64_t chunk_terms,
+                            int nchunks, const double* tabs, double2* out, double2* partial, uint8_t* per_term,
+                            int enumerate, double gre, double gim, hipStream_t st) {
   if (nb == 0) return hipSuccess;
-  NX_DISPATCH_W(q.W, k_amplitude<WW><<<dim3(blocks_for(nb, 256)), dim3(256), 0, st>>>(q, coef, bits, nb, t0, t1, tabs, out, per_term, enumerate, gre, gim));
+  constexpr int XB = 2;
+  const dim3 grid(blocks_for(nb, 256 * XB), nchunks);
+  const int final_g = nchunks == 1;
+  double2* dst = final_g ? out : partial;
+  NX_DISPATCH_W(q.W, k_amplitude<WW, XB><<<grid, dim3(256), 0, st>>>(q, coef, bits, nb, chunk_terms, tabs, dst,
+                                                                      per_term, enumerate, final_g, gre, gim));
+  return hipGetLastError();
+}
+
+hipError_t launch_amp_reduce(const double2* partial, int64_t nb, int nchunks, double2* out, double gre, double gim,
+                             hipStream_t st) {
+  if (nb == 0 || nchunks <= 1) return hipSuccess;
+  k_amp_reduce<<<dim3(blocks_for(nb, 256)), dim3(256), 0, st>>>(partial, nb, nchunks, out, gre, gim);
   return hipGetLastError();
 }
 

@@ -993,11 +993,25 @@ static void amplitudes_impl(const nx_state* s, int64_t nbits, const uint64_t* d_
                             uint8_t* d_per_term, int enumerate) {
   const QForms& q = qform_of(s);
   CoefDev c = coef_of(s);
+  // enough waves to cover the GPU several times: split the term loop into
+  // chunks when the strings alone give too few waves
+  const int64_t chi = s->chi();
+  const int64_t waves_x = (nbits + 127) / 128;
+  int64_t nchunks = waves_x >= 8192 ? 1 : (8192 + waves_x - 1) / waves_x;
+  nchunks = std::min<int64_t>(std::min<int64_t>(nchunks, chi), 64);
+  if (nchunks < 1) nchunks = 1;
+  const int64_t chunk_terms = (chi + nchunks - 1) / nchunks;
+  nchunks = (chi + chunk_terms - 1) / chunk_terms;
+  DevBuf partial(nchunks > 1 ? (size_t)nchunks * nbits * 16 : 0);
   ProfScope ps("k_amplitude");
-  hip_check(launch_amplitude(q.desc(), c.c, d_bits, nbits, 0, s->chi(), ctx()->tabs, (double2*)d_out, d_per_term,
-                             enumerate, s->g.real(), s->g.imag(), stream()),
+  hip_check(launch_amplitude(q.desc(), c.c, d_bits, nbits, chunk_terms, (int)nchunks, ctx()->tabs, (double2*)d_out,
+                             partial.as<double2>(), d_per_term, enumerate, s->g.real(), s->g.imag(), stream()),
             "k_amplitude");
   ps.end();
+  hip_check(launch_amp_reduce(partial.as<double2>(), nbits, (int)nchunks, (double2*)d_out, s->g.real(), s->g.imag(),
+                              stream()),
+            "k_amp_reduce");
+  sync();
 }
 
 int nx_state_amplitudes(const nx_state* s, int64_t nbits, const uint64_t* bits, int32_t words, double* out,
