@@ -270,5 +270,5 @@ def test_config1_full_size(nx):
         x = np.array([(int(bits[b, 0]) >> q) & 1 for q in range(n)], np.uint8)
         val_o, (zo, ko, ro) = o.amplitude(x, per_term=True)
         assert np.array_equal(pt[:, b] == 255, zo)
-        assert np.array_equal(pt[~zo, b], ko[~zo]) and np.array_equal(rr, ro)
+        assert np.array_equal(pt[~zo, b], ko[~zo]) and np.array_equal(rr[~zo], ro[~zo])
         assert abs(vals[b] - val_o) <= TOL
