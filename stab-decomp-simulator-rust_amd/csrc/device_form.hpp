@@ -122,7 +122,7 @@ __device__ __forceinline__ bool owns(const Form<W>& f, int r, int sl) {
 
 // left_multiplication/cx.rs:13-51
 template <int W>
-__device__ void left_cx(Form<W>& f, int c, int t) {
+__device__ __forceinline__ void left_cx(Form<W>& f, int c, int t) {
   u64 Gc[W], Ft[W], Mt[W];
   bcast<W>(f.G, c, Gc);
   bcast<W>(f.F, t, Ft);
@@ -148,7 +148,7 @@ __device__ void left_cx(Form<W>& f, int c, int t) {
 
 // left_multiplication/cz.rs:12-35
 template <int W>
-__device__ void left_cz(Form<W>& f, int a, int b) {
+__device__ __forceinline__ void left_cz(Form<W>& f, int a, int b) {
   u64 Ga[W], Gb[W];
   bcast<W>(f.G, a, Ga);
   bcast<W>(f.G, b, Gb);
@@ -167,7 +167,7 @@ __device__ void left_cz(Form<W>& f, int a, int b) {
 
 // left_multiplication/s.rs:13-33 (add = 3 for S: gamma *= -i; 1 for Sdg)
 template <int W>
-__device__ void left_s_like(Form<W>& f, int q, int add) {
+__device__ __forceinline__ void left_s_like(Form<W>& f, int q, int add) {
 #pragma unroll
   for (int sl = 0; sl < W; ++sl)
     if (owns<W>(f, q, sl)) {
@@ -179,7 +179,7 @@ __device__ void left_s_like(Form<W>& f, int q, int add) {
 
 // left_multiplication/z.rs:7-14
 template <int W>
-__device__ void left_z(Form<W>& f, int q) {
+__device__ __forceinline__ void left_z(Form<W>& f, int q) {
 #pragma unroll
   for (int sl = 0; sl < W; ++sl)
     if (owns<W>(f, q, sl)) f.g[sl] = (f.g[sl] + 2) & 3;
@@ -187,7 +187,7 @@ __device__ void left_z(Form<W>& f, int q) {
 
 // left_multiplication/x.rs:12-48
 template <int W>
-__device__ void left_x(Form<W>& f, int q) {
+__device__ __forceinline__ void left_x(Form<W>& f, int q) {
   u64 Fq[W], Mq[W];
   bcast<W>(f.F, q, Fq);
   bcast<W>(f.M, q, Mq);
@@ -203,7 +203,7 @@ __device__ void left_x(Form<W>& f, int q) {
 
 // left_multiplication/y.rs:4-11
 template <int W>
-__device__ void left_y(Form<W>& f, int q) {
+__device__ __forceinline__ void left_y(Form<W>& f, int q) {
   left_z<W>(f, q);
   left_x<W>(f, q);
   f.ph = (f.ph + 2) & 7;
@@ -213,7 +213,7 @@ __device__ void left_y(Form<W>& f, int q) {
 
 // right_multiplication/s.rs:8-24
 template <int W>
-__device__ void right_s(Form<W>& f, int q) {
+__device__ __forceinline__ void right_s(Form<W>& f, int q) {
 #pragma unroll
   for (int sl = 0; sl < W; ++sl) {
     int fq = getbit<W>(f.F[sl], q);
@@ -224,7 +224,7 @@ __device__ void right_s(Form<W>& f, int q) {
 
 // right_multiplication/cx.rs:7-26 (one gate)
 template <int W>
-__device__ void right_cx(Form<W>& f, int c, int t) {
+__device__ __forceinline__ void right_cx(Form<W>& f, int c, int t) {
 #pragma unroll
   for (int sl = 0; sl < W; ++sl) {
     flipbit<W>(f.G[sl], c, getbit<W>(f.G[sl], t));
@@ -236,7 +236,7 @@ __device__ void right_cx(Form<W>& f, int c, int t) {
 // right_multiplication/cz.rs:8-35 for every i in D (D excludes p): the
 // sequential loop of resolve_superposition.rs:171-173 / discard.rs:174-178
 template <int W>
-__device__ void right_cz_fan(Form<W>& f, int p, const u64 (&D)[W]) {
+__device__ __forceinline__ void right_cz_fan(Form<W>& f, int p, const u64 (&D)[W]) {
 #pragma unroll
   for (int sl = 0; sl < W; ++sl) {
     int fp = getbit<W>(f.F[sl], p);
@@ -252,7 +252,7 @@ __device__ void right_cz_fan(Form<W>& f, int p, const u64 (&D)[W]) {
 
 // right_cx(p, i) for every i in D (resolve_superposition.rs:167-170)
 template <int W>
-__device__ void right_cx_fan_ctrl(Form<W>& f, int p, const u64 (&D)[W]) {
+__device__ __forceinline__ void right_cx_fan_ctrl(Form<W>& f, int p, const u64 (&D)[W]) {
 #pragma unroll
   for (int sl = 0; sl < W; ++sl) {
     flipbit<W>(f.G[sl], p, parand<W>(f.G[sl], D));
@@ -267,7 +267,7 @@ __device__ void right_cx_fan_ctrl(Form<W>& f, int p, const u64 (&D)[W]) {
 // right_cx(i, p) for every i in D (resolve_superposition.rs:160-165,
 // discard.rs:152-158)
 template <int W>
-__device__ void right_cx_fan_targ(Form<W>& f, int p, const u64 (&D)[W]) {
+__device__ __forceinline__ void right_cx_fan_targ(Form<W>& f, int p, const u64 (&D)[W]) {
 #pragma unroll
   for (int sl = 0; sl < W; ++sl) {
     if (getbit<W>(f.G[sl], p)) {
@@ -284,7 +284,7 @@ __device__ void right_cx_fan_targ(Form<W>& f, int p, const u64 (&D)[W]) {
 
 // resolve_superposition.rs:6-113 ; delta in units of i (0:+1 1:+i 2:-1 3:-i)
 template <int W>
-__device__ int resolve(Form<W>& f, const u64 (&t)[W], const u64 (&u)[W], int delta) {
+__device__ __forceinline__ int resolve(Form<W>& f, const u64 (&t)[W], const u64 (&u)[W], int delta) {
   bool same = true;
 #pragma unroll
   for (int w = 0; w < W; ++w) same = same && (t[w] == u[w]);
@@ -344,7 +344,7 @@ __device__ int resolve(Form<W>& f, const u64 (&t)[W], const u64 (&u)[W], int del
 
 // left_multiplication/h.rs:13-65
 template <int W>
-__device__ int left_h(Form<W>& f, int q) {
+__device__ __forceinline__ int left_h(Form<W>& f, int q) {
   u64 Gq[W], Fq[W], Mq[W], t[W], u[W];
   bcast<W>(f.G, q, Gq);
   bcast<W>(f.F, q, Fq);
@@ -368,7 +368,7 @@ __device__ int left_h(Form<W>& f, int q) {
 
 // project.rs:33-76 ; returns error, sets *dead (impossible) / *det
 template <int W>
-__device__ int project(Form<W>& f, int q, int outcome, int* dead, int* det) {
+__device__ __forceinline__ int project(Form<W>& f, int q, int outcome, int* dead, int* det) {
   u64 Gq[W], gv[W];
   bcast<W>(f.G, q, Gq);
 #pragma unroll
@@ -433,7 +433,7 @@ __device__ __forceinline__ void remove_row(Form<W>& f, u64 (&X)[W][W], int q) {
 
 // discard.rs:21-182
 template <int W>
-__device__ int discard(Form<W>& f, int q) {
+__device__ __forceinline__ int discard(Form<W>& f, int q) {
   u64 valid[W];
 #pragma unroll
   for (int w = 0; w < W; ++w) {

@@ -476,6 +476,13 @@ struct PairRegs {
 };
 
 template <int W>
+__device__ __forceinline__ void clearbit(u64 (&x)[W], int pw, int pb) {
+#pragma unroll
+  for (int w = 0; w < W; ++w)
+    if (w == pw) x[w] &= ~(1ull << pb);
+}
+
+template <int W>
 __device__ __forceinline__ int bcast_int(const int (&x)[W], int r) {
   const int sl = r >> 6;
   int val = x[0];
@@ -485,54 +492,53 @@ __device__ __forceinline__ int bcast_int(const int (&x)[W], int r) {
   return rl32(val, r & 63);
 }
 
-// substitute x_p = c ^ <r, x> (r excludes p) into the form and the
-// remaining constraint rows
-template <int W>
+// substitute x_p = c ^ <r, x> (r excludes p) into the form; with CONS also
+// into the remaining constraint rows.  Rows of eliminated variables and
+// padding rows may hold junk: they are never broadcast again, and every
+// column of an eliminated variable is cleared in all rows.
+template <int W, bool CONS>
 __device__ __forceinline__ void subst(PairRegs<W>& P, int p, const u64 (&r)[W], int c) {
   const int ap = bcast_int<W>(P.a, p);
   u64 Bp[W];
   bcast<W>(P.B, p, Bp);
   P.c0 = (P.c0 + 2 * ap * c) & 7;
-  const int sgn = 1 - 2 * c;
-  const bool aodd = ap & 1;
+  // a_l += ap (1 - 2c) r_l + 2 (c + r_l) B_pl ;  B_l ^= r_l ((ap odd) r ^ Bp) ^ B_pl r
+  const int lin = c ? (4 - ap) & 3 : ap;
+  const int pw = p >> 6, pb = p & 63;
 #pragma unroll
   for (int sl = 0; sl < W; ++sl) {
-    const int l = 64 * sl + P.lane;
     const int rl = (int)((r[sl] >> P.lane) & 1ull);
-    const int bpl = getbit<W>(P.B[sl], p);
-    const bool live = ((P.rem[sl] >> P.lane) & 1ull) && l != p;
-    if (live) {
-      P.a[sl] = (P.a[sl] + ap * sgn * rl + 2 * (c * bpl + rl * bpl)) & 3;
+    const int bpl = (int)((wsel<W>(P.B[sl], pw) >> pb) & 1ull);
+    P.a[sl] = (P.a[sl] + (rl ? lin : 0) + ((c ^ rl) & bpl ? 2 : 0)) & 3;
+    const u64 mr = 0ull - (u64)rl, mb = 0ull - (u64)bpl;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const u64 t = ((ap & 1) ? r[w] : 0ull) ^ Bp[w];
+      P.B[sl][w] ^= (t & mr) ^ (r[w] & mb);
+    }
+    // clear the diagonal bit of this lane's row and column p
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+      if (w == sl) P.B[sl][w] &= ~(1ull << P.lane);
+    clearbit<W>(P.B[sl], pw, pb);
+    if (CONS) {
+      const int ha = (int)((wsel<W>(P.ca[sl], pw) >> pb) & 1ull);
+      const int hb = (int)((wsel<W>(P.cb[sl], pw) >> pb) & 1ull);
+      const u64 ma = 0ull - (u64)ha, mb2 = 0ull - (u64)hb;
 #pragma unroll
       for (int w = 0; w < W; ++w) {
-        u64 d = 0;
-        if (rl) d ^= (aodd ? r[w] : 0ull) ^ Bp[w];
-        if (bpl) d ^= r[w];
-        P.B[sl][w] ^= d;
+        P.ca[sl][w] ^= r[w] & ma;
+        P.cb[sl][w] ^= r[w] & mb2;
       }
-      setbitv<W>(P.B[sl], l, 0);
-    }
-    setbitv<W>(P.B[sl], p, 0);
-    // constraint rows of both terms
-    const int ha = getbit<W>(P.ca[sl], p);
-    const int hb = getbit<W>(P.cb[sl], p);
-    if (ha) {
-#pragma unroll
-      for (int w = 0; w < W; ++w) P.ca[sl][w] ^= r[w];
-      setbitv<W>(P.ca[sl], p, 0);
-    }
-    if (hb) {
-#pragma unroll
-      for (int w = 0; w < W; ++w) P.cb[sl][w] ^= r[w];
-      setbitv<W>(P.cb[sl], p, 0);
-    }
-    const u64 ba = __ballot(ha), bb = __ballot(hb);
-    if (c) {
-      P.rhsa[sl] ^= ba;
-      P.rhsb[sl] ^= bb;
+      clearbit<W>(P.ca[sl], pw, pb);
+      clearbit<W>(P.cb[sl], pw, pb);
+      if (c) {
+        P.rhsa[sl] ^= __ballot(ha);
+        P.rhsb[sl] ^= __ballot(hb);
+      }
     }
   }
-  flipbit<W>(P.rem, p, getbit<W>(P.rem, p));
+  clearbit<W>(P.rem, pw, pb);
 }
 
 // returns 1 if a constraint is inconsistent (the sum is zero)
@@ -544,7 +550,7 @@ __device__ __forceinline__ int take_constraint(PairRegs<W>& P, int useb, int c) 
   if (!anyv<W>(row)) return rhs;
   const int p = ffsv<W>(row);
   flipbit<W>(row, p, 1);
-  subst<W>(P, p, row, rhs);
+  subst<W, true>(P, p, row, rhs);
   return 0;
 }
 
@@ -557,7 +563,8 @@ __device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int*
     if (take_constraint<W>(P, 1, c)) { *zero = 1; return; }
   while (anyv<W>(P.rem)) {
     const int p = uni(ffsv<W>(P.rem));
-    flipbit<W>(P.rem, p, 1);
+    const int pw = p >> 6, pb = p & 63;
+    clearbit<W>(P.rem, pw, pb);
     u64 Bp[W];
     bcast<W>(P.B, p, Bp);
     const int ap = bcast_int<W>(P.a, p);
@@ -566,29 +573,30 @@ __device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int*
       if (ap == 0) P.mag -= 2;
       else { P.mag -= 1; P.c0 = (P.c0 + (ap == 1 ? 1 : 7)) & 7; }
     } else if (ap & 1) {
-      const int add = (ap == 1) ? 3 : 1;  // a_l -= sigma (sigma = +1 for a_p = 1, -1 for a_p = 3)
+      // sum over x_p: sqrt2 zeta^s i^{-s lambda}, lambda = <B_p, x>, s = +-1
+      const int add = (ap == 1) ? 3 : 1;
       P.mag -= 1;
       P.c0 = (P.c0 + (ap == 1 ? 1 : 7)) & 7;
 #pragma unroll
       for (int sl = 0; sl < W; ++sl) {
-        const int l = 64 * sl + P.lane;
-        const int bpl = getbit<W>(P.B[sl], p);
-        const bool live = ((P.rem[sl] >> P.lane) & 1ull);
-        if (live && bpl) {
-          P.a[sl] = (P.a[sl] + add) & 3;
+        const int bpl = (int)((wsel<W>(P.B[sl], pw) >> pb) & 1ull);
+        const u64 mb = 0ull - (u64)bpl;
+        P.a[sl] = (P.a[sl] + (bpl ? add : 0)) & 3;
 #pragma unroll
-          for (int w = 0; w < W; ++w) P.B[sl][w] ^= Bp[w];
-          setbitv<W>(P.B[sl], l, 0);
-        }
-        setbitv<W>(P.B[sl], p, 0);
+        for (int w = 0; w < W; ++w) P.B[sl][w] ^= Bp[w] & mb;
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+          if (w == sl) P.B[sl][w] &= ~(1ull << P.lane);
+        clearbit<W>(P.B[sl], pw, pb);
       }
     } else {
+      // sum over x_p: 2 [lambda = a_p / 2]; then substitute for x_q
       P.mag -= 2;
 #pragma unroll
-      for (int sl = 0; sl < W; ++sl) setbitv<W>(P.B[sl], p, 0);
+      for (int sl = 0; sl < W; ++sl) clearbit<W>(P.B[sl], pw, pb);
       const int qv = ffsv<W>(Bp);
       flipbit<W>(Bp, qv, 1);
-      subst<W>(P, qv, Bp, ap >> 1);
+      subst<W, false>(P, qv, Bp, ap >> 1);
     }
   }
   *zero = 0;
