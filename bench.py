@@ -88,6 +88,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=16.0)
+    ap.add_argument("--config", type=int, default=1, choices=(1, 3),
+                    help="1: amplitude sweep + norm (the metric); 3: Pauli-loss forward vs backward")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 1 GiB device copy (PMC calibration)")
     args = ap.parse_args()
@@ -105,6 +107,8 @@ def main():
 
     L = _lib.lib()
     _lib.check(L.nx_init(0))
+    if args.config == 3:
+        return bench_cfg3(args)
     cfg = WL.config(1)
     n = cfg["n"]
     t0 = time.perf_counter()
@@ -188,6 +192,43 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def bench_cfg3(args):
+    """Differentiation workload: n=48, 1500 Clifford + 30 T, 128 Paulis of
+    weight 2-6 with N(0,1) weights, coefficients perturbed by N(0, 1e-3).
+    Forward: L = sum_k w_k exp_value(P_k); backward: dL/dc (Gram matvec)."""
+    import numpy as np
+    import workloads as WL
+    from necstar import PauliString, QuantumState
+    cfg = WL.config(3)
+    n = cfg["n"]
+    st = QuantumState.from_circuit(WL.to_circuit(n, cfg["gates"]))
+    chi = st.stabilizer_rank
+    rng = np.random.default_rng(cfg["perturb_seed"])
+    c = st.coefficients() + 1e-3 * (rng.standard_normal(chi) + 1j * rng.standard_normal(chi))
+    st.set_coefficients(c)
+    paulis = [PauliString.from_str("".join("IXYZ"[x] for x in reversed(p))) for p in cfg["paulis"]]
+    w = cfg["weights"]
+    for _ in range(args.warmup):
+        st.exp_values(paulis[:4])
+        st.grad_exp_values(paulis[:4], w[:4])
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ev = st.exp_values(paulis)
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        loss, grad = st.grad_exp_values(paulis, w)
+    t2 = time.perf_counter()
+    fwd, bwd = (t1 - t0) / args.steps, (t2 - t1) / args.steps
+    pairs = len(paulis) * chi * (chi + 1) // 2
+    print(json.dumps({
+        "metric": "coefficient backward / forward cost, cfg3", "value": bwd / fwd, "unit": "ratio",
+        "higher_is_better": False, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "forward_s": fwd, "backward_s": bwd, "pairs_per_pass": pairs,
+        "forward_inner_products_per_s": pairs / fwd, "loss": float(np.dot(w, ev)), "loss_bwd": loss,
+        "config": {"workload": "cfg3: n=48, 1500 Clifford + 30 T, 128 Paulis (weight 2-6)", "chi": chi},
+    }), flush=True)
 
 
 if __name__ == "__main__":

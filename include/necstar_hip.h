@@ -125,10 +125,15 @@ int nx_state_global_factor(const nx_state *s, double *re, double *im);
  * the same values in StabilizerCHForm::inner_product (inner_product.rs:14-48). */
 int nx_state_pair_table(const nx_state *s, uint32_t *out);
 
+/* exp_value for npauli dense Pauli strings ops[npauli][n] (little-endian,
+ * 0 I, 1 X, 2 Y, 3 Z): out[k] as exp_value.rs:7-61 computes it. */
+int nx_state_exp_values(const nx_state *s, int32_t npauli, const int32_t *ops, double *out);
+
 /* Coefficient backward of L = sum_k w_k exp_value(P_k) with respect to the
  * decomposition coefficients: grad = dL/d(conj c) * 2 = 2 |g|^2 sum_k w_k G_k c,
  * (G_k)_{ji} = <phi_j|P_k phi_i>.  paulis: npauli dense strings ops[npauli][n].
- * grad: complex128 [chi].  No reference counterpart (an addition for
+ * grad: complex128 [chi] (the PyTorch convention for a real loss of complex
+ * leaves); *loss receives L.  Up to 16384 terms.  No reference counterpart (an addition for
  * differentiation; the forward is exp_value.rs:7-61). */
 int nx_state_grad_exp_values(const nx_state *s, int32_t npauli, const int32_t *ops, const double *weights,
                              double *loss, double *grad);

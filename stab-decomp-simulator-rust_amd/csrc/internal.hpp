@@ -83,5 +83,9 @@ hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, const int64_t* off
                         const double* tabs, double2* partial, int nblocks, uint32_t* exact_out, int64_t exact_ld,
                         hipStream_t st);
 hipError_t launch_reduce(const double2* partial, int64_t n, double2* out, hipStream_t st);
+// y += w * G c for the Hermitian G whose lower triangle T[j*ld + i] (j >= i)
+// holds exact pair scalars
+hipError_t launch_gram(const uint32_t* T, int64_t ld, int64_t chi, const double2* c, double w, const double* tabs,
+                       double2* y, hipStream_t st);
 
 }  // namespace nx

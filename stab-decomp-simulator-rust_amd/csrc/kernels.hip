@@ -740,6 +740,72 @@ __global__ __launch_bounds__(256) void k_pairs(QFormDev X, QFormDev Y, const int
   if (lane == 0) partial[wave] = make_double2(acc_re, acc_im);
 }
 
+// ---------------------------------------------------------------------------
+// Conjugate Gram matvec from exact pair scalars.  T[j * ld + i] holds
+// <phi_j | P phi_i> for j >= i (k | r << 8, 0xFFFFFFFF = zero); G is
+// Hermitian, G_ij = conj(G_ji).  y += w * G c, in two ordered passes.
+__device__ __forceinline__ double2 pair_value(uint32_t e, const double* tabs) {
+  if (e == 0xFFFFFFFFu) return make_double2(0.0, 0.0);
+  const int k = (int)(e & 7u), r = (int)(e >> 8);
+  const double nrm = tabs[TAB_POW + r];
+  return make_double2(tabs[TAB_COS + k] * nrm, tabs[TAB_SIN + k] * nrm);
+}
+
+// y_j += w * sum_{i <= j} G_ji c_i : one block per row j, coalesced along i
+__global__ __launch_bounds__(256) void k_gram_rows(const uint32_t* __restrict__ T, int64_t ld, int64_t chi,
+                                                   const double2* __restrict__ c, double w,
+                                                   const double* __restrict__ tabs, double2* __restrict__ y) {
+  __shared__ double sr[256], si[256];
+  const int64_t j = blockIdx.x;
+  double ar = 0.0, ai = 0.0;
+  for (int64_t i = threadIdx.x; i <= j; i += blockDim.x) {
+    const double2 g = pair_value(T[(size_t)j * ld + i], tabs);
+    const double2 ci = c[i];
+    ar = ar + (g.x * ci.x - g.y * ci.y);
+    ai = ai + (g.x * ci.y + g.y * ci.x);
+  }
+  sr[threadIdx.x] = ar;
+  si[threadIdx.x] = ai;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      sr[threadIdx.x] += sr[threadIdx.x + st];
+      si[threadIdx.x] += si[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double2 v = y[j];
+    v.x = v.x + w * sr[0];
+    v.y = v.y + w * si[0];
+    y[j] = v;
+  }
+}
+
+// y_i += w * sum_{j > i} conj(G_ji) c_j : one thread per column i, rows in
+// order (threads of a wave read consecutive entries of row j)
+__global__ __launch_bounds__(256) void k_gram_cols(const uint32_t* __restrict__ T, int64_t ld, int64_t chi,
+                                                   const double2* __restrict__ c, double w,
+                                                   const double* __restrict__ tabs, double2* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t jstart = (int64_t)blockIdx.x * blockDim.x + 1;  // smallest row any thread of the block needs
+  double ar = 0.0, ai = 0.0;
+  for (int64_t j = jstart; j < chi; ++j) {
+    if (i < chi && j > i) {
+      const double2 g = pair_value(T[(size_t)j * ld + i], tabs);
+      const double2 cj = c[j];
+      ar = ar + (g.x * cj.x + g.y * cj.y);  // conj(g) * c_j
+      ai = ai + (g.x * cj.y - g.y * cj.x);
+    }
+  }
+  if (i < chi) {
+    double2 v = y[i];
+    v.x = v.x + w * ar;
+    v.y = v.y + w * ai;
+    y[i] = v;
+  }
+}
+
 // single-block ordered reduction of the per-wave partial sums
 __global__ void k_reduce(const double2* __restrict__ partial, int64_t n, double2* __restrict__ out) {
   __shared__ double sr[256], si[256];
@@ -835,6 +901,16 @@ hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, const int64_t* off
   NX_DISPATCH_W(X.W, k_pairs<WW><<<dim3(nblocks), dim3(256), 0, st>>>(X, Y, offsets, nfixed, nother, triangle, J,
                                                                       mode, cx, cy, tabs, partial, exact_out,
                                                                       exact_ld));
+  return hipGetLastError();
+}
+
+hipError_t launch_gram(const uint32_t* T, int64_t ld, int64_t chi, const double2* c, double w, const double* tabs,
+                       double2* y, hipStream_t st) {
+  if (chi == 0) return hipSuccess;
+  k_gram_rows<<<dim3((unsigned)chi), dim3(256), 0, st>>>(T, ld, chi, c, w, tabs, y);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_gram_cols<<<dim3(blocks_for(chi, 256)), dim3(256), 0, st>>>(T, ld, chi, c, w, tabs, y);
   return hipGetLastError();
 }
 

@@ -873,6 +873,10 @@ static std::vector<ProgOp> pauli_prog(const nx_state* s, int dense, int32_t len,
   return p;
 }
 
+static Forms run_pairs_prep(Forms& f, const std::vector<ProgOp>& prog, int n) {
+  return run_program(f, prog, n, nullptr);
+}
+
 // exp_value.rs:7-61: sum over i <= j of c_j* c_i <phi_j|P phi_i>
 static double exp_value(const nx_state* s, const std::vector<ProgOp>& prog) {
   if (s->chi() == 0) return 0.0;
@@ -1205,11 +1209,51 @@ int nx_state_pair_table(const nx_state* s, uint32_t* out) {
   })
 }
 
+int nx_state_exp_values(const nx_state* s, int32_t npauli, const int32_t* ops, double* out) {
+  NX_TRY({
+    for (int32_t k = 0; k < npauli; ++k)
+      out[k] = exp_value(s, pauli_prog(s, 1, s->n, ops + (size_t)k * s->n, nullptr));
+  })
+}
+
 int nx_state_grad_exp_values(const nx_state* s, int32_t npauli, const int32_t* ops, const double* weights,
                              double* loss, double* grad) {
   NX_TRY({
-    (void)s; (void)npauli; (void)ops; (void)weights; (void)loss; (void)grad;
-    fail(NX_E_INTERNAL, "nx_state_grad_exp_values: not implemented yet");
+    const int64_t chi = s->chi();
+    std::vector<std::vector<ProgOp>> progs;
+    for (int32_t k = 0; k < npauli; ++k) progs.push_back(pauli_prog(s, 1, s->n, ops + (size_t)k * s->n, nullptr));
+    if (chi == 0) {
+      *loss = 0.0;
+      return NX_OK;
+    }
+    if (chi > 16384) fail(NX_E_VALUE, "the dense Gram backward supports up to 16384 terms (got %lld)", (long long)chi);
+    const QForms& q = qform_of(s);
+    CoefDev c = coef_of(s);
+    DevBuf T((size_t)chi * chi * 4), y((size_t)chi * 16);
+    hip_check(hipMemsetAsync(y.p, 0, (size_t)chi * 16, stream()), "hipMemsetAsync");
+    for (int32_t k = 0; k < npauli; ++k) {
+      Forms pf = clone_forms(s->forms);
+      if (!progs[k].empty()) pf = run_pairs_prep(pf, progs[k], s->n);
+      QForms qp = make_qform(pf);
+      // exact <phi_j | P_k phi_i> for j >= i into T[j * chi + i]
+      run_pairs(q, qp, chi, chi, true, 3, c, c, T.as<uint32_t>(), chi);
+      ProfScope ps("k_gram");
+      hip_check(launch_gram(T.as<uint32_t>(), chi, chi, c.c, weights[k], ctx()->tabs, y.as<double2>(), stream()),
+                "k_gram");
+      ps.end();
+    }
+    std::vector<double> yh(2 * chi);
+    d2h(yh.data(), y.p, (size_t)chi * 16);
+    sync();
+    const double g2 = s->g.real() * s->g.real() + s->g.imag() * s->g.imag();
+    double acc = 0.0;
+    for (int64_t j = 0; j < chi; ++j) {
+      const cplx cj = s->coeff_value(j);
+      acc += cj.real() * yh[2 * j] + cj.imag() * yh[2 * j + 1];  // Re(conj(c_j) y_j)
+      grad[2 * j] = 2.0 * g2 * yh[2 * j];
+      grad[2 * j + 1] = 2.0 * g2 * yh[2 * j + 1];
+    }
+    *loss = acc * g2;
   })
 }
 

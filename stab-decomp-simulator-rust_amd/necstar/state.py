@@ -128,6 +128,49 @@ class QuantumState:
                                        opp, qsp, C.byref(v)))
         return v.value
 
+    def _dense_ops(self, paulis):
+        """Dense little-endian op rows [len(paulis), n] with the length checks
+        of exp_value.rs:9-27."""
+        n = self.num_qubits
+        rows = np.zeros((max(len(paulis), 1), max(n, 1)), np.int32)
+        for k, p in enumerate(paulis):
+            if p._dense:
+                if len(p._ops) != n:
+                    raise ValueError(f"Invalid Pauli string length: expected {n}, found {len(p._ops)}.")
+                rows[k, :n] = p._ops
+            else:
+                mx = max(p._qubits) if p._qubits else 0
+                if mx >= n:
+                    raise ValueError(f"Invalid Pauli string length: expected {n}, found {mx + 1}.")
+                for o, q in zip(p._ops, p._qubits):
+                    rows[k, q] = o
+        return rows
+
+    def exp_values(self, paulis) -> np.ndarray:
+        """exp_value for a list of PauliStrings (one call through the C-ABI)."""
+        paulis = list(paulis)
+        rows = self._dense_ops(paulis)
+        out = np.zeros(max(len(paulis), 1), np.float64)
+        check(lib().nx_state_exp_values(self._h, len(paulis), rows.ctypes.data_as(C.POINTER(C.c_int32)),
+                                        out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out[: len(paulis)]
+
+    def grad_exp_values(self, paulis, weights):
+        """L = sum_k w_k exp_value(P_k) and dL/dc in PyTorch's convention for
+        complex leaves (2 dL/d conj(c) = 2 |g|^2 sum_k w_k G_k c)."""
+        paulis = list(paulis)
+        w = np.ascontiguousarray(np.asarray(weights, np.float64))
+        if w.shape != (len(paulis),):
+            raise ValueError("one weight per Pauli string is required")
+        rows = self._dense_ops(paulis)
+        chi = self.stabilizer_rank
+        grad = np.zeros(2 * max(chi, 1), np.float64)
+        loss = C.c_double()
+        check(lib().nx_state_grad_exp_values(self._h, len(paulis), rows.ctypes.data_as(C.POINTER(C.c_int32)),
+                                             w.ctypes.data_as(C.POINTER(C.c_double)), C.byref(loss),
+                                             grad.ctypes.data_as(C.POINTER(C.c_double))))
+        return loss.value, grad[: 2 * chi].view(np.complex128).copy()
+
     def amplitudes(self, bitstrings, per_term: bool = False):
         """<x|psi> for a batch of bitstrings (API addition).  bitstrings:
         uint64 array [B] (n <= 64) or [B, words], little-endian (qubit j =
