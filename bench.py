@@ -90,6 +90,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=16.0)
     ap.add_argument("--config", type=int, default=1, choices=(1, 3),
                     help="1: amplitude sweep + norm (the metric); 3: Pauli-loss forward vs backward")
+    ap.add_argument("--sample-shots", type=int, default=0,
+                    help="also time QuantumState.sample over all 64 qubits with this many shots (cfg1)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 1 GiB device copy (PMC calibration)")
     args = ap.parse_args()
@@ -152,6 +154,18 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
 
+    sampling = None
+    if args.sample_shots > 0:
+        _lib.check(L.nx_profile_enable(1))
+        ts = time.perf_counter()
+        counts = st.sample(list(range(n)), args.sample_shots, seed=cfg["sample_seed"])
+        te = time.perf_counter() - ts
+        pms, pcnt = C.c_double(), C.c_int64()
+        _lib.check(L.nx_profile_get(b"k_pairs", C.byref(pms), C.byref(pcnt)))
+        _lib.check(L.nx_profile_enable(0))
+        sampling = {"shots": args.sample_shots, "seconds": te, "distinct_outcomes": len(counts),
+                    "pair_kernel_launches": pcnt.value, "pair_kernel_ms": pms.value}
+
     ms_step = (t1 - t0) * 1e3 / args.steps
     amp_units = chi * B
     pair_units = chi * (chi + 1) // 2
@@ -188,7 +202,7 @@ def main():
         "amplitude_evals_per_s": amp_rate, "inner_products_per_s": pair_rate,
         "kernel_ms": {k: v[0] for k, v in prof.items()},
         "norm": norm.value, "compile_s": compile_s,
-        "batched_vs_plain": None,
+        "batched_vs_plain": None, "sampling": sampling,
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
