@@ -28,6 +28,7 @@ extern "C" {
 #endif
 
 typedef struct nx_state nx_state;
+typedef struct nx_batch nx_batch;
 
 enum {
   NX_OK = 0,
@@ -133,15 +134,38 @@ int nx_state_exp_values(const nx_state *s, int32_t npauli, const int32_t *ops, d
  * decomposition coefficients: grad = dL/d(conj c) * 2 = 2 |g|^2 sum_k w_k G_k c,
  * (G_k)_{ji} = <phi_j|P_k phi_i>.  paulis: npauli dense strings ops[npauli][n].
  * grad: complex128 [chi] (the PyTorch convention for a real loss of complex
- * leaves); *loss receives L.  Up to 16384 terms.  No reference counterpart (an addition for
+ * leaves); *loss receives L.  Up to 32768 terms.  No reference counterpart (an addition for
  * differentiation; the forward is exp_value.rs:7-61). */
 int nx_state_grad_exp_values(const nx_state *s, int32_t npauli, const int32_t *ops, const double *weights,
                              double *loss, double *grad);
 
+/* Batches (many circuits of uneven n and chi in one launch plan).  State b
+ * (copied at creation; later changes to it are not seen) gets its norm
+ * (norm.rs:32-35) and the exp_value (exp_value.rs:7-61) of npauli[b] dense
+ * Pauli strings; ops concatenates the strings of all states (npauli[b]
+ * strings of length n_b each).  nx_batch_run writes, for state b in order,
+ * norm_b then exp_value_b,1 .. exp_value_b,npauli[b].  All pairs of all
+ * states and all strings of a state run in one persistent pair launch per
+ * row-word width.  flags: NX_BATCH_AUTOTUNE times a grid of launch shapes
+ * (pair-chunk length, wave count) once and keeps the fastest (cached per
+ * shape class); NX_BATCH_GRAPH captures the launches into a HIP graph that
+ * every nx_batch_run replays.  The reference has no batch API: this is the
+ * batched form of calling QuantumState::norm and exp_value per circuit. */
+enum { NX_BATCH_AUTOTUNE = 1, NX_BATCH_GRAPH = 2 };
+int nx_batch_create(int32_t nstates, const nx_state *const *states, const int32_t *npauli, const int32_t *ops,
+                    int32_t flags, nx_batch **out);
+int nx_batch_num_outputs(const nx_batch *b, int64_t *out);
+/* pair-chunk length, total waves and variant words (1: up to 64 strings per
+ * job, 2: up to 128) of the batch's launch plan */
+int nx_batch_shape(const nx_batch *b, int32_t *J, int64_t *waves, int32_t *variant_words);
+int nx_batch_run(nx_batch *b, double *out);
+void nx_batch_free(nx_batch *b);
+
 /* Kernel timing on the library's own stream (HIP events around each
  * amplitude / pair / qform / program launch; callers' events on other
  * streams cannot see these launches).  kernel: "k_amplitude", "k_pairs",
- * "k_qform", "k_program".  No reference counterpart (measurement aid). */
+ * "k_pairs_var", "k_gramc", "k_qform", "k_program", "batch_pairs" (a whole
+ * nx_batch_run launch plan).  No reference counterpart (measurement aid). */
 int nx_profile_enable(int on);
 int nx_profile_get(const char *kernel, double *total_ms, int64_t *count);
 

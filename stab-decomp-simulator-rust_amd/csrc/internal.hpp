@@ -62,6 +62,42 @@ struct CoefDev {
   const double2* c;
 };
 
+// Ket-side variants of a term set, bit-sliced: variant k of term t is the
+// term's quadratic form with
+//   a_r  -> a_r + 2 * bit k of hi[(t*R + r)*VW + k/64]      (row r)
+//   rhs_c -> rhs_c ^ bit k of rhs[(t*R + c)*VW + k/64]       (constraint c)
+//   p0   -> p0 + sum_b 2^b * bit k of ph[(t*3 + b)*VW + k/64] (mod 8)
+// and the same B, constraint rows and magnitude.  A Pauli image P phi_t is
+// such a variant: a Pauli left-multiplication changes only s, gamma and the
+// phase of the CH-form, so G, F, M and v (hence B and the constraint rows)
+// stay those of phi_t.
+struct VarDev {
+  const uint64_t* hi;
+  const uint64_t* rhs;
+  const uint64_t* ph;
+};
+
+// One pair job: the Hermitian sum over kets f in [0, nfixed) of Y and bras
+// o in [f, nother) (triangle) or [0, nother) of X, for every variant of
+// the ket; the job owns waves [wave0, wave0 + nwaves) of the launch.
+// row_off[f] is the first unit (J consecutive bras) of ket row f.  With gw
+// set, the weighted pair values sum_k wts[k] <X_o | Y_f variant k> are
+// written (accumulate = 0) or added (accumulate = 1) at gw[o * gw_ld + f].
+struct PairJobDev {
+  QFormDev X, Y;
+  VarDev V;
+  CoefDev cx, cy;
+  const int64_t* row_off;
+  int64_t nfixed, nother;
+  int32_t triangle, nvar;
+  int64_t wave0, nwaves;
+  int64_t out_base;
+  double2* gw;
+  int64_t gw_ld;
+  const double* wts;
+  int32_t accumulate;
+};
+
 // FP64 tables computed on the host with the same libm calls as the
 // reference (cos/sin of k*pi/4, powf(2, -r/2)).
 enum : int { TAB_COS = 0, TAB_SIN = 8, TAB_RMAX = 1024, TAB_POW = 16 + TAB_RMAX, TAB_SIZE = 16 + 2 * TAB_RMAX + 1 };
@@ -83,9 +119,20 @@ hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, const int64_t* off
                         const double* tabs, double2* partial, int nblocks, uint32_t* exact_out, int64_t exact_ld,
                         hipStream_t st);
 hipError_t launch_reduce(const double2* partial, int64_t n, double2* out, hipStream_t st);
-// y += w * G c for the Hermitian G whose lower triangle T[j*ld + i] (j >= i)
-// holds exact pair scalars
-hipError_t launch_gram(const uint32_t* T, int64_t ld, int64_t chi, const double2* c, double w, const double* tabs,
-                       double2* y, hipStream_t st);
+
+// Pauli variants of every term of `in`: variant k applies the ops
+// vops[voff[k] .. voff[k+1]) (q << 2 | op, op 1 = X, 2 = Y, 3 = Z) in order.
+hipError_t launch_pauli_var(const FormsDev& in, const int32_t* voff, const int32_t* vops, int nvar, int VW,
+                            uint64_t* vhi, uint64_t* vrhs, uint64_t* vph, hipStream_t st);
+// waves [wave_base, wave_base + nwaves) of a launch plan whose jobs all have
+// word width W; wave_job[w] = job of wave w; partial holds 64 * VW doubles
+// per wave of the plan (the real Hermitian sums per variant)
+hipError_t launch_pairs_var(int W, int VW, const PairJobDev* jobs, const int32_t* wave_job, int64_t wave_base,
+                            int64_t nwaves, int J, const double* tabs, double* partial, hipStream_t st);
+// out[jobs[j].out_base + k] = sum over the job's waves of partial (k < nvar)
+hipError_t launch_reduce_var(const PairJobDev* jobs, int njobs, int VW, const double* partial, double* out,
+                             hipStream_t st);
+// y = G c for the Hermitian G whose lower triangle is Gw[j*ld + i] (j >= i)
+hipError_t launch_gramc(const double2* Gw, int64_t ld, int64_t chi, const double2* c, double2* y, hipStream_t st);
 
 }  // namespace nx

@@ -18,61 +18,9 @@
 
 #include "device_form.hpp"
 #include "internal.hpp"
+#include "pair_core.hpp"
 
 namespace nx {
-
-__device__ __forceinline__ size_t rowidx(int64_t t, int R, int r, int W, int w) {
-  return ((size_t)t * R + r) * W + w;
-}
-
-template <int W>
-__device__ __forceinline__ void load_form(Form<W>& f, const FormsDev& in, int64_t t, int lane) {
-  const int R = 64 * in.W;
-  f.lane = lane;
-  f.n = in.n;
-#pragma unroll
-  for (int sl = 0; sl < W; ++sl) {
-    const int r = 64 * sl + lane;
-    const bool okr = sl < in.W;
-#pragma unroll
-    for (int w = 0; w < W; ++w) {
-      const bool ok = okr && w < in.W;
-      f.G[sl][w] = ok ? in.G[rowidx(t, R, r, in.W, w)] : 0ull;
-      f.F[sl][w] = ok ? in.F[rowidx(t, R, r, in.W, w)] : 0ull;
-      f.M[sl][w] = ok ? in.M[rowidx(t, R, r, in.W, w)] : 0ull;
-    }
-    f.g[sl] = okr ? (int)in.g[(size_t)t * R + r] : 0;
-  }
-#pragma unroll
-  for (int w = 0; w < W; ++w) {
-    f.v[w] = w < in.W ? in.v[(size_t)t * in.W + w] : 0ull;
-    f.s[w] = w < in.W ? in.s[(size_t)t * in.W + w] : 0ull;
-  }
-  f.ph = in.ph[t];
-}
-
-template <int W>
-__device__ __forceinline__ void store_form(const Form<W>& f, const FormsDev& out, int64_t t) {
-  const int R = 64 * out.W;
-#pragma unroll
-  for (int sl = 0; sl < W; ++sl) {
-    if (sl >= out.W) continue;
-    const int r = 64 * sl + f.lane;
-#pragma unroll
-    for (int w = 0; w < W; ++w) {
-      if (w >= out.W) continue;
-      out.G[rowidx(t, R, r, out.W, w)] = f.G[sl][w];
-      out.F[rowidx(t, R, r, out.W, w)] = f.F[sl][w];
-      out.M[rowidx(t, R, r, out.W, w)] = f.M[sl][w];
-    }
-    out.g[(size_t)t * R + r] = (uint8_t)f.g[sl];
-  }
-  if (f.lane < out.W) {
-    out.v[(size_t)t * out.W + f.lane] = wsel<W>(f.v, f.lane);
-    out.s[(size_t)t * out.W + f.lane] = wsel<W>(f.s, f.lane);
-  }
-  if (f.lane == 0) out.ph[t] = (uint8_t)f.ph;
-}
 
 // ---------------------------------------------------------------------------
 template <int W>
@@ -462,173 +410,13 @@ __global__ void k_amp_reduce(const double2* __restrict__ part, int64_t nb, int n
   out[b] = make_double2(r * gre - i * gim, r * gim + i * gre);
 }
 
-// ---------------------------------------------------------------------------
-// Pair kernel.
-template <int W>
-struct PairRegs {
-  u64 B[W][W];
-  int a[W];
-  u64 ca[W][W], cb[W][W];
-  u64 rhsa[W], rhsb[W];
-  u64 rem[W];
-  int c0, mag;
-  int lane;
-};
-
-template <int W>
-__device__ __forceinline__ void clearbit(u64 (&x)[W], int pw, int pb) {
-#pragma unroll
-  for (int w = 0; w < W; ++w)
-    if (w == pw) x[w] &= ~(1ull << pb);
-}
-
-template <int W>
-__device__ __forceinline__ int bcast_int(const int (&x)[W], int r) {
-  const int sl = r >> 6;
-  int val = x[0];
-#pragma unroll
-  for (int k = 1; k < W; ++k)
-    if (sl == k) val = x[k];
-  return rl32(val, r & 63);
-}
-
-// substitute x_p = c ^ <r, x> (r excludes p) into the form; with CONS also
-// into the remaining constraint rows.  Rows of eliminated variables and
-// padding rows may hold junk: they are never broadcast again, and every
-// column of an eliminated variable is cleared in all rows.
-template <int W, bool CONS>
-__device__ __forceinline__ void subst(PairRegs<W>& P, int p, const u64 (&r)[W], int c) {
-  const int ap = bcast_int<W>(P.a, p);
-  u64 Bp[W];
-  bcast<W>(P.B, p, Bp);
-  P.c0 = (P.c0 + 2 * ap * c) & 7;
-  // a_l += ap (1 - 2c) r_l + 2 (c + r_l) B_pl ;  B_l ^= r_l ((ap odd) r ^ Bp) ^ B_pl r
-  const int lin = c ? (4 - ap) & 3 : ap;
-  const int pw = p >> 6, pb = p & 63;
-#pragma unroll
-  for (int sl = 0; sl < W; ++sl) {
-    const int rl = (int)((r[sl] >> P.lane) & 1ull);
-    const int bpl = (int)((wsel<W>(P.B[sl], pw) >> pb) & 1ull);
-    P.a[sl] = (P.a[sl] + (rl ? lin : 0) + ((c ^ rl) & bpl ? 2 : 0)) & 3;
-    const u64 mr = 0ull - (u64)rl, mb = 0ull - (u64)bpl;
-#pragma unroll
-    for (int w = 0; w < W; ++w) {
-      const u64 t = ((ap & 1) ? r[w] : 0ull) ^ Bp[w];
-      P.B[sl][w] ^= (t & mr) ^ (r[w] & mb);
-    }
-    // clear the diagonal bit of this lane's row and column p
-#pragma unroll
-    for (int w = 0; w < W; ++w)
-      if (w == sl) P.B[sl][w] &= ~(1ull << P.lane);
-    clearbit<W>(P.B[sl], pw, pb);
-    if (CONS) {
-      const int ha = (int)((wsel<W>(P.ca[sl], pw) >> pb) & 1ull);
-      const int hb = (int)((wsel<W>(P.cb[sl], pw) >> pb) & 1ull);
-      const u64 ma = 0ull - (u64)ha, mb2 = 0ull - (u64)hb;
-#pragma unroll
-      for (int w = 0; w < W; ++w) {
-        P.ca[sl][w] ^= r[w] & ma;
-        P.cb[sl][w] ^= r[w] & mb2;
-      }
-      clearbit<W>(P.ca[sl], pw, pb);
-      clearbit<W>(P.cb[sl], pw, pb);
-      if (c) {
-        P.rhsa[sl] ^= __ballot(ha);
-        P.rhsb[sl] ^= __ballot(hb);
-      }
-    }
-  }
-  clearbit<W>(P.rem, pw, pb);
-}
-
-// returns 1 if a constraint is inconsistent (the sum is zero)
-template <int W>
-__device__ __forceinline__ int take_constraint(PairRegs<W>& P, int useb, int c) {
-  u64 row[W];
-  bcast<W>(useb ? P.cb : P.ca, c, row);
-  const int rhs = useb ? getbit<W>(P.rhsb, c) : getbit<W>(P.rhsa, c);
-  if (!anyv<W>(row)) return rhs;
-  const int p = ffsv<W>(row);
-  flipbit<W>(row, p, 1);
-  subst<W, true>(P, p, row, rhs);
-  return 0;
-}
-
-// exact <bra|ket>: returns zero flag, K (mod 8), r
-template <int W>
-__device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int* zero, int* K, int* R) {
-  for (int c = 0; c < nca; ++c)
-    if (take_constraint<W>(P, 0, c)) { *zero = 1; return; }
-  for (int c = 0; c < ncb; ++c)
-    if (take_constraint<W>(P, 1, c)) { *zero = 1; return; }
-  while (anyv<W>(P.rem)) {
-    const int p = uni(ffsv<W>(P.rem));
-    const int pw = p >> 6, pb = p & 63;
-    clearbit<W>(P.rem, pw, pb);
-    u64 Bp[W];
-    bcast<W>(P.B, p, Bp);
-    const int ap = bcast_int<W>(P.a, p);
-    if (!anyv<W>(Bp)) {
-      if (ap == 2) { *zero = 1; return; }
-      if (ap == 0) P.mag -= 2;
-      else { P.mag -= 1; P.c0 = (P.c0 + (ap == 1 ? 1 : 7)) & 7; }
-    } else if (ap & 1) {
-      // sum over x_p: sqrt2 zeta^s i^{-s lambda}, lambda = <B_p, x>, s = +-1
-      const int add = (ap == 1) ? 3 : 1;
-      P.mag -= 1;
-      P.c0 = (P.c0 + (ap == 1 ? 1 : 7)) & 7;
-#pragma unroll
-      for (int sl = 0; sl < W; ++sl) {
-        const int bpl = (int)((wsel<W>(P.B[sl], pw) >> pb) & 1ull);
-        const u64 mb = 0ull - (u64)bpl;
-        P.a[sl] = (P.a[sl] + (bpl ? add : 0)) & 3;
-#pragma unroll
-        for (int w = 0; w < W; ++w) P.B[sl][w] ^= Bp[w] & mb;
-#pragma unroll
-        for (int w = 0; w < W; ++w)
-          if (w == sl) P.B[sl][w] &= ~(1ull << P.lane);
-        clearbit<W>(P.B[sl], pw, pb);
-      }
-    } else {
-      // sum over x_p: 2 [lambda = a_p / 2]; then substitute for x_q
-      P.mag -= 2;
-#pragma unroll
-      for (int sl = 0; sl < W; ++sl) clearbit<W>(P.B[sl], pw, pb);
-      const int qv = ffsv<W>(Bp);
-      flipbit<W>(Bp, qv, 1);
-      subst<W, false>(P, qv, Bp, ap >> 1);
-    }
-  }
-  *zero = 0;
-  *K = P.c0;
-  *R = P.mag;
-}
-
-template <int W>
-__device__ __forceinline__ void load_qrows(const QFormDev& q, int64_t t, int lane, u64 (&B)[W][W], int (&a)[W],
-                                           u64 (&C)[W][W], u64 (&rhs)[W]) {
-  const int R = 64 * W;
-#pragma unroll
-  for (int sl = 0; sl < W; ++sl) {
-    const int r = 64 * sl + lane;
-#pragma unroll
-    for (int w = 0; w < W; ++w) {
-      B[sl][w] = q.B[rowidx(t, R, r, W, w)];
-      C[sl][w] = q.C[rowidx(t, R, r, W, w)];
-    }
-    a[sl] = q.a[(size_t)t * R + r];
-  }
-#pragma unroll
-  for (int w = 0; w < W; ++w) rhs[w] = q.rhs[(size_t)t * W + w];
-}
-
+// Coefficients: exact Scalars (ck, cr, cz) when scalar != 0, else c.
 // Work decomposition: fixed index f in [0, nfixed) pairs with the other
 // indices o in [lo_f, nother), lo_f = f (triangle) or 0 (full), cut into
 // units of J; offsets[f] is the first unit of row f (prefix sum on the
 // host).  mode bit 0: fixed index is the ket (else the bra); mode bit 1:
 // Hermitian real sum (diagonal once, off-diagonal twice, norm.rs:16-27 and
 // exp_value.rs:38-58) else the complex sum (inner_product.rs:12-17).
-// Coefficients: exact Scalars (ck, cr, cz) when scalar != 0, else c.
 template <int W>
 __global__ __launch_bounds__(256) void k_pairs(QFormDev X, QFormDev Y, const int64_t* __restrict__ offsets,
                                                int64_t nfixed, int64_t nother, int triangle, int J, int mode,
@@ -740,72 +528,6 @@ __global__ __launch_bounds__(256) void k_pairs(QFormDev X, QFormDev Y, const int
   if (lane == 0) partial[wave] = make_double2(acc_re, acc_im);
 }
 
-// ---------------------------------------------------------------------------
-// Conjugate Gram matvec from exact pair scalars.  T[j * ld + i] holds
-// <phi_j | P phi_i> for j >= i (k | r << 8, 0xFFFFFFFF = zero); G is
-// Hermitian, G_ij = conj(G_ji).  y += w * G c, in two ordered passes.
-__device__ __forceinline__ double2 pair_value(uint32_t e, const double* tabs) {
-  if (e == 0xFFFFFFFFu) return make_double2(0.0, 0.0);
-  const int k = (int)(e & 7u), r = (int)(e >> 8);
-  const double nrm = tabs[TAB_POW + r];
-  return make_double2(tabs[TAB_COS + k] * nrm, tabs[TAB_SIN + k] * nrm);
-}
-
-// y_j += w * sum_{i <= j} G_ji c_i : one block per row j, coalesced along i
-__global__ __launch_bounds__(256) void k_gram_rows(const uint32_t* __restrict__ T, int64_t ld, int64_t chi,
-                                                   const double2* __restrict__ c, double w,
-                                                   const double* __restrict__ tabs, double2* __restrict__ y) {
-  __shared__ double sr[256], si[256];
-  const int64_t j = blockIdx.x;
-  double ar = 0.0, ai = 0.0;
-  for (int64_t i = threadIdx.x; i <= j; i += blockDim.x) {
-    const double2 g = pair_value(T[(size_t)j * ld + i], tabs);
-    const double2 ci = c[i];
-    ar = ar + (g.x * ci.x - g.y * ci.y);
-    ai = ai + (g.x * ci.y + g.y * ci.x);
-  }
-  sr[threadIdx.x] = ar;
-  si[threadIdx.x] = ai;
-  __syncthreads();
-  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) {
-      sr[threadIdx.x] += sr[threadIdx.x + st];
-      si[threadIdx.x] += si[threadIdx.x + st];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    double2 v = y[j];
-    v.x = v.x + w * sr[0];
-    v.y = v.y + w * si[0];
-    y[j] = v;
-  }
-}
-
-// y_i += w * sum_{j > i} conj(G_ji) c_j : one thread per column i, rows in
-// order (threads of a wave read consecutive entries of row j)
-__global__ __launch_bounds__(256) void k_gram_cols(const uint32_t* __restrict__ T, int64_t ld, int64_t chi,
-                                                   const double2* __restrict__ c, double w,
-                                                   const double* __restrict__ tabs, double2* __restrict__ y) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t jstart = (int64_t)blockIdx.x * blockDim.x + 1;  // smallest row any thread of the block needs
-  double ar = 0.0, ai = 0.0;
-  for (int64_t j = jstart; j < chi; ++j) {
-    if (i < chi && j > i) {
-      const double2 g = pair_value(T[(size_t)j * ld + i], tabs);
-      const double2 cj = c[j];
-      ar = ar + (g.x * cj.x + g.y * cj.y);  // conj(g) * c_j
-      ai = ai + (g.x * cj.y - g.y * cj.x);
-    }
-  }
-  if (i < chi) {
-    double2 v = y[i];
-    v.x = v.x + w * ar;
-    v.y = v.y + w * ai;
-    y[i] = v;
-  }
-}
-
 // single-block ordered reduction of the per-wave partial sums
 __global__ void k_reduce(const double2* __restrict__ partial, int64_t n, double2* __restrict__ out) {
   __shared__ double sr[256], si[256];
@@ -901,16 +623,6 @@ hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, const int64_t* off
   NX_DISPATCH_W(X.W, k_pairs<WW><<<dim3(nblocks), dim3(256), 0, st>>>(X, Y, offsets, nfixed, nother, triangle, J,
                                                                       mode, cx, cy, tabs, partial, exact_out,
                                                                       exact_ld));
-  return hipGetLastError();
-}
-
-hipError_t launch_gram(const uint32_t* T, int64_t ld, int64_t chi, const double2* c, double w, const double* tabs,
-                       double2* y, hipStream_t st) {
-  if (chi == 0) return hipSuccess;
-  k_gram_rows<<<dim3((unsigned)chi), dim3(256), 0, st>>>(T, ld, chi, c, w, tabs, y);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  k_gram_cols<<<dim3(blocks_for(chi, 256)), dim3(256), 0, st>>>(T, ld, chi, c, w, tabs, y);
   return hipGetLastError();
 }
 

@@ -13,11 +13,13 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "internal.hpp"
@@ -890,7 +892,316 @@ static double exp_value(const nx_state* s, const std::vector<ProgOp>& prog) {
   return sum.real() * (s->g.real() * s->g.real() + s->g.imag() * s->g.imag());
 }
 
+// ---------------------------------------------------------------------------
+// Pauli variants and variant pair plans.  One k_pairs_var launch per word
+// width evaluates, for many jobs at once (a job = one term set and up to
+// 64*VW Pauli strings), every pair (bra o >= ket f) with all of the job's
+// Paulis applied to the ket; the identity string gives the norm.
+
+// sparse Pauli op lists, q << 2 | op (1 X, 2 Y, 3 Z), in application order
+struct PauliSet {
+  std::vector<int32_t> off{0}, ops;
+  int size() const { return (int)off.size() - 1; }
+  void add(const std::vector<ProgOp>& prog) {
+    for (const ProgOp& o : prog) ops.push_back(o.a << 2 | (o.code == OP_X ? 1 : (o.code == OP_Y ? 2 : 3)));
+    off.push_back((int32_t)ops.size());
+  }
+};
+
+struct VarSet {
+  DevBuf hi, rhs, ph;
+  int nvar = 0, VW = 1;
+  VarDev desc() const { return VarDev{hi.as<uint64_t>(), rhs.as<uint64_t>(), ph.as<uint64_t>()}; }
+};
+
+// variants first .. first + count - 1 of ps, for every term of f
+static VarSet pauli_vars(const Forms& f, const PauliSet& ps, int first, int count, int VW) {
+  if (count < 1 || count > 64 * VW) fail(NX_E_INTERNAL, "bad variant count %d", count);
+  VarSet vs;
+  vs.nvar = count;
+  vs.VW = VW;
+  const size_t R = 64 * (size_t)f.W, chi = (size_t)f.chi;
+  vs.hi.alloc(chi * R * VW * 8);
+  vs.rhs.alloc(chi * R * VW * 8);
+  vs.ph.alloc(chi * 3 * VW * 8);
+  std::vector<int32_t> off(count + 1), ops;
+  for (int k = 0; k < count; ++k) {
+    off[k] = (int32_t)ops.size();
+    ops.insert(ops.end(), ps.ops.begin() + ps.off[first + k], ps.ops.begin() + ps.off[first + k + 1]);
+  }
+  off[count] = (int32_t)ops.size();
+  if (ops.empty()) ops.push_back(0);
+  DevBuf doff(off.size() * 4), dops(ops.size() * 4);
+  h2d(doff.p, off.data(), off.size() * 4);
+  h2d(dops.p, ops.data(), ops.size() * 4);
+  hip_check(launch_pauli_var(f.desc(), doff.as<int32_t>(), dops.as<int32_t>(), count, VW, vs.hi.as<uint64_t>(),
+                             vs.rhs.as<uint64_t>(), vs.ph.as<uint64_t>(), stream()),
+            "k_pauli_var");
+  return vs;
+}
+
+struct VarJob {
+  const QForms* X = nullptr;
+  const QForms* Y = nullptr;
+  const VarSet* V = nullptr;
+  CoefDev cx{}, cy{};
+  int64_t out_base = 0;
+  double2* gw = nullptr;
+  int64_t gw_ld = 0;
+  const double* wts = nullptr;
+  int accumulate = 0;
+};
+
+struct VarPlan {
+  int VW = 1, J = 16;
+  int64_t total_waves = 0, nout = 0;
+  int njobs = 0;
+  struct Launch {
+    int W;
+    int64_t wave_base, nwaves;
+  };
+  std::vector<Launch> launches;
+  DevBuf jobs, wave_job, row_off, partial, out;
+  void enqueue(hipStream_t st) const {
+    for (const Launch& L : launches)
+      hip_check(launch_pairs_var(L.W, VW, jobs.as<PairJobDev>(), wave_job.as<int32_t>(), L.wave_base, L.nwaves, J,
+                                 ctx()->tabs, partial.as<double>(), st),
+                "k_pairs_var");
+    hip_check(launch_reduce_var(jobs.as<PairJobDev>(), njobs, VW, partial.as<double>(), out.as<double>(), st),
+              "k_reduce_var");
+  }
+};
+
+static int64_t units_of(int64_t chi, int J) {
+  int64_t u = 0;
+  for (int64_t f = 0; f < chi; ++f) u += (chi - f + J - 1) / J;
+  return u;
+}
+
+// Builds the device job table.  Jobs are grouped by word width (one launch
+// each); within a launch the waves are shared out in proportion to each
+// job's work (units x n^2), at least one wave per job and at most one per
+// unit.
+static VarPlan make_plan(const std::vector<VarJob>& in, int VW, int J, int64_t target_waves) {
+  VarPlan pl;
+  pl.VW = VW;
+  pl.J = J;
+  std::vector<int> order(in.size());
+  for (size_t i = 0; i < in.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return in[a].X->W < in[b].X->W; });
+  std::vector<int64_t> units(in.size());
+  double work_total = 0.0;
+  int64_t rows_total = 0;
+  for (size_t i = 0; i < in.size(); ++i) {
+    units[i] = units_of(in[i].X->chi, J);
+    const double n = (double)in[i].X->n;
+    work_total += (double)units[i] * n * n;
+    rows_total += in[i].X->chi + 1;
+  }
+  std::vector<int64_t> roff;
+  roff.reserve(rows_total);
+  std::vector<int64_t> roff_base(in.size());
+  std::vector<PairJobDev> hj(in.size());
+  std::vector<int32_t> wj;
+  int64_t wave = 0;
+  int curW = -1;
+  for (int idx : order) {
+    const VarJob& j = in[idx];
+    const int64_t chi = j.X->chi;
+    if (j.X->W != curW) {
+      curW = j.X->W;
+      pl.launches.push_back({curW, wave, 0});
+    }
+    roff_base[idx] = (int64_t)roff.size();
+    int64_t acc = 0;
+    for (int64_t f = 0; f < chi; ++f) {
+      roff.push_back(acc);
+      acc += (chi - f + J - 1) / J;
+    }
+    roff.push_back(acc);
+    const double n = (double)j.X->n;
+    int64_t nw = work_total > 0 ? (int64_t)std::llround((double)units[idx] * n * n / work_total * target_waves) : 1;
+    nw = std::max<int64_t>(1, std::min<int64_t>(nw, std::max<int64_t>(units[idx], 1)));
+    PairJobDev& d = hj[idx];
+    d.X = j.X->desc();
+    d.Y = j.Y->desc();
+    d.V = j.V->desc();
+    d.cx = j.cx;
+    d.cy = j.cy;
+    d.row_off = nullptr;  // set below
+    d.nfixed = chi;
+    d.nother = chi;
+    d.triangle = 1;
+    d.nvar = j.V->nvar;
+    d.wave0 = wave;
+    d.nwaves = nw;
+    d.out_base = j.out_base;
+    d.gw = j.gw;
+    d.gw_ld = j.gw_ld;
+    d.wts = j.wts;
+    d.accumulate = j.accumulate;
+    for (int64_t w = 0; w < nw; ++w) wj.push_back(idx);
+    wave += nw;
+    pl.launches.back().nwaves += nw;
+    pl.nout = std::max<int64_t>(pl.nout, j.out_base + j.V->nvar);
+  }
+  pl.total_waves = wave;
+  pl.njobs = (int)in.size();
+  pl.row_off.alloc(std::max<size_t>(roff.size(), 1) * 8);
+  h2d(pl.row_off.p, roff.data(), roff.size() * 8);
+  for (size_t i = 0; i < in.size(); ++i) hj[i].row_off = pl.row_off.as<int64_t>() + roff_base[i];
+  pl.jobs.alloc(std::max<size_t>(hj.size(), 1) * sizeof(PairJobDev));
+  h2d(pl.jobs.p, hj.data(), hj.size() * sizeof(PairJobDev));
+  pl.wave_job.alloc(std::max<size_t>(wj.size(), 1) * 4);
+  h2d(pl.wave_job.p, wj.data(), wj.size() * 4);
+  pl.partial.alloc(std::max<int64_t>(wave, 1) * 64 * VW * 8);
+  pl.out.alloc(std::max<int64_t>(pl.nout, 1) * 8);
+  return pl;
+}
+
+// Launch-shape autotuner: pair-chunk length J and total wave count, timed
+// with HIP events on the library stream; results are cached per (word
+// width, variant words, log2 of the pair count).
+struct TuneKey {
+  int W, VW, lg;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(W, VW, lg) < std::tie(o.W, o.VW, o.lg);
+  }
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, std::pair<int, int64_t>> g_tune;
+
+static const int64_t DEFAULT_WAVES = 16384;
+
+static TuneKey tune_key(const std::vector<VarJob>& jobs, int VW) {
+  int W = 0;
+  double pairs = 0.0;
+  for (const VarJob& j : jobs) {
+    W = std::max(W, j.X->W);
+    pairs += 0.5 * (double)j.X->chi * (double)(j.X->chi + 1);
+  }
+  return TuneKey{W, VW, (int)std::floor(std::log2(std::max(pairs, 1.0)))};
+}
+
+static std::pair<int, int64_t> tuned_shape(const std::vector<VarJob>& jobs, int VW) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(tune_key(jobs, VW));
+  if (it != g_tune.end()) return it->second;
+  return {16, DEFAULT_WAVES};
+}
+
+static std::pair<int, int64_t> autotune(const std::vector<VarJob>& jobs, int VW) {
+  static const int Js[] = {4, 8, 16, 32};
+  static const int64_t Ws[] = {4096, 8192, 16384, 32768};
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "hipEventCreate");
+  hip_check(hipEventCreate(&e1), "hipEventCreate");
+  float best = 1e30f;
+  std::pair<int, int64_t> arg{16, DEFAULT_WAVES};
+  for (int J : Js)
+    for (int64_t nw : Ws) {
+      VarPlan pl = make_plan(jobs, VW, J, nw);
+      pl.enqueue(stream());  // warm-up
+      hip_check(hipEventRecord(e0, stream()), "hipEventRecord");
+      pl.enqueue(stream());
+      hip_check(hipEventRecord(e1, stream()), "hipEventRecord");
+      hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+      float ms = 0.f;
+      hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+      if (ms < best) {
+        best = ms;
+        arg = {J, nw};
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune[tune_key(jobs, VW)] = arg;
+  return arg;
+}
+
+// Dense Pauli strings -> PauliSet, with the length checks of exp_value.rs:9-27
+static void add_dense_paulis(const nx_state* s, int32_t npauli, const int32_t* ops, PauliSet& ps) {
+  for (int32_t k = 0; k < npauli; ++k) ps.add(pauli_prog(s, 1, s->n, ops + (size_t)k * s->n, nullptr));
+}
+
+// Unnormalised <P_k> for every string of ps (exp_value.rs:7-61), all
+// strings per pair in one pass; with gw, also the weighted pair matrix.
+static std::vector<double> exp_values_var(const nx_state* s, const PauliSet& ps, double2* gw = nullptr,
+                                          const double* d_wts = nullptr) {
+  const int np = ps.size();
+  std::vector<double> res(np, 0.0);
+  if (s->chi() == 0 || np == 0) return res;
+  const int VW = np <= 64 ? 1 : 2;
+  const int per = 64 * VW;
+  const QForms& q = qform_of(s);
+  CoefDev c = coef_of(s);
+  std::vector<VarSet> vs;
+  vs.reserve((np + per - 1) / per);
+  for (int first = 0; first < np; first += per) vs.push_back(pauli_vars(s->forms, ps, first, std::min(per, np - first), VW));
+  std::vector<double> outh;
+  // chunks that write gw run one after another (each adds to the matrix);
+  // without gw all chunks share one launch
+  const size_t nchunks = vs.size();
+  const size_t groups = gw ? nchunks : 1;
+  for (size_t gi = 0; gi < groups; ++gi) {
+    std::vector<VarJob> jobs;
+    for (size_t ci = (gw ? gi : 0); ci < (gw ? gi + 1 : nchunks); ++ci) {
+      VarJob j;
+      j.X = &q;
+      j.Y = &q;
+      j.V = &vs[ci];
+      j.cx = c;
+      j.cy = c;
+      j.out_base = (int64_t)ci * per;
+      if (gw) {
+        j.gw = gw;
+        j.gw_ld = s->chi();
+        j.wts = d_wts + ci * per;
+        j.accumulate = ci > 0;
+      }
+      jobs.push_back(j);
+    }
+    const auto shape = tuned_shape(jobs, VW);
+    VarPlan pl = make_plan(jobs, VW, shape.first, shape.second);
+    {
+      ProfScope ps_("k_pairs_var");
+      pl.enqueue(stream());
+      ps_.end();
+    }
+    std::vector<double> o(pl.nout);
+    d2h(o.data(), pl.out.p, pl.nout * 8);
+    sync();
+    for (size_t ci = (gw ? gi : 0); ci < (gw ? gi + 1 : nchunks); ++ci)
+      for (int k = 0; k < vs[ci].nvar; ++k) res[ci * per + k] = o[ci * per + k];
+  }
+  const double g2 = s->g.real() * s->g.real() + s->g.imag() * s->g.imag();
+  for (double& r : res) r *= g2;
+  return res;
+}
+
 }  // namespace nx
+
+// ---------------------------------------------------------------------------
+// a batch of states with their Pauli lists, evaluated by one launch plan
+struct nx_batch {
+  std::vector<nx_state*> states;  // private clones
+  std::vector<nx::VarSet> vars;
+  std::vector<int64_t> state_out;
+  std::vector<int32_t> state_np;
+  nx::VarPlan plan;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int64_t nout = 0, waves = 0;
+  int J = 16, VW = 1;
+  ~nx_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    plan = nx::VarPlan();
+    vars.clear();
+    for (nx_state* s : states) delete s;
+  }
+};
 
 // ---------------------------------------------------------------------------
 // C-ABI
@@ -1211,8 +1522,10 @@ int nx_state_pair_table(const nx_state* s, uint32_t* out) {
 
 int nx_state_exp_values(const nx_state* s, int32_t npauli, const int32_t* ops, double* out) {
   NX_TRY({
-    for (int32_t k = 0; k < npauli; ++k)
-      out[k] = exp_value(s, pauli_prog(s, 1, s->n, ops + (size_t)k * s->n, nullptr));
+    PauliSet ps;
+    add_dense_paulis(s, npauli, ops, ps);
+    const std::vector<double> r = exp_values_var(s, ps);
+    for (int32_t k = 0; k < npauli; ++k) out[k] = r[k];
   })
 }
 
@@ -1220,42 +1533,147 @@ int nx_state_grad_exp_values(const nx_state* s, int32_t npauli, const int32_t* o
                              double* loss, double* grad) {
   NX_TRY({
     const int64_t chi = s->chi();
-    std::vector<std::vector<ProgOp>> progs;
-    for (int32_t k = 0; k < npauli; ++k) progs.push_back(pauli_prog(s, 1, s->n, ops + (size_t)k * s->n, nullptr));
-    if (chi == 0) {
-      *loss = 0.0;
-      return NX_OK;
-    }
-    if (chi > 16384) fail(NX_E_VALUE, "the dense Gram backward supports up to 16384 terms (got %lld)", (long long)chi);
-    const QForms& q = qform_of(s);
+    PauliSet ps;
+    add_dense_paulis(s, npauli, ops, ps);
+    for (int64_t j = 0; j < 2 * chi; ++j) grad[j] = 0.0;
+    *loss = 0.0;
+    if (chi == 0 || npauli == 0) return NX_OK;
+    if (chi > 32768) fail(NX_E_VALUE, "the dense Gram backward supports up to 32768 terms (got %lld)", (long long)chi);
+    DevBuf G((size_t)chi * chi * 16), y((size_t)chi * 16), dw((size_t)npauli * 8);
+    h2d(dw.p, weights, (size_t)npauli * 8);
+    // G_ji = sum_k w_k <phi_j | P_k phi_i> (j >= i), and the forward values
+    const std::vector<double> ev = exp_values_var(s, ps, G.as<double2>(), dw.as<double>());
     CoefDev c = coef_of(s);
-    DevBuf T((size_t)chi * chi * 4), y((size_t)chi * 16);
-    hip_check(hipMemsetAsync(y.p, 0, (size_t)chi * 16, stream()), "hipMemsetAsync");
-    for (int32_t k = 0; k < npauli; ++k) {
-      Forms pf = clone_forms(s->forms);
-      if (!progs[k].empty()) pf = run_pairs_prep(pf, progs[k], s->n);
-      QForms qp = make_qform(pf);
-      // exact <phi_j | P_k phi_i> for j >= i into T[j * chi + i]
-      run_pairs(q, qp, chi, chi, true, 3, c, c, T.as<uint32_t>(), chi);
-      ProfScope ps("k_gram");
-      hip_check(launch_gram(T.as<uint32_t>(), chi, chi, c.c, weights[k], ctx()->tabs, y.as<double2>(), stream()),
-                "k_gram");
-      ps.end();
+    {
+      ProfScope pg("k_gramc");
+      hip_check(launch_gramc(G.as<double2>(), chi, chi, c.c, y.as<double2>(), stream()), "k_gramc");
+      pg.end();
     }
     std::vector<double> yh(2 * chi);
     d2h(yh.data(), y.p, (size_t)chi * 16);
     sync();
     const double g2 = s->g.real() * s->g.real() + s->g.imag() * s->g.imag();
+    for (int64_t j = 0; j < 2 * chi; ++j) grad[j] = 2.0 * g2 * yh[j];
     double acc = 0.0;
-    for (int64_t j = 0; j < chi; ++j) {
-      const cplx cj = s->coeff_value(j);
-      acc += cj.real() * yh[2 * j] + cj.imag() * yh[2 * j + 1];  // Re(conj(c_j) y_j)
-      grad[2 * j] = 2.0 * g2 * yh[2 * j];
-      grad[2 * j + 1] = 2.0 * g2 * yh[2 * j + 1];
-    }
-    *loss = acc * g2;
+    for (int32_t k = 0; k < npauli; ++k) acc += weights[k] * ev[k];
+    *loss = acc;
   })
 }
+
+// ---------------------------------------------------------------------------
+// batches
+
+int nx_batch_create(int32_t nstates, const nx_state* const* states, const int32_t* npauli, const int32_t* ops,
+                    int32_t flags, nx_batch** out) {
+  NX_TRY({
+    *out = nullptr;
+    if (nstates < 0) fail(NX_E_VALUE, "negative state count");
+    std::unique_ptr<nx_batch> b(new nx_batch());
+    // Pauli lists first: every length error is reported before any device work
+    std::vector<PauliSet> sets(nstates);
+    size_t pos = 0;
+    int maxvar = 1;
+    for (int32_t i = 0; i < nstates; ++i) {
+      if (npauli[i] < 0) fail(NX_E_VALUE, "negative Pauli count for state %d", i);
+      sets[i].add({});  // identity: the norm
+      add_dense_paulis(states[i], npauli[i], ops + pos, sets[i]);
+      pos += (size_t)npauli[i] * states[i]->n;
+      maxvar = std::max(maxvar, 1 + npauli[i]);
+    }
+    const int VW = maxvar <= 64 ? 1 : 2;
+    const int per = 64 * VW;
+    b->state_out.resize(nstates);
+    b->state_np.assign(npauli, npauli + nstates);
+    int64_t nout = 0;
+    for (int32_t i = 0; i < nstates; ++i) {
+      b->state_out[i] = nout;
+      nout += 1 + npauli[i];
+    }
+    b->nout = nout;
+    b->states.reserve(nstates);
+    for (int32_t i = 0; i < nstates; ++i) b->states.push_back(clone_state(states[i]));
+    std::vector<VarJob> jobs;
+    size_t njobs = 0;
+    for (int32_t i = 0; i < nstates; ++i)
+      if (b->states[i]->chi() > 0) njobs += (sets[i].size() + per - 1) / per;
+    b->vars.reserve(njobs);
+    for (int32_t i = 0; i < nstates; ++i) {
+      const nx_state* st = b->states[i];
+      if (st->chi() == 0) continue;
+      const QForms& q = qform_of(st);
+      CoefDev c = coef_of(st);
+      for (int first = 0; first < sets[i].size(); first += per) {
+        b->vars.push_back(pauli_vars(st->forms, sets[i], first, std::min(per, sets[i].size() - first), VW));
+        VarJob j;
+        j.X = &q;
+        j.Y = &q;
+        j.V = &b->vars.back();
+        j.cx = c;
+        j.cy = c;
+        j.out_base = b->state_out[i] + first;
+        jobs.push_back(j);
+      }
+    }
+    b->VW = VW;
+    if (!jobs.empty()) {
+      std::pair<int, int64_t> shape = (flags & NX_BATCH_AUTOTUNE) ? autotune(jobs, VW) : tuned_shape(jobs, VW);
+      b->J = shape.first;
+      b->waves = shape.second;
+      b->plan = make_plan(jobs, VW, b->J, b->waves);
+      if (b->plan.nout < nout) b->plan.out.alloc(nout * 8);
+      b->plan.nout = nout;
+      sync();
+      if (flags & NX_BATCH_GRAPH) {
+        hipStream_t st = stream();
+        hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        try {
+          b->plan.enqueue(st);
+        } catch (...) {
+          hipGraph_t g = nullptr;
+          (void)hipStreamEndCapture(st, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        hip_check(hipStreamEndCapture(st, &b->graph), "hipStreamEndCapture");
+        hip_check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+      }
+    }
+    *out = b.release();
+  })
+}
+
+int nx_batch_num_outputs(const nx_batch* b, int64_t* out) { NX_TRY(*out = b->nout) }
+
+int nx_batch_shape(const nx_batch* b, int32_t* J, int64_t* waves, int32_t* variant_words) {
+  NX_TRY(*J = b->J; *waves = b->plan.total_waves; *variant_words = b->VW)
+}
+
+int nx_batch_run(nx_batch* b, double* out) {
+  NX_TRY({
+    std::vector<double> o(b->nout, 0.0);
+    if (b->plan.njobs > 0) {
+      ProfScope ps("batch_pairs");
+      if (b->exec) hip_check(hipGraphLaunch(b->exec, stream()), "hipGraphLaunch");
+      else b->plan.enqueue(stream());
+      ps.end();
+      d2h(o.data(), b->plan.out.p, (size_t)b->nout * 8);
+      sync();
+    }
+    for (size_t i = 0; i < b->states.size(); ++i) {
+      const nx_state* st = b->states[i];
+      const double g2 = st->g.real() * st->g.real() + st->g.imag() * st->g.imag();
+      const int64_t base = b->state_out[i];
+      if (st->chi() == 0) {
+        for (int k = 0; k <= b->state_np[i]; ++k) out[base + k] = 0.0;
+        continue;
+      }
+      out[base] = std::sqrt(o[base] * g2);  // norm.rs:32-35
+      for (int k = 1; k <= b->state_np[i]; ++k) out[base + k] = o[base + k] * g2;
+    }
+  })
+}
+
+void nx_batch_free(nx_batch* b) { delete b; }
 
 int nx_profile_enable(int on) {
   NX_TRY({

@@ -1,0 +1,632 @@
+// variants.hip -- pair sums over many ket variants at once (gfx950, wave64).
+//
+//  k_pauli_var  one wavefront per term: the bit-sliced differences between
+//               the quadratic form of P_k phi_t and that of phi_t for up to
+//               64*VW Pauli strings P_k
+//  k_pairs_var  one wavefront per (bra, ket) pair, as k_pairs, but the
+//               elimination carries every ket variant along: B, the pivot
+//               order, the parity of each linear coefficient and the
+//               constraint rows are the same for all variants (a variant
+//               only adds even amounts to the linear coefficients and flips
+//               constraint right-hand sides), so the GF(2) work is done
+//               once and the variant-dependent quantities (high bit of each
+//               linear coefficient, right-hand sides, Z8 phase, zero flag)
+//               are u64 masks with one bit per variant.  Lane k then turns
+//               variant k's exact value into FP64 and accumulates it.
+//  k_reduce_var per job and variant, the ordered sum of the wave partials
+//  k_gramc_*    y = G c for a Hermitian FP64 pair matrix (weighted Paulis)
+#include <hip/hip_runtime.h>
+
+#include "device_form.hpp"
+#include "internal.hpp"
+#include "pair_core.hpp"
+
+namespace nx {
+
+// b += x per variant, b and x in Z8 as three bit planes
+__device__ __forceinline__ void z8_add(u64& b0, u64& b1, u64& b2, u64 x0, u64 x1, u64 x2) {
+  const u64 c0 = b0 & x0;
+  b0 ^= x0;
+  const u64 t = b1 ^ x1;
+  const u64 c1 = (b1 & x1) | (c0 & t);
+  b1 = t ^ c0;
+  b2 ^= x2 ^ c1;
+}
+
+template <int W, int VW>
+struct VarRegs {
+  u64 hi[W][VW];               // row r: variants whose a_r carries an extra 2
+  u64 ra[W][VW], rb[W][VW];    // constraint row c: variants whose rhs is flipped
+  u64 d0[VW], d1[VW], d2[VW];  // uniform: Z8 phase offset of each variant
+  u64 z[VW];                   // uniform: variants whose sum is already zero
+};
+
+template <int W, int VW>
+__device__ __forceinline__ void bcast_v(const u64 (&X)[W][VW], int r, u64 (&out)[VW]) {
+  const int sl = r >> 6, ln = r & 63;
+#pragma unroll
+  for (int v = 0; v < VW; ++v) {
+    u64 val = X[0][v];
+#pragma unroll
+    for (int k = 1; k < W; ++k)
+      if (sl == k) val = X[k][v];
+    out[v] = rl64(val, ln);
+  }
+}
+
+template <int W, int VW>
+__device__ __forceinline__ bool all_zero(const VarRegs<W, VW>& V) {
+  u64 a = ~0ull;
+#pragma unroll
+  for (int v = 0; v < VW; ++v) a &= V.z[v];
+  return a == ~0ull;
+}
+
+// subst<W, CONS> of pair_core.hpp for every variant: x_p = c_k ^ <r, x>
+// with c_k = c ^ bit k of E.  Base quantities (a, B, constraint rows, base
+// rhs) are updated as in subst with c; the variant masks pick up the
+// differences: a_l changes by r_l (c_k ? -a_p,k : a_p,k) + 2 ((c_k ^ r_l) &
+// B_pl), which differs from the base change by 2 (r_l (h_p,k ^ (a_p odd &
+// e_k)) ^ (B_pl & e_k)); the phase gains 2 (a_p,k c_k mod 4) per variant.
+template <int W, int VW, bool CONS>
+__device__ __forceinline__ void subst_v(PairRegs<W>& P, VarRegs<W, VW>& V, int p, const u64 (&r)[W], int c,
+                                        const u64 (&E)[VW]) {
+  const int ap = bcast_int<W>(P.a, p);
+  u64 H[VW];
+  bcast_v<W, VW>(V.hi, p, H);
+  u64 Bp[W];
+  bcast<W>(P.B, p, Bp);
+  u64 L[VW];
+#pragma unroll
+  for (int v = 0; v < VW; ++v) {
+    const u64 cm = c ? ~E[v] : E[v];                  // variants with c_k = 1
+    const u64 lo = (ap & 1) ? cm : 0ull;              // (a_p,k c_k) mod 2
+    const u64 hi = cm & ((ap & 2) ? ~H[v] : H[v]);    // high bit of a_p,k c_k
+    z8_add(V.d0[v], V.d1[v], V.d2[v], 0ull, lo, hi);
+    L[v] = H[v] ^ ((ap & 1) ? E[v] : 0ull);
+  }
+  const int lin = c ? (4 - ap) & 3 : ap;
+  const int pw = p >> 6, pb = p & 63;
+#pragma unroll
+  for (int sl = 0; sl < W; ++sl) {
+    const int rl = (int)((r[sl] >> P.lane) & 1ull);
+    const int bpl = (int)((wsel<W>(P.B[sl], pw) >> pb) & 1ull);
+    P.a[sl] = (P.a[sl] + (rl ? lin : 0) + ((c ^ rl) & bpl ? 2 : 0)) & 3;
+#pragma unroll
+    for (int v = 0; v < VW; ++v) V.hi[sl][v] ^= (rl ? L[v] : 0ull) ^ (bpl ? E[v] : 0ull);
+    const u64 mr = 0ull - (u64)rl, mb = 0ull - (u64)bpl;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const u64 t = ((ap & 1) ? r[w] : 0ull) ^ Bp[w];
+      P.B[sl][w] ^= (t & mr) ^ (r[w] & mb);
+    }
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+      if (w == sl) P.B[sl][w] &= ~(1ull << P.lane);
+    clearbit<W>(P.B[sl], pw, pb);
+    if (CONS) {
+      const int ha = (int)((wsel<W>(P.ca[sl], pw) >> pb) & 1ull);
+      const int hb = (int)((wsel<W>(P.cb[sl], pw) >> pb) & 1ull);
+      const u64 ma = 0ull - (u64)ha, mb2 = 0ull - (u64)hb;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        P.ca[sl][w] ^= r[w] & ma;
+        P.cb[sl][w] ^= r[w] & mb2;
+      }
+      clearbit<W>(P.ca[sl], pw, pb);
+      clearbit<W>(P.cb[sl], pw, pb);
+      if (c) {
+        P.rhsa[sl] ^= __ballot(ha);
+        P.rhsb[sl] ^= __ballot(hb);
+      }
+#pragma unroll
+      for (int v = 0; v < VW; ++v) {
+        V.ra[sl][v] ^= ha ? E[v] : 0ull;
+        V.rb[sl][v] ^= hb ? E[v] : 0ull;
+      }
+    }
+  }
+  clearbit<W>(P.rem, pw, pb);
+}
+
+template <int W, int VW>
+__device__ __forceinline__ void take_constraint_v(PairRegs<W>& P, VarRegs<W, VW>& V, int useb, int c) {
+  u64 row[W];
+  bcast<W>(useb ? P.cb : P.ca, c, row);
+  const int rhs = useb ? getbit<W>(P.rhsb, c) : getbit<W>(P.rhsa, c);
+  u64 E[VW];
+  bcast_v<W, VW>(useb ? V.rb : V.ra, c, E);
+  if (!anyv<W>(row)) {
+    // 0 = rhs_k: the sum vanishes for the variants with rhs_k = 1
+#pragma unroll
+    for (int v = 0; v < VW; ++v) V.z[v] |= rhs ? ~E[v] : E[v];
+    return;
+  }
+  const int p = ffsv<W>(row);
+  flipbit<W>(row, p, 1);
+  subst_v<W, VW, true>(P, V, p, row, rhs, E);
+}
+
+// pair_eval of pair_core.hpp for every variant.  Base phase changes that
+// are the same for all variants go to P.c0, the rest to the planes.
+template <int W, int VW>
+__device__ __forceinline__ void pair_eval_v(PairRegs<W>& P, VarRegs<W, VW>& V, int nca, int ncb) {
+  for (int c = 0; c < nca; ++c) {
+    take_constraint_v<W, VW>(P, V, 0, c);
+    if (all_zero<W, VW>(V)) return;
+  }
+  for (int c = 0; c < ncb; ++c) {
+    take_constraint_v<W, VW>(P, V, 1, c);
+    if (all_zero<W, VW>(V)) return;
+  }
+  while (anyv<W>(P.rem)) {
+    const int p = uni(ffsv<W>(P.rem));
+    const int pw = p >> 6, pb = p & 63;
+    clearbit<W>(P.rem, pw, pb);
+    u64 Bp[W];
+    bcast<W>(P.B, p, Bp);
+    const int ap = bcast_int<W>(P.a, p);
+    u64 H[VW], m[VW];
+    bcast_v<W, VW>(V.hi, p, H);
+#pragma unroll
+    for (int v = 0; v < VW; ++v) m[v] = (ap & 2) ? ~H[v] : H[v];  // variants with a_p,k in {2, 3}
+    if (!anyv<W>(Bp)) {
+      if (!(ap & 1)) {
+        // a_p,k = 2: 1 + i^2 = 0; a_p,k = 0: factor 2
+#pragma unroll
+        for (int v = 0; v < VW; ++v) V.z[v] |= m[v];
+        if (all_zero<W, VW>(V)) return;
+        P.mag -= 2;
+      } else {
+        // 1 + i^{a}: sqrt2 zeta (a = 1) or sqrt2 zeta^7 (a = 3)
+        P.mag -= 1;
+        P.c0 = (P.c0 + 1) & 7;
+#pragma unroll
+        for (int v = 0; v < VW; ++v) z8_add(V.d0[v], V.d1[v], V.d2[v], 0ull, m[v], m[v]);
+      }
+    } else if (ap & 1) {
+      const int add = (ap == 1) ? 3 : 1;
+      P.mag -= 1;
+      P.c0 = (P.c0 + 1) & 7;
+#pragma unroll
+      for (int v = 0; v < VW; ++v) z8_add(V.d0[v], V.d1[v], V.d2[v], 0ull, m[v], m[v]);
+#pragma unroll
+      for (int sl = 0; sl < W; ++sl) {
+        const int bpl = (int)((wsel<W>(P.B[sl], pw) >> pb) & 1ull);
+        const u64 mb = 0ull - (u64)bpl;
+        P.a[sl] = (P.a[sl] + (bpl ? add : 0)) & 3;
+#pragma unroll
+        for (int v = 0; v < VW; ++v) V.hi[sl][v] ^= H[v] & mb;
+#pragma unroll
+        for (int w = 0; w < W; ++w) P.B[sl][w] ^= Bp[w] & mb;
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+          if (w == sl) P.B[sl][w] &= ~(1ull << P.lane);
+        clearbit<W>(P.B[sl], pw, pb);
+      }
+    } else {
+      P.mag -= 2;
+#pragma unroll
+      for (int sl = 0; sl < W; ++sl) clearbit<W>(P.B[sl], pw, pb);
+      const int qv = ffsv<W>(Bp);
+      flipbit<W>(Bp, qv, 1);
+      subst_v<W, VW, false>(P, V, qv, Bp, ap >> 1, H);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pauli variants.  For P applied to phi_t (gate_application.rs:186-210 op
+// order, left_multiplication/{x,y,z}.rs): Z_q adds 2 to gamma_q/2, X_q
+// flips s by u_q = (F_q & ~v) ^ (M_q & v) and adds 4 beta + 2 gamma_q/2 to
+// the phase, Y_q = Z_q, X_q, then +2.  With ds the total change of s, the
+// quadratic form's a_j = gamma_j/2 + 2 (par(s & w_j) ^ c_j) gains
+// 2 (zy_j ^ par(ds & w_j)), the constraint rhs (s at the positions with
+// v = 0, in order) flips by ds there, and p0 = phase changes by dph.
+template <int W, int VW>
+__global__ __launch_bounds__(256) void k_pauli_var(FormsDev in, const int32_t* __restrict__ voff,
+                                                   const int32_t* __restrict__ vops, int nvar,
+                                                   u64* __restrict__ vhi, u64* __restrict__ vrhs,
+                                                   u64* __restrict__ vph) {
+  const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (t >= in.chi) return;
+  Form<W> f;
+  load_form<W>(f, in, t, lane);
+  const int n = in.n, R = 64 * W;
+  u64 Wm[W][W];
+  int kpos[W];
+#pragma unroll
+  for (int sl = 0; sl < W; ++sl) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) Wm[sl][w] = (f.M[sl][w] & ~f.v[w]) ^ (f.F[sl][w] & f.v[w]);
+    kpos[sl] = -1;
+  }
+  int nc = 0;
+  for (int k = 0; k < n; ++k) {
+    if (getbit<W>(f.v, k)) continue;
+#pragma unroll
+    for (int sl = 0; sl < W; ++sl)
+      if (owns<W>(f, nc, sl)) kpos[sl] = k;
+    ++nc;
+  }
+  u64 hacc[W][VW], racc[W][VW], p0[VW], p1[VW], p2[VW];
+#pragma unroll
+  for (int v = 0; v < VW; ++v) {
+    p0[v] = p1[v] = p2[v] = 0ull;
+#pragma unroll
+    for (int sl = 0; sl < W; ++sl) hacc[sl][v] = racc[sl][v] = 0ull;
+  }
+  for (int k = 0; k < nvar; ++k) {
+    u64 s[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) s[w] = f.s[w];
+    int ph = f.ph;
+    int zy[W];
+#pragma unroll
+    for (int sl = 0; sl < W; ++sl) zy[sl] = 0;
+    const int e1 = voff[k + 1];
+    for (int e = voff[k]; e < e1; ++e) {
+      const int code = vops[e];
+      const int q = code >> 2, op = code & 3;
+      int gadd = 0;
+      if (op == 2 || op == 3) {
+#pragma unroll
+        for (int sl = 0; sl < W; ++sl)
+          if (owns<W>(f, q, sl)) zy[sl] ^= 1;
+        gadd = 2;
+      }
+      if (op == 1 || op == 2) {
+        u64 Fq[W], Mq[W];
+        bcast<W>(f.F, q, Fq);
+        bcast<W>(f.M, q, Mq);
+        const int gq = (bcast_g<W>(f, q) + gadd) & 3;
+        int beta = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+          beta += __builtin_popcountll((Mq[w] & ~f.v[w] & s[w]) ^ (Fq[w] & f.v[w] & (Mq[w] ^ s[w])));
+        ph = (ph + 4 * (beta & 1) + 2 * gq) & 7;
+#pragma unroll
+        for (int w = 0; w < W; ++w) s[w] ^= (Fq[w] & ~f.v[w]) ^ (Mq[w] & f.v[w]);
+      }
+      if (op == 2) ph = (ph + 2) & 7;
+    }
+    u64 ds[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) ds[w] = s[w] ^ f.s[w];
+    const int dph = (ph - f.ph) & 7;
+    const int kv = k >> 6;
+    const u64 bit = 1ull << (k & 63);
+#pragma unroll
+    for (int sl = 0; sl < W; ++sl) {
+      const bool real_row = 64 * sl + lane < n;
+      const int hb = real_row ? (zy[sl] ^ parand<W>(ds, Wm[sl])) : 0;
+      const int rb = kpos[sl] >= 0 ? getbit<W>(ds, kpos[sl]) : 0;
+#pragma unroll
+      for (int v = 0; v < VW; ++v)
+        if (v == kv) {
+          if (hb) hacc[sl][v] |= bit;
+          if (rb) racc[sl][v] |= bit;
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < VW; ++v)
+      if (v == kv) {
+        if (dph & 1) p0[v] |= bit;
+        if (dph & 2) p1[v] |= bit;
+        if (dph & 4) p2[v] |= bit;
+      }
+  }
+#pragma unroll
+  for (int sl = 0; sl < W; ++sl) {
+    const size_t r = (size_t)t * R + 64 * sl + lane;
+#pragma unroll
+    for (int v = 0; v < VW; ++v) {
+      vhi[r * VW + v] = hacc[sl][v];
+      vrhs[r * VW + v] = racc[sl][v];
+    }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int v = 0; v < VW; ++v) {
+      vph[((size_t)t * 3 + 0) * VW + v] = p0[v];
+      vph[((size_t)t * 3 + 1) * VW + v] = p1[v];
+      vph[((size_t)t * 3 + 2) * VW + v] = p2[v];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Hermitian pair sums over ket variants.  Each wave belongs to one job
+// (wave_job) and walks that job's units with the job's wave stride; lane k
+// accumulates variant 64*v + k in acc[v]:
+//   sum_f sum_{o >= f} (o == f ? 1 : 2) Re(conj(c_o) c_f <X_o | Y_f^(k)>)
+// which is exp_value.rs:38-58 for Y_f^(k) = P_k phi_f.
+template <int W, int VW>
+__global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict__ jobs,
+                                                   const int32_t* __restrict__ wave_job, int64_t wave_base,
+                                                   int64_t nwaves_launch, int J, const double* __restrict__ tabs,
+                                                   double* __restrict__ partial) {
+  const int lane = threadIdx.x & 63;
+  const int64_t lw = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (lw >= nwaves_launch) return;
+  const int64_t wave = wave_base + lw;
+  const int jb = uni(wave_job[wave]);
+  const PairJobDev& jd = jobs[jb];
+  const QFormDev& X = jd.X;
+  const QFormDev& Y = jd.Y;
+  const int64_t nfixed = jd.nfixed, nother = jd.nother;
+  const int64_t nunits = jd.row_off[nfixed];
+  const int n = X.n, R = 64 * W;
+  u64 valid[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const int hi = n - 64 * w;
+    valid[w] = hi >= 64 ? ~0ull : (hi <= 0 ? 0ull : ((1ull << hi) - 1ull));
+  }
+  u64 invalid[VW];
+#pragma unroll
+  for (int v = 0; v < VW; ++v) {
+    const int lo = jd.nvar - 64 * v;
+    invalid[v] = lo >= 64 ? 0ull : (lo <= 0 ? ~0ull : ~((1ull << lo) - 1ull));
+  }
+  double wk[VW];
+#pragma unroll
+  for (int v = 0; v < VW; ++v) wk[v] = (jd.gw && 64 * v + lane < jd.nvar) ? jd.wts[64 * v + lane] : 0.0;
+  double acc[VW];
+#pragma unroll
+  for (int v = 0; v < VW; ++v) acc[v] = 0.0;
+  for (int64_t u = wave - jd.wave0; u < nunits; u += jd.nwaves) {
+    int64_t lo_r = 0, hi_r = nfixed;
+    while (hi_r - lo_r > 1) {
+      const int64_t mid = (lo_r + hi_r) >> 1;
+      if (jd.row_off[mid] <= u) lo_r = mid;
+      else hi_r = mid;
+    }
+    const int64_t fx = lo_r;
+    const int64_t base = jd.triangle ? fx : 0;
+    const int64_t o_lo = base + (u - jd.row_off[fx]) * J;
+    const int64_t o_hi = (o_lo + J < nother) ? o_lo + J : nother;
+    u64 Bf[W][W], Cf[W][W], rf[W];
+    int af[W];
+    load_qrows<W>(Y, fx, lane, Bf, af, Cf, rf);
+    const int ncf = uni(Y.nc[fx]), kf = uni(Y.k[fx]), pf = uni(Y.p0[fx]);
+    u64 vh[W][VW], vr[W][VW], vp0[VW], vp1[VW], vp2[VW];
+#pragma unroll
+    for (int sl = 0; sl < W; ++sl) {
+      const size_t r = (size_t)fx * R + 64 * sl + lane;
+#pragma unroll
+      for (int v = 0; v < VW; ++v) {
+        vh[sl][v] = jd.V.hi[r * VW + v];
+        vr[sl][v] = jd.V.rhs[r * VW + v];
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < VW; ++v) {
+      vp0[v] = jd.V.ph[((size_t)fx * 3 + 0) * VW + v];
+      vp1[v] = jd.V.ph[((size_t)fx * 3 + 1) * VW + v];
+      vp2[v] = jd.V.ph[((size_t)fx * 3 + 2) * VW + v];
+    }
+    for (int64_t o = o_lo; o < o_hi; ++o) {
+      u64 Bo[W][W], Co[W][W], ro[W];
+      int ao[W];
+      load_qrows<W>(X, o, lane, Bo, ao, Co, ro);
+      const int nco = uni(X.nc[o]), ko = uni(X.k[o]), po = uni(X.p0[o]);
+      PairRegs<W> P;
+      VarRegs<W, VW> V;
+      P.lane = lane;
+#pragma unroll
+      for (int sl = 0; sl < W; ++sl) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          P.B[sl][w] = Bf[sl][w] ^ Bo[sl][w];
+          P.ca[sl][w] = Co[sl][w];
+          P.cb[sl][w] = Cf[sl][w];
+        }
+        P.a[sl] = (af[sl] - ao[sl]) & 3;
+#pragma unroll
+        for (int v = 0; v < VW; ++v) {
+          V.hi[sl][v] = vh[sl][v];
+          V.rb[sl][v] = vr[sl][v];
+          V.ra[sl][v] = 0ull;
+        }
+      }
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        P.rhsa[w] = ro[w];
+        P.rhsb[w] = rf[w];
+        P.rem[w] = valid[w];
+      }
+#pragma unroll
+      for (int v = 0; v < VW; ++v) {
+        V.d0[v] = vp0[v];
+        V.d1[v] = vp1[v];
+        V.d2[v] = vp2[v];
+        V.z[v] = invalid[v];
+      }
+      P.c0 = (pf - po) & 7;
+      P.mag = kf + ko;
+      pair_eval_v<W, VW>(P, V, nco, ncf);
+      // (c_bra.conj() * c_ket).into(), shared by the variants
+      double cr_, ci_;
+      if (jd.cx.scalar) {
+        const int zb = jd.cx.cz[o], zk = jd.cy.cz[fx];
+        if (zb || zk) {
+          cr_ = 0.0;
+          ci_ = 0.0;
+        } else {
+          const int kk = ((8 - jd.cx.ck[o]) % 8 + jd.cy.ck[fx]) % 8;
+          const int rr = jd.cx.cr[o] + jd.cy.cr[fx];
+          const double nn = tabs[TAB_POW + rr];
+          cr_ = tabs[TAB_COS + kk] * nn;
+          ci_ = tabs[TAB_SIN + kk] * nn;
+        }
+      } else {
+        const double2 b0 = jd.cx.c[o], k0 = jd.cy.c[fx];
+        const double br = b0.x, bi = -b0.y;
+        cr_ = br * k0.x - bi * k0.y;
+        ci_ = br * k0.y + bi * k0.x;
+      }
+      const bool diag = (o == fx);
+      double gr = 0.0, gi = 0.0;
+#pragma unroll
+      for (int v = 0; v < VW; ++v) {
+        const int zk = (int)((V.z[v] >> lane) & 1ull);
+        if (!zk) {
+          const int D = (int)((V.d0[v] >> lane) & 1ull) | ((int)((V.d1[v] >> lane) & 1ull) << 1) |
+                        ((int)((V.d2[v] >> lane) & 1ull) << 2);
+          const int K = (P.c0 + D) & 7;
+          const double nrm = tabs[TAB_POW + P.mag];
+          const double ir = tabs[TAB_COS + K] * nrm, ii = tabs[TAB_SIN + K] * nrm;
+          const double tr = cr_ * ir - ci_ * ii;
+          acc[v] = diag ? acc[v] + tr : acc[v] + (tr + tr);
+          gr = gr + wk[v] * ir;
+          gi = gi + wk[v] * ii;
+        }
+      }
+      if (jd.gw) {
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          gr = gr + __shfl_xor(gr, off);
+          gi = gi + __shfl_xor(gi, off);
+        }
+        if (lane == 0) {
+          double2* dst = jd.gw + (size_t)o * jd.gw_ld + fx;
+          if (jd.accumulate) {
+            double2 old = *dst;
+            old.x = old.x + gr;
+            old.y = old.y + gi;
+            *dst = old;
+          } else {
+            *dst = make_double2(gr, gi);
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < VW; ++v) partial[(size_t)wave * (64 * VW) + 64 * v + lane] = acc[v];
+}
+
+// one block per job: thread i sums variant i % NV over the job's waves
+// w = wave0 + part, part + P, ... in order (P = 256 / NV parts), then the
+// parts are added in order
+template <int VW>
+__global__ __launch_bounds__(256) void k_reduce_var(const PairJobDev* __restrict__ jobs,
+                                                    const double* __restrict__ partial, double* __restrict__ out) {
+  constexpr int NV = 64 * VW, NP = 256 / NV;
+  __shared__ double sh[256];
+  const PairJobDev& jd = jobs[blockIdx.x];
+  const int v = threadIdx.x % NV, part = threadIdx.x / NV;
+  double s = 0.0;
+  for (int64_t w = part; w < jd.nwaves; w += NP) s = s + partial[(size_t)(jd.wave0 + w) * NV + v];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  if (part == 0 && v < jd.nvar) {
+    double t = sh[v];
+    for (int k = 1; k < NP; ++k) t = t + sh[k * NV + v];
+    out[jd.out_base + v] = t;
+  }
+}
+
+// y_j = sum_{i <= j} G_ji c_i : one block per row j
+__global__ __launch_bounds__(256) void k_gramc_rows(const double2* __restrict__ G, int64_t ld, int64_t chi,
+                                                    const double2* __restrict__ c, double2* __restrict__ y) {
+  __shared__ double sr[256], si[256];
+  const int64_t j = blockIdx.x;
+  double ar = 0.0, ai = 0.0;
+  for (int64_t i = threadIdx.x; i <= j; i += blockDim.x) {
+    const double2 g = G[(size_t)j * ld + i];
+    const double2 ci = c[i];
+    ar = ar + (g.x * ci.x - g.y * ci.y);
+    ai = ai + (g.x * ci.y + g.y * ci.x);
+  }
+  sr[threadIdx.x] = ar;
+  si[threadIdx.x] = ai;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      sr[threadIdx.x] += sr[threadIdx.x + st];
+      si[threadIdx.x] += si[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) y[j] = make_double2(sr[0], si[0]);
+}
+
+// y_i += sum_{j > i} conj(G_ji) c_j : one thread per column i
+__global__ __launch_bounds__(256) void k_gramc_cols(const double2* __restrict__ G, int64_t ld, int64_t chi,
+                                                    const double2* __restrict__ c, double2* __restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t jstart = (int64_t)blockIdx.x * blockDim.x + 1;
+  double ar = 0.0, ai = 0.0;
+  for (int64_t j = jstart; j < chi; ++j) {
+    if (i < chi && j > i) {
+      const double2 g = G[(size_t)j * ld + i];
+      const double2 cj = c[j];
+      ar = ar + (g.x * cj.x + g.y * cj.y);
+      ai = ai + (g.x * cj.y - g.y * cj.x);
+    }
+  }
+  if (i < chi) {
+    double2 v = y[i];
+    v.x = v.x + ar;
+    v.y = v.y + ai;
+    y[i] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+static inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
+
+#define NX_DISPATCH_WV(W_, VW_, ...)                                                         \
+  switch ((W_) * 8 + (VW_)) {                                                               \
+    case 9: { constexpr int WW = 1, VV = 1; __VA_ARGS__; } break;                           \
+    case 10: { constexpr int WW = 1, VV = 2; __VA_ARGS__; } break;                          \
+    case 17: { constexpr int WW = 2, VV = 1; __VA_ARGS__; } break;                          \
+    case 18: { constexpr int WW = 2, VV = 2; __VA_ARGS__; } break;                          \
+    case 25: { constexpr int WW = 3, VV = 1; __VA_ARGS__; } break;                          \
+    case 26: { constexpr int WW = 3, VV = 2; __VA_ARGS__; } break;                          \
+    case 33: { constexpr int WW = 4, VV = 1; __VA_ARGS__; } break;                          \
+    case 34: { constexpr int WW = 4, VV = 2; __VA_ARGS__; } break;                          \
+    default: return hipErrorInvalidValue;                                                   \
+  }
+
+hipError_t launch_pauli_var(const FormsDev& in, const int32_t* voff, const int32_t* vops, int nvar, int VW,
+                            uint64_t* vhi, uint64_t* vrhs, uint64_t* vph, hipStream_t st) {
+  if (in.chi == 0) return hipSuccess;
+  if (nvar > 64 * VW) return hipErrorInvalidValue;
+  NX_DISPATCH_WV(in.W, VW,
+                 k_pauli_var<WW, VV><<<dim3(nblk(in.chi, 4)), dim3(256), 0, st>>>(in, voff, vops, nvar, vhi, vrhs,
+                                                                                   vph));
+  return hipGetLastError();
+}
+
+hipError_t launch_pairs_var(int W, int VW, const PairJobDev* jobs, const int32_t* wave_job, int64_t wave_base,
+                            int64_t nwaves, int J, const double* tabs, double* partial, hipStream_t st) {
+  if (nwaves == 0) return hipSuccess;
+  NX_DISPATCH_WV(W, VW,
+                 k_pairs_var<WW, VV><<<dim3(nblk(nwaves, 4)), dim3(256), 0, st>>>(jobs, wave_job, wave_base, nwaves,
+                                                                                  J, tabs, partial));
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_var(const PairJobDev* jobs, int njobs, int VW, const double* partial, double* out,
+                             hipStream_t st) {
+  if (njobs == 0) return hipSuccess;
+  if (VW == 1) k_reduce_var<1><<<dim3(njobs), dim3(256), 0, st>>>(jobs, partial, out);
+  else if (VW == 2) k_reduce_var<2><<<dim3(njobs), dim3(256), 0, st>>>(jobs, partial, out);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_gramc(const double2* Gw, int64_t ld, int64_t chi, const double2* c, double2* y, hipStream_t st) {
+  if (chi == 0) return hipSuccess;
+  k_gramc_rows<<<dim3((unsigned)chi), dim3(256), 0, st>>>(Gw, ld, chi, c, y);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_gramc_cols<<<dim3(nblk(chi, 256)), dim3(256), 0, st>>>(Gw, ld, chi, c, y);
+  return hipGetLastError();
+}
+
+}  // namespace nx

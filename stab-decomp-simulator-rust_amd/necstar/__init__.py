@@ -9,5 +9,6 @@ from .gate import QuantumGate
 from .circuit import QuantumCircuit
 from .pauli_string import PauliString
 from .state import QuantumState
+from .batch import Batch
 
-__all__ = ["QuantumGate", "QuantumCircuit", "QuantumState", "PauliString"]
+__all__ = ["QuantumGate", "QuantumCircuit", "QuantumState", "PauliString", "Batch"]
