@@ -88,8 +88,12 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=16.0)
-    ap.add_argument("--config", type=int, default=1, choices=(1, 3),
-                    help="1: amplitude sweep + norm (the metric); 3: Pauli-loss forward vs backward")
+    ap.add_argument("--config", type=int, default=1, choices=(1, 2, 3),
+                    help="1: amplitude sweep + norm (the metric); 2: batched vs plain norm + 64 Pauli "
+                         "expectations over 256 circuits; 3: Pauli-loss forward vs backward")
+    ap.add_argument("--circuits", type=int, default=0, help="cfg2: use only the first N circuits")
+    ap.add_argument("--plain-paulis", type=int, default=2,
+                    help="cfg2: Pauli strings per circuit timed on the plain path (then scaled to 64)")
     ap.add_argument("--sample-shots", type=int, default=0,
                     help="also time QuantumState.sample over all 64 qubits with this many shots (cfg1)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -111,6 +115,8 @@ def main():
     _lib.check(L.nx_init(0))
     if args.config == 3:
         return bench_cfg3(args)
+    if args.config == 2:
+        return bench_cfg2(args)
     cfg = WL.config(1)
     n = cfg["n"]
     t0 = time.perf_counter()
@@ -206,6 +212,86 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def pauli_text(ops):
+    return "".join("IXYZ"[x] for x in reversed(ops))
+
+
+def bench_cfg2(args):
+    """Batching workload: 256 circuits, n ~ U[16,128], 30 n Cliffords,
+    t ~ U[4,32], each with its norm and 64 Pauli expectations (weight 2-6).
+    Batched: one necstar.Batch (autotuned, HIP graph) evaluates all of them
+    per run.  Plain: the reference's per-call API, QuantumState.norm() and
+    exp_value(P) one string at a time, on every circuit for the first
+    --plain-paulis strings; its per-circuit time is norm + 64 x (mean string
+    time).  Both legs run in the same process on the same device-resident
+    states."""
+    import ctypes as C
+    import numpy as np
+    import workloads as WL
+    from necstar import Batch, PauliString, QuantumState
+    from necstar import _lib
+    L = _lib.lib()
+    cfg = WL.config(2)
+    circs = cfg["circuits"][: args.circuits] if args.circuits else cfg["circuits"]
+    t0 = time.perf_counter()
+    states = [QuantumState.from_circuit(WL.to_circuit(c["n"], c["gates"])) for c in circs]
+    compile_s = time.perf_counter() - t0
+    paulis = [[PauliString.from_str(pauli_text(p)) for p in c["paulis"]] for c in circs]
+    chis = np.array([s.stabilizer_rank for s in states])
+    ns = np.array([c["n"] for c in circs])
+    print(f"# cfg2: {len(states)} circuits compiled in {compile_s:.1f} s, chi max {chis.max()}, "
+          f"sum chi^2/2 {float((chis.astype(float) ** 2).sum() / 2):.3e}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    b = Batch(states, paulis, autotune=True, graph=True)
+    create_s = time.perf_counter() - t0
+    J, waves, vw = b.shape
+    for _ in range(args.warmup):
+        b.run()
+    _lib.check(L.nx_profile_enable(1))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        norms, vals = b.run()
+    t1 = time.perf_counter()
+    ms, cnt = C.c_double(), C.c_int64()
+    _lib.check(L.nx_profile_get(b"batch_pairs", C.byref(ms), C.byref(cnt)))
+    _lib.check(L.nx_profile_enable(0))
+    batched_s = (t1 - t0) / args.steps
+    graph_ms = ms.value / max(cnt.value, 1)
+    print(f"# batched run {batched_s:.3f} s (J={J}, waves={waves})", file=sys.stderr, flush=True)
+    # plain per-call path
+    K = max(1, min(args.plain_paulis, 64))
+    plain_s, maxdiff = 0.0, 0.0
+    t_start = time.perf_counter()
+    for i, (st, pl) in enumerate(zip(states, paulis)):
+        ta = time.perf_counter()
+        nrm = st.norm()
+        tb = time.perf_counter()
+        ev = [st.exp_value(p) for p in pl[:K]]
+        tc = time.perf_counter()
+        plain_s += (tb - ta) + (tc - tb) / K * len(pl)
+        maxdiff = max(maxdiff, abs(nrm - norms[i]) / max(1.0, abs(nrm)),
+                      float(np.max(np.abs(np.array(ev) - vals[i][:K]) / np.maximum(1.0, np.abs(ev)))))
+    plain_wall = time.perf_counter() - t_start
+    nc = len(states)
+    pairs = float((chis.astype(float) * (chis + 1) / 2).sum())
+    print(json.dumps({
+        "metric": "cfg2 batched vs plain, circuits/s (norm + 64 Pauli expectations per circuit)",
+        "value": nc / batched_s, "unit": "circuits/s", "higher_is_better": True,
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "batched_s": batched_s, "batched_graph_ms": graph_ms, "plain_s_est": plain_s,
+        "plain_circuits_per_s": nc / plain_s, "batched_vs_plain": plain_s / batched_s,
+        "plain_sample": f"norm + {K} of 64 strings per circuit timed ({plain_wall:.1f} s wall), "
+                        f"string time scaled to 64",
+        "max_rel_diff_batched_vs_plain": maxdiff,
+        "pair_evaluations_per_run": pairs, "pair_string_values_per_s": pairs * 65 / batched_s,
+        "shape": {"J": J, "waves": waves, "variant_words": vw}, "compile_s": compile_s, "batch_create_s": create_s,
+        "config": {"workload": "cfg2: 256 circuits, n U[16,128], 30n Cliffords, t U[4,32], norm + 64 Paulis "
+                               "(weight 2-6) each; sampling not included",
+                   "circuits": nc, "chi_max": int(chis.max()), "chi_mean": float(chis.mean()),
+                   "n_mean": float(ns.mean())},
+    }), flush=True)
 
 
 def bench_cfg3(args):

@@ -91,7 +91,8 @@ struct PairJobDev {
   int64_t nfixed, nother;
   int32_t triangle, nvar;
   int64_t wave0, nwaves;
-  int64_t out_base;
+  int64_t out_base;  // variant k -> out[out_base + k]
+  int64_t base_out;  // >= 0: the base (ket itself, identity) -> out[base_out]
   double2* gw;
   int64_t gw_ld;
   const double* wts;
@@ -125,11 +126,13 @@ hipError_t launch_reduce(const double2* partial, int64_t n, double2* out, hipStr
 hipError_t launch_pauli_var(const FormsDev& in, const int32_t* voff, const int32_t* vops, int nvar, int VW,
                             uint64_t* vhi, uint64_t* vrhs, uint64_t* vph, hipStream_t st);
 // waves [wave_base, wave_base + nwaves) of a launch plan whose jobs all have
-// word width W; wave_job[w] = job of wave w; partial holds 64 * VW doubles
-// per wave of the plan (the real Hermitian sums per variant)
+// word width W; wave_job[w] = job of wave w; partial holds 64 * VW + 1
+// doubles per wave of the plan (the real Hermitian sums per variant, then
+// the base)
 hipError_t launch_pairs_var(int W, int VW, const PairJobDev* jobs, const int32_t* wave_job, int64_t wave_base,
                             int64_t nwaves, int J, const double* tabs, double* partial, hipStream_t st);
-// out[jobs[j].out_base + k] = sum over the job's waves of partial (k < nvar)
+// out[jobs[j].out_base + k] = sum over the job's waves of partial (k < nvar),
+// and the base sum to out[jobs[j].base_out] when that is >= 0
 hipError_t launch_reduce_var(const PairJobDev* jobs, int njobs, int VW, const double* partial, double* out,
                              hipStream_t st);
 // y = G c for the Hermitian G whose lower triangle is Gw[j*ld + i] (j >= i)

@@ -92,11 +92,16 @@ __device__ __forceinline__ int bcast_int(const int (&x)[W], int r) {
   return rl32(val, r & 63);
 }
 
-// substitute x_p = c ^ <r, x> (r excludes p) into the form; with CONS also
-// into the remaining constraint rows.  Rows of eliminated variables and
+// Constraint rows a substitution must update: none (Gauss phase), the bra
+// and ket rows (while bra constraints are taken; they come first), or the
+// ket rows only (bra rows are never read again once all are taken).
+enum : int { CONS_NONE = 0, CONS_AB = 1, CONS_B = 2 };
+
+// substitute x_p = c ^ <r, x> (r excludes p) into the form and into the
+// constraint rows that CONS names.  Rows of eliminated variables and
 // padding rows may hold junk: they are never broadcast again, and every
 // column of an eliminated variable is cleared in all rows.
-template <int W, bool CONS>
+template <int W, int CONS>
 __device__ __forceinline__ void subst(PairRegs<W>& P, int p, const u64 (&r)[W], int c) {
   const int ap = bcast_int<W>(P.a, p);
   u64 Bp[W];
@@ -121,36 +126,36 @@ __device__ __forceinline__ void subst(PairRegs<W>& P, int p, const u64 (&r)[W], 
     for (int w = 0; w < W; ++w)
       if (w == sl) P.B[sl][w] &= ~(1ull << P.lane);
     clearbit<W>(P.B[sl], pw, pb);
-    if (CONS) {
+    if (CONS == CONS_AB) {
       const int ha = (int)((wsel<W>(P.ca[sl], pw) >> pb) & 1ull);
-      const int hb = (int)((wsel<W>(P.cb[sl], pw) >> pb) & 1ull);
-      const u64 ma = 0ull - (u64)ha, mb2 = 0ull - (u64)hb;
+      const u64 ma = 0ull - (u64)ha;
 #pragma unroll
-      for (int w = 0; w < W; ++w) {
-        P.ca[sl][w] ^= r[w] & ma;
-        P.cb[sl][w] ^= r[w] & mb2;
-      }
+      for (int w = 0; w < W; ++w) P.ca[sl][w] ^= r[w] & ma;
       clearbit<W>(P.ca[sl], pw, pb);
+      if (c) P.rhsa[sl] ^= __ballot(ha);
+    }
+    if (CONS != CONS_NONE) {
+      const int hb = (int)((wsel<W>(P.cb[sl], pw) >> pb) & 1ull);
+      const u64 mb2 = 0ull - (u64)hb;
+#pragma unroll
+      for (int w = 0; w < W; ++w) P.cb[sl][w] ^= r[w] & mb2;
       clearbit<W>(P.cb[sl], pw, pb);
-      if (c) {
-        P.rhsa[sl] ^= __ballot(ha);
-        P.rhsb[sl] ^= __ballot(hb);
-      }
+      if (c) P.rhsb[sl] ^= __ballot(hb);
     }
   }
   clearbit<W>(P.rem, pw, pb);
 }
 
 // returns 1 if a constraint is inconsistent (the sum is zero)
-template <int W>
-__device__ __forceinline__ int take_constraint(PairRegs<W>& P, int useb, int c) {
+template <int W, bool USEB>
+__device__ __forceinline__ int take_constraint(PairRegs<W>& P, int c) {
   u64 row[W];
-  bcast<W>(useb ? P.cb : P.ca, c, row);
-  const int rhs = useb ? getbit<W>(P.rhsb, c) : getbit<W>(P.rhsa, c);
+  bcast<W>(USEB ? P.cb : P.ca, c, row);
+  const int rhs = USEB ? getbit<W>(P.rhsb, c) : getbit<W>(P.rhsa, c);
   if (!anyv<W>(row)) return rhs;
   const int p = ffsv<W>(row);
   flipbit<W>(row, p, 1);
-  subst<W, true>(P, p, row, rhs);
+  subst<W, USEB ? CONS_B : CONS_AB>(P, p, row, rhs);
   return 0;
 }
 
@@ -158,9 +163,9 @@ __device__ __forceinline__ int take_constraint(PairRegs<W>& P, int useb, int c) 
 template <int W>
 __device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int* zero, int* K, int* R) {
   for (int c = 0; c < nca; ++c)
-    if (take_constraint<W>(P, 0, c)) { *zero = 1; return; }
+    if (take_constraint<W, false>(P, c)) { *zero = 1; return; }
   for (int c = 0; c < ncb; ++c)
-    if (take_constraint<W>(P, 1, c)) { *zero = 1; return; }
+    if (take_constraint<W, true>(P, c)) { *zero = 1; return; }
   while (anyv<W>(P.rem)) {
     const int p = uni(ffsv<W>(P.rem));
     const int pw = p >> 6, pb = p & 63;
@@ -196,7 +201,7 @@ __device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int*
       for (int sl = 0; sl < W; ++sl) clearbit<W>(P.B[sl], pw, pb);
       const int qv = ffsv<W>(Bp);
       flipbit<W>(Bp, qv, 1);
-      subst<W, false>(P, qv, Bp, ap >> 1);
+      subst<W, CONS_NONE>(P, qv, Bp, ap >> 1);
     }
   }
   *zero = 0;

@@ -916,7 +916,7 @@ struct VarSet {
 
 // variants first .. first + count - 1 of ps, for every term of f
 static VarSet pauli_vars(const Forms& f, const PauliSet& ps, int first, int count, int VW) {
-  if (count < 1 || count > 64 * VW) fail(NX_E_INTERNAL, "bad variant count %d", count);
+  if (count < 0 || count > 64 * VW) fail(NX_E_INTERNAL, "bad variant count %d", count);
   VarSet vs;
   vs.nvar = count;
   vs.VW = VW;
@@ -946,6 +946,7 @@ struct VarJob {
   const VarSet* V = nullptr;
   CoefDev cx{}, cy{};
   int64_t out_base = 0;
+  int64_t base_out = -1;  // >= 0: also the base sum (norm^2 / |g|^2) there
   double2* gw = nullptr;
   int64_t gw_ld = 0;
   const double* wts = nullptr;
@@ -972,31 +973,29 @@ struct VarPlan {
   }
 };
 
-static int64_t units_of(int64_t chi, int J) {
-  int64_t u = 0;
-  for (int64_t f = 0; f < chi; ++f) u += (chi - f + J - 1) / J;
-  return u;
-}
-
 // Builds the device job table.  Jobs are grouped by word width (one launch
 // each); within a launch the waves are shared out in proportion to each
 // job's work (units x n^2), at least one wave per job and at most one per
-// unit.
-static VarPlan make_plan(const std::vector<VarJob>& in, int VW, int J, int64_t target_waves) {
+// unit.  row_frac < 1 keeps only that fraction of each job's ket rows (a
+// shortened plan of the same shape, for timing launch shapes).
+static VarPlan make_plan(const std::vector<VarJob>& in, int VW, int J, int64_t target_waves, double row_frac = 1.0) {
   VarPlan pl;
   pl.VW = VW;
   pl.J = J;
   std::vector<int> order(in.size());
   for (size_t i = 0; i < in.size(); ++i) order[i] = (int)i;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return in[a].X->W < in[b].X->W; });
-  std::vector<int64_t> units(in.size());
+  std::vector<int64_t> units(in.size()), nrows(in.size());
   double work_total = 0.0;
   int64_t rows_total = 0;
   for (size_t i = 0; i < in.size(); ++i) {
-    units[i] = units_of(in[i].X->chi, J);
+    const int64_t chi = in[i].X->chi;
+    nrows[i] = row_frac >= 1.0 ? chi : std::max<int64_t>(1, (int64_t)std::ceil(row_frac * (double)chi));
+    units[i] = 0;
+    for (int64_t f = 0; f < nrows[i]; ++f) units[i] += (chi - f + J - 1) / J;
     const double n = (double)in[i].X->n;
     work_total += (double)units[i] * n * n;
-    rows_total += in[i].X->chi + 1;
+    rows_total += nrows[i] + 1;
   }
   std::vector<int64_t> roff;
   roff.reserve(rows_total);
@@ -1014,7 +1013,7 @@ static VarPlan make_plan(const std::vector<VarJob>& in, int VW, int J, int64_t t
     }
     roff_base[idx] = (int64_t)roff.size();
     int64_t acc = 0;
-    for (int64_t f = 0; f < chi; ++f) {
+    for (int64_t f = 0; f < nrows[idx]; ++f) {
       roff.push_back(acc);
       acc += (chi - f + J - 1) / J;
     }
@@ -1029,13 +1028,14 @@ static VarPlan make_plan(const std::vector<VarJob>& in, int VW, int J, int64_t t
     d.cx = j.cx;
     d.cy = j.cy;
     d.row_off = nullptr;  // set below
-    d.nfixed = chi;
+    d.nfixed = nrows[idx];
     d.nother = chi;
     d.triangle = 1;
     d.nvar = j.V->nvar;
     d.wave0 = wave;
     d.nwaves = nw;
     d.out_base = j.out_base;
+    d.base_out = j.base_out;
     d.gw = j.gw;
     d.gw_ld = j.gw_ld;
     d.wts = j.wts;
@@ -1043,7 +1043,7 @@ static VarPlan make_plan(const std::vector<VarJob>& in, int VW, int J, int64_t t
     for (int64_t w = 0; w < nw; ++w) wj.push_back(idx);
     wave += nw;
     pl.launches.back().nwaves += nw;
-    pl.nout = std::max<int64_t>(pl.nout, j.out_base + j.V->nvar);
+    pl.nout = std::max<int64_t>(pl.nout, std::max<int64_t>(j.out_base + j.V->nvar, j.base_out + 1));
   }
   pl.total_waves = wave;
   pl.njobs = (int)in.size();
@@ -1054,7 +1054,7 @@ static VarPlan make_plan(const std::vector<VarJob>& in, int VW, int J, int64_t t
   h2d(pl.jobs.p, hj.data(), hj.size() * sizeof(PairJobDev));
   pl.wave_job.alloc(std::max<size_t>(wj.size(), 1) * 4);
   h2d(pl.wave_job.p, wj.data(), wj.size() * 4);
-  pl.partial.alloc(std::max<int64_t>(wave, 1) * 64 * VW * 8);
+  pl.partial.alloc(std::max<int64_t>(wave, 1) * (64 * VW + 1) * 8);
   pl.out.alloc(std::max<int64_t>(pl.nout, 1) * 8);
   return pl;
 }
@@ -1090,9 +1090,14 @@ static std::pair<int, int64_t> tuned_shape(const std::vector<VarJob>& jobs, int 
   return {16, DEFAULT_WAVES};
 }
 
+// Each candidate is timed on a shortened plan (about 2^24 pairs: the same
+// jobs with a fraction of their ket rows).
 static std::pair<int, int64_t> autotune(const std::vector<VarJob>& jobs, int VW) {
   static const int Js[] = {4, 8, 16, 32};
   static const int64_t Ws[] = {4096, 8192, 16384, 32768};
+  double pairs = 0.0;
+  for (const VarJob& j : jobs) pairs += 0.5 * (double)j.X->chi * (double)(j.X->chi + 1);
+  const double frac = std::min(1.0, 16777216.0 / std::max(pairs, 1.0));
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "hipEventCreate");
   hip_check(hipEventCreate(&e1), "hipEventCreate");
@@ -1100,7 +1105,7 @@ static std::pair<int, int64_t> autotune(const std::vector<VarJob>& jobs, int VW)
   std::pair<int, int64_t> arg{16, DEFAULT_WAVES};
   for (int J : Js)
     for (int64_t nw : Ws) {
-      VarPlan pl = make_plan(jobs, VW, J, nw);
+      VarPlan pl = make_plan(jobs, VW, J, nw, frac);
       pl.enqueue(stream());  // warm-up
       hip_check(hipEventRecord(e0, stream()), "hipEventRecord");
       pl.enqueue(stream());
@@ -1575,10 +1580,9 @@ int nx_batch_create(int32_t nstates, const nx_state* const* states, const int32_
     int maxvar = 1;
     for (int32_t i = 0; i < nstates; ++i) {
       if (npauli[i] < 0) fail(NX_E_VALUE, "negative Pauli count for state %d", i);
-      sets[i].add({});  // identity: the norm
       add_dense_paulis(states[i], npauli[i], ops + pos, sets[i]);
       pos += (size_t)npauli[i] * states[i]->n;
-      maxvar = std::max(maxvar, 1 + npauli[i]);
+      maxvar = std::max(maxvar, npauli[i]);
     }
     const int VW = maxvar <= 64 ? 1 : 2;
     const int per = 64 * VW;
@@ -1595,14 +1599,16 @@ int nx_batch_create(int32_t nstates, const nx_state* const* states, const int32_
     std::vector<VarJob> jobs;
     size_t njobs = 0;
     for (int32_t i = 0; i < nstates; ++i)
-      if (b->states[i]->chi() > 0) njobs += (sets[i].size() + per - 1) / per;
+      if (b->states[i]->chi() > 0) njobs += std::max(1, (sets[i].size() + per - 1) / per);
     b->vars.reserve(njobs);
     for (int32_t i = 0; i < nstates; ++i) {
       const nx_state* st = b->states[i];
       if (st->chi() == 0) continue;
       const QForms& q = qform_of(st);
       CoefDev c = coef_of(st);
-      for (int first = 0; first < sets[i].size(); first += per) {
+      // the first job of a state also yields its norm (the base, i.e. the
+      // identity string, comes with every variant pass)
+      for (int first = 0; first == 0 || first < sets[i].size(); first += per) {
         b->vars.push_back(pauli_vars(st->forms, sets[i], first, std::min(per, sets[i].size() - first), VW));
         VarJob j;
         j.X = &q;
@@ -1610,7 +1616,8 @@ int nx_batch_create(int32_t nstates, const nx_state* const* states, const int32_
         j.V = &b->vars.back();
         j.cx = c;
         j.cy = c;
-        j.out_base = b->state_out[i] + first;
+        j.out_base = b->state_out[i] + 1 + first;
+        j.base_out = first == 0 ? b->state_out[i] : -1;
         jobs.push_back(j);
       }
     }

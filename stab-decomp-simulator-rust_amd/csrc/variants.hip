@@ -36,9 +36,10 @@ __device__ __forceinline__ void z8_add(u64& b0, u64& b1, u64& b2, u64 x0, u64 x1
 template <int W, int VW>
 struct VarRegs {
   u64 hi[W][VW];               // row r: variants whose a_r carries an extra 2
-  u64 ra[W][VW], rb[W][VW];    // constraint row c: variants whose rhs is flipped
+  u64 rb[W][VW];               // ket constraint row c: variants whose rhs is flipped
   u64 d0[VW], d1[VW], d2[VW];  // uniform: Z8 phase offset of each variant
   u64 z[VW];                   // uniform: variants whose sum is already zero
+  int bz;                      // uniform: the base (identity) sum is zero or not wanted
 };
 
 template <int W, int VW>
@@ -56,7 +57,7 @@ __device__ __forceinline__ void bcast_v(const u64 (&X)[W][VW], int r, u64 (&out)
 
 template <int W, int VW>
 __device__ __forceinline__ bool all_zero(const VarRegs<W, VW>& V) {
-  u64 a = ~0ull;
+  u64 a = V.bz ? ~0ull : 0ull;
 #pragma unroll
   for (int v = 0; v < VW; ++v) a &= V.z[v];
   return a == ~0ull;
@@ -67,8 +68,10 @@ __device__ __forceinline__ bool all_zero(const VarRegs<W, VW>& V) {
 // rhs) are updated as in subst with c; the variant masks pick up the
 // differences: a_l changes by r_l (c_k ? -a_p,k : a_p,k) + 2 ((c_k ^ r_l) &
 // B_pl), which differs from the base change by 2 (r_l (h_p,k ^ (a_p odd &
-// e_k)) ^ (B_pl & e_k)); the phase gains 2 (a_p,k c_k mod 4) per variant.
-template <int W, int VW, bool CONS>
+// e_k)) ^ (B_pl & e_k)); the phase gains 2 (a_p,k c_k mod 4) per variant and
+// 2 (a_p c mod 4) for the base.  Bra constraints carry no variant flips
+// (E = 0 while they are taken).
+template <int W, int VW, int CONS>
 __device__ __forceinline__ void subst_v(PairRegs<W>& P, VarRegs<W, VW>& V, int p, const u64 (&r)[W], int c,
                                         const u64 (&E)[VW]) {
   const int ap = bcast_int<W>(P.a, p);
@@ -76,13 +79,18 @@ __device__ __forceinline__ void subst_v(PairRegs<W>& P, VarRegs<W, VW>& V, int p
   bcast_v<W, VW>(V.hi, p, H);
   u64 Bp[W];
   bcast<W>(P.B, p, Bp);
+  P.c0 = (P.c0 + 2 * ap * c) & 7;
   u64 L[VW];
 #pragma unroll
   for (int v = 0; v < VW; ++v) {
+    // variant phase minus base phase, 2 (a_p,k c_k - a_p c) mod 8, as the
+    // sum of the full variant term and 8 - 2 a_p c (both bit-sliced)
     const u64 cm = c ? ~E[v] : E[v];                  // variants with c_k = 1
     const u64 lo = (ap & 1) ? cm : 0ull;              // (a_p,k c_k) mod 2
     const u64 hi = cm & ((ap & 2) ? ~H[v] : H[v]);    // high bit of a_p,k c_k
     z8_add(V.d0[v], V.d1[v], V.d2[v], 0ull, lo, hi);
+    const int nb = (8 - 2 * ap * c) & 7;
+    z8_add(V.d0[v], V.d1[v], V.d2[v], 0ull, (nb & 2) ? ~0ull : 0ull, (nb & 4) ? ~0ull : 0ull);
     L[v] = H[v] ^ ((ap & 1) ? E[v] : 0ull);
   }
   const int lin = c ? (4 - ap) & 3 : ap;
@@ -104,59 +112,64 @@ __device__ __forceinline__ void subst_v(PairRegs<W>& P, VarRegs<W, VW>& V, int p
     for (int w = 0; w < W; ++w)
       if (w == sl) P.B[sl][w] &= ~(1ull << P.lane);
     clearbit<W>(P.B[sl], pw, pb);
-    if (CONS) {
+    if (CONS == CONS_AB) {
       const int ha = (int)((wsel<W>(P.ca[sl], pw) >> pb) & 1ull);
-      const int hb = (int)((wsel<W>(P.cb[sl], pw) >> pb) & 1ull);
-      const u64 ma = 0ull - (u64)ha, mb2 = 0ull - (u64)hb;
+      const u64 ma = 0ull - (u64)ha;
 #pragma unroll
-      for (int w = 0; w < W; ++w) {
-        P.ca[sl][w] ^= r[w] & ma;
-        P.cb[sl][w] ^= r[w] & mb2;
-      }
+      for (int w = 0; w < W; ++w) P.ca[sl][w] ^= r[w] & ma;
       clearbit<W>(P.ca[sl], pw, pb);
-      clearbit<W>(P.cb[sl], pw, pb);
-      if (c) {
-        P.rhsa[sl] ^= __ballot(ha);
-        P.rhsb[sl] ^= __ballot(hb);
-      }
+      if (c) P.rhsa[sl] ^= __ballot(ha);
+    }
+    if (CONS != CONS_NONE) {
+      const int hb = (int)((wsel<W>(P.cb[sl], pw) >> pb) & 1ull);
+      const u64 mb2 = 0ull - (u64)hb;
 #pragma unroll
-      for (int v = 0; v < VW; ++v) {
-        V.ra[sl][v] ^= ha ? E[v] : 0ull;
-        V.rb[sl][v] ^= hb ? E[v] : 0ull;
+      for (int w = 0; w < W; ++w) P.cb[sl][w] ^= r[w] & mb2;
+      clearbit<W>(P.cb[sl], pw, pb);
+      if (c) P.rhsb[sl] ^= __ballot(hb);
+      if (CONS == CONS_B) {
+#pragma unroll
+        for (int v = 0; v < VW; ++v) V.rb[sl][v] ^= hb ? E[v] : 0ull;
       }
     }
   }
   clearbit<W>(P.rem, pw, pb);
 }
 
-template <int W, int VW>
-__device__ __forceinline__ void take_constraint_v(PairRegs<W>& P, VarRegs<W, VW>& V, int useb, int c) {
+template <int W, int VW, bool USEB>
+__device__ __forceinline__ void take_constraint_v(PairRegs<W>& P, VarRegs<W, VW>& V, int c) {
   u64 row[W];
-  bcast<W>(useb ? P.cb : P.ca, c, row);
-  const int rhs = useb ? getbit<W>(P.rhsb, c) : getbit<W>(P.rhsa, c);
+  bcast<W>(USEB ? P.cb : P.ca, c, row);
+  const int rhs = USEB ? getbit<W>(P.rhsb, c) : getbit<W>(P.rhsa, c);
   u64 E[VW];
-  bcast_v<W, VW>(useb ? V.rb : V.ra, c, E);
+  if (USEB) {
+    bcast_v<W, VW>(V.rb, c, E);
+  } else {
+#pragma unroll
+    for (int v = 0; v < VW; ++v) E[v] = 0ull;
+  }
   if (!anyv<W>(row)) {
     // 0 = rhs_k: the sum vanishes for the variants with rhs_k = 1
+    if (rhs) V.bz = 1;
 #pragma unroll
     for (int v = 0; v < VW; ++v) V.z[v] |= rhs ? ~E[v] : E[v];
     return;
   }
   const int p = ffsv<W>(row);
   flipbit<W>(row, p, 1);
-  subst_v<W, VW, true>(P, V, p, row, rhs, E);
+  subst_v<W, VW, USEB ? CONS_B : CONS_AB>(P, V, p, row, rhs, E);
 }
 
-// pair_eval of pair_core.hpp for every variant.  Base phase changes that
-// are the same for all variants go to P.c0, the rest to the planes.
+// pair_eval of pair_core.hpp for every variant and for the base (the ket
+// itself).  P.c0 follows the base; the planes hold each variant's offset.
 template <int W, int VW>
 __device__ __forceinline__ void pair_eval_v(PairRegs<W>& P, VarRegs<W, VW>& V, int nca, int ncb) {
   for (int c = 0; c < nca; ++c) {
-    take_constraint_v<W, VW>(P, V, 0, c);
+    take_constraint_v<W, VW, false>(P, V, c);
     if (all_zero<W, VW>(V)) return;
   }
   for (int c = 0; c < ncb; ++c) {
-    take_constraint_v<W, VW>(P, V, 1, c);
+    take_constraint_v<W, VW, true>(P, V, c);
     if (all_zero<W, VW>(V)) return;
   }
   while (anyv<W>(P.rem)) {
@@ -166,30 +179,31 @@ __device__ __forceinline__ void pair_eval_v(PairRegs<W>& P, VarRegs<W, VW>& V, i
     u64 Bp[W];
     bcast<W>(P.B, p, Bp);
     const int ap = bcast_int<W>(P.a, p);
-    u64 H[VW], m[VW];
+    u64 H[VW];
     bcast_v<W, VW>(V.hi, p, H);
-#pragma unroll
-    for (int v = 0; v < VW; ++v) m[v] = (ap & 2) ? ~H[v] : H[v];  // variants with a_p,k in {2, 3}
     if (!anyv<W>(Bp)) {
       if (!(ap & 1)) {
         // a_p,k = 2: 1 + i^2 = 0; a_p,k = 0: factor 2
+        if (ap == 2) V.bz = 1;
 #pragma unroll
-        for (int v = 0; v < VW; ++v) V.z[v] |= m[v];
+        for (int v = 0; v < VW; ++v) V.z[v] |= (ap & 2) ? ~H[v] : H[v];
         if (all_zero<W, VW>(V)) return;
         P.mag -= 2;
       } else {
-        // 1 + i^{a}: sqrt2 zeta (a = 1) or sqrt2 zeta^7 (a = 3)
+        // 1 + i^a: sqrt2 zeta (a = 1) or sqrt2 zeta^7 (a = 3); a variant
+        // whose a_p differs from the base by 2 gets the other one: offset
+        // 7 - 1 = 6 (base a_p = 1) or 1 - 7 = 2 (base a_p = 3)
         P.mag -= 1;
-        P.c0 = (P.c0 + 1) & 7;
+        P.c0 = (P.c0 + (ap == 1 ? 1 : 7)) & 7;
 #pragma unroll
-        for (int v = 0; v < VW; ++v) z8_add(V.d0[v], V.d1[v], V.d2[v], 0ull, m[v], m[v]);
+        for (int v = 0; v < VW; ++v) z8_add(V.d0[v], V.d1[v], V.d2[v], 0ull, H[v], ap == 1 ? H[v] : 0ull);
       }
     } else if (ap & 1) {
       const int add = (ap == 1) ? 3 : 1;
       P.mag -= 1;
-      P.c0 = (P.c0 + 1) & 7;
+      P.c0 = (P.c0 + (ap == 1 ? 1 : 7)) & 7;
 #pragma unroll
-      for (int v = 0; v < VW; ++v) z8_add(V.d0[v], V.d1[v], V.d2[v], 0ull, m[v], m[v]);
+      for (int v = 0; v < VW; ++v) z8_add(V.d0[v], V.d1[v], V.d2[v], 0ull, H[v], ap == 1 ? H[v] : 0ull);
 #pragma unroll
       for (int sl = 0; sl < W; ++sl) {
         const int bpl = (int)((wsel<W>(P.B[sl], pw) >> pb) & 1ull);
@@ -210,7 +224,7 @@ __device__ __forceinline__ void pair_eval_v(PairRegs<W>& P, VarRegs<W, VW>& V, i
       for (int sl = 0; sl < W; ++sl) clearbit<W>(P.B[sl], pw, pb);
       const int qv = ffsv<W>(Bp);
       flipbit<W>(Bp, qv, 1);
-      subst_v<W, VW, false>(P, V, qv, Bp, ap >> 1, H);
+      subst_v<W, VW, CONS_NONE>(P, V, qv, Bp, ap >> 1, H);
     }
   }
 }
@@ -339,14 +353,16 @@ __global__ __launch_bounds__(256) void k_pauli_var(FormsDev in, const int32_t* _
 // ---------------------------------------------------------------------------
 // Hermitian pair sums over ket variants.  Each wave belongs to one job
 // (wave_job) and walks that job's units with the job's wave stride; lane k
-// accumulates variant 64*v + k in acc[v]:
+// accumulates variant 64*v + k in acc[v], every lane the base in accb:
 //   sum_f sum_{o >= f} (o == f ? 1 : 2) Re(conj(c_o) c_f <X_o | Y_f^(k)>)
-// which is exp_value.rs:38-58 for Y_f^(k) = P_k phi_f.
+// which is exp_value.rs:38-58 for Y_f^(k) = P_k phi_f (norm.rs:16-27 for
+// the base).  Per wave, partial holds 64*VW variant sums then the base.
 template <int W, int VW>
 __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict__ jobs,
                                                    const int32_t* __restrict__ wave_job, int64_t wave_base,
                                                    int64_t nwaves_launch, int J, const double* __restrict__ tabs,
                                                    double* __restrict__ partial) {
+  constexpr int NVS = 64 * VW + 1;
   const int lane = threadIdx.x & 63;
   const int64_t lw = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (lw >= nwaves_launch) return;
@@ -358,6 +374,7 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
   const int64_t nfixed = jd.nfixed, nother = jd.nother;
   const int64_t nunits = jd.row_off[nfixed];
   const int n = X.n, R = 64 * W;
+  const int want_base = jd.base_out >= 0;
   u64 valid[W];
 #pragma unroll
   for (int w = 0; w < W; ++w) {
@@ -373,7 +390,7 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
   double wk[VW];
 #pragma unroll
   for (int v = 0; v < VW; ++v) wk[v] = (jd.gw && 64 * v + lane < jd.nvar) ? jd.wts[64 * v + lane] : 0.0;
-  double acc[VW];
+  double acc[VW], accb = 0.0;
 #pragma unroll
   for (int v = 0; v < VW; ++v) acc[v] = 0.0;
   for (int64_t u = wave - jd.wave0; u < nunits; u += jd.nwaves) {
@@ -387,20 +404,8 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
     const int64_t base = jd.triangle ? fx : 0;
     const int64_t o_lo = base + (u - jd.row_off[fx]) * J;
     const int64_t o_hi = (o_lo + J < nother) ? o_lo + J : nother;
-    u64 Bf[W][W], Cf[W][W], rf[W];
-    int af[W];
-    load_qrows<W>(Y, fx, lane, Bf, af, Cf, rf);
     const int ncf = uni(Y.nc[fx]), kf = uni(Y.k[fx]), pf = uni(Y.p0[fx]);
-    u64 vh[W][VW], vr[W][VW], vp0[VW], vp1[VW], vp2[VW];
-#pragma unroll
-    for (int sl = 0; sl < W; ++sl) {
-      const size_t r = (size_t)fx * R + 64 * sl + lane;
-#pragma unroll
-      for (int v = 0; v < VW; ++v) {
-        vh[sl][v] = jd.V.hi[r * VW + v];
-        vr[sl][v] = jd.V.rhs[r * VW + v];
-      }
-    }
+    u64 vp0[VW], vp1[VW], vp2[VW];
 #pragma unroll
     for (int v = 0; v < VW; ++v) {
       vp0[v] = jd.V.ph[((size_t)fx * 3 + 0) * VW + v];
@@ -408,6 +413,10 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
       vp2[v] = jd.V.ph[((size_t)fx * 3 + 2) * VW + v];
     }
     for (int64_t o = o_lo; o < o_hi; ++o) {
+      // ket rows are re-read per pair (cache hits) to keep registers free
+      u64 Bf[W][W], Cf[W][W], rf[W];
+      int af[W];
+      load_qrows<W>(Y, fx, lane, Bf, af, Cf, rf);
       u64 Bo[W][W], Co[W][W], ro[W];
       int ao[W];
       load_qrows<W>(X, o, lane, Bo, ao, Co, ro);
@@ -424,11 +433,11 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
           P.cb[sl][w] = Cf[sl][w];
         }
         P.a[sl] = (af[sl] - ao[sl]) & 3;
+        const size_t r = (size_t)fx * R + 64 * sl + lane;
 #pragma unroll
         for (int v = 0; v < VW; ++v) {
-          V.hi[sl][v] = vh[sl][v];
-          V.rb[sl][v] = vr[sl][v];
-          V.ra[sl][v] = 0ull;
+          V.hi[sl][v] = jd.V.hi[r * VW + v];
+          V.rb[sl][v] = jd.V.rhs[r * VW + v];
         }
       }
 #pragma unroll
@@ -444,6 +453,7 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
         V.d2[v] = vp2[v];
         V.z[v] = invalid[v];
       }
+      V.bz = !want_base;
       P.c0 = (pf - po) & 7;
       P.mag = kf + ko;
       pair_eval_v<W, VW>(P, V, nco, ncf);
@@ -468,6 +478,13 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
         ci_ = br * k0.y + bi * k0.x;
       }
       const bool diag = (o == fx);
+      const double nrm = tabs[TAB_POW + P.mag];
+      if (!V.bz) {
+        const int K = P.c0;
+        const double ir = tabs[TAB_COS + K] * nrm, ii = tabs[TAB_SIN + K] * nrm;
+        const double tr = cr_ * ir - ci_ * ii;
+        accb = diag ? accb + tr : accb + (tr + tr);
+      }
       double gr = 0.0, gi = 0.0;
 #pragma unroll
       for (int v = 0; v < VW; ++v) {
@@ -476,7 +493,6 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
           const int D = (int)((V.d0[v] >> lane) & 1ull) | ((int)((V.d1[v] >> lane) & 1ull) << 1) |
                         ((int)((V.d2[v] >> lane) & 1ull) << 2);
           const int K = (P.c0 + D) & 7;
-          const double nrm = tabs[TAB_POW + P.mag];
           const double ir = tabs[TAB_COS + K] * nrm, ii = tabs[TAB_SIN + K] * nrm;
           const double tr = cr_ * ir - ci_ * ii;
           acc[v] = diag ? acc[v] + tr : acc[v] + (tr + tr);
@@ -505,27 +521,30 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
     }
   }
 #pragma unroll
-  for (int v = 0; v < VW; ++v) partial[(size_t)wave * (64 * VW) + 64 * v + lane] = acc[v];
+  for (int v = 0; v < VW; ++v) partial[(size_t)wave * NVS + 64 * v + lane] = acc[v];
+  if (lane == 0) partial[(size_t)wave * NVS + 64 * VW] = accb;
 }
 
-// one block per job: thread i sums variant i % NV over the job's waves
-// w = wave0 + part, part + P, ... in order (P = 256 / NV parts), then the
-// parts are added in order
+// one block per job: thread i sums value i % NVS (64*VW variants, then the
+// base) over the job's waves w = wave0 + part, part + NP, ... in order
+// (NP = 256 / NVS parts), then the parts are added in order
 template <int VW>
 __global__ __launch_bounds__(256) void k_reduce_var(const PairJobDev* __restrict__ jobs,
                                                     const double* __restrict__ partial, double* __restrict__ out) {
-  constexpr int NV = 64 * VW, NP = 256 / NV;
+  constexpr int NVS = 64 * VW + 1, NP = 256 / NVS;
   __shared__ double sh[256];
   const PairJobDev& jd = jobs[blockIdx.x];
-  const int v = threadIdx.x % NV, part = threadIdx.x / NV;
+  const int v = threadIdx.x % NVS, part = threadIdx.x / NVS;
   double s = 0.0;
-  for (int64_t w = part; w < jd.nwaves; w += NP) s = s + partial[(size_t)(jd.wave0 + w) * NV + v];
+  if (part < NP)
+    for (int64_t w = part; w < jd.nwaves; w += NP) s = s + partial[(size_t)(jd.wave0 + w) * NVS + v];
   sh[threadIdx.x] = s;
   __syncthreads();
-  if (part == 0 && v < jd.nvar) {
+  if (part == 0) {
     double t = sh[v];
-    for (int k = 1; k < NP; ++k) t = t + sh[k * NV + v];
-    out[jd.out_base + v] = t;
+    for (int k = 1; k < NP; ++k) t = t + sh[k * NVS + v];
+    if (v < jd.nvar) out[jd.out_base + v] = t;
+    if (v == 64 * VW && jd.base_out >= 0) out[jd.base_out] = t;
   }
 }
 
