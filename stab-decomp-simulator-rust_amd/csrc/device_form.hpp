@@ -31,6 +31,12 @@ __device__ __forceinline__ u64 rl64(u64 x, int lane) {
 }
 __device__ __forceinline__ int rl32(int x, int lane) { return __builtin_amdgcn_readlane(x, lane); }
 __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+// wave-uniform copy (scalar registers) of a value every lane holds equally
+__device__ __forceinline__ u64 uni64(u64 x) {
+  const u32 lo = (u32)__builtin_amdgcn_readfirstlane((int)(u32)x);
+  const u32 hi = (u32)__builtin_amdgcn_readfirstlane((int)(u32)(x >> 32));
+  return ((u64)hi << 32) | lo;
+}
 __device__ __forceinline__ int par64(u64 x) { return __builtin_popcountll(x) & 1; }
 __device__ __forceinline__ int ffs64(u64 x) { return __builtin_ctzll(x); }
 
@@ -46,17 +52,23 @@ template <int W>
 __device__ __forceinline__ int getbit(const u64 (&x)[W], int c) {
   return (int)((wsel<W>(x, c >> 6) >> (c & 63)) & 1ull);
 }
+// The word updates below touch every word with a mask that is zero except
+// in word c/64: a conditional store x[k] would be folded by the compiler into
+// a store at the run-time index c/64, which sends the array to scratch.
 template <int W>
 __device__ __forceinline__ void flipbit(u64 (&x)[W], int c, int b) {
+  const u64 m = ((u64)(b & 1)) << (c & 63);
 #pragma unroll
-  for (int k = 0; k < W; ++k)
-    if ((c >> 6) == k) x[k] ^= ((u64)(b & 1)) << (c & 63);
+  for (int k = 0; k < W; ++k) x[k] ^= ((c >> 6) == k) ? m : 0ull;
 }
 template <int W>
 __device__ __forceinline__ void setbitv(u64 (&x)[W], int c, int b) {
+  const u64 clr = ~(1ull << (c & 63)), set = ((u64)(b & 1)) << (c & 63);
 #pragma unroll
-  for (int k = 0; k < W; ++k)
-    if ((c >> 6) == k) x[k] = (x[k] & ~(1ull << (c & 63))) | (((u64)(b & 1)) << (c & 63));
+  for (int k = 0; k < W; ++k) {
+    const bool here = (c >> 6) == k;
+    x[k] = (x[k] & (here ? clr : ~0ull)) | (here ? set : 0ull);
+  }
 }
 template <int W>
 __device__ __forceinline__ bool anyv(const u64 (&x)[W]) {

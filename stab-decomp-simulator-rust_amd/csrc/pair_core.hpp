@@ -77,9 +77,9 @@ struct PairRegs {
 
 template <int W>
 __device__ __forceinline__ void clearbit(u64 (&x)[W], int pw, int pb) {
+  const u64 m = ~(1ull << pb);
 #pragma unroll
-  for (int w = 0; w < W; ++w)
-    if (w == pw) x[w] &= ~(1ull << pb);
+  for (int w = 0; w < W; ++w) x[w] &= (w == pw) ? m : ~0ull;
 }
 
 template <int W>
@@ -159,13 +159,33 @@ __device__ __forceinline__ int take_constraint(PairRegs<W>& P, int c) {
   return 0;
 }
 
+// keeps the wave-uniform elimination state in scalar registers (the
+// compiler otherwise carries some of it in vector registers through the
+// loops below)
+template <int W>
+__device__ __forceinline__ void uniform_state(PairRegs<W>& P) {
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    P.rem[w] = uni64(P.rem[w]);
+    P.rhsa[w] = uni64(P.rhsa[w]);
+    P.rhsb[w] = uni64(P.rhsb[w]);
+  }
+  P.c0 = uni(P.c0);
+  P.mag = uni(P.mag);
+}
+
 // exact <bra|ket>: returns zero flag, K (mod 8), r
 template <int W>
 __device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int* zero, int* K, int* R) {
-  for (int c = 0; c < nca; ++c)
-    if (take_constraint<W, false>(P, c)) { *zero = 1; return; }
-  for (int c = 0; c < ncb; ++c)
-    if (take_constraint<W, true>(P, c)) { *zero = 1; return; }
+  uniform_state<W>(P);
+  for (int c = 0; c < nca; ++c) {
+    if (uni(take_constraint<W, false>(P, c))) { *zero = 1; return; }
+    uniform_state<W>(P);
+  }
+  for (int c = 0; c < ncb; ++c) {
+    if (uni(take_constraint<W, true>(P, c))) { *zero = 1; return; }
+    uniform_state<W>(P);
+  }
   while (anyv<W>(P.rem)) {
     const int p = uni(ffsv<W>(P.rem));
     const int pw = p >> 6, pb = p & 63;
@@ -203,6 +223,7 @@ __device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int*
       flipbit<W>(Bp, qv, 1);
       subst<W, CONS_NONE>(P, qv, Bp, ap >> 1);
     }
+    uniform_state<W>(P);
   }
   *zero = 0;
   *K = P.c0;
@@ -224,7 +245,7 @@ __device__ __forceinline__ void load_qrows(const QFormDev& q, int64_t t, int lan
     a[sl] = q.a[(size_t)t * R + r];
   }
 #pragma unroll
-  for (int w = 0; w < W; ++w) rhs[w] = q.rhs[(size_t)t * W + w];
+  for (int w = 0; w < W; ++w) rhs[w] = uni64(q.rhs[(size_t)t * W + w]);
 }
 
 }  // namespace nx

@@ -60,7 +60,28 @@ __device__ __forceinline__ bool all_zero(const VarRegs<W, VW>& V) {
   u64 a = V.bz ? ~0ull : 0ull;
 #pragma unroll
   for (int v = 0; v < VW; ++v) a &= V.z[v];
-  return a == ~0ull;
+  return uni(a == ~0ull) != 0;
+}
+
+// keeps the wave-uniform elimination state in scalar registers
+template <int W, int VW>
+__device__ __forceinline__ void uniform_state(PairRegs<W>& P, VarRegs<W, VW>& V) {
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    P.rem[w] = uni64(P.rem[w]);
+    P.rhsa[w] = uni64(P.rhsa[w]);
+    P.rhsb[w] = uni64(P.rhsb[w]);
+  }
+  P.c0 = uni(P.c0);
+  P.mag = uni(P.mag);
+  V.bz = uni(V.bz);
+#pragma unroll
+  for (int v = 0; v < VW; ++v) {
+    V.z[v] = uni64(V.z[v]);
+    V.d0[v] = uni64(V.d0[v]);
+    V.d1[v] = uni64(V.d1[v]);
+    V.d2[v] = uni64(V.d2[v]);
+  }
 }
 
 // subst<W, CONS> of pair_core.hpp for every variant: x_p = c_k ^ <r, x>
@@ -164,14 +185,18 @@ __device__ __forceinline__ void take_constraint_v(PairRegs<W>& P, VarRegs<W, VW>
 // itself).  P.c0 follows the base; the planes hold each variant's offset.
 template <int W, int VW>
 __device__ __forceinline__ void pair_eval_v(PairRegs<W>& P, VarRegs<W, VW>& V, int nca, int ncb) {
+  uniform_state<W, VW>(P, V);
   for (int c = 0; c < nca; ++c) {
     take_constraint_v<W, VW, false>(P, V, c);
+    uniform_state<W, VW>(P, V);
     if (all_zero<W, VW>(V)) return;
   }
   for (int c = 0; c < ncb; ++c) {
     take_constraint_v<W, VW, true>(P, V, c);
+    uniform_state<W, VW>(P, V);
     if (all_zero<W, VW>(V)) return;
   }
+  uniform_state<W, VW>(P, V);
   while (anyv<W>(P.rem)) {
     const int p = uni(ffsv<W>(P.rem));
     const int pw = p >> 6, pb = p & 63;
@@ -226,6 +251,7 @@ __device__ __forceinline__ void pair_eval_v(PairRegs<W>& P, VarRegs<W, VW>& V, i
       flipbit<W>(Bp, qv, 1);
       subst_v<W, VW, CONS_NONE>(P, V, qv, Bp, ap >> 1, H);
     }
+    uniform_state<W, VW>(P, V);
   }
 }
 
@@ -408,9 +434,9 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
     u64 vp0[VW], vp1[VW], vp2[VW];
 #pragma unroll
     for (int v = 0; v < VW; ++v) {
-      vp0[v] = jd.V.ph[((size_t)fx * 3 + 0) * VW + v];
-      vp1[v] = jd.V.ph[((size_t)fx * 3 + 1) * VW + v];
-      vp2[v] = jd.V.ph[((size_t)fx * 3 + 2) * VW + v];
+      vp0[v] = uni64(jd.V.ph[((size_t)fx * 3 + 0) * VW + v]);
+      vp1[v] = uni64(jd.V.ph[((size_t)fx * 3 + 1) * VW + v]);
+      vp2[v] = uni64(jd.V.ph[((size_t)fx * 3 + 2) * VW + v]);
     }
     for (int64_t o = o_lo; o < o_hi; ++o) {
       // ket rows are re-read per pair (cache hits) to keep registers free
