@@ -38,6 +38,10 @@ __device__ __forceinline__ u64 uni64(u64 x) {
   return ((u64)hi << 32) | lo;
 }
 __device__ __forceinline__ int par64(u64 x) { return __builtin_popcountll(x) & 1; }
+__device__ __forceinline__ u32 lo32(u64 x) { return (u32)x; }
+__device__ __forceinline__ u32 hi32(u64 x) { return (u32)(x >> 32); }
+// a ^ b ^ c in one gfx950 three-input bitwise operation (truth table 0x96)
+__device__ __forceinline__ u32 xor3(u32 a, u32 b, u32 c) { return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96); }
 __device__ __forceinline__ int ffs64(u64 x) { return __builtin_ctzll(x); }
 
 template <int W>

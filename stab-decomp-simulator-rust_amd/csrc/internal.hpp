@@ -44,6 +44,7 @@ struct FormsDev {
 // Quadratic-form rewrite of each term (see k_qform).
 struct QFormDev {
   uint64_t *Q, *B, *C;       // [chi][R][W]
+  uint64_t* CT;              // [chi][R][W] C transposed: row j = constraints containing x_j
   uint8_t* a;                // [chi][R]
   uint64_t *alo, *ahi, *rhs; // [chi][W]
   int32_t *nc, *k;           // [chi]
@@ -109,6 +110,8 @@ hipError_t launch_program(const FormsDev& in, const FormsDev& out, const ProgOp*
 hipError_t launch_kron(const FormsDev& A, const FormsDev& B, const FormsDev& O, hipStream_t st);
 hipError_t launch_gather(const FormsDev& in, const FormsDev& O, const int64_t* idx, hipStream_t st);
 hipError_t launch_qform(const FormsDev& in, const QFormDev& q, hipStream_t st);
+// bitstrings per thread of the amplitude kernel (a block holds 256 threads)
+inline int amp_strings_per_thread(int W) { return W <= 2 ? 4 : 2; }
 // nchunks > 1: partial must hold nchunks * nb values
 hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const uint64_t* bits, int64_t nb,
                             int64_t chunk_terms, int nchunks, const double* tabs, double2* out, double2* partial,

@@ -253,6 +253,21 @@ __global__ __launch_bounds__(256) void k_qform(FormsDev in, QFormDev q) {
     if (getbit<W>(f.s, k)) flipbit<W>(rhs, nc, 1);
     ++nc;
   }
+  // C transposed: row j holds the constraints (bits over c) containing x_j
+  u64 CT[W][W];
+#pragma unroll
+  for (int sl = 0; sl < W; ++sl)
+#pragma unroll
+    for (int w = 0; w < W; ++w) CT[sl][w] = 0ull;
+  for (int j = 0; j < n; ++j) {
+#pragma unroll
+    for (int sw = 0; sw < W; ++sw) {
+      const u64 col = __ballot(getbit<W>(C[sw], j));
+#pragma unroll
+      for (int sl = 0; sl < W; ++sl)
+        if (owns<W>(f, j, sl)) CT[sl][sw] = col;
+    }
+  }
   // store
 #pragma unroll
   for (int sl = 0; sl < W; ++sl) {
@@ -262,6 +277,7 @@ __global__ __launch_bounds__(256) void k_qform(FormsDev in, QFormDev q) {
       q.Q[rowidx(t, R, r, W, w)] = Q[sl][w];
       q.B[rowidx(t, R, r, W, w)] = Bm[sl][w];
       q.C[rowidx(t, R, r, W, w)] = C[sl][w];
+      q.CT[rowidx(t, R, r, W, w)] = CT[sl][w];
     }
     q.a[(size_t)t * R + r] = (uint8_t)a[sl];
   }
@@ -286,28 +302,91 @@ __global__ __launch_bounds__(256) void k_qform(FormsDev in, QFormDev q) {
 
 // ---------------------------------------------------------------------------
 // Amplitudes.  Each thread evaluates XB bitstrings; blockIdx.y selects a
-// chunk of terms [t0, t1).  Per term and string: the support test (C x =
-// rhs), the linear part a.x mod 4 from two bit planes, and the quadratic
-// part sum_{l<m} Q_lm x_l x_m = par(x & XOR_{l in x} Q_l) (Q_l holds only
-// bits m > l).  The FP64 arithmetic per term is the reference's
-// to_statevector arithmetic (Scalar::to_complex, * omega, * coefficient,
-// +=) in term order within the chunk; chunks are summed in chunk order by
-// k_amp_reduce, which also applies the global factor.
+// chunk of terms [t0, t1).  Per term and string x: the support test
+// C x = rhs, the linear part a.x mod 4 from two bit planes, and the
+// quadratic part sum_{l<m} Q_lm x_l x_m = par(x & XOR_{l in x} Q_l) (Q_l
+// holds only bits m > l).  Both XOR_{l in x} Q_l and C x = XOR_{l in x} CT_l
+// are read from per-term nibble tables in LDS: for nibble position b
+// (variables 4b .. 4b+3) and value v,
+//   T[b][v] = (XOR_{k : v_k = 1} Q_{4b+k}, XOR_{k : v_k = 1} CT_{4b+k}),
+// so a string costs one lookup per nibble.  The block stages the next
+// term's rows while it evaluates the current one (two table buffers).  The
+// FP64 arithmetic per term is the reference's to_statevector arithmetic
+// (Scalar::to_complex, * omega, * coefficient, +=) in term order within
+// the chunk; chunks are summed in chunk order by k_amp_reduce, which also
+// applies the global factor.
+template <int W>
+struct AmpLds {
+  static constexpr int EW = 2 * W;           // words per table entry (Q part, CT part)
+  static constexpr int NB = 16 * W;          // nibble positions
+  static constexpr int RAW = 64 * W * EW;    // raw rows of one term
+  static constexpr int TAB = NB * 16 * EW;   // one table
+  static constexpr size_t BYTES = (size_t)(RAW + 2 * TAB) * 8;
+};
+
+template <int W>
+__device__ __forceinline__ void amp_load_raw(const QFormDev& q, int64_t t, u64 (&pre)[(AmpLds<W>::RAW + 255) / 256]) {
+  constexpr int EW = AmpLds<W>::EW, RAW = AmpLds<W>::RAW;
+  const int R = 64 * W;
+#pragma unroll
+  for (int i = 0; i < (RAW + 255) / 256; ++i) {
+    const int e = threadIdx.x + 256 * i;
+    pre[i] = 0ull;
+    if (e < RAW) {
+      const int row = e / EW, ww = e % EW;
+      pre[i] = ww < W ? q.Q[rowidx(t, R, row, W, ww)] : q.CT[rowidx(t, R, row, W, ww - W)];
+    }
+  }
+}
+
+template <int W>
+__device__ __forceinline__ void amp_store_raw(u64* raw, const u64 (&pre)[(AmpLds<W>::RAW + 255) / 256]) {
+  constexpr int RAW = AmpLds<W>::RAW;
+#pragma unroll
+  for (int i = 0; i < (RAW + 255) / 256; ++i) {
+    const int e = threadIdx.x + 256 * i;
+    if (e < RAW) raw[e] = pre[i];
+  }
+}
+
+template <int W>
+__device__ __forceinline__ void amp_build(const u64* raw, u64* tab) {
+  constexpr int EW = AmpLds<W>::EW, NE = AmpLds<W>::NB * 16;
+#pragma unroll
+  for (int i = 0; i < (NE + 255) / 256; ++i) {
+    const int e = threadIdx.x + 256 * i;
+    if (e < NE) {
+      const int b = e >> 4, v = e & 15;
+#pragma unroll
+      for (int ww = 0; ww < EW; ++ww) {
+        u64 val = 0ull;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if ((v >> k) & 1) val ^= raw[(4 * b + k) * EW + ww];
+        tab[e * EW + ww] = val;
+      }
+    }
+  }
+}
+
 template <int W, int XB>
 __global__ __launch_bounds__(256) void k_amplitude(QFormDev q, const double2* __restrict__ coef,
                                                    const u64* __restrict__ bits, int64_t nb, int64_t chunk_terms,
                                                    const double* __restrict__ tabs, double2* __restrict__ out,
                                                    uint8_t* __restrict__ per_term, int enumerate, int final_g,
                                                    double gre, double gim) {
+  using L = AmpLds<W>;
+  constexpr int NB = L::NB, NPRE = (L::RAW + 255) / 256;
+  extern __shared__ u64 amp_lds[];
+  u64* raw = amp_lds;
+  u64* tab_base = amp_lds + L::RAW;  // two tables of L::TAB words
   __shared__ double s_cos[8], s_sin[8];
   if (threadIdx.x < 8) {
     s_cos[threadIdx.x] = tabs[TAB_COS + threadIdx.x];
     s_sin[threadIdx.x] = tabs[TAB_SIN + threadIdx.x];
   }
-  __syncthreads();
   const int64_t t0 = (int64_t)blockIdx.y * chunk_terms;
   const int64_t t1 = (t0 + chunk_terms < q.chi) ? t0 + chunk_terms : q.chi;
-  const int n = q.n, R = 64 * W;
   int64_t bidx[XB];
   bool active[XB];
   u64 x[XB][W];
@@ -324,67 +403,76 @@ __global__ __launch_bounds__(256) void k_amplitude(QFormDev q, const double2* __
   double are[XB], aim[XB];
 #pragma unroll
   for (int j = 0; j < XB; ++j) { are[j] = 0.0; aim[j] = 0.0; }
+  u64 pre[NPRE];
+  if (t0 < t1) {
+    amp_load_raw<W>(q, t0, pre);
+    amp_store_raw<W>(raw, pre);
+  }
+  __syncthreads();
+  if (t0 < t1) amp_build<W>(raw, tab_base);
+  __syncthreads();
   for (int64_t t = t0; t < t1; ++t) {
-    const int nc = uni(q.nc[t]);
+    const bool more = t + 1 < t1;
+    if (more) amp_load_raw<W>(q, t + 1, pre);  // in flight during the evaluation below
+    const u64* tab = tab_base + ((t - t0) & 1) * L::TAB;
     const int k = uni(q.k[t]);
     const int p0 = uni(q.p0[t]);
-    int bad[XB];
+    u64 alo[W], ahi[W], rhs[W];
 #pragma unroll
-    for (int j = 0; j < XB; ++j) bad[j] = 0;
-    const u64* Cr = q.C + (size_t)t * R * W;
-    for (int c = 0; c < nc; ++c) {
-      const int rhs = (int)((q.rhs[(size_t)t * W + (c >> 6)] >> (c & 63)) & 1ull);
-#pragma unroll
-      for (int j = 0; j < XB; ++j) {
-        int p = rhs;
-#pragma unroll
-        for (int w = 0; w < W; ++w) p += __builtin_popcountll(Cr[(size_t)c * W + w] & x[j][w]);
-        bad[j] |= p & 1;
-      }
-    }
-    u64 acc[XB][W];
-#pragma unroll
-    for (int j = 0; j < XB; ++j)
-#pragma unroll
-      for (int w = 0; w < W; ++w) acc[j][w] = 0ull;
-    const u64* Qr = q.Q + (size_t)t * R * W;
-#pragma unroll
-    for (int wl = 0; wl < W; ++wl) {
-      const int lmax = (n - 64 * wl) < 64 ? (n - 64 * wl) : 64;
-      for (int lb = 0; lb < lmax; ++lb) {
-        u64 Ql[W];
-#pragma unroll
-        for (int w = 0; w < W; ++w) Ql[w] = Qr[(size_t)(64 * wl + lb) * W + w];
-#pragma unroll
-        for (int j = 0; j < XB; ++j) {
-          const u64 m = 0ull - ((x[j][wl] >> lb) & 1ull);
-#pragma unroll
-          for (int w = 0; w < W; ++w) acc[j][w] ^= Ql[w] & m;
-        }
-      }
+    for (int w = 0; w < W; ++w) {
+      alo[w] = uni64(q.alo[(size_t)t * W + w]);
+      ahi[w] = uni64(q.ahi[(size_t)t * W + w]);
+      rhs[w] = uni64(q.rhs[(size_t)t * W + w]);
     }
     const double nrm = tabs[TAB_POW + k];
     const double2 c = coef[t];
+    const uint4* tab4 = (const uint4*)tab;  // entry = W uint4: u64 words Q[0..W) then CT[0..W)
 #pragma unroll
     for (int j = 0; j < XB; ++j) {
-      int ax = 0;
+      u64 acc[W], y[W];
 #pragma unroll
-      for (int w = 0; w < W; ++w)
-        ax += __builtin_popcountll(x[j][w] & q.alo[(size_t)t * W + w]) +
-              2 * __builtin_popcountll(x[j][w] & q.ahi[(size_t)t * W + w]);
-      const int quad = parand<W>(acc[j], x[j]);
-      const int ph = (p0 + 2 * ax + 4 * quad) & 7;
-      if (per_term && active[j]) per_term[(size_t)t * nb + bidx[j]] = bad[j] ? (uint8_t)0xFF : (uint8_t)ph;
-      double zr = 0.0, zi = 0.0;
-      if (!bad[j]) {
-        zr = s_cos[ph] * nrm;
-        zi = s_sin[ph] * nrm;
+      for (int w = 0; w < W; ++w) { acc[w] = 0ull; y[w] = 0ull; }
+      // all 16 W nibbles (those above n are 0 and select the zero entry);
+      // pairs of lookups are combined before they are added in
+#pragma unroll
+      for (int b = 0; b < NB; b += 2) {
+        const int v0 = (int)((x[j][b >> 4] >> (4 * (b & 15))) & 15ull);
+        const int v1 = (int)((x[j][(b + 1) >> 4] >> (4 * ((b + 1) & 15))) & 15ull);
+        const uint4* e0 = tab4 + (b * 16 + v0) * W;
+        const uint4* e1 = tab4 + ((b + 1) * 16 + v1) * W;
+#pragma unroll
+        for (int h = 0; h < W; ++h) {
+          // uint4 h of an entry holds u64 words 2h, 2h + 1 of (Q[0..W), CT[0..W))
+          const uint4 a0 = e0[h], a1 = e1[h];
+          const int wlo = 2 * h, whi = 2 * h + 1;
+          u64& dlo = wlo < W ? acc[wlo < W ? wlo : 0] : y[(wlo - W) < W ? wlo - W : 0];
+          u64& dhi = whi < W ? acc[whi < W ? whi : 0] : y[(whi - W) < W ? whi - W : 0];
+          dlo = ((u64)xor3(hi32(dlo), a0.y, a1.y) << 32) | xor3(lo32(dlo), a0.x, a1.x);
+          dhi = ((u64)xor3(hi32(dhi), a0.w, a1.w) << 32) | xor3(lo32(dhi), a0.z, a1.z);
+        }
       }
-      const double wr = zr * 1.0 - zi * 0.0, wi = zr * 0.0 + zi * 1.0;
-      const double tr = wr * c.x - wi * c.y, ti = wr * c.y + wi * c.x;
+      int ax = 0, bad = 0;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        ax += __builtin_popcountll(x[j][w] & alo[w]) + 2 * __builtin_popcountll(x[j][w] & ahi[w]);
+        bad |= (y[w] != rhs[w]);
+      }
+      const int quad = parand<W>(acc, x[j]);
+      const int ph = (p0 + 2 * ax + 4 * quad) & 7;
+      if (per_term && active[j]) per_term[(size_t)t * nb + bidx[j]] = bad ? (uint8_t)0xFF : (uint8_t)ph;
+      // Scalar::to_complex; the reference then multiplies by omega = 1,
+      // which leaves every value reachable here unchanged (no table entry
+      // is +-0 except sin(0) = +0), so that product is not formed
+      const double zr = bad ? 0.0 : s_cos[ph] * nrm;
+      const double zi = bad ? 0.0 : s_sin[ph] * nrm;
+      const double tr = zr * c.x - zi * c.y, ti = zr * c.y + zi * c.x;
       are[j] = are[j] + tr;
       aim[j] = aim[j] + ti;
     }
+    if (more) amp_store_raw<W>(raw, pre);
+    __syncthreads();
+    if (more) amp_build<W>(raw, tab_base + ((t + 1 - t0) & 1) * L::TAB);
+    __syncthreads();
   }
 #pragma unroll
   for (int j = 0; j < XB; ++j) {
@@ -600,12 +688,29 @@ hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const u64* b
                             int nchunks, const double* tabs, double2* out, double2* partial, uint8_t* per_term,
                             int enumerate, double gre, double gim, hipStream_t st) {
   if (nb == 0) return hipSuccess;
-  constexpr int XB = 2;
-  const dim3 grid(blocks_for(nb, 256 * XB), nchunks);
   const int final_g = nchunks == 1;
   double2* dst = final_g ? out : partial;
-  NX_DISPATCH_W(q.W, k_amplitude<WW, XB><<<grid, dim3(256), 0, st>>>(q, coef, bits, nb, chunk_terms, tabs, dst,
-                                                                      per_term, enumerate, final_g, gre, gim));
+  const int xb = amp_strings_per_thread(q.W);
+  const dim3 grid(blocks_for(nb, 256 * xb), nchunks);
+#define NX_AMP_LAUNCH(WW, XBV)                                                                                   \
+  {                                                                                                               \
+    const size_t sh = AmpLds<WW>::BYTES;                                                                          \
+    if (sh > 64 * 1024) {                                                                                         \
+      hipError_t e = hipFuncSetAttribute((const void*)k_amplitude<WW, XBV>,                                      \
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);                    \
+      if (e != hipSuccess) return e;                                                                              \
+    }                                                                                                             \
+    k_amplitude<WW, XBV><<<grid, dim3(256), sh, st>>>(q, coef, bits, nb, chunk_terms, tabs, dst, per_term,        \
+                                                       enumerate, final_g, gre, gim);                             \
+  }
+  switch (q.W) {
+    case 1: NX_AMP_LAUNCH(1, 4) break;
+    case 2: NX_AMP_LAUNCH(2, 4) break;
+    case 3: NX_AMP_LAUNCH(3, 2) break;
+    case 4: NX_AMP_LAUNCH(4, 2) break;
+    default: return hipErrorInvalidValue;
+  }
+#undef NX_AMP_LAUNCH
   return hipGetLastError();
 }
 

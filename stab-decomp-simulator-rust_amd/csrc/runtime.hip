@@ -254,7 +254,7 @@ struct Forms {
 };
 
 struct QForms {
-  DevBuf Q, B, C, a, alo, ahi, rhs, nc, k, p0;
+  DevBuf Q, B, C, CT, a, alo, ahi, rhs, nc, k, p0;
   int64_t chi = 0;
   int n = 0, W = 1;
   void alloc(int64_t chi_, int n_) {
@@ -265,6 +265,7 @@ struct QForms {
     Q.alloc(chi * R * W * 8);
     B.alloc(chi * R * W * 8);
     C.alloc(chi * R * W * 8);
+    CT.alloc(chi * R * W * 8);
     a.alloc(chi * R);
     alo.alloc(chi * W * 8);
     ahi.alloc(chi * W * 8);
@@ -275,7 +276,8 @@ struct QForms {
   }
   QFormDev desc() const {
     QFormDev d;
-    d.Q = Q.as<uint64_t>(); d.B = B.as<uint64_t>(); d.C = C.as<uint64_t>(); d.a = a.as<uint8_t>();
+    d.Q = Q.as<uint64_t>(); d.B = B.as<uint64_t>(); d.C = C.as<uint64_t>(); d.CT = CT.as<uint64_t>();
+    d.a = a.as<uint8_t>();
     d.alo = alo.as<uint64_t>(); d.ahi = ahi.as<uint64_t>(); d.rhs = rhs.as<uint64_t>();
     d.nc = nc.as<int32_t>(); d.k = k.as<int32_t>(); d.p0 = p0.as<uint8_t>();
     d.chi = chi; d.n = n; d.W = W;
@@ -1316,7 +1318,8 @@ static void amplitudes_impl(const nx_state* s, int64_t nbits, const uint64_t* d_
   // enough waves to cover the GPU several times: split the term loop into
   // chunks when the strings alone give too few waves
   const int64_t chi = s->chi();
-  const int64_t waves_x = (nbits + 127) / 128;
+  const int64_t per_wave = 64 * amp_strings_per_thread(q.W);
+  const int64_t waves_x = (nbits + per_wave - 1) / per_wave;
   int64_t nchunks = waves_x >= 8192 ? 1 : (8192 + waves_x - 1) / waves_x;
   nchunks = std::min<int64_t>(std::min<int64_t>(nchunks, chi), 64);
   if (nchunks < 1) nchunks = 1;
