@@ -88,7 +88,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=16.0)
-    ap.add_argument("--config", type=int, default=1, choices=(1, 2, 3),
+    ap.add_argument("--config", type=int, default=1, choices=(1, 2, 3, 4),
                     help="1: amplitude sweep + norm (the metric); 2: batched vs plain norm + 64 Pauli "
                          "expectations over 256 circuits; 3: Pauli-loss forward vs backward")
     ap.add_argument("--circuits", type=int, default=0, help="cfg2: use only the first N circuits")
@@ -117,6 +117,8 @@ def main():
         return bench_cfg3(args)
     if args.config == 2:
         return bench_cfg2(args)
+    if args.config == 4:
+        return bench_cfg4(args)
     cfg = WL.config(1)
     n = cfg["n"]
     t0 = time.perf_counter()
@@ -291,6 +293,59 @@ def bench_cfg2(args):
                                "(weight 2-6) each; sampling not included",
                    "circuits": nc, "chi_max": int(chis.max()), "chi_mean": float(chis.mean()),
                    "n_mean": float(ns.mean())},
+    }), flush=True)
+
+
+def bench_cfg4(args):
+    """Large amplitude sweep: n=128 (two words per row), 6000 Clifford + 40 T
+    gates, <x|psi> for 2^16 random 128-bit strings with inputs and outputs
+    resident in HBM.  One step is one pass over all chi x 2^16 (term, string)
+    evaluations."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    import workloads as WL
+    from necstar import QuantumState
+    from necstar import _lib
+    L = _lib.lib()
+    cfg = WL.config(4)
+    n = cfg["n"]
+    t0 = time.perf_counter()
+    st = QuantumState.from_circuit(WL.to_circuit(n, cfg["gates"]))
+    compile_s = time.perf_counter() - t0
+    chi = st.stabilizer_rank
+    print(f"# cfg4: chi {chi}, compiled in {compile_s:.1f} s", file=sys.stderr, flush=True)
+    bits = cfg["bitstrings"]
+    B = bits.shape[0]
+    dev = torch.device("cuda:0")
+    d_bits = torch.from_numpy(bits.view(np.int64).copy()).to(dev)
+    d_out = torch.empty(2 * B, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    bits_p = C.cast(d_bits.data_ptr(), C.POINTER(C.c_uint64))
+    out_p = C.cast(d_out.data_ptr(), C.POINTER(C.c_double))
+    for _ in range(args.warmup):
+        _lib.check(L.nx_state_amplitudes_device(st._h, B, bits_p, 2, out_p))
+    torch.cuda.synchronize()
+    _lib.check(L.nx_profile_enable(1))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _lib.check(L.nx_state_amplitudes_device(st._h, B, bits_p, 2, out_p))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms, cnt = C.c_double(), C.c_int64()
+    _lib.check(L.nx_profile_get(b"k_amplitude", C.byref(ms), C.byref(cnt)))
+    _lib.check(L.nx_profile_enable(0))
+    step = (t1 - t0) / args.steps
+    evals = float(chi) * B
+    kms = ms.value / max(cnt.value, 1)
+    amp = d_out.view(torch.complex128).cpu().numpy()
+    print(json.dumps({
+        "metric": "cfg4 CH-form amplitude evals/s, n=128", "value": evals / step, "unit": "amplitude evals/s",
+        "higher_is_better": True, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": step * 1e3, "k_amplitude_ms": kms, "kernel_evals_per_s": evals / (kms / 1e3),
+        "compile_s": compile_s, "sum_abs2": float(np.sum(np.abs(amp) ** 2)),
+        "config": {"workload": "cfg4: n=128, 6000 Clifford + 40 T, 2^16 strings of 128 bits", "chi": chi,
+                   "bitstrings": B, "sampling": "not run (each chi^2 norm is ~3e9 pair evaluations)"},
     }), flush=True)
 
 

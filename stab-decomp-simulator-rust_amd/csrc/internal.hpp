@@ -83,7 +83,8 @@ struct VarDev {
 // the ket; the job owns waves [wave0, wave0 + nwaves) of the launch.
 // row_off[f] is the first unit (J consecutive bras) of ket row f.  With gw
 // set, the weighted pair values sum_k wts[k] <X_o | Y_f variant k> are
-// written (accumulate = 0) or added (accumulate = 1) at gw[o * gw_ld + f].
+// written (accumulate = 0) or added (accumulate = 1) at gw[o * gw_ld + f],
+// and their conjugates at gw[f * gw_ld + o].
 struct PairJobDev {
   QFormDev X, Y;
   VarDev V;
@@ -138,7 +139,7 @@ hipError_t launch_pairs_var(int W, int VW, const PairJobDev* jobs, const int32_t
 // and the base sum to out[jobs[j].base_out] when that is >= 0
 hipError_t launch_reduce_var(const PairJobDev* jobs, int njobs, int VW, const double* partial, double* out,
                              hipStream_t st);
-// y = G c for the Hermitian G whose lower triangle is Gw[j*ld + i] (j >= i)
+// y = G c for the Hermitian G held in full in Gw[j*ld + i]
 hipError_t launch_gramc(const double2* Gw, int64_t ld, int64_t chi, const double2* c, double2* y, hipStream_t st);
 
 }  // namespace nx

@@ -14,7 +14,7 @@
 //               are u64 masks with one bit per variant.  Lane k then turns
 //               variant k's exact value into FP64 and accumulates it.
 //  k_reduce_var per job and variant, the ordered sum of the wave partials
-//  k_gramc_*    y = G c for a Hermitian FP64 pair matrix (weighted Paulis)
+//  k_gramc      y = G c for the weighted FP64 pair matrix (both triangles)
 #include <hip/hip_runtime.h>
 
 #include "device_form.hpp"
@@ -533,14 +533,23 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
           gi = gi + __shfl_xor(gi, off);
         }
         if (lane == 0) {
+          // G[o][f] and, off the diagonal, G[f][o] = conj(G[o][f])
           double2* dst = jd.gw + (size_t)o * jd.gw_ld + fx;
+          double2* dsT = jd.gw + (size_t)fx * jd.gw_ld + o;
           if (jd.accumulate) {
             double2 old = *dst;
             old.x = old.x + gr;
             old.y = old.y + gi;
             *dst = old;
+            if (o != fx) {
+              double2 oT = *dsT;
+              oT.x = oT.x + gr;
+              oT.y = oT.y - gi;
+              *dsT = oT;
+            }
           } else {
             *dst = make_double2(gr, gi);
+            if (o != fx) *dsT = make_double2(gr, -gi);
           }
         }
       }
@@ -551,36 +560,39 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
   if (lane == 0) partial[(size_t)wave * NVS + 64 * VW] = accb;
 }
 
-// one block per job: thread i sums value i % NVS (64*VW variants, then the
-// base) over the job's waves w = wave0 + part, part + NP, ... in order
-// (NP = 256 / NVS parts), then the parts are added in order
+// block (job, g) sums values 8g .. 8g+7 of the job (64*VW variants, then
+// the base): thread (part, k) adds the partials of waves wave0 + part,
+// part + 32, ... of value 8g + k in order, then the 32 parts are added in
+// order.
 template <int VW>
 __global__ __launch_bounds__(256) void k_reduce_var(const PairJobDev* __restrict__ jobs,
                                                     const double* __restrict__ partial, double* __restrict__ out) {
-  constexpr int NVS = 64 * VW + 1, NP = 256 / NVS;
+  constexpr int NVS = 64 * VW + 1;
   __shared__ double sh[256];
   const PairJobDev& jd = jobs[blockIdx.x];
-  const int v = threadIdx.x % NVS, part = threadIdx.x / NVS;
+  const int k = threadIdx.x & 7, part = threadIdx.x >> 3;
+  const int v = 8 * (int)blockIdx.y + k;
   double s = 0.0;
-  if (part < NP)
-    for (int64_t w = part; w < jd.nwaves; w += NP) s = s + partial[(size_t)(jd.wave0 + w) * NVS + v];
+  if (v < NVS)
+    for (int64_t w = part; w < jd.nwaves; w += 32) s = s + partial[(size_t)(jd.wave0 + w) * NVS + v];
   sh[threadIdx.x] = s;
   __syncthreads();
-  if (part == 0) {
-    double t = sh[v];
-    for (int k = 1; k < NP; ++k) t = t + sh[k * NVS + v];
+  if (part == 0 && v < NVS) {
+    double t = sh[k];
+    for (int q = 1; q < 32; ++q) t = t + sh[8 * q + k];
     if (v < jd.nvar) out[jd.out_base + v] = t;
     if (v == 64 * VW && jd.base_out >= 0) out[jd.base_out] = t;
   }
 }
 
-// y_j = sum_{i <= j} G_ji c_i : one block per row j
-__global__ __launch_bounds__(256) void k_gramc_rows(const double2* __restrict__ G, int64_t ld, int64_t chi,
-                                                    const double2* __restrict__ c, double2* __restrict__ y) {
+// y_j = sum_i G_ji c_i over the full (both triangles written) matrix: one
+// block per row j, threads stride over i, tree sum in LDS
+__global__ __launch_bounds__(256) void k_gramc(const double2* __restrict__ G, int64_t ld, int64_t chi,
+                                               const double2* __restrict__ c, double2* __restrict__ y) {
   __shared__ double sr[256], si[256];
   const int64_t j = blockIdx.x;
   double ar = 0.0, ai = 0.0;
-  for (int64_t i = threadIdx.x; i <= j; i += blockDim.x) {
+  for (int64_t i = threadIdx.x; i < chi; i += blockDim.x) {
     const double2 g = G[(size_t)j * ld + i];
     const double2 ci = c[i];
     ar = ar + (g.x * ci.x - g.y * ci.y);
@@ -597,28 +609,6 @@ __global__ __launch_bounds__(256) void k_gramc_rows(const double2* __restrict__ 
     __syncthreads();
   }
   if (threadIdx.x == 0) y[j] = make_double2(sr[0], si[0]);
-}
-
-// y_i += sum_{j > i} conj(G_ji) c_j : one thread per column i
-__global__ __launch_bounds__(256) void k_gramc_cols(const double2* __restrict__ G, int64_t ld, int64_t chi,
-                                                    const double2* __restrict__ c, double2* __restrict__ y) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t jstart = (int64_t)blockIdx.x * blockDim.x + 1;
-  double ar = 0.0, ai = 0.0;
-  for (int64_t j = jstart; j < chi; ++j) {
-    if (i < chi && j > i) {
-      const double2 g = G[(size_t)j * ld + i];
-      const double2 cj = c[j];
-      ar = ar + (g.x * cj.x + g.y * cj.y);
-      ai = ai + (g.x * cj.y - g.y * cj.x);
-    }
-  }
-  if (i < chi) {
-    double2 v = y[i];
-    v.x = v.x + ar;
-    v.y = v.y + ai;
-    y[i] = v;
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -659,18 +649,15 @@ hipError_t launch_pairs_var(int W, int VW, const PairJobDev* jobs, const int32_t
 hipError_t launch_reduce_var(const PairJobDev* jobs, int njobs, int VW, const double* partial, double* out,
                              hipStream_t st) {
   if (njobs == 0) return hipSuccess;
-  if (VW == 1) k_reduce_var<1><<<dim3(njobs), dim3(256), 0, st>>>(jobs, partial, out);
-  else if (VW == 2) k_reduce_var<2><<<dim3(njobs), dim3(256), 0, st>>>(jobs, partial, out);
+  if (VW == 1) k_reduce_var<1><<<dim3(njobs, (64 * 1 + 1 + 7) / 8), dim3(256), 0, st>>>(jobs, partial, out);
+  else if (VW == 2) k_reduce_var<2><<<dim3(njobs, (64 * 2 + 1 + 7) / 8), dim3(256), 0, st>>>(jobs, partial, out);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
 hipError_t launch_gramc(const double2* Gw, int64_t ld, int64_t chi, const double2* c, double2* y, hipStream_t st) {
   if (chi == 0) return hipSuccess;
-  k_gramc_rows<<<dim3((unsigned)chi), dim3(256), 0, st>>>(Gw, ld, chi, c, y);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  k_gramc_cols<<<dim3(nblk(chi, 256)), dim3(256), 0, st>>>(Gw, ld, chi, c, y);
+  k_gramc<<<dim3((unsigned)chi), dim3(256), 0, st>>>(Gw, ld, chi, c, y);
   return hipGetLastError();
 }
 

@@ -272,3 +272,33 @@ def test_config1_full_size(nx):
         assert np.array_equal(pt[:, b] == 255, zo)
         assert np.array_equal(pt[~zo, b], ko[~zo]) and np.array_equal(rr[~zo], ro[~zo])
         assert abs(vals[b] - val_o) <= TOL
+
+
+def test_config4_full_size(nx):
+    """cfg4 at its real shape (n=128, 6000 Cliffords, 40 T: chi = 78,732 on
+    two-word rows).  The oracle cannot compile it in test time, so two
+    independent device paths are compared: the amplitude kernel's <x|psi>
+    against inner_product(|x>, psi) evaluated by the pair kernel (the
+    reference defines the amplitude through the same CH-form algebra,
+    amplitude.rs:28-44 vs inner_product.rs:14-48)."""
+    cfg = WL.config(4)
+    n = cfg["n"]
+    st = nx.QuantumState.from_circuit(WL.to_circuit(n, cfg["gates"]))
+    chi = st.stabilizer_rank
+    say(f"cfg4 chi={chi}")
+    assert chi == 78732
+    # random 128-bit strings: amplitudes of order 2^-64, compared relative
+    # to their own size (zero amplitudes must be zero on both paths)
+    bits = cfg["bitstrings"][:6]
+    amp = st.amplitudes(bits)
+    for b in range(len(bits)):
+        qc = nx.QuantumCircuit(n)
+        for w in range(2):
+            word = int(bits[b, w])
+            for q in range(64):
+                if (word >> q) & 1:
+                    qc.apply_x(64 * w + q)
+        basis = nx.QuantumState.from_circuit(qc)
+        ip = basis.inner_product(st)
+        scale = max(abs(ip), abs(amp[b]), 1e-300)
+        assert abs(ip - amp[b]) <= 1e-12 * scale, (b, ip, amp[b])
