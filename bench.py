@@ -169,10 +169,10 @@ def main():
         counts = st.sample(list(range(n)), args.sample_shots, seed=cfg["sample_seed"])
         te = time.perf_counter() - ts
         pms, pcnt = C.c_double(), C.c_int64()
-        _lib.check(L.nx_profile_get(b"k_pairs", C.byref(pms), C.byref(pcnt)))
+        _lib.check(L.nx_profile_get(b"k_pairs_var", C.byref(pms), C.byref(pcnt)))
         _lib.check(L.nx_profile_enable(0))
-        sampling = {"shots": args.sample_shots, "seconds": te, "distinct_outcomes": len(counts),
-                    "pair_kernel_launches": pcnt.value, "pair_kernel_ms": pms.value}
+        sampling = {"shots": args.sample_shots, "qubits": n, "seconds": te, "distinct_outcomes": len(counts),
+                    "projection_passes": pcnt.value, "pass_kernel_ms": pms.value}
 
     ms_step = (t1 - t0) * 1e3 / args.steps
     amp_units = chi * B

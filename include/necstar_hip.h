@@ -121,6 +121,16 @@ int nx_state_get_coefficients(const nx_state *s, double *re_im);
 int nx_state_set_coefficients(nx_state *s, const double *re_im);
 int nx_state_global_factor(const nx_state *s, double *re, double *im);
 
+/* norm_squared() of the state projected (projection.rs:10-31, unnormalised)
+ * onto outcome fixed_b[i] of qubit fixed_q[i] for every i, and onto each of
+ * the 2^nlook outcomes of qubits look_q: out[k] for outcome (k >> i) & 1 on
+ * look_q[i].  nlook <= 7.  One device pass for all 2^nlook outcomes (the
+ * projections enter the pair sums as constraints); sample() is built on it.
+ * No reference counterpart (the reference clones, projects and evaluates
+ * norm() once per outcome). */
+int nx_state_projection_norms(const nx_state *s, int32_t nfixed, const int32_t *fixed_q, const int32_t *fixed_b,
+                              int32_t nlook, const int32_t *look_q, double *out);
+
 /* Exact pair scalars <phi_i|phi_j> for all i, j < chi (chi*chi uint32:
  * 0xFFFFFFFF zero, else k | r << 8).  Parity aid; the reference computes
  * the same values in StabilizerCHForm::inner_product (inner_product.rs:14-48). */

@@ -99,6 +99,15 @@ struct PairJobDev {
   int64_t gw_ld;
   const double* wts;
   int32_t accumulate;
+  // unit constraints (sampling): x_{uq[u]} = ub[u*(VW+1)] for the base, the
+  // variants in masks ub[u*(VW+1) + 1 .. VW] take the other value; the ket
+  // then carries no variant tables (V unused)
+  int32_t nunit;
+  const int32_t* uq;
+  const uint64_t* ub;
+  // split_ab != 0: the variant sums are written as their dyadic part A at
+  // out[out_base + k] and sqrt(2) part B at out[out_base + nvar + k]
+  int32_t split_ab;
 };
 
 // FP64 tables computed on the host with the same libm calls as the
@@ -130,11 +139,13 @@ hipError_t launch_reduce(const double2* partial, int64_t n, double2* out, hipStr
 hipError_t launch_pauli_var(const FormsDev& in, const int32_t* voff, const int32_t* vops, int nvar, int VW,
                             uint64_t* vhi, uint64_t* vrhs, uint64_t* vph, hipStream_t st);
 // waves [wave_base, wave_base + nwaves) of a launch plan whose jobs all have
-// word width W; wave_job[w] = job of wave w; partial holds 64 * VW + 1
-// doubles per wave of the plan (the real Hermitian sums per variant, then
-// the base)
-hipError_t launch_pairs_var(int W, int VW, const PairJobDev* jobs, const int32_t* wave_job, int64_t wave_base,
-                            int64_t nwaves, int J, const double* tabs, double* partial, hipStream_t st);
+// word width W; wave_job[w] = job of wave w; partial holds 2 (64 * VW + 1)
+// doubles per wave of the plan: the real Hermitian sums per variant and the
+// base, then their sqrt(2) parts (exact sums of Scalar-coefficient states
+// are A + sqrt(2) B with A, B sums of powers of two)
+hipError_t launch_pairs_var(int W, int VW, int units, const PairJobDev* jobs, const int32_t* wave_job,
+                            int64_t wave_base, int64_t nwaves, int J, const double* tabs, double* partial,
+                            hipStream_t st);
 // out[jobs[j].out_base + k] = sum over the job's waves of partial (k < nvar),
 // and the base sum to out[jobs[j].base_out] when that is >= 0
 hipError_t launch_reduce_var(const PairJobDev* jobs, int njobs, int VW, const double* partial, double* out,

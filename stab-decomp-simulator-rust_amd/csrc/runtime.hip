@@ -816,48 +816,6 @@ static int measure_single(nx_state* s, int q, ChaCha12Rng& rng) {
   return res;
 }
 
-struct SampleCtx {
-  std::vector<int> q, bit;
-  std::vector<uint64_t> bits, counts;
-  ChaCha12Rng* rng;
-};
-
-// sampling.rs:80-189
-static void recursive_sample(const nx_state* s, SampleCtx& cx, size_t idx, uint64_t shots, uint64_t lo,
-                             uint64_t hi) {
-  if (shots == 0) return;
-  if (idx == cx.q.size()) {
-    cx.bits.push_back(lo);
-    cx.bits.push_back(hi);
-    cx.counts.push_back(shots);
-    return;
-  }
-  const int q = cx.q[idx], b = cx.bit[idx];
-  StateGuard zero(clone_state(s)), one(clone_state(s));
-  project_unnormalized(zero.p, q, 0);
-  project_unnormalized(one.p, q, 1);
-  // the reference evaluates state_zero.norm_squared() twice; it is a pure
-  // function of the state, so one evaluation gives the same value
-  const double nz = norm_squared(zero.p), no = norm_squared(one.p);
-  double pz = nz / (nz + no);
-  if (!std::isnan(pz)) pz = pz < 0.0 ? 0.0 : (pz > 1.0 ? 1.0 : pz);
-  if (!(pz >= 0.0)) fail(NX_E_VALUE, "p < 0 or is NaN in binomial distribution");
-  const uint64_t nzeros = binomial(*cx.rng, shots, pz);
-  const uint64_t nones = shots - nzeros;
-  if (nzeros > 0) {
-    discard(zero.p, q);
-    recursive_sample(zero.p, cx, idx + 1, nzeros, lo, hi);
-  }
-  if (nones > 0) {
-    uint64_t nlo = lo, nhi = hi;
-    if (b < 64) nlo |= 1ull << b;
-    else nhi |= 1ull << (b - 64);
-    apply_program_all(one.p, {{OP_X, q, 0, 0}});
-    discard(one.p, q);
-    recursive_sample(one.p, cx, idx + 1, nones, nlo, nhi);
-  }
-}
-
 // Pauli program (gate_application.rs:186-210) with the length checks of
 // exp_value.rs:9-27
 static std::vector<ProgOp> pauli_prog(const nx_state* s, int dense, int32_t len, const int32_t* ops,
@@ -945,7 +903,9 @@ static VarSet pauli_vars(const Forms& f, const PauliSet& ps, int first, int coun
 struct VarJob {
   const QForms* X = nullptr;
   const QForms* Y = nullptr;
-  const VarSet* V = nullptr;
+  const VarSet* V = nullptr;  // null: unit-constraint job (nvar variants, no ket tables)
+  int nvar = 0;
+  int split_ab = 0;           // write the A and B parts of the sums separately
   CoefDev cx{}, cy{};
   int64_t out_base = 0;
   int64_t base_out = -1;  // >= 0: also the base sum (norm^2 / |g|^2) there
@@ -956,7 +916,7 @@ struct VarJob {
 };
 
 struct VarPlan {
-  int VW = 1, J = 16;
+  int VW = 1, J = 16, units = 0;
   int64_t total_waves = 0, nout = 0;
   int njobs = 0;
   struct Launch {
@@ -967,8 +927,8 @@ struct VarPlan {
   DevBuf jobs, wave_job, row_off, partial, out;
   void enqueue(hipStream_t st) const {
     for (const Launch& L : launches)
-      hip_check(launch_pairs_var(L.W, VW, jobs.as<PairJobDev>(), wave_job.as<int32_t>(), L.wave_base, L.nwaves, J,
-                                 ctx()->tabs, partial.as<double>(), st),
+      hip_check(launch_pairs_var(L.W, VW, units, jobs.as<PairJobDev>(), wave_job.as<int32_t>(), L.wave_base,
+                                 L.nwaves, J, ctx()->tabs, partial.as<double>(), st),
                 "k_pairs_var");
     hip_check(launch_reduce_var(jobs.as<PairJobDev>(), njobs, VW, partial.as<double>(), out.as<double>(), st),
               "k_reduce_var");
@@ -1026,14 +986,18 @@ static VarPlan make_plan(const std::vector<VarJob>& in, int VW, int J, int64_t t
     PairJobDev& d = hj[idx];
     d.X = j.X->desc();
     d.Y = j.Y->desc();
-    d.V = j.V->desc();
+    d.V = j.V ? j.V->desc() : VarDev{nullptr, nullptr, nullptr};
     d.cx = j.cx;
     d.cy = j.cy;
     d.row_off = nullptr;  // set below
     d.nfixed = nrows[idx];
     d.nother = chi;
     d.triangle = 1;
-    d.nvar = j.V->nvar;
+    d.nvar = j.V ? j.V->nvar : j.nvar;
+    d.nunit = 0;
+    d.uq = nullptr;
+    d.ub = nullptr;
+    d.split_ab = j.split_ab;
     d.wave0 = wave;
     d.nwaves = nw;
     d.out_base = j.out_base;
@@ -1045,10 +1009,12 @@ static VarPlan make_plan(const std::vector<VarJob>& in, int VW, int J, int64_t t
     for (int64_t w = 0; w < nw; ++w) wj.push_back(idx);
     wave += nw;
     pl.launches.back().nwaves += nw;
-    pl.nout = std::max<int64_t>(pl.nout, std::max<int64_t>(j.out_base + j.V->nvar, j.base_out + 1));
+    pl.nout = std::max<int64_t>(pl.nout, std::max<int64_t>(j.out_base + (j.split_ab ? 2 : 1) * d.nvar,
+                                                           j.base_out + 1));
   }
   pl.total_waves = wave;
   pl.njobs = (int)in.size();
+  pl.units = (!in.empty() && in[0].V == nullptr) ? 1 : 0;
   pl.row_off.alloc(std::max<size_t>(roff.size(), 1) * 8);
   h2d(pl.row_off.p, roff.data(), roff.size() * 8);
   for (size_t i = 0; i < in.size(); ++i) hj[i].row_off = pl.row_off.as<int64_t>() + roff_base[i];
@@ -1056,7 +1022,7 @@ static VarPlan make_plan(const std::vector<VarJob>& in, int VW, int J, int64_t t
   h2d(pl.jobs.p, hj.data(), hj.size() * sizeof(PairJobDev));
   pl.wave_job.alloc(std::max<size_t>(wj.size(), 1) * 4);
   h2d(pl.wave_job.p, wj.data(), wj.size() * 4);
-  pl.partial.alloc(std::max<int64_t>(wave, 1) * (64 * VW + 1) * 8);
+  pl.partial.alloc(std::max<int64_t>(wave, 1) * 2 * (64 * VW + 1) * 8);
   pl.out.alloc(std::max<int64_t>(pl.nout, 1) * 8);
   return pl;
 }
@@ -1185,6 +1151,170 @@ static std::vector<double> exp_values_var(const nx_state* s, const PauliSet& ps,
   const double g2 = s->g.real() * s->g.real() + s->g.imag() * s->g.imag();
   for (double& r : res) r *= g2;
   return res;
+}
+
+// One state's Z-basis projection norms, 2^d outcomes per pass.  The plan
+// (job table, unit offsets, wave map) is built once; a pass only uploads the
+// unit constraints: the fixed outcomes of the already processed qubits and
+// the d look-ahead qubits whose 2^d outcomes are the variants (variant k
+// takes outcome (k >> i) & 1 on look-ahead qubit i).
+struct ProjPlan {
+  const nx_state* s = nullptr;
+  VarPlan plan;
+  DevBuf uq, ub;
+  std::vector<PairJobDev> hj;
+  int VW = 1, maxunit = 0;
+  double g2 = 1.0;
+  void init(const nx_state* st, int max_units, int max_look) {
+    s = st;
+    VW = (1 << max_look) <= 64 ? 1 : 2;
+    maxunit = max_units;
+    VarJob j;
+    j.X = &qform_of(st);
+    j.Y = j.X;
+    j.cx = coef_of(st);
+    j.cy = j.cx;
+    j.nvar = 1 << max_look;
+    j.split_ab = 1;
+    const auto shape = tuned_shape({j}, VW);
+    plan = make_plan({j}, VW, shape.first, shape.second);
+    hj.resize(1);
+    d2h(hj.data(), plan.jobs.p, sizeof(PairJobDev));
+    uq.alloc((size_t)std::max(max_units, 1) * 4);
+    ub.alloc((size_t)std::max(max_units, 1) * (VW + 1) * 8);
+    sync();
+    g2 = st->g.real() * st->g.real() + st->g.imag() * st->g.imag();
+  }
+  // norm_squared / |g|^2 of Pi psi for the 2^d outcomes of qubits look[0..d),
+  // as exact parts: value = A[k] + sqrt(2) B[k] (A, B = the returned halves)
+  std::vector<double> run(const std::vector<int>& fq, const std::vector<int>& fb, const std::vector<int>& look) {
+    const int d = (int)look.size(), nv = 1 << d;
+    std::vector<int32_t> q;
+    std::vector<uint64_t> b;
+    for (size_t i = 0; i < fq.size(); ++i) {
+      q.push_back(fq[i]);
+      b.push_back((uint64_t)fb[i]);
+      for (int v = 0; v < VW; ++v) b.push_back(0ull);
+    }
+    for (int i = 0; i < d; ++i) {
+      q.push_back(look[i]);
+      b.push_back(0ull);
+      for (int v = 0; v < VW; ++v) {
+        uint64_t m = 0;
+        for (int k = 0; k < 64; ++k)
+          if (64 * v + k < nv && (((64 * v + k) >> i) & 1)) m |= 1ull << k;
+        b.push_back(m);
+      }
+    }
+    if ((int)q.size() > maxunit) fail(NX_E_INTERNAL, "too many projected qubits");
+    h2d(uq.p, q.data(), q.size() * 4);
+    h2d(ub.p, b.data(), b.size() * 8);
+    hj[0].nunit = (int32_t)q.size();
+    hj[0].uq = uq.as<int32_t>();
+    hj[0].ub = ub.as<uint64_t>();
+    hj[0].nvar = nv;
+    h2d(plan.jobs.p, hj.data(), sizeof(PairJobDev));
+    {
+      ProfScope ps("k_pairs_var");
+      plan.enqueue(stream());
+      ps.end();
+    }
+    std::vector<double> o(2 * nv);
+    d2h(o.data(), plan.out.p, (size_t)nv * 16);
+    sync();
+    return o;
+  }
+  // the reference's norm_squared: (sum) * |g|^2, the sum being A + sqrt(2) B
+  double value(double A, double B) const { return (A + 1.4142135623730951 * B) * g2; }
+};
+
+// Norms of every node of a DFS window: the window rooted at depth `root`
+// holds, for relative depth j = 1 .. d and outcome index k < 2^j (bit i =
+// outcome of processed qubit root + i), the norm of that node as the sum of
+// its look-ahead leaves.
+struct Window {
+  int root = 0, d = 0;
+  std::vector<std::vector<double>> la, lb;  // exact parts A, B of node [j][k]
+};
+
+struct SampleCtx {
+  std::vector<int> q, bit;
+  std::vector<uint64_t> bits, counts;
+  std::vector<int> path;  // outcomes of the processed qubits so far
+  ChaCha12Rng* rng;
+  ProjPlan pp;
+  int look = 6;
+  int64_t passes = 0;
+};
+
+static Window open_window(SampleCtx& cx, int root) {
+  Window w;
+  w.root = root;
+  w.d = std::min<int>(cx.look, (int)cx.q.size() - root);
+  std::vector<int> fq(cx.q.begin(), cx.q.begin() + root), fb(cx.path.begin(), cx.path.begin() + root);
+  std::vector<int> lq(cx.q.begin() + root, cx.q.begin() + root + w.d);
+  w.la.resize(w.d + 1);
+  w.lb.resize(w.d + 1);
+  const std::vector<double> ab = cx.pp.run(fq, fb, lq);
+  const size_t nv = (size_t)1 << w.d;
+  w.la[w.d].assign(ab.begin(), ab.begin() + nv);
+  w.lb[w.d].assign(ab.begin() + nv, ab.begin() + 2 * nv);
+  ++cx.passes;
+  // inner nodes: exact sums of their leaves (powers of two in A and B)
+  for (int j = w.d; j >= 2; --j) {
+    const size_t m = (size_t)1 << (j - 1);
+    w.la[j - 1].assign(m, 0.0);
+    w.lb[j - 1].assign(m, 0.0);
+    for (size_t k = 0; k < m; ++k) {
+      w.la[j - 1][k] = w.la[j][k] + w.la[j][k | m];
+      w.lb[j - 1][k] = w.lb[j][k] + w.lb[j][k | m];
+    }
+  }
+  return w;
+}
+
+// sampling.rs:80-189.  The norm of a child is ||Pi psi||^2 for the projector
+// onto the outcomes chosen so far (the reference projects, then discards
+// the measured qubit, which leaves the norm unchanged); each window pass
+// yields the norms of all nodes up to `look` levels below its root, so the
+// draws below happen in the reference's DFS order without further passes.
+static void recursive_sample(SampleCtx& cx, size_t idx, uint64_t shots, uint64_t lo, uint64_t hi, const Window* w,
+                             uint64_t rel) {
+  if (shots == 0) return;
+  if (idx == cx.q.size()) {
+    cx.bits.push_back(lo);
+    cx.bits.push_back(hi);
+    cx.counts.push_back(shots);
+    return;
+  }
+  Window own;
+  if (!w || (int)idx + 1 > w->root + w->d) {
+    own = open_window(cx, (int)idx);
+    w = &own;
+    rel = 0;
+  }
+  const int j = (int)idx - w->root + 1;  // children's relative depth
+  const uint64_t r1 = rel | ((uint64_t)1 << (j - 1));
+  const double nz = cx.pp.value(w->la[j][rel], w->lb[j][rel]), no = cx.pp.value(w->la[j][r1], w->lb[j][r1]);
+  double pz = nz / (nz + no);
+  if (!std::isnan(pz)) pz = pz < 0.0 ? 0.0 : (pz > 1.0 ? 1.0 : pz);
+  if (!(pz >= 0.0)) fail(NX_E_VALUE, "p < 0 or is NaN in binomial distribution");
+  const uint64_t nzeros = binomial(*cx.rng, shots, pz);
+  const uint64_t nones = shots - nzeros;
+  const int b = cx.bit[idx];
+  if (nzeros > 0) {
+    cx.path.push_back(0);
+    recursive_sample(cx, idx + 1, nzeros, lo, hi, w, rel);
+    cx.path.pop_back();
+  }
+  if (nones > 0) {
+    uint64_t nlo = lo, nhi = hi;
+    if (b < 64) nlo |= 1ull << b;
+    else nhi |= 1ull << (b - 64);
+    cx.path.push_back(1);
+    recursive_sample(cx, idx + 1, nones, nlo, nhi, w, rel | ((uint64_t)1 << (j - 1)));
+    cx.path.pop_back();
+  }
 }
 
 }  // namespace nx
@@ -1440,12 +1570,43 @@ int nx_state_sample(const nx_state* s, int32_t nq, const int32_t* qargs, uint64_
       cx.q.push_back(p.first);
       cx.bit.push_back(p.second);
     }
-    recursive_sample(s, cx, 0, shots, 0, 0);
+    if (shots > 0 && nq > 0 && s->chi() == 0)  // every norm is 0: p = 0/0
+      fail(NX_E_VALUE, "p < 0 or is NaN in binomial distribution");
+    if (shots > 0 && nq > 0) {
+      cx.pp.init(s, nq, cx.look);
+      recursive_sample(cx, 0, shots, 0, 0, nullptr, 0);
+    } else if (shots > 0) {
+      cx.bits.push_back(0);
+      cx.bits.push_back(0);
+      cx.counts.push_back(shots);
+    }
     const int64_t k = (int64_t)cx.counts.size();
     if (k > max_out) fail(NX_E_VALUE, "sample output buffer too small (%lld outcomes)", (long long)k);
     std::memcpy(out_bits, cx.bits.data(), k * 16);
     std::memcpy(out_counts, cx.counts.data(), k * 8);
     *nout = k;
+  })
+}
+
+int nx_state_projection_norms(const nx_state* s, int32_t nfixed, const int32_t* fixed_q, const int32_t* fixed_b,
+                              int32_t nlook, const int32_t* look_q, double* out) {
+  NX_TRY({
+    if (nlook < 0 || nlook > 7) fail(NX_E_VALUE, "between 0 and 7 look-ahead qubits are supported");
+    if (nfixed < 0) fail(NX_E_VALUE, "negative qubit count");
+    std::vector<int32_t> all(fixed_q, fixed_q + nfixed);
+    all.insert(all.end(), look_q, look_q + nlook);
+    validate_qargs(s, (int32_t)all.size(), all.data());
+    const int nv = 1 << nlook;
+    if (s->chi() == 0) {
+      for (int k = 0; k < nv; ++k) out[k] = 0.0;
+      return NX_OK;
+    }
+    ProjPlan pp;
+    pp.init(s, (int)all.size(), nlook);
+    std::vector<int> fq(fixed_q, fixed_q + nfixed), fb(nfixed), lq(look_q, look_q + nlook);
+    for (int i = 0; i < nfixed; ++i) fb[i] = fixed_b[i] ? 1 : 0;
+    const std::vector<double> r = pp.run(fq, fb, lq);
+    for (int k = 0; k < nv; ++k) out[k] = pp.value(r[k], r[nv + k]);
   })
 }
 

@@ -33,9 +33,24 @@ __device__ __forceinline__ void z8_add(u64& b0, u64& b1, u64& b2, u64 x0, u64 x1
   b2 ^= x2 ^ c1;
 }
 
+// Exact real part of w * zeta^k * 2^{-s/2} (w = 1 or 2): cos(k pi/4) is 0,
+// +-1 or +-2^{-1/2}, so the value is +-w 2^{-t/2} with t = s or s + 1; it
+// is added to A when t is even and as +-w 2^{-(t+1)/2} to B when t is odd,
+// so that the sum is A + sqrt(2) B.  Each addend is a power of two, so A and
+// B are exact while their exponent span fits a double's 53 bits.
+__device__ __forceinline__ void add_exact(int k, int s, double w, double& A, double& B) {
+  k &= 7;
+  if (k == 2 || k == 6) return;
+  const double sg = (k == 0 || k == 1 || k == 7) ? w : -w;
+  const int t = (k & 1) ? s + 1 : s;
+  if (t & 1) B = B + ldexp(sg, -((t + 1) >> 1));
+  else A = A + ldexp(sg, -(t >> 1));
+}
+
 template <int W, int VW>
 struct VarRegs {
   u64 hi[W][VW];               // row r: variants whose a_r carries an extra 2
+  u64 ra[W][VW];               // bra constraint row c: variants whose rhs is flipped (unit mode)
   u64 rb[W][VW];               // ket constraint row c: variants whose rhs is flipped
   u64 d0[VW], d1[VW], d2[VW];  // uniform: Z8 phase offset of each variant
   u64 z[VW];                   // uniform: variants whose sum is already zero
@@ -92,7 +107,7 @@ __device__ __forceinline__ void uniform_state(PairRegs<W>& P, VarRegs<W, VW>& V)
 // e_k)) ^ (B_pl & e_k)); the phase gains 2 (a_p,k c_k mod 4) per variant and
 // 2 (a_p c mod 4) for the base.  Bra constraints carry no variant flips
 // (E = 0 while they are taken).
-template <int W, int VW, int CONS>
+template <int W, int VW, int CONS, bool UNITS>
 __device__ __forceinline__ void subst_v(PairRegs<W>& P, VarRegs<W, VW>& V, int p, const u64 (&r)[W], int c,
                                         const u64 (&E)[VW]) {
   const int ap = bcast_int<W>(P.a, p);
@@ -140,6 +155,10 @@ __device__ __forceinline__ void subst_v(PairRegs<W>& P, VarRegs<W, VW>& V, int p
       for (int w = 0; w < W; ++w) P.ca[sl][w] ^= r[w] & ma;
       clearbit<W>(P.ca[sl], pw, pb);
       if (c) P.rhsa[sl] ^= __ballot(ha);
+      if (UNITS) {
+#pragma unroll
+        for (int v = 0; v < VW; ++v) V.ra[sl][v] ^= ha ? E[v] : 0ull;
+      }
     }
     if (CONS != CONS_NONE) {
       const int hb = (int)((wsel<W>(P.cb[sl], pw) >> pb) & 1ull);
@@ -148,7 +167,7 @@ __device__ __forceinline__ void subst_v(PairRegs<W>& P, VarRegs<W, VW>& V, int p
       for (int w = 0; w < W; ++w) P.cb[sl][w] ^= r[w] & mb2;
       clearbit<W>(P.cb[sl], pw, pb);
       if (c) P.rhsb[sl] ^= __ballot(hb);
-      if (CONS == CONS_B) {
+      if (CONS == CONS_B || UNITS) {
 #pragma unroll
         for (int v = 0; v < VW; ++v) V.rb[sl][v] ^= hb ? E[v] : 0ull;
       }
@@ -157,7 +176,7 @@ __device__ __forceinline__ void subst_v(PairRegs<W>& P, VarRegs<W, VW>& V, int p
   clearbit<W>(P.rem, pw, pb);
 }
 
-template <int W, int VW, bool USEB>
+template <int W, int VW, bool USEB, bool UNITS>
 __device__ __forceinline__ void take_constraint_v(PairRegs<W>& P, VarRegs<W, VW>& V, int c) {
   u64 row[W];
   bcast<W>(USEB ? P.cb : P.ca, c, row);
@@ -165,6 +184,8 @@ __device__ __forceinline__ void take_constraint_v(PairRegs<W>& P, VarRegs<W, VW>
   u64 E[VW];
   if (USEB) {
     bcast_v<W, VW>(V.rb, c, E);
+  } else if (UNITS) {
+    bcast_v<W, VW>(V.ra, c, E);
   } else {
 #pragma unroll
     for (int v = 0; v < VW; ++v) E[v] = 0ull;
@@ -178,21 +199,40 @@ __device__ __forceinline__ void take_constraint_v(PairRegs<W>& P, VarRegs<W, VW>
   }
   const int p = ffsv<W>(row);
   flipbit<W>(row, p, 1);
-  subst_v<W, VW, USEB ? CONS_B : CONS_AB>(P, V, p, row, rhs, E);
+  subst_v<W, VW, USEB ? CONS_B : CONS_AB, UNITS>(P, V, p, row, rhs, E);
 }
 
 // pair_eval of pair_core.hpp for every variant and for the base (the ket
 // itself).  P.c0 follows the base; the planes hold each variant's offset.
-template <int W, int VW>
-__device__ __forceinline__ void pair_eval_v(PairRegs<W>& P, VarRegs<W, VW>& V, int nca, int ncb) {
+// With UNITS, the unit constraints x_{uq[u]} = ub base bit (flipped for the
+// variants in the following VW masks) are substituted first: they restrict
+// the sum to one Z-basis outcome per variant on those qubits, i.e. the pair
+// value <bra| Pi |ket> of the projector Pi = |b><b| on them.
+template <int W, int VW, bool UNITS>
+__device__ __forceinline__ void pair_eval_v(PairRegs<W>& P, VarRegs<W, VW>& V, int nca, int ncb, int nunit,
+                                            const int32_t* __restrict__ uq, const u64* __restrict__ ub) {
   uniform_state<W, VW>(P, V);
+  if (UNITS) {
+    u64 r0[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) r0[w] = 0ull;
+    for (int u = 0; u < nunit; ++u) {
+      const int q = uni(uq[u]);
+      const int b = (int)uni64(ub[(size_t)u * (VW + 1)]);
+      u64 E[VW];
+#pragma unroll
+      for (int v = 0; v < VW; ++v) E[v] = uni64(ub[(size_t)u * (VW + 1) + 1 + v]);
+      subst_v<W, VW, CONS_AB, UNITS>(P, V, q, r0, b, E);
+      uniform_state<W, VW>(P, V);
+    }
+  }
   for (int c = 0; c < nca; ++c) {
-    take_constraint_v<W, VW, false>(P, V, c);
+    take_constraint_v<W, VW, false, UNITS>(P, V, c);
     uniform_state<W, VW>(P, V);
     if (all_zero<W, VW>(V)) return;
   }
   for (int c = 0; c < ncb; ++c) {
-    take_constraint_v<W, VW, true>(P, V, c);
+    take_constraint_v<W, VW, true, UNITS>(P, V, c);
     uniform_state<W, VW>(P, V);
     if (all_zero<W, VW>(V)) return;
   }
@@ -249,7 +289,7 @@ __device__ __forceinline__ void pair_eval_v(PairRegs<W>& P, VarRegs<W, VW>& V, i
       for (int sl = 0; sl < W; ++sl) clearbit<W>(P.B[sl], pw, pb);
       const int qv = ffsv<W>(Bp);
       flipbit<W>(Bp, qv, 1);
-      subst_v<W, VW, CONS_NONE>(P, V, qv, Bp, ap >> 1, H);
+      subst_v<W, VW, CONS_NONE, UNITS>(P, V, qv, Bp, ap >> 1, H);
     }
     uniform_state<W, VW>(P, V);
   }
@@ -383,7 +423,7 @@ __global__ __launch_bounds__(256) void k_pauli_var(FormsDev in, const int32_t* _
 //   sum_f sum_{o >= f} (o == f ? 1 : 2) Re(conj(c_o) c_f <X_o | Y_f^(k)>)
 // which is exp_value.rs:38-58 for Y_f^(k) = P_k phi_f (norm.rs:16-27 for
 // the base).  Per wave, partial holds 64*VW variant sums then the base.
-template <int W, int VW>
+template <int W, int VW, bool UNITS>
 __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict__ jobs,
                                                    const int32_t* __restrict__ wave_job, int64_t wave_base,
                                                    int64_t nwaves_launch, int J, const double* __restrict__ tabs,
@@ -416,9 +456,12 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
   double wk[VW];
 #pragma unroll
   for (int v = 0; v < VW; ++v) wk[v] = (jd.gw && 64 * v + lane < jd.nvar) ? jd.wts[64 * v + lane] : 0.0;
-  double acc[VW], accb = 0.0;
+  // acc: FP64 sums (complex coefficients), or the dyadic part A of the
+  // exact sums (exact Scalar coefficients) with the sqrt(2) part in accB
+  const bool exact = jd.cx.scalar && jd.cy.scalar;
+  double acc[VW], accB[VW], accb = 0.0, accbB = 0.0;
 #pragma unroll
-  for (int v = 0; v < VW; ++v) acc[v] = 0.0;
+  for (int v = 0; v < VW; ++v) acc[v] = accB[v] = 0.0;
   for (int64_t u = wave - jd.wave0; u < nunits; u += jd.nwaves) {
     int64_t lo_r = 0, hi_r = nfixed;
     while (hi_r - lo_r > 1) {
@@ -434,9 +477,9 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
     u64 vp0[VW], vp1[VW], vp2[VW];
 #pragma unroll
     for (int v = 0; v < VW; ++v) {
-      vp0[v] = uni64(jd.V.ph[((size_t)fx * 3 + 0) * VW + v]);
-      vp1[v] = uni64(jd.V.ph[((size_t)fx * 3 + 1) * VW + v]);
-      vp2[v] = uni64(jd.V.ph[((size_t)fx * 3 + 2) * VW + v]);
+      vp0[v] = UNITS ? 0ull : uni64(jd.V.ph[((size_t)fx * 3 + 0) * VW + v]);
+      vp1[v] = UNITS ? 0ull : uni64(jd.V.ph[((size_t)fx * 3 + 1) * VW + v]);
+      vp2[v] = UNITS ? 0ull : uni64(jd.V.ph[((size_t)fx * 3 + 2) * VW + v]);
     }
     for (int64_t o = o_lo; o < o_hi; ++o) {
       // ket rows are re-read per pair (cache hits) to keep registers free
@@ -462,8 +505,9 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
         const size_t r = (size_t)fx * R + 64 * sl + lane;
 #pragma unroll
         for (int v = 0; v < VW; ++v) {
-          V.hi[sl][v] = jd.V.hi[r * VW + v];
-          V.rb[sl][v] = jd.V.rhs[r * VW + v];
+          V.hi[sl][v] = UNITS ? 0ull : jd.V.hi[r * VW + v];
+          V.rb[sl][v] = UNITS ? 0ull : jd.V.rhs[r * VW + v];
+          V.ra[sl][v] = 0ull;
         }
       }
 #pragma unroll
@@ -482,21 +526,16 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
       V.bz = !want_base;
       P.c0 = (pf - po) & 7;
       P.mag = kf + ko;
-      pair_eval_v<W, VW>(P, V, nco, ncf);
+      pair_eval_v<W, VW, UNITS>(P, V, nco, ncf, jd.nunit, jd.uq, jd.ub);
       // (c_bra.conj() * c_ket).into(), shared by the variants
       double cr_, ci_;
-      if (jd.cx.scalar) {
-        const int zb = jd.cx.cz[o], zk = jd.cy.cz[fx];
-        if (zb || zk) {
-          cr_ = 0.0;
-          ci_ = 0.0;
-        } else {
-          const int kk = ((8 - jd.cx.ck[o]) % 8 + jd.cy.ck[fx]) % 8;
-          const int rr = jd.cx.cr[o] + jd.cy.cr[fx];
-          const double nn = tabs[TAB_POW + rr];
-          cr_ = tabs[TAB_COS + kk] * nn;
-          ci_ = tabs[TAB_SIN + kk] * nn;
-        }
+      int kc = 0, rc = 0, czero = 0;  // exact coefficient product zeta^kc 2^{-rc/2}
+      if (exact) {
+        czero = jd.cx.cz[o] | jd.cy.cz[fx];
+        kc = ((8 - jd.cx.ck[o]) % 8 + jd.cy.ck[fx]) % 8;
+        rc = jd.cx.cr[o] + jd.cy.cr[fx];
+        cr_ = 0.0;
+        ci_ = 0.0;
       } else {
         const double2 b0 = jd.cx.c[o], k0 = jd.cy.c[fx];
         const double br = b0.x, bi = -b0.y;
@@ -505,11 +544,16 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
       }
       const bool diag = (o == fx);
       const double nrm = tabs[TAB_POW + P.mag];
+      const double w2 = diag ? 1.0 : 2.0;
       if (!V.bz) {
         const int K = P.c0;
-        const double ir = tabs[TAB_COS + K] * nrm, ii = tabs[TAB_SIN + K] * nrm;
-        const double tr = cr_ * ir - ci_ * ii;
-        accb = diag ? accb + tr : accb + (tr + tr);
+        if (exact) {
+          if (!czero) add_exact(kc + K, rc + P.mag, w2, accb, accbB);
+        } else {
+          const double ir = tabs[TAB_COS + K] * nrm, ii = tabs[TAB_SIN + K] * nrm;
+          const double tr = cr_ * ir - ci_ * ii;
+          accb = diag ? accb + tr : accb + (tr + tr);
+        }
       }
       double gr = 0.0, gi = 0.0;
 #pragma unroll
@@ -520,8 +564,12 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
                         ((int)((V.d2[v] >> lane) & 1ull) << 2);
           const int K = (P.c0 + D) & 7;
           const double ir = tabs[TAB_COS + K] * nrm, ii = tabs[TAB_SIN + K] * nrm;
-          const double tr = cr_ * ir - ci_ * ii;
-          acc[v] = diag ? acc[v] + tr : acc[v] + (tr + tr);
+          if (exact) {
+            if (!czero) add_exact(kc + K, rc + P.mag, w2, acc[v], accB[v]);
+          } else {
+            const double tr = cr_ * ir - ci_ * ii;
+            acc[v] = diag ? acc[v] + tr : acc[v] + (tr + tr);
+          }
           gr = gr + wk[v] * ir;
           gi = gi + wk[v] * ii;
         }
@@ -555,32 +603,56 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
       }
     }
   }
+  // per wave: the NVS sums (A parts), then their sqrt(2) parts (B)
 #pragma unroll
-  for (int v = 0; v < VW; ++v) partial[(size_t)wave * NVS + 64 * v + lane] = acc[v];
-  if (lane == 0) partial[(size_t)wave * NVS + 64 * VW] = accb;
+  for (int v = 0; v < VW; ++v) {
+    partial[(size_t)wave * 2 * NVS + 64 * v + lane] = acc[v];
+    partial[(size_t)wave * 2 * NVS + NVS + 64 * v + lane] = accB[v];
+  }
+  if (lane == 0) {
+    partial[(size_t)wave * 2 * NVS + 64 * VW] = accb;
+    partial[(size_t)wave * 2 * NVS + NVS + 64 * VW] = accbB;
+  }
 }
 
 // block (job, g) sums values 8g .. 8g+7 of the job (64*VW variants, then
 // the base): thread (part, k) adds the partials of waves wave0 + part,
-// part + 32, ... of value 8g + k in order, then the 32 parts are added in
-// order.
+// part + 32, ... of value 8g + k in order, for the A and the B part, then
+// the 32 parts are added in order and the value is A + sqrt(2) B.
 template <int VW>
 __global__ __launch_bounds__(256) void k_reduce_var(const PairJobDev* __restrict__ jobs,
                                                     const double* __restrict__ partial, double* __restrict__ out) {
   constexpr int NVS = 64 * VW + 1;
-  __shared__ double sh[256];
+  constexpr double SQRT2 = 1.4142135623730951;  // nearest double to sqrt(2)
+  __shared__ double shA[256], shB[256];
   const PairJobDev& jd = jobs[blockIdx.x];
   const int k = threadIdx.x & 7, part = threadIdx.x >> 3;
   const int v = 8 * (int)blockIdx.y + k;
-  double s = 0.0;
+  double a = 0.0, b = 0.0;
   if (v < NVS)
-    for (int64_t w = part; w < jd.nwaves; w += 32) s = s + partial[(size_t)(jd.wave0 + w) * NVS + v];
-  sh[threadIdx.x] = s;
+    for (int64_t w = part; w < jd.nwaves; w += 32) {
+      const size_t base = (size_t)(jd.wave0 + w) * 2 * NVS;
+      a = a + partial[base + v];
+      b = b + partial[base + NVS + v];
+    }
+  shA[threadIdx.x] = a;
+  shB[threadIdx.x] = b;
   __syncthreads();
   if (part == 0 && v < NVS) {
-    double t = sh[k];
-    for (int q = 1; q < 32; ++q) t = t + sh[8 * q + k];
-    if (v < jd.nvar) out[jd.out_base + v] = t;
+    double ta = shA[k], tb = shB[k];
+    for (int q = 1; q < 32; ++q) {
+      ta = ta + shA[8 * q + k];
+      tb = tb + shB[8 * q + k];
+    }
+    const double t = ta + SQRT2 * tb;
+    if (v < jd.nvar) {
+      if (jd.split_ab) {
+        out[jd.out_base + v] = ta;
+        out[jd.out_base + jd.nvar + v] = tb;
+      } else {
+        out[jd.out_base + v] = t;
+      }
+    }
     if (v == 64 * VW && jd.base_out >= 0) out[jd.base_out] = t;
   }
 }
@@ -626,6 +698,10 @@ static inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1
     case 34: { constexpr int WW = 4, VV = 2; __VA_ARGS__; } break;                          \
     default: return hipErrorInvalidValue;                                                   \
   }
+#define NX_LAUNCH_PAIRS_VAR(UU)                                                                              \
+  NX_DISPATCH_WV(W, VW,                                                                                      \
+                 k_pairs_var<WW, VV, UU><<<dim3(nblk(nwaves, 4)), dim3(256), 0, st>>>(jobs, wave_job, wave_base, \
+                                                                                      nwaves, J, tabs, partial))
 
 hipError_t launch_pauli_var(const FormsDev& in, const int32_t* voff, const int32_t* vops, int nvar, int VW,
                             uint64_t* vhi, uint64_t* vrhs, uint64_t* vph, hipStream_t st) {
@@ -637,12 +713,12 @@ hipError_t launch_pauli_var(const FormsDev& in, const int32_t* voff, const int32
   return hipGetLastError();
 }
 
-hipError_t launch_pairs_var(int W, int VW, const PairJobDev* jobs, const int32_t* wave_job, int64_t wave_base,
-                            int64_t nwaves, int J, const double* tabs, double* partial, hipStream_t st) {
+hipError_t launch_pairs_var(int W, int VW, int units, const PairJobDev* jobs, const int32_t* wave_job,
+                            int64_t wave_base, int64_t nwaves, int J, const double* tabs, double* partial,
+                            hipStream_t st) {
   if (nwaves == 0) return hipSuccess;
-  NX_DISPATCH_WV(W, VW,
-                 k_pairs_var<WW, VV><<<dim3(nblk(nwaves, 4)), dim3(256), 0, st>>>(jobs, wave_job, wave_base, nwaves,
-                                                                                  J, tabs, partial));
+  if (units) NX_LAUNCH_PAIRS_VAR(true)
+  else NX_LAUNCH_PAIRS_VAR(false)
   return hipGetLastError();
 }
 

@@ -146,6 +146,21 @@ class QuantumState:
                     rows[k, q] = o
         return rows
 
+    def projection_norms(self, look, fixed=()):
+        """norm_squared() after projecting onto outcome b of qubit q for each
+        (q, b) in `fixed` and onto every outcome of the qubits in `look`
+        (API addition; k-th entry: outcome (k >> i) & 1 on look[i])."""
+        look = [int(q) for q in look]
+        fq = np.array([int(q) for q, _ in fixed] + [0], np.int32)
+        fb = np.array([1 if b else 0 for _, b in fixed] + [0], np.int32)
+        lq = np.array(look + [0], np.int32)
+        out = np.zeros(1 << len(look), np.float64)
+        P = C.POINTER(C.c_int32)
+        check(lib().nx_state_projection_norms(self._h, len(fixed), fq.ctypes.data_as(P), fb.ctypes.data_as(P),
+                                              len(look), lq.ctypes.data_as(P),
+                                              out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
     def exp_values(self, paulis) -> np.ndarray:
         """exp_value for a list of PauliStrings (one call through the C-ABI)."""
         paulis = list(paulis)
