@@ -25,7 +25,7 @@ namespace nx {
 // ---------------------------------------------------------------------------
 template <int W>
 __global__ __launch_bounds__(256) void k_init_zero(FormsDev out) {
-  const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t t = (int64_t)blockIdx.x * 4 + uni((int)(threadIdx.x >> 6));  // uniform in the wave
   const int lane = threadIdx.x & 63;
   if (t >= out.chi) return;
   Form<W> f;
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(256) void k_init_zero(FormsDev out) {
 template <int W>
 __global__ __launch_bounds__(256) void k_program(FormsDev in, FormsDev out, const ProgOp* __restrict__ prog,
                                                  int nprog, int4* __restrict__ status) {
-  const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t t = (int64_t)blockIdx.x * 4 + uni((int)(threadIdx.x >> 6));  // uniform in the wave
   const int lane = threadIdx.x & 63;
   if (t >= in.chi) return;
   Form<W> f;
@@ -182,7 +182,7 @@ __global__ void k_gather(FormsDev in, FormsDev O, const int64_t* __restrict__ id
 // k of F with v_k = 0: par(F[:,k] & x) = s_k.
 template <int W>
 __global__ __launch_bounds__(256) void k_qform(FormsDev in, QFormDev q) {
-  const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t t = (int64_t)blockIdx.x * 4 + uni((int)(threadIdx.x >> 6));  // uniform in the wave
   const int lane = threadIdx.x & 63;
   if (t >= in.chi) return;
   Form<W> f;
@@ -512,7 +512,7 @@ __global__ __launch_bounds__(256) void k_pairs(QFormDev X, QFormDev Y, const int
                                                double2* __restrict__ partial, uint32_t* __restrict__ exact_out,
                                                int64_t exact_ld) {
   const int lane = threadIdx.x & 63;
-  const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t wave = (int64_t)blockIdx.x * 4 + uni((int)(threadIdx.x >> 6));  // uniform in the wave
   const int64_t nwaves = (int64_t)gridDim.x * 4;
   const int64_t nunits = offsets[nfixed];
   const int n = X.n;

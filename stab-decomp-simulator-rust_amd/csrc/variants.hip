@@ -308,7 +308,7 @@ __global__ __launch_bounds__(256) void k_pauli_var(FormsDev in, const int32_t* _
                                                    const int32_t* __restrict__ vops, int nvar,
                                                    u64* __restrict__ vhi, u64* __restrict__ vrhs,
                                                    u64* __restrict__ vph) {
-  const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t t = (int64_t)blockIdx.x * 4 + uni((int)(threadIdx.x >> 6));  // uniform in the wave
   const int lane = threadIdx.x & 63;
   if (t >= in.chi) return;
   Form<W> f;
@@ -430,7 +430,7 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
                                                    double* __restrict__ partial) {
   constexpr int NVS = 64 * VW + 1;
   const int lane = threadIdx.x & 63;
-  const int64_t lw = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t lw = (int64_t)blockIdx.x * 4 + uni((int)(threadIdx.x >> 6));  // uniform in the wave
   if (lw >= nwaves_launch) return;
   const int64_t wave = wave_base + lw;
   const int jb = uni(wave_job[wave]);
