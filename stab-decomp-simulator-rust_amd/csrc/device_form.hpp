@@ -44,6 +44,11 @@ __device__ __forceinline__ u32 hi32(u64 x) { return (u32)(x >> 32); }
 __device__ __forceinline__ u32 xor3(u32 a, u32 b, u32 c) { return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96); }
 __device__ __forceinline__ int ffs64(u64 x) { return __builtin_ctzll(x); }
 
+// word of variable c in a row of W words (always word 0 when W = 1, which
+// the compiler cannot infer from c < 64 by itself)
+template <int W>
+__device__ __forceinline__ int wordof(int c) { return W == 1 ? 0 : (c >> 6); }
+
 template <int W>
 __device__ __forceinline__ u64 wsel(const u64 (&x)[W], int w) {
   u64 r = x[0];
@@ -54,7 +59,7 @@ __device__ __forceinline__ u64 wsel(const u64 (&x)[W], int w) {
 }
 template <int W>
 __device__ __forceinline__ int getbit(const u64 (&x)[W], int c) {
-  return (int)((wsel<W>(x, c >> 6) >> (c & 63)) & 1ull);
+  return (int)((wsel<W>(x, wordof<W>(c)) >> (c & 63)) & 1ull);
 }
 // The word updates below touch every word with a mask that is zero except
 // in word c/64: a conditional store x[k] would be folded by the compiler into
@@ -63,14 +68,14 @@ template <int W>
 __device__ __forceinline__ void flipbit(u64 (&x)[W], int c, int b) {
   const u64 m = ((u64)(b & 1)) << (c & 63);
 #pragma unroll
-  for (int k = 0; k < W; ++k) x[k] ^= ((c >> 6) == k) ? m : 0ull;
+  for (int k = 0; k < W; ++k) x[k] ^= (wordof<W>(c) == k) ? m : 0ull;
 }
 template <int W>
 __device__ __forceinline__ void setbitv(u64 (&x)[W], int c, int b) {
   const u64 clr = ~(1ull << (c & 63)), set = ((u64)(b & 1)) << (c & 63);
 #pragma unroll
   for (int k = 0; k < W; ++k) {
-    const bool here = (c >> 6) == k;
+    const bool here = wordof<W>(c) == k;
     x[k] = (x[k] & (here ? clr : ~0ull)) | (here ? set : 0ull);
   }
 }
@@ -110,7 +115,7 @@ struct Form {
 // broadcast row r of X (uniform r) to all lanes
 template <int W>
 __device__ __forceinline__ void bcast(const u64 (&X)[W][W], int r, u64 (&out)[W]) {
-  const int sl = r >> 6, ln = r & 63;
+  const int sl = wordof<W>(r), ln = r & 63;
 #pragma unroll
   for (int w = 0; w < W; ++w) {
     u64 val = X[0][w];
@@ -122,7 +127,7 @@ __device__ __forceinline__ void bcast(const u64 (&X)[W][W], int r, u64 (&out)[W]
 }
 template <int W>
 __device__ __forceinline__ int bcast_g(const Form<W>& f, int r) {
-  const int sl = r >> 6;
+  const int sl = wordof<W>(r);
   int val = f.g[0];
 #pragma unroll
   for (int k = 1; k < W; ++k)
@@ -131,7 +136,7 @@ __device__ __forceinline__ int bcast_g(const Form<W>& f, int r) {
 }
 template <int W>
 __device__ __forceinline__ bool owns(const Form<W>& f, int r, int sl) {
-  return f.lane == (r & 63) && sl == (r >> 6);
+  return f.lane == (r & 63) && sl == wordof<W>(r);
 }
 
 // ---- left multiplications (row operations) -------------------------------

@@ -84,7 +84,7 @@ __device__ __forceinline__ void clearbit(u64 (&x)[W], int pw, int pb) {
 
 template <int W>
 __device__ __forceinline__ int bcast_int(const int (&x)[W], int r) {
-  const int sl = r >> 6;
+  const int sl = wordof<W>(r);
   int val = x[0];
 #pragma unroll
   for (int k = 1; k < W; ++k)
@@ -109,7 +109,7 @@ __device__ __forceinline__ void subst(PairRegs<W>& P, int p, const u64 (&r)[W], 
   P.c0 = (P.c0 + 2 * ap * c) & 7;
   // a_l += ap (1 - 2c) r_l + 2 (c + r_l) B_pl ;  B_l ^= r_l ((ap odd) r ^ Bp) ^ B_pl r
   const int lin = c ? (4 - ap) & 3 : ap;
-  const int pw = p >> 6, pb = p & 63;
+  const int pw = wordof<W>(p), pb = p & 63;
 #pragma unroll
   for (int sl = 0; sl < W; ++sl) {
     const int rl = (int)((r[sl] >> P.lane) & 1ull);
@@ -188,7 +188,7 @@ __device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int*
   }
   while (anyv<W>(P.rem)) {
     const int p = uni(ffsv<W>(P.rem));
-    const int pw = p >> 6, pb = p & 63;
+    const int pw = wordof<W>(p), pb = p & 63;
     clearbit<W>(P.rem, pw, pb);
     u64 Bp[W];
     bcast<W>(P.B, p, Bp);

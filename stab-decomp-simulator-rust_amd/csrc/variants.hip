@@ -59,7 +59,7 @@ struct VarRegs {
 
 template <int W, int VW>
 __device__ __forceinline__ void bcast_v(const u64 (&X)[W][VW], int r, u64 (&out)[VW]) {
-  const int sl = r >> 6, ln = r & 63;
+  const int sl = wordof<W>(r), ln = r & 63;
 #pragma unroll
   for (int v = 0; v < VW; ++v) {
     u64 val = X[0][v];
@@ -130,7 +130,7 @@ __device__ __forceinline__ void subst_v(PairRegs<W>& P, VarRegs<W, VW>& V, int p
     L[v] = H[v] ^ ((ap & 1) ? E[v] : 0ull);
   }
   const int lin = c ? (4 - ap) & 3 : ap;
-  const int pw = p >> 6, pb = p & 63;
+  const int pw = wordof<W>(p), pb = p & 63;
 #pragma unroll
   for (int sl = 0; sl < W; ++sl) {
     const int rl = (int)((r[sl] >> P.lane) & 1ull);
@@ -239,7 +239,7 @@ __device__ __forceinline__ void pair_eval_v(PairRegs<W>& P, VarRegs<W, VW>& V, i
   uniform_state<W, VW>(P, V);
   while (anyv<W>(P.rem)) {
     const int p = uni(ffsv<W>(P.rem));
-    const int pw = p >> 6, pb = p & 63;
+    const int pw = wordof<W>(p), pb = p & 63;
     clearbit<W>(P.rem, pw, pb);
     u64 Bp[W];
     bcast<W>(P.B, p, Bp);
