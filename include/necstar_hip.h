@@ -107,6 +107,14 @@ int nx_state_measure(nx_state *s, int32_t nq, const int32_t *qargs, const uint8_
 int nx_state_sample(const nx_state *s, int32_t nq, const int32_t *qargs, uint64_t shots, const uint8_t *seed32,
                     int64_t max_out, uint64_t *out_bits, uint64_t *out_counts, int64_t *nout);
 
+/* QuantumCircuit::random_clifford (python/src/circuit.rs:73-79,
+ * crates/necstar/src/circuit/random_clifford.rs:20-27,
+ * crates/stabilizer-ch-form-rust/src/circuit/random_clifford.rs:7-218):
+ * the gate list (op, q0, q1, 0) of a uniformly random Clifford on n <= 1024
+ * qubits, drawn from the seed as the reference draws it.  At most
+ * 2 n^2 + 3 n gates; capacity max_gates, *ngates receives the count. */
+int nx_random_clifford(int32_t n, const uint8_t *seed32, int64_t max_gates, int32_t *gates, int64_t *ngates);
+
 /* Parity export of term i in the reference's layout: G, F, M as n*n bytes
  * (row-major), gamma n bytes (k mod 8), v, s n bytes, phase 1 byte
  * (form/mod.rs:11-22).  No reference counterpart (tables are private). */

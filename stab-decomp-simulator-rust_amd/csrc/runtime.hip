@@ -25,6 +25,7 @@
 #include "internal.hpp"
 #include "necstar_hip.h"
 #include "rng.hpp"
+#include "random_clifford.hpp"
 
 using cplx = std::complex<double>;
 
@@ -1861,6 +1862,21 @@ int nx_profile_get(const char* kernel, double* total_ms, int64_t* count) {
     auto it = g_prof.acc.find(kernel ? kernel : "");
     *total_ms = it == g_prof.acc.end() ? 0.0 : it->second.first;
     *count = it == g_prof.acc.end() ? 0 : it->second.second;
+  })
+}
+
+int nx_random_clifford(int32_t n, const uint8_t* seed32, int64_t max_gates, int32_t* gates, int64_t* ngates) {
+  NX_TRY({
+    if (n < 0) fail(NX_E_VALUE, "negative qubit count");
+    if (n > 1024) fail(NX_E_VALUE, "random Clifford circuits are limited to 1024 qubits");
+    uint8_t seed[32];
+    seed_or_entropy(seed32, seed);
+    ChaCha12Rng rng(seed);
+    const std::vector<int32_t> g = random_clifford_gates(n, rng);
+    const int64_t k = (int64_t)g.size() / 4;
+    if (k > max_gates) fail(NX_E_VALUE, "gate buffer too small (%lld gates)", (long long)k);
+    if (k) std::memcpy(gates, g.data(), g.size() * sizeof(int32_t));
+    *ngates = k;
   })
 }
 

@@ -17,6 +17,7 @@ _U8P, _U32P, _DP = C.POINTER(C.c_uint8), C.POINTER(C.c_uint32), C.POINTER(C.c_do
 SIGNATURES = {
     "nx_last_error": (C.c_char_p, []),
     "nx_init": (C.c_int, [C.c_int]),
+    "nx_random_clifford": (C.c_int, [_I32, _U8P, _I64, _I32P, _I64P]),
     "nx_state_compile": (C.c_int, [C.c_int, _I64, _I32P, C.POINTER(_P)]),
     "nx_state_clone": (C.c_int, [_P, C.POINTER(_P)]),
     "nx_state_free": (None, [_P]),

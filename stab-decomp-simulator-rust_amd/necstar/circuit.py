@@ -119,9 +119,28 @@ class QuantumCircuit:
 
     @staticmethod
     def random_clifford(n: int, seed=None) -> "QuantumCircuit":
-        raise NotImplementedError(
-            "random_clifford (Bravyi-Maslov sampling, crates/stabilizer-ch-form-rust/src/circuit/"
-            "random_clifford.rs) is outside the hot path and not provided")
+        """python/src/circuit.rs:73-79: a uniformly random Clifford circuit
+        (Bravyi-Maslov; crates/stabilizer-ch-form-rust/src/circuit/
+        random_clifford.rs:7-218), drawn on the host by the library from the
+        seed's StdRng stream."""
+        import ctypes as C
+
+        import numpy as np
+
+        from ._lib import check, lib
+        from .state import _seed_ptr
+
+        n = _usize(n)
+        _buf, sp = _seed_ptr(seed)
+        cap = 2 * n * n + 3 * n + 1 if n <= 1024 else 1  # larger n is rejected by the library
+        rows = np.zeros((cap, 4), np.int32)
+        count = C.c_int64()
+        check(lib().nx_random_clifford(min(n, 2 ** 31 - 1), sp, cap,
+                                       rows.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(count)))
+        qc = QuantumCircuit(n)
+        qc._gates = [QuantumGate(int(r[0]), tuple(int(q) for q in r[1:1 + G.ARITY[r[0]]]))
+                     for r in rows[: count.value]]
+        return qc
 
     def append(self, other: "QuantumCircuit") -> None:
         self._gates.extend(other._gates)
