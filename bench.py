@@ -170,9 +170,12 @@ def main():
         te = time.perf_counter() - ts
         pms, pcnt = C.c_double(), C.c_int64()
         _lib.check(L.nx_profile_get(b"k_pairs_var", C.byref(pms), C.byref(pcnt)))
+        sms, scnt = C.c_double(), C.c_int64()
+        _lib.check(L.nx_profile_get(b"k_pairs_sym", C.byref(sms), C.byref(scnt)))
         _lib.check(L.nx_profile_enable(0))
         sampling = {"shots": args.sample_shots, "qubits": n, "seconds": te, "distinct_outcomes": len(counts),
-                    "projection_passes": pcnt.value, "pass_kernel_ms": pms.value}
+                    "uniform_levels": st.uniform_prefix(list(range(n - 1, -1, -1))),
+                    "uniform_pass_ms": sms.value, "projection_passes": pcnt.value, "pass_kernel_ms": pms.value}
 
     ms_step = (t1 - t0) * 1e3 / args.steps
     amp_units = chi * B

@@ -139,6 +139,14 @@ int nx_state_global_factor(const nx_state *s, double *re, double *im);
 int nx_state_projection_norms(const nx_state *s, int32_t nfixed, const int32_t *fixed_q, const int32_t *fixed_b,
                               int32_t nlook, const int32_t *look_q, double *out);
 
+/* The largest D such that the Z-basis marginal of qubits qargs[0..D) is
+ * provably uniform (every outcome has probability 2^-D): no pair value
+ * <phi_i| Z^t |phi_j> with t != 0 on those qubits is nonzero, decided for
+ * all 2^D - 1 strings at once by one symbolic pass over the pairs (D <= 128).
+ * sample() draws p = 1/2 without norms for that many levels.  No reference
+ * counterpart (sampling.rs:80-189 evaluates every node's norms). */
+int nx_state_uniform_prefix(const nx_state *s, int32_t nq, const int32_t *qargs, int32_t *out);
+
 /* Exact pair scalars <phi_i|phi_j> for all i, j < chi (chi*chi uint32:
  * 0xFFFFFFFF zero, else k | r << 8).  Parity aid; the reference computes
  * the same values in StabilizerCHForm::inner_product (inner_product.rs:14-48). */

@@ -150,6 +150,12 @@ hipError_t launch_pairs_var(int W, int VW, int units, const PairJobDev* jobs, co
 // and the base sum to out[jobs[j].base_out] when that is >= 0
 hipError_t launch_reduce_var(const PairJobDev* jobs, int njobs, int VW, const double* partial, double* out,
                              hipStream_t st);
+// atomicMin into *dmin (entering as ncoord + 1) of the smallest D such that
+// some pair value <X_o| Z^t |X_f> (o >= f) with t != 0 supported on the
+// first D coordinates is nonzero; coord[r] = coordinate of qubit r or -1;
+// VW = ceil(ncoord / 64); offsets and J as for launch_pairs (triangle)
+hipError_t launch_pairs_sym(const QFormDev& X, const int64_t* offsets, int J, int VW, const int32_t* coord, int* dmin,
+                            int nblocks, hipStream_t st);
 // y = G c for the Hermitian G held in full in Gw[j*ld + i]
 hipError_t launch_gramc(const double2* Gw, int64_t ld, int64_t chi, const double2* c, double2* y, hipStream_t st);
 

@@ -1238,7 +1238,43 @@ struct Window {
   std::vector<std::vector<double>> la, lb;  // exact parts A, B of node [j][k]
 };
 
+// The largest D such that the Z-basis marginal of the qubits order[0..D)
+// is provably uniform: no pair value <phi_o| Z^t |phi_f> with t != 0
+// supported on them is nonzero, so <psi| Z^t |psi> = 0 for all such t and
+// ||Pi_b psi||^2 = 2^-D ||psi||^2 for every outcome b on them.  A sufficient
+// condition (cancellation between pairs is not used); at most 128.
+static int uniform_prefix(const nx_state* s, const std::vector<int>& order) {
+  const int64_t chi = s->chi();
+  const int ncoord = std::min<int>((int)order.size(), 128);
+  if (chi == 0 || ncoord == 0) return 0;
+  const QForms& q = qform_of(s);
+  const int J = 16;
+  std::vector<int64_t> off(chi + 1);
+  off[0] = 0;
+  for (int64_t f = 0; f < chi; ++f) off[f + 1] = off[f] + (chi - f + J - 1) / J;
+  std::vector<int32_t> coord(s->n, -1);
+  for (int k = 0; k < ncoord; ++k) coord[order[k]] = k;
+  const int32_t init = ncoord + 1;
+  DevBuf doff(off.size() * 8), dcoord(coord.size() * 4), dmin(4);
+  h2d(doff.p, off.data(), off.size() * 8);
+  h2d(dcoord.p, coord.data(), coord.size() * 4);
+  h2d(dmin.p, &init, 4);
+  const int64_t units = off[chi];
+  int nblocks = (int)std::min<int64_t>((units + 3) / 4, 4096);
+  if (nblocks < 1) nblocks = 1;
+  ProfScope ps("k_pairs_sym");
+  hip_check(launch_pairs_sym(q.desc(), doff.as<int64_t>(), J, ncoord > 64 ? 2 : 1, dcoord.as<int32_t>(),
+                             dmin.as<int>(), nblocks, stream()),
+            "k_pairs_sym");
+  ps.end();
+  int32_t d = 0;
+  d2h(&d, dmin.p, 4);
+  sync();
+  return d - 1;
+}
+
 struct SampleCtx {
+  int uniform = 0;  // the first `uniform` qubits of q have a uniform marginal
   std::vector<int> q, bit;
   std::vector<uint64_t> bits, counts;
   std::vector<int> path;  // outcomes of the processed qubits so far
@@ -1289,17 +1325,21 @@ static void recursive_sample(SampleCtx& cx, size_t idx, uint64_t shots, uint64_t
     return;
   }
   Window own;
-  if (!w || (int)idx + 1 > w->root + w->d) {
-    own = open_window(cx, (int)idx);
-    w = &own;
-    rel = 0;
+  double pz = 0.5;  // both children hold half of a uniform marginal's mass
+  int j = 0;        // children's relative depth in the window
+  if ((int)idx >= cx.uniform) {
+    if (!w || (int)idx + 1 > w->root + w->d) {
+      own = open_window(cx, (int)idx);
+      w = &own;
+      rel = 0;
+    }
+    j = (int)idx - w->root + 1;
+    const uint64_t r1 = rel | ((uint64_t)1 << (j - 1));
+    const double nz = cx.pp.value(w->la[j][rel], w->lb[j][rel]), no = cx.pp.value(w->la[j][r1], w->lb[j][r1]);
+    pz = nz / (nz + no);
+    if (!std::isnan(pz)) pz = pz < 0.0 ? 0.0 : (pz > 1.0 ? 1.0 : pz);
+    if (!(pz >= 0.0)) fail(NX_E_VALUE, "p < 0 or is NaN in binomial distribution");
   }
-  const int j = (int)idx - w->root + 1;  // children's relative depth
-  const uint64_t r1 = rel | ((uint64_t)1 << (j - 1));
-  const double nz = cx.pp.value(w->la[j][rel], w->lb[j][rel]), no = cx.pp.value(w->la[j][r1], w->lb[j][r1]);
-  double pz = nz / (nz + no);
-  if (!std::isnan(pz)) pz = pz < 0.0 ? 0.0 : (pz > 1.0 ? 1.0 : pz);
-  if (!(pz >= 0.0)) fail(NX_E_VALUE, "p < 0 or is NaN in binomial distribution");
   const uint64_t nzeros = binomial(*cx.rng, shots, pz);
   const uint64_t nones = shots - nzeros;
   const int b = cx.bit[idx];
@@ -1313,7 +1353,7 @@ static void recursive_sample(SampleCtx& cx, size_t idx, uint64_t shots, uint64_t
     if (b < 64) nlo |= 1ull << b;
     else nhi |= 1ull << (b - 64);
     cx.path.push_back(1);
-    recursive_sample(cx, idx + 1, nones, nlo, nhi, w, rel | ((uint64_t)1 << (j - 1)));
+    recursive_sample(cx, idx + 1, nones, nlo, nhi, w, j ? rel | ((uint64_t)1 << (j - 1)) : 0);
     cx.path.pop_back();
   }
 }
@@ -1574,6 +1614,8 @@ int nx_state_sample(const nx_state* s, int32_t nq, const int32_t* qargs, uint64_
     if (shots > 0 && nq > 0 && s->chi() == 0)  // every norm is 0: p = 0/0
       fail(NX_E_VALUE, "p < 0 or is NaN in binomial distribution");
     if (shots > 0 && nq > 0) {
+      // at least the last level keeps its window: it also reports a zero state
+      cx.uniform = std::min(uniform_prefix(s, cx.q), nq - 1);
       cx.pp.init(s, nq, cx.look);
       recursive_sample(cx, 0, shots, 0, 0, nullptr, 0);
     } else if (shots > 0) {
@@ -1586,6 +1628,13 @@ int nx_state_sample(const nx_state* s, int32_t nq, const int32_t* qargs, uint64_
     std::memcpy(out_bits, cx.bits.data(), k * 16);
     std::memcpy(out_counts, cx.counts.data(), k * 8);
     *nout = k;
+  })
+}
+
+int nx_state_uniform_prefix(const nx_state* s, int32_t nq, const int32_t* qargs, int32_t* out) {
+  NX_TRY({
+    validate_qargs(s, nq, qargs);
+    *out = uniform_prefix(s, std::vector<int>(qargs, qargs + nq));
   })
 }
 

@@ -106,8 +106,8 @@ __device__ __forceinline__ void uniform_state(PairRegs<W>& P, VarRegs<W, VW>& V)
 // B_pl), which differs from the base change by 2 (r_l (h_p,k ^ (a_p odd &
 // e_k)) ^ (B_pl & e_k)); the phase gains 2 (a_p,k c_k mod 4) per variant and
 // 2 (a_p c mod 4) for the base.  Bra constraints carry no variant flips
-// (E = 0 while they are taken).
-template <int W, int VW, int CONS, bool UNITS>
+// (E = 0 while they are taken).  PH = false skips the phase planes.
+template <int W, int VW, int CONS, bool UNITS, bool PH = true>
 __device__ __forceinline__ void subst_v(PairRegs<W>& P, VarRegs<W, VW>& V, int p, const u64 (&r)[W], int c,
                                         const u64 (&E)[VW]) {
   const int ap = bcast_int<W>(P.a, p);
@@ -119,14 +119,16 @@ __device__ __forceinline__ void subst_v(PairRegs<W>& P, VarRegs<W, VW>& V, int p
   u64 L[VW];
 #pragma unroll
   for (int v = 0; v < VW; ++v) {
-    // variant phase minus base phase, 2 (a_p,k c_k - a_p c) mod 8, as the
-    // sum of the full variant term and 8 - 2 a_p c (both bit-sliced)
-    const u64 cm = c ? ~E[v] : E[v];                  // variants with c_k = 1
-    const u64 lo = (ap & 1) ? cm : 0ull;              // (a_p,k c_k) mod 2
-    const u64 hi = cm & ((ap & 2) ? ~H[v] : H[v]);    // high bit of a_p,k c_k
-    z8_add(V.d0[v], V.d1[v], V.d2[v], 0ull, lo, hi);
-    const int nb = (8 - 2 * ap * c) & 7;
-    z8_add(V.d0[v], V.d1[v], V.d2[v], 0ull, (nb & 2) ? ~0ull : 0ull, (nb & 4) ? ~0ull : 0ull);
+    if (PH) {
+      // variant phase minus base phase, 2 (a_p,k c_k - a_p c) mod 8, as the
+      // sum of the full variant term and 8 - 2 a_p c (both bit-sliced)
+      const u64 cm = c ? ~E[v] : E[v];                  // variants with c_k = 1
+      const u64 lo = (ap & 1) ? cm : 0ull;              // (a_p,k c_k) mod 2
+      const u64 hi = cm & ((ap & 2) ? ~H[v] : H[v]);    // high bit of a_p,k c_k
+      z8_add(V.d0[v], V.d1[v], V.d2[v], 0ull, lo, hi);
+      const int nb = (8 - 2 * ap * c) & 7;
+      z8_add(V.d0[v], V.d1[v], V.d2[v], 0ull, (nb & 2) ? ~0ull : 0ull, (nb & 4) ? ~0ull : 0ull);
+    }
     L[v] = H[v] ^ ((ap & 1) ? E[v] : 0ull);
   }
   const int lin = c ? (4 - ap) & 3 : ap;
@@ -292,6 +294,266 @@ __device__ __forceinline__ void pair_eval_v(PairRegs<W>& P, VarRegs<W, VW>& V, i
       subst_v<W, VW, CONS_NONE, UNITS>(P, V, qv, Bp, ap >> 1, H);
     }
     uniform_state<W, VW>(P, V);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Uniform Z-marginals.  Read the ket variants as symbols: variant k is Z on
+// coordinate k of a qubit order, and bit k of a row's mask is the
+// coefficient of t_k in a linear form over GF(2).  Every update of the
+// masks above is linear in them (a substitution's constant for a variant
+// is c ^ <H_p, t>, a rank-1 update adds H_p), so after the same elimination
+// the row masks are the linear forms themselves.  Z^t never moves the
+// support, so the constraint phase is the same for every t; after it,
+// <bra| Z^t |ket> vanishes exactly when an isolated variable with even
+// coefficient a_p + 2 <H_p, t> = 2 (mod 4), i.e. <H_p, t> != a_p / 2.  The
+// pair is therefore nonzero exactly for the t with <H_p, t> = a_p / 2 at
+// every isolated even pivot p: those equations are collected, one per lane
+// (lane e % 64 of slot e / 64 holds equation e), constants in em.
+// Returns 1 if the pair vanishes for every t (an inconsistent constraint).
+template <int W, int VW, bool USEB>
+__device__ __forceinline__ int take_constraint_sym(PairRegs<W>& P, VarRegs<W, VW>& V, int c) {
+  u64 row[W];
+  bcast<W>(USEB ? P.cb : P.ca, c, row);
+  const int rhs = USEB ? getbit<W>(P.rhsb, c) : getbit<W>(P.rhsa, c);
+  if (!anyv<W>(row)) return rhs;
+  const int p = ffsv<W>(row);
+  flipbit<W>(row, p, 1);
+  u64 E[VW];
+#pragma unroll
+  for (int v = 0; v < VW; ++v) E[v] = 0ull;
+  subst_v<W, VW, USEB ? CONS_B : CONS_AB, false, false>(P, V, p, row, rhs, E);
+  return 0;
+}
+
+template <int W, int VW>
+__device__ __forceinline__ int pair_eval_sym(PairRegs<W>& P, VarRegs<W, VW>& V, int nca, int ncb,
+                                             u64 (&EQ)[W][VW], u64 (&em)[W], int& neq) {
+  uniform_state<W>(P);
+  for (int c = 0; c < nca; ++c) {
+    if (uni(take_constraint_sym<W, VW, false>(P, V, c))) return 1;
+    uniform_state<W>(P);
+  }
+  for (int c = 0; c < ncb; ++c) {
+    if (uni(take_constraint_sym<W, VW, true>(P, V, c))) return 1;
+    uniform_state<W>(P);
+  }
+  neq = 0;
+#pragma unroll
+  for (int s = 0; s < W; ++s) em[s] = 0ull;
+  while (anyv<W>(P.rem)) {
+    const int p = uni(ffsv<W>(P.rem));
+    const int pw = wordof<W>(p), pb = p & 63;
+    clearbit<W>(P.rem, pw, pb);
+    u64 Bp[W];
+    bcast<W>(P.B, p, Bp);
+    const int ap = bcast_int<W>(P.a, p);
+    u64 H[VW];
+    bcast_v<W, VW>(V.hi, p, H);
+    if (!anyv<W>(Bp)) {
+      if (!(ap & 1)) {
+        const int slot = neq >> 6, ln = neq & 63;
+#pragma unroll
+        for (int s = 0; s < W; ++s) {
+#pragma unroll
+          for (int v = 0; v < VW; ++v) EQ[s][v] = (s == slot && P.lane == ln) ? H[v] : EQ[s][v];
+          if (s == slot && (ap & 2)) em[s] |= 1ull << ln;
+        }
+        ++neq;
+      }
+    } else if (ap & 1) {
+      const int add = (ap == 1) ? 3 : 1;
+#pragma unroll
+      for (int sl = 0; sl < W; ++sl) {
+        const int bpl = (int)((wsel<W>(P.B[sl], pw) >> pb) & 1ull);
+        const u64 mb = 0ull - (u64)bpl;
+        P.a[sl] = (P.a[sl] + (bpl ? add : 0)) & 3;
+#pragma unroll
+        for (int v = 0; v < VW; ++v) V.hi[sl][v] ^= H[v] & mb;
+#pragma unroll
+        for (int w = 0; w < W; ++w) P.B[sl][w] ^= Bp[w] & mb;
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+          if (w == sl) P.B[sl][w] &= ~(1ull << P.lane);
+        clearbit<W>(P.B[sl], pw, pb);
+      }
+    } else {
+#pragma unroll
+      for (int sl = 0; sl < W; ++sl) clearbit<W>(P.B[sl], pw, pb);
+      const int qv = ffsv<W>(Bp);
+      flipbit<W>(Bp, qv, 1);
+      subst_v<W, VW, CONS_NONE, false, false>(P, V, qv, Bp, ap >> 1, H);
+    }
+    uniform_state<W>(P);
+    neq = uni(neq);
+#pragma unroll
+    for (int s = 0; s < W; ++s) em[s] = uni64(em[s]);
+  }
+  return 0;
+}
+
+// Smallest D <= lim such that the equations have a solution t != 0
+// supported on coordinates [0, D); -1 if there is none.  Columns
+// are taken in order by a lane-parallel elimination over the equation rows:
+// with all constants zero, the first column that is not a pivot (it lies in
+// the span of the earlier columns) gives D; otherwise D is the first column
+// after which every non-pivot row has a zero constant (the constants lie in
+// the span of the columns so far).
+template <int W, int VW>
+__device__ __forceinline__ int first_solution(u64 (&EQ)[W][VW], u64 (&em)[W], int neq, int lim, int lane) {
+  u64 valid[W], used[W];
+  u64 anye = 0ull;
+#pragma unroll
+  for (int s = 0; s < W; ++s) {
+    const int cnt = neq - 64 * s;
+    valid[s] = cnt >= 64 ? ~0ull : (cnt <= 0 ? 0ull : ((1ull << cnt) - 1ull));
+    used[s] = 0ull;
+    em[s] &= valid[s];
+    anye |= em[s];
+  }
+  const bool homog = anye == 0ull;
+  for (int c = 0; c < lim; ++c) {
+    const int cw = c >> 6, cb = c & 63;
+    int ps = -1;
+    u64 cand = 0ull;
+    u64 hasm[W];
+#pragma unroll
+    for (int s = 0; s < W; ++s) {
+      u64 colw = EQ[s][0];
+#pragma unroll
+      for (int v = 1; v < VW; ++v)
+        if (cw == v) colw = EQ[s][v];
+      hasm[s] = uni64(__ballot((int)((colw >> cb) & 1ull))) & valid[s];
+      const u64 cs = hasm[s] & ~used[s];
+      if (ps < 0 && cs != 0ull) {
+        ps = s;
+        cand = cs;
+      }
+    }
+    if (ps < 0) {
+      if (homog) return c + 1;
+      continue;
+    }
+    const int pl = (int)__builtin_ctzll(cand);
+    u64 piv[VW];
+#pragma unroll
+    for (int v = 0; v < VW; ++v) {
+      u64 x = EQ[0][v];
+#pragma unroll
+      for (int s = 1; s < W; ++s)
+        if (ps == s) x = EQ[s][v];
+      piv[v] = rl64(x, pl);
+    }
+    u64 pe = 0ull;
+#pragma unroll
+    for (int s = 0; s < W; ++s)
+      if (ps == s) pe = (em[s] >> pl) & 1ull;
+#pragma unroll
+    for (int s = 0; s < W; ++s) {
+      const u64 own = (ps == s) ? (1ull << pl) : 0ull;
+      used[s] |= own;
+      const u64 hit = hasm[s] & ~own;
+      const bool mine = (hit >> lane) & 1ull;
+#pragma unroll
+      for (int v = 0; v < VW; ++v) EQ[s][v] ^= mine ? piv[v] : 0ull;
+      if (pe) em[s] ^= hit;
+    }
+    if (!homog) {
+      u64 pend = 0ull;
+#pragma unroll
+      for (int s = 0; s < W; ++s) pend |= em[s] & ~used[s];
+      if (pend == 0ull) return c + 1;
+    }
+  }
+  return -1;
+}
+
+// Over all pairs (bra o >= ket f) of one term set, the smallest D for which
+// some <phi_o| Z^t |phi_f> with t != 0 supported on the first D coordinates
+// is nonzero: atomicMin into *dmin (which enters as ncoord + 1).  coord[r]
+// is the coordinate of qubit r (-1 when it is not tracked).  Units and
+// offsets as in k_pairs (triangle, J bras per unit).
+template <int W, int VW>
+__global__ __launch_bounds__(256) void k_pairs_sym(QFormDev X, const int64_t* __restrict__ offsets, int64_t chi,
+                                                   int J, const int32_t* __restrict__ coord,
+                                                   int* __restrict__ dmin) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t)blockIdx.x * 4 + uni((int)(threadIdx.x >> 6));  // uniform in the wave
+  const int64_t nwaves = (int64_t)gridDim.x * 4;
+  const int64_t nunits = offsets[chi];
+  const int n = X.n;
+  u64 valid[W];
+  u64 h0[W][VW];
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const int hi = n - 64 * w;
+    valid[w] = hi >= 64 ? ~0ull : (hi <= 0 ? 0ull : ((1ull << hi) - 1ull));
+    const int r = 64 * w + lane;
+    const int cr = r < n ? coord[r] : -1;
+#pragma unroll
+    for (int v = 0; v < VW; ++v) h0[w][v] = (cr >= 0 && (cr >> 6) == v) ? (1ull << (cr & 63)) : 0ull;
+  }
+  for (int64_t u = wave; u < nunits; u += nwaves) {
+    int64_t lo_r = 0, hi_r = chi;
+    while (hi_r - lo_r > 1) {
+      const int64_t mid = (lo_r + hi_r) >> 1;
+      if (offsets[mid] <= u) lo_r = mid;
+      else hi_r = mid;
+    }
+    const int64_t fx = lo_r;
+    const int64_t o_lo = fx + (u - offsets[fx]) * J;
+    const int64_t o_hi = (o_lo + J < chi) ? o_lo + J : chi;
+    const int ncf = uni(X.nc[fx]);
+    for (int64_t o = o_lo; o < o_hi; ++o) {
+      const int cap = uni(__hip_atomic_load(dmin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      if (cap <= 1) return;
+      u64 Bf[W][W], Cf[W][W], rf[W];
+      int af[W];
+      load_qrows<W>(X, fx, lane, Bf, af, Cf, rf);
+      u64 Bo[W][W], Co[W][W], ro[W];
+      int ao[W];
+      load_qrows<W>(X, o, lane, Bo, ao, Co, ro);
+      const int nco = uni(X.nc[o]);
+      PairRegs<W> P;
+      VarRegs<W, VW> V;
+      P.lane = lane;
+#pragma unroll
+      for (int sl = 0; sl < W; ++sl) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          P.B[sl][w] = Bf[sl][w] ^ Bo[sl][w];
+          P.ca[sl][w] = Co[sl][w];
+          P.cb[sl][w] = Cf[sl][w];
+        }
+        P.a[sl] = (af[sl] - ao[sl]) & 3;
+#pragma unroll
+        for (int v = 0; v < VW; ++v) {
+          V.hi[sl][v] = h0[sl][v];
+          V.ra[sl][v] = V.rb[sl][v] = 0ull;
+        }
+      }
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        P.rhsa[w] = ro[w];
+        P.rhsb[w] = rf[w];
+        P.rem[w] = valid[w];
+      }
+#pragma unroll
+      for (int v = 0; v < VW; ++v) V.d0[v] = V.d1[v] = V.d2[v] = V.z[v] = 0ull;
+      V.bz = 0;
+      P.c0 = 0;
+      P.mag = 0;
+      u64 EQ[W][VW], em[W];
+#pragma unroll
+      for (int s = 0; s < W; ++s)
+#pragma unroll
+        for (int v = 0; v < VW; ++v) EQ[s][v] = 0ull;
+      int neq = 0;
+      if (pair_eval_sym<W, VW>(P, V, nco, ncf, EQ, em, neq)) continue;
+      // only D < cap can lower the minimum
+      const int d = first_solution<W, VW>(EQ, em, neq, cap - 1, lane);
+      if (d > 0 && lane == 0) atomicMin(dmin, d);
+    }
   }
 }
 
@@ -719,6 +981,14 @@ hipError_t launch_pairs_var(int W, int VW, int units, const PairJobDev* jobs, co
   if (nwaves == 0) return hipSuccess;
   if (units) NX_LAUNCH_PAIRS_VAR(true)
   else NX_LAUNCH_PAIRS_VAR(false)
+  return hipGetLastError();
+}
+
+hipError_t launch_pairs_sym(const QFormDev& X, const int64_t* offsets, int J, int VW, const int32_t* coord, int* dmin,
+                            int nblocks, hipStream_t st) {
+  if (X.chi == 0) return hipSuccess;
+  NX_DISPATCH_WV(X.W, VW,
+                 k_pairs_sym<WW, VV><<<dim3(nblocks), dim3(256), 0, st>>>(X, offsets, X.chi, J, coord, dmin));
   return hipGetLastError();
 }
 

@@ -161,6 +161,14 @@ class QuantumState:
                                               out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 
+    def uniform_prefix(self, qargs) -> int:
+        """The largest D such that the Z-basis marginal of qargs[:D] is
+        provably uniform (API addition; sample() skips those levels)."""
+        arr, ptr = _i32_array(qargs)
+        out = C.c_int32()
+        check(lib().nx_state_uniform_prefix(self._h, len(qargs), ptr, C.byref(out)))
+        return out.value
+
     def exp_values(self, paulis) -> np.ndarray:
         """exp_value for a list of PauliStrings (one call through the C-ABI)."""
         paulis = list(paulis)
