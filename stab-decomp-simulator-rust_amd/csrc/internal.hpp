@@ -108,6 +108,9 @@ struct PairJobDev {
   // split_ab != 0: the variant sums are written as their dyadic part A at
   // out[out_base + k] and sqrt(2) part B at out[out_base + nvar + k]
   int32_t split_ab;
+  // variables already substituted in X and Y (forms projected by
+  // launch_qproject): they take no part in the elimination
+  uint64_t removed[4];
 };
 
 // FP64 tables computed on the host with the same libm calls as the
@@ -156,6 +159,10 @@ hipError_t launch_reduce_var(const PairJobDev* jobs, int njobs, int VW, const do
 // VW = ceil(ncoord / 64); offsets and J as for launch_pairs (triangle)
 hipError_t launch_pairs_sym(const QFormDev& X, const int64_t* offsets, int J, int VW, const int32_t* coord, int* dmin,
                             int nblocks, hipStream_t st);
+// out = in with x_{uq[u]} = ub[u] substituted into every term's form, u <
+// nunit (the fields the pair kernels read: B, C, a, rhs, p0, nc, k)
+hipError_t launch_qproject(const QFormDev& in, const QFormDev& out, const int32_t* uq, const int32_t* ub, int nunit,
+                           hipStream_t st);
 // y = G c for the Hermitian G held in full in Gw[j*ld + i]
 hipError_t launch_gramc(const double2* Gw, int64_t ld, int64_t chi, const double2* c, double2* y, hipStream_t st);
 

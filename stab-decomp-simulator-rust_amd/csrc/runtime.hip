@@ -275,6 +275,20 @@ struct QForms {
     k.alloc(chi * 4);
     p0.alloc(chi);
   }
+  // only the fields the pair kernels read (projected forms)
+  void alloc_pair(int64_t chi_, int n_) {
+    chi = chi_;
+    n = n_;
+    W = words_for(n_);
+    const size_t R = 64 * (size_t)W;
+    B.alloc(chi * R * W * 8);
+    C.alloc(chi * R * W * 8);
+    a.alloc(chi * R);
+    rhs.alloc(chi * W * 8);
+    nc.alloc(chi * 4);
+    k.alloc(chi * 4);
+    p0.alloc(chi);
+  }
   QFormDev desc() const {
     QFormDev d;
     d.Q = Q.as<uint64_t>(); d.B = B.as<uint64_t>(); d.C = C.as<uint64_t>(); d.CT = CT.as<uint64_t>();
@@ -999,6 +1013,7 @@ static VarPlan make_plan(const std::vector<VarJob>& in, int VW, int J, int64_t t
     d.uq = nullptr;
     d.ub = nullptr;
     d.split_ab = j.split_ab;
+    for (int w = 0; w < 4; ++w) d.removed[w] = 0ull;
     d.wave0 = wave;
     d.nwaves = nw;
     d.out_base = j.out_base;
@@ -1162,7 +1177,8 @@ static std::vector<double> exp_values_var(const nx_state* s, const PauliSet& ps,
 struct ProjPlan {
   const nx_state* s = nullptr;
   VarPlan plan;
-  DevBuf uq, ub;
+  DevBuf uq, ub, fq_d, fb_d;
+  QForms proj;  // the forms projected onto the fixed outcomes of a window
   std::vector<PairJobDev> hj;
   int VW = 1, maxunit = 0;
   double g2 = 1.0;
@@ -1183,6 +1199,9 @@ struct ProjPlan {
     d2h(hj.data(), plan.jobs.p, sizeof(PairJobDev));
     uq.alloc((size_t)std::max(max_units, 1) * 4);
     ub.alloc((size_t)std::max(max_units, 1) * (VW + 1) * 8);
+    fq_d.alloc((size_t)std::max(max_units, 1) * 4);
+    fb_d.alloc((size_t)std::max(max_units, 1) * 4);
+    proj.alloc_pair(st->chi(), st->n);
     sync();
     g2 = st->g.real() * st->g.real() + st->g.imag() * st->g.imag();
   }
@@ -1190,13 +1209,29 @@ struct ProjPlan {
   // as exact parts: value = A[k] + sqrt(2) B[k] (A, B = the returned halves)
   std::vector<double> run(const std::vector<int>& fq, const std::vector<int>& fb, const std::vector<int>& look) {
     const int d = (int)look.size(), nv = 1 << d;
+    if ((int)(fq.size() + look.size()) > maxunit) fail(NX_E_INTERNAL, "too many projected qubits");
+    // the fixed outcomes are substituted once per term (k_qproject); the
+    // look-ahead qubits stay unit constraints with one outcome per variant
+    const QForms& q0 = qform_of(s);
+    for (int w = 0; w < 4; ++w) hj[0].removed[w] = 0ull;
+    if (!fq.empty()) {
+      std::vector<int32_t> fq32(fq.begin(), fq.end()), fb32(fb.begin(), fb.end());
+      h2d(fq_d.p, fq32.data(), fq32.size() * 4);
+      h2d(fb_d.p, fb32.data(), fb32.size() * 4);
+      {
+        ProfScope ps("k_qproject");
+        hip_check(launch_qproject(q0.desc(), proj.desc(), fq_d.as<int32_t>(), fb_d.as<int32_t>(), (int)fq.size(),
+                                  stream()),
+                  "k_qproject");
+        ps.end();
+      }
+      hj[0].X = hj[0].Y = proj.desc();
+      for (int qq : fq) hj[0].removed[qq >> 6] |= 1ull << (qq & 63);
+    } else {
+      hj[0].X = hj[0].Y = q0.desc();
+    }
     std::vector<int32_t> q;
     std::vector<uint64_t> b;
-    for (size_t i = 0; i < fq.size(); ++i) {
-      q.push_back(fq[i]);
-      b.push_back((uint64_t)fb[i]);
-      for (int v = 0; v < VW; ++v) b.push_back(0ull);
-    }
     for (int i = 0; i < d; ++i) {
       q.push_back(look[i]);
       b.push_back(0ull);
@@ -1207,9 +1242,10 @@ struct ProjPlan {
         b.push_back(m);
       }
     }
-    if ((int)q.size() > maxunit) fail(NX_E_INTERNAL, "too many projected qubits");
-    h2d(uq.p, q.data(), q.size() * 4);
-    h2d(ub.p, b.data(), b.size() * 8);
+    if (!q.empty()) {
+      h2d(uq.p, q.data(), q.size() * 4);
+      h2d(ub.p, b.data(), b.size() * 8);
+    }
     hj[0].nunit = (int32_t)q.size();
     hj[0].uq = uq.as<int32_t>();
     hj[0].ub = ub.as<uint64_t>();

@@ -298,6 +298,64 @@ __device__ __forceinline__ void pair_eval_v(PairRegs<W>& P, VarRegs<W, VW>& V, i
 }
 
 // ---------------------------------------------------------------------------
+// Projection of every term's quadratic form onto x_{uq[u]} = ub[u], u <
+// nunit in order: the substitutions pair_eval_v makes for unit constraints,
+// applied to one form at a time.  A unit substitution x_p = c is affine in
+// the form: a_l gains 2 c B_pl, the phase 2 a_p c, and every constraint
+// holding x_p loses it and flips its right-hand side by c.  A pair of
+// projected forms (with the substituted variables removed from the
+// elimination) therefore carries what the pair of unprojected forms carries
+// after the same substitutions, and the prefix of a sampling window costs
+// chi instead of chi^2 substitutions.  Rows of substituted variables keep
+// junk that is never read.
+template <int W>
+__global__ __launch_bounds__(256) void k_qproject(QFormDev in, QFormDev out, const int32_t* __restrict__ uq,
+                                                  const int32_t* __restrict__ ub, int nunit) {
+  const int64_t t = (int64_t)blockIdx.x * 4 + uni((int)(threadIdx.x >> 6));  // uniform in the wave
+  const int lane = threadIdx.x & 63;
+  if (t >= in.chi) return;
+  const int R = 64 * W;
+  PairRegs<W> P;
+  P.lane = lane;
+  load_qrows<W>(in, t, lane, P.B, P.a, P.ca, P.rhsa);
+#pragma unroll
+  for (int sl = 0; sl < W; ++sl)
+#pragma unroll
+    for (int w = 0; w < W; ++w) P.cb[sl][w] = 0ull;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    P.rhsb[w] = 0ull;
+    P.rem[w] = 0ull;
+  }
+  P.c0 = uni((int)in.p0[t]);
+  P.mag = 0;
+  u64 r0[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) r0[w] = 0ull;
+  for (int u = 0; u < nunit; ++u) {
+    subst<W, CONS_AB>(P, uni(uq[u]), r0, uni(ub[u]));
+    uniform_state<W>(P);
+  }
+#pragma unroll
+  for (int sl = 0; sl < W; ++sl) {
+    const int r = 64 * sl + lane;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      out.B[rowidx(t, R, r, W, w)] = P.B[sl][w];
+      out.C[rowidx(t, R, r, W, w)] = P.ca[sl][w];
+    }
+    out.a[(size_t)t * R + r] = (uint8_t)P.a[sl];
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) out.rhs[(size_t)t * W + w] = P.rhsa[w];
+    out.p0[t] = (uint8_t)P.c0;
+    out.nc[t] = in.nc[t];
+    out.k[t] = in.k[t];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Uniform Z-marginals.  Read the ket variants as symbols: variant k is Z on
 // coordinate k of a qubit order, and bit k of a row's mask is the
 // coefficient of t_k in a linear form over GF(2).  Every update of the
@@ -708,6 +766,7 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
   for (int w = 0; w < W; ++w) {
     const int hi = n - 64 * w;
     valid[w] = hi >= 64 ? ~0ull : (hi <= 0 ? 0ull : ((1ull << hi) - 1ull));
+    valid[w] &= ~uni64(jd.removed[w]);
   }
   u64 invalid[VW];
 #pragma unroll
@@ -981,6 +1040,20 @@ hipError_t launch_pairs_var(int W, int VW, int units, const PairJobDev* jobs, co
   if (nwaves == 0) return hipSuccess;
   if (units) NX_LAUNCH_PAIRS_VAR(true)
   else NX_LAUNCH_PAIRS_VAR(false)
+  return hipGetLastError();
+}
+
+hipError_t launch_qproject(const QFormDev& in, const QFormDev& out, const int32_t* uq, const int32_t* ub, int nunit,
+                           hipStream_t st) {
+  if (in.chi == 0) return hipSuccess;
+  const dim3 g(nblk(in.chi, 4)), b(256);
+  switch (in.W) {
+    case 1: k_qproject<1><<<g, b, 0, st>>>(in, out, uq, ub, nunit); break;
+    case 2: k_qproject<2><<<g, b, 0, st>>>(in, out, uq, ub, nunit); break;
+    case 3: k_qproject<3><<<g, b, 0, st>>>(in, out, uq, ub, nunit); break;
+    case 4: k_qproject<4><<<g, b, 0, st>>>(in, out, uq, ub, nunit); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
