@@ -164,10 +164,13 @@ def main():
 
     sampling = None
     if args.sample_shots > 0:
-        _lib.check(L.nx_profile_enable(1))
         ts = time.perf_counter()
         counts = st.sample(list(range(n)), args.sample_shots, seed=cfg["sample_seed"])
         te = time.perf_counter() - ts
+        # again with per-launch timing (synchronizing) for the pass counts
+        _lib.check(L.nx_profile_enable(1))
+        counts2 = st.sample(list(range(n)), args.sample_shots, seed=cfg["sample_seed"])
+        assert counts2 == counts
         pms, pcnt = C.c_double(), C.c_int64()
         _lib.check(L.nx_profile_get(b"k_pairs_var", C.byref(pms), C.byref(pcnt)))
         sms, scnt = C.c_double(), C.c_int64()

@@ -170,6 +170,29 @@ struct DevBuf {
   T* as() const { return (T*)p; }
 };
 
+// page-locked host memory (staging for small per-pass uploads)
+struct PinnedBuf {
+  void* p = nullptr;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  void alloc(size_t bytes) {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+      p = nullptr;
+      fail(NX_E_DEVICE, "out of pinned host memory (%zu bytes)", bytes);
+    }
+  }
+  template <class T>
+  T* as(size_t off = 0) const {
+    return (T*)((char*)p + off);
+  }
+};
+
 static hipStream_t stream() { return ctx()->stream; }
 static void sync() { hip_check(hipStreamSynchronize(stream()), "hipStreamSynchronize"); }
 static void h2d(void* d, const void* h, size_t b) {
@@ -940,13 +963,14 @@ struct VarPlan {
   };
   std::vector<Launch> launches;
   DevBuf jobs, wave_job, row_off, partial, out;
-  void enqueue(hipStream_t st) const {
+  // jobs_dev: the job table to read (default: this plan's own)
+  void enqueue(hipStream_t st, const PairJobDev* jobs_dev = nullptr) const {
+    const PairJobDev* jd = jobs_dev ? jobs_dev : jobs.as<PairJobDev>();
     for (const Launch& L : launches)
-      hip_check(launch_pairs_var(L.W, VW, units, jobs.as<PairJobDev>(), wave_job.as<int32_t>(), L.wave_base,
-                                 L.nwaves, J, ctx()->tabs, partial.as<double>(), st),
+      hip_check(launch_pairs_var(L.W, VW, units, jd, wave_job.as<int32_t>(), L.wave_base, L.nwaves, J, ctx()->tabs,
+                                 partial.as<double>(), st),
                 "k_pairs_var");
-    hip_check(launch_reduce_var(jobs.as<PairJobDev>(), njobs, VW, partial.as<double>(), out.as<double>(), st),
-              "k_reduce_var");
+    hip_check(launch_reduce_var(jd, njobs, VW, partial.as<double>(), out.as<double>(), st), "k_reduce_var");
   }
 };
 
@@ -1177,11 +1201,15 @@ static std::vector<double> exp_values_var(const nx_state* s, const PauliSet& ps,
 struct ProjPlan {
   const nx_state* s = nullptr;
   VarPlan plan;
-  DevBuf uq, ub, fq_d, fb_d;
   QForms proj;  // the forms projected onto the fixed outcomes of a window
   std::vector<PairJobDev> hj;
   int VW = 1, maxunit = 0;
   double g2 = 1.0;
+  // one upload per pass: [job | fixed qubits | fixed outcomes | look-ahead
+  // qubits | look-ahead outcome masks], staged in page-locked memory
+  DevBuf stage_d;
+  PinnedBuf stage_h, res_h;
+  size_t o_fq = 0, o_fb = 0, o_uq = 0, o_ub = 0, stage_bytes = 0;
   void init(const nx_state* st, int max_units, int max_look) {
     s = st;
     VW = (1 << max_look) <= 64 ? 1 : 2;
@@ -1197,10 +1225,16 @@ struct ProjPlan {
     plan = make_plan({j}, VW, shape.first, shape.second);
     hj.resize(1);
     d2h(hj.data(), plan.jobs.p, sizeof(PairJobDev));
-    uq.alloc((size_t)std::max(max_units, 1) * 4);
-    ub.alloc((size_t)std::max(max_units, 1) * (VW + 1) * 8);
-    fq_d.alloc((size_t)std::max(max_units, 1) * 4);
-    fb_d.alloc((size_t)std::max(max_units, 1) * 4);
+    const size_t mu = (size_t)std::max(max_units, 1);
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    o_fq = up(sizeof(PairJobDev));
+    o_fb = o_fq + up(mu * 4);
+    o_uq = o_fb + up(mu * 4);
+    o_ub = o_uq + up(mu * 4);
+    stage_bytes = o_ub + up(mu * (VW + 1) * 8);
+    stage_d.alloc(stage_bytes);
+    stage_h.alloc(stage_bytes);
+    res_h.alloc((size_t)2 * 64 * VW * 8);
     proj.alloc_pair(st->chi(), st->n);
     sync();
     g2 = st->g.real() * st->g.real() + st->g.imag() * st->g.imag();
@@ -1210,56 +1244,57 @@ struct ProjPlan {
   std::vector<double> run(const std::vector<int>& fq, const std::vector<int>& fb, const std::vector<int>& look) {
     const int d = (int)look.size(), nv = 1 << d;
     if ((int)(fq.size() + look.size()) > maxunit) fail(NX_E_INTERNAL, "too many projected qubits");
+    if (nv > 64 * VW) fail(NX_E_INTERNAL, "too many look-ahead qubits");
     // the fixed outcomes are substituted once per term (k_qproject); the
     // look-ahead qubits stay unit constraints with one outcome per variant
     const QForms& q0 = qform_of(s);
-    for (int w = 0; w < 4; ++w) hj[0].removed[w] = 0ull;
+    char* dbase = (char*)stage_d.p;
+    PairJobDev& job = hj[0];
+    for (int w = 0; w < 4; ++w) job.removed[w] = 0ull;
     if (!fq.empty()) {
-      std::vector<int32_t> fq32(fq.begin(), fq.end()), fb32(fb.begin(), fb.end());
-      h2d(fq_d.p, fq32.data(), fq32.size() * 4);
-      h2d(fb_d.p, fb32.data(), fb32.size() * 4);
-      {
-        ProfScope ps("k_qproject");
-        hip_check(launch_qproject(q0.desc(), proj.desc(), fq_d.as<int32_t>(), fb_d.as<int32_t>(), (int)fq.size(),
-                                  stream()),
-                  "k_qproject");
-        ps.end();
+      for (size_t i = 0; i < fq.size(); ++i) {
+        stage_h.as<int32_t>(o_fq)[i] = fq[i];
+        stage_h.as<int32_t>(o_fb)[i] = fb[i];
+        job.removed[fq[i] >> 6] |= 1ull << (fq[i] & 63);
       }
-      hj[0].X = hj[0].Y = proj.desc();
-      for (int qq : fq) hj[0].removed[qq >> 6] |= 1ull << (qq & 63);
+      job.X = job.Y = proj.desc();
     } else {
-      hj[0].X = hj[0].Y = q0.desc();
+      job.X = job.Y = q0.desc();
     }
-    std::vector<int32_t> q;
-    std::vector<uint64_t> b;
+    int32_t* uq = stage_h.as<int32_t>(o_uq);
+    uint64_t* ub = stage_h.as<uint64_t>(o_ub);
     for (int i = 0; i < d; ++i) {
-      q.push_back(look[i]);
-      b.push_back(0ull);
+      uq[i] = look[i];
+      uint64_t* e = ub + (size_t)i * (VW + 1);
+      e[0] = 0ull;
       for (int v = 0; v < VW; ++v) {
         uint64_t m = 0;
         for (int k = 0; k < 64; ++k)
           if (64 * v + k < nv && (((64 * v + k) >> i) & 1)) m |= 1ull << k;
-        b.push_back(m);
+        e[1 + v] = m;
       }
     }
-    if (!q.empty()) {
-      h2d(uq.p, q.data(), q.size() * 4);
-      h2d(ub.p, b.data(), b.size() * 8);
-    }
-    hj[0].nunit = (int32_t)q.size();
-    hj[0].uq = uq.as<int32_t>();
-    hj[0].ub = ub.as<uint64_t>();
-    hj[0].nvar = nv;
-    h2d(plan.jobs.p, hj.data(), sizeof(PairJobDev));
-    {
-      ProfScope ps("k_pairs_var");
-      plan.enqueue(stream());
+    job.nunit = (int32_t)d;
+    job.uq = (const int32_t*)(dbase + o_uq);
+    job.ub = (const uint64_t*)(dbase + o_ub);
+    job.nvar = nv;
+    std::memcpy(stage_h.p, &job, sizeof(PairJobDev));
+    h2d(stage_d.p, stage_h.p, stage_bytes);
+    if (!fq.empty()) {
+      ProfScope ps("k_qproject");
+      hip_check(launch_qproject(q0.desc(), proj.desc(), (const int32_t*)(dbase + o_fq),
+                                (const int32_t*)(dbase + o_fb), (int)fq.size(), stream()),
+                "k_qproject");
       ps.end();
     }
-    std::vector<double> o(2 * nv);
-    d2h(o.data(), plan.out.p, (size_t)nv * 16);
+    {
+      ProfScope ps("k_pairs_var");
+      plan.enqueue(stream(), (const PairJobDev*)dbase);
+      ps.end();
+    }
+    d2h(res_h.p, plan.out.p, (size_t)nv * 16);
     sync();
-    return o;
+    return std::vector<double>(res_h.as<double>(), res_h.as<double>() + 2 * nv);
   }
   // the reference's norm_squared: (sum) * |g|^2, the sum being A + sqrt(2) B
   double value(double A, double B) const { return (A + 1.4142135623730951 * B) * g2; }
