@@ -111,7 +111,13 @@ struct PairJobDev {
   // variables already substituted in X and Y (forms projected by
   // launch_qproject): they take no part in the elimination
   uint64_t removed[4];
+  // reduction segments seg0 .. seg0 + nseg - 1 (RED_SEG waves each)
+  int64_t seg0;
+  int32_t nseg;
 };
+
+// waves per first-stage reduction segment
+enum : int { RED_SEG = 128 };
 
 // FP64 tables computed on the host with the same libm calls as the
 // reference (cos/sin of k*pi/4, powf(2, -r/2)).
@@ -150,9 +156,11 @@ hipError_t launch_pairs_var(int W, int VW, int units, const PairJobDev* jobs, co
                             int64_t wave_base, int64_t nwaves, int J, const double* tabs, double* partial,
                             hipStream_t st);
 // out[jobs[j].out_base + k] = sum over the job's waves of partial (k < nvar),
-// and the base sum to out[jobs[j].base_out] when that is >= 0
-hipError_t launch_reduce_var(const PairJobDev* jobs, int njobs, int VW, const double* partial, double* out,
-                             hipStream_t st);
+// and the base sum to out[jobs[j].base_out] when that is >= 0: first per
+// segment of RED_SEG waves (seg_job[s] = job of segment s) into seg_part,
+// then per job over its segments, both in a fixed order
+hipError_t launch_reduce_var(const PairJobDev* jobs, int njobs, int VW, const double* partial, const int32_t* seg_job,
+                             int64_t nseg, double* seg_part, double* out, hipStream_t st);
 // atomicMin into *dmin (entering as ncoord + 1) of the smallest D such that
 // some pair value <X_o| Z^t |X_f> (o >= f) with t != 0 supported on the
 // first D coordinates is nonzero; coord[r] = coordinate of qubit r or -1;

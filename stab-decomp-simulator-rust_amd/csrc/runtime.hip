@@ -962,7 +962,8 @@ struct VarPlan {
     int64_t wave_base, nwaves;
   };
   std::vector<Launch> launches;
-  DevBuf jobs, wave_job, row_off, partial, out;
+  int64_t nseg = 0;
+  DevBuf jobs, wave_job, row_off, partial, out, seg_job, seg_part;
   // jobs_dev: the job table to read (default: this plan's own)
   void enqueue(hipStream_t st, const PairJobDev* jobs_dev = nullptr) const {
     const PairJobDev* jd = jobs_dev ? jobs_dev : jobs.as<PairJobDev>();
@@ -970,7 +971,9 @@ struct VarPlan {
       hip_check(launch_pairs_var(L.W, VW, units, jd, wave_job.as<int32_t>(), L.wave_base, L.nwaves, J, ctx()->tabs,
                                  partial.as<double>(), st),
                 "k_pairs_var");
-    hip_check(launch_reduce_var(jd, njobs, VW, partial.as<double>(), out.as<double>(), st), "k_reduce_var");
+    hip_check(launch_reduce_var(jd, njobs, VW, partial.as<double>(), seg_job.as<int32_t>(), nseg,
+                                seg_part.as<double>(), out.as<double>(), st),
+              "k_reduce_var");
   }
 };
 
@@ -1054,6 +1057,16 @@ static VarPlan make_plan(const std::vector<VarJob>& in, int VW, int J, int64_t t
   }
   pl.total_waves = wave;
   pl.njobs = (int)in.size();
+  std::vector<int32_t> sj;
+  for (size_t i = 0; i < in.size(); ++i) {
+    hj[i].seg0 = (int64_t)sj.size();
+    hj[i].nseg = (int32_t)((hj[i].nwaves + RED_SEG - 1) / RED_SEG);
+    for (int32_t k = 0; k < hj[i].nseg; ++k) sj.push_back((int32_t)i);
+  }
+  pl.nseg = (int64_t)sj.size();
+  pl.seg_job.alloc(std::max<size_t>(sj.size(), 1) * 4);
+  h2d(pl.seg_job.p, sj.data(), sj.size() * 4);
+  pl.seg_part.alloc(std::max<int64_t>(pl.nseg, 1) * 2 * (64 * VW + 1) * 8);
   pl.units = (!in.empty() && in[0].V == nullptr) ? 1 : 0;
   pl.row_off.alloc(std::max<size_t>(roff.size(), 1) * 8);
   h2d(pl.row_off.p, roff.data(), roff.size() * 8);

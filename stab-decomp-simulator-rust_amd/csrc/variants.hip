@@ -936,34 +936,41 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
   }
 }
 
-// block (job, g) sums values 8g .. 8g+7 of the job (64*VW variants, then
-// the base): thread (part, k) adds the partials of waves wave0 + part,
-// part + 32, ... of value 8g + k in order, for the A and the B part, then
-// the 32 parts are added in order and the value is A + sqrt(2) B.
+// First stage: block b sums, for each of the 2 (64 VW + 1) values of a wave
+// (the variant and base sums, A parts then sqrt(2) parts), the partials of
+// the waves of segment b in wave order; a thread per value, so each wave's
+// row is one coalesced read.
+template <int VW>
+__global__ __launch_bounds__(256) void k_reduce_seg(const PairJobDev* __restrict__ jobs,
+                                                    const int32_t* __restrict__ seg_job,
+                                                    const double* __restrict__ partial,
+                                                    double* __restrict__ seg_part) {
+  constexpr int NV2 = 2 * (64 * VW + 1);
+  const int64_t b = blockIdx.x;
+  const PairJobDev& jd = jobs[seg_job[b]];
+  const int64_t w0 = jd.wave0 + (b - jd.seg0) * RED_SEG;
+  const int64_t wend = jd.wave0 + jd.nwaves;
+  const int64_t w1 = (w0 + RED_SEG < wend) ? w0 + RED_SEG : wend;
+  for (int t = threadIdx.x; t < NV2; t += 256) {
+    double a = 0.0;
+    for (int64_t w = w0; w < w1; ++w) a = a + partial[(size_t)w * NV2 + t];
+    seg_part[(size_t)b * NV2 + t] = a;
+  }
+}
+
+// Second stage: block j adds job j's segment sums in segment order; the
+// value is A + sqrt(2) B (or A and B apart with split_ab).
 template <int VW>
 __global__ __launch_bounds__(256) void k_reduce_var(const PairJobDev* __restrict__ jobs,
-                                                    const double* __restrict__ partial, double* __restrict__ out) {
+                                                    const double* __restrict__ seg_part, double* __restrict__ out) {
   constexpr int NVS = 64 * VW + 1;
   constexpr double SQRT2 = 1.4142135623730951;  // nearest double to sqrt(2)
-  __shared__ double shA[256], shB[256];
   const PairJobDev& jd = jobs[blockIdx.x];
-  const int k = threadIdx.x & 7, part = threadIdx.x >> 3;
-  const int v = 8 * (int)blockIdx.y + k;
-  double a = 0.0, b = 0.0;
-  if (v < NVS)
-    for (int64_t w = part; w < jd.nwaves; w += 32) {
-      const size_t base = (size_t)(jd.wave0 + w) * 2 * NVS;
-      a = a + partial[base + v];
-      b = b + partial[base + NVS + v];
-    }
-  shA[threadIdx.x] = a;
-  shB[threadIdx.x] = b;
-  __syncthreads();
-  if (part == 0 && v < NVS) {
-    double ta = shA[k], tb = shB[k];
-    for (int q = 1; q < 32; ++q) {
-      ta = ta + shA[8 * q + k];
-      tb = tb + shB[8 * q + k];
+  for (int v = threadIdx.x; v < NVS; v += 256) {
+    double ta = 0.0, tb = 0.0;
+    for (int64_t s = jd.seg0; s < jd.seg0 + jd.nseg; ++s) {
+      ta = ta + seg_part[(size_t)s * 2 * NVS + v];
+      tb = tb + seg_part[(size_t)s * 2 * NVS + NVS + v];
     }
     const double t = ta + SQRT2 * tb;
     if (v < jd.nvar) {
@@ -1065,12 +1072,18 @@ hipError_t launch_pairs_sym(const QFormDev& X, const int64_t* offsets, int J, in
   return hipGetLastError();
 }
 
-hipError_t launch_reduce_var(const PairJobDev* jobs, int njobs, int VW, const double* partial, double* out,
-                             hipStream_t st) {
-  if (njobs == 0) return hipSuccess;
-  if (VW == 1) k_reduce_var<1><<<dim3(njobs, (64 * 1 + 1 + 7) / 8), dim3(256), 0, st>>>(jobs, partial, out);
-  else if (VW == 2) k_reduce_var<2><<<dim3(njobs, (64 * 2 + 1 + 7) / 8), dim3(256), 0, st>>>(jobs, partial, out);
-  else return hipErrorInvalidValue;
+hipError_t launch_reduce_var(const PairJobDev* jobs, int njobs, int VW, const double* partial, const int32_t* seg_job,
+                             int64_t nseg, double* seg_part, double* out, hipStream_t st) {
+  if (njobs == 0 || nseg == 0) return hipSuccess;
+  if (VW == 1) {
+    k_reduce_seg<1><<<dim3((unsigned)nseg), dim3(256), 0, st>>>(jobs, seg_job, partial, seg_part);
+    k_reduce_var<1><<<dim3(njobs), dim3(256), 0, st>>>(jobs, seg_part, out);
+  } else if (VW == 2) {
+    k_reduce_seg<2><<<dim3((unsigned)nseg), dim3(256), 0, st>>>(jobs, seg_job, partial, seg_part);
+    k_reduce_var<2><<<dim3(njobs), dim3(256), 0, st>>>(jobs, seg_part, out);
+  } else {
+    return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
