@@ -239,8 +239,63 @@ __device__ __forceinline__ void pair_eval_v(PairRegs<W>& P, VarRegs<W, VW>& V, i
     if (all_zero<W, VW>(V)) return;
   }
   uniform_state<W, VW>(P, V);
-  while (anyv<W>(P.rem)) {
-    const int p = uni(ffsv<W>(P.rem));
+  while (true) {
+    // Variables with no coupling left are final (as in pair_eval): the
+    // base's factors come from ballots of the coefficients; each variant's
+    // difference needs the variable's high-bit mask H, read per variable.
+    u64 nz[W];
+    bool more = false, iso_any = false;
+#pragma unroll
+    for (int sl = 0; sl < W; ++sl) {
+      bool rownz = false;
+#pragma unroll
+      for (int w = 0; w < W; ++w) rownz = rownz || (P.B[sl][w] != 0ull);
+      nz[sl] = uni64(__ballot(rownz)) & P.rem[sl];
+      more = more || (nz[sl] != 0ull);
+      iso_any = iso_any || ((P.rem[sl] & ~nz[sl]) != 0ull);
+    }
+    if (iso_any) {
+      int ne = 0, n1 = 0, n3 = 0;
+#pragma unroll
+      for (int sl = 0; sl < W; ++sl) {
+        const u64 iso = P.rem[sl] & ~nz[sl];
+        const u64 m2 = uni64(__ballot(P.a[sl] == 2)) & iso;
+        const u64 m1 = uni64(__ballot(P.a[sl] == 1)) & iso;
+        const u64 m3 = uni64(__ballot(P.a[sl] == 3)) & iso;
+        const u64 me = iso & ~(m1 | m3);
+        if (m2) V.bz = 1;
+        ne += __builtin_popcountll(me);
+        n1 += __builtin_popcountll(m1);
+        n3 += __builtin_popcountll(m3);
+        // a_p,k = 2: 1 + i^2 = 0; a_p,k = 0: factor 2
+        for (u64 m = me; m; m &= m - 1) {
+          const int q = 64 * sl + __builtin_ctzll(m);
+          u64 H[VW];
+          bcast_v<W, VW>(V.hi, q, H);
+          const bool two = (m2 >> (q & 63)) & 1ull;
+#pragma unroll
+          for (int v = 0; v < VW; ++v) V.z[v] |= two ? ~H[v] : H[v];
+        }
+        // 1 + i^a: sqrt2 zeta (a = 1) or sqrt2 zeta^7 (a = 3); a variant
+        // whose a_p differs from the base by 2 gets the other one: offset
+        // 7 - 1 = 6 (base a_p = 1) or 1 - 7 = 2 (base a_p = 3)
+        for (u64 m = m1 | m3; m; m &= m - 1) {
+          const int q = 64 * sl + __builtin_ctzll(m);
+          u64 H[VW];
+          bcast_v<W, VW>(V.hi, q, H);
+          const bool one = (m1 >> (q & 63)) & 1ull;
+#pragma unroll
+          for (int v = 0; v < VW; ++v) z8_add(V.d0[v], V.d1[v], V.d2[v], 0ull, H[v], one ? H[v] : 0ull);
+        }
+        P.rem[sl] = nz[sl];
+      }
+      P.mag -= 2 * ne + n1 + n3;
+      P.c0 = (P.c0 + n1 + 7 * n3) & 7;
+      uniform_state<W, VW>(P, V);
+      if (all_zero<W, VW>(V)) return;
+    }
+    if (!more) break;
+    const int p = uni(ffsv<W>(nz));
     const int pw = wordof<W>(p), pb = p & 63;
     clearbit<W>(P.rem, pw, pb);
     u64 Bp[W];
@@ -248,24 +303,7 @@ __device__ __forceinline__ void pair_eval_v(PairRegs<W>& P, VarRegs<W, VW>& V, i
     const int ap = bcast_int<W>(P.a, p);
     u64 H[VW];
     bcast_v<W, VW>(V.hi, p, H);
-    if (!anyv<W>(Bp)) {
-      if (!(ap & 1)) {
-        // a_p,k = 2: 1 + i^2 = 0; a_p,k = 0: factor 2
-        if (ap == 2) V.bz = 1;
-#pragma unroll
-        for (int v = 0; v < VW; ++v) V.z[v] |= (ap & 2) ? ~H[v] : H[v];
-        if (all_zero<W, VW>(V)) return;
-        P.mag -= 2;
-      } else {
-        // 1 + i^a: sqrt2 zeta (a = 1) or sqrt2 zeta^7 (a = 3); a variant
-        // whose a_p differs from the base by 2 gets the other one: offset
-        // 7 - 1 = 6 (base a_p = 1) or 1 - 7 = 2 (base a_p = 3)
-        P.mag -= 1;
-        P.c0 = (P.c0 + (ap == 1 ? 1 : 7)) & 7;
-#pragma unroll
-        for (int v = 0; v < VW; ++v) z8_add(V.d0[v], V.d1[v], V.d2[v], 0ull, H[v], ap == 1 ? H[v] : 0ull);
-      }
-    } else if (ap & 1) {
+    if (ap & 1) {
       const int add = (ap == 1) ? 3 : 1;
       P.mag -= 1;
       P.c0 = (P.c0 + (ap == 1 ? 1 : 7)) & 7;
