@@ -186,13 +186,16 @@ __device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int*
     if (uni(take_constraint<W, true>(P, c))) { *zero = 1; return; }
     uniform_state<W>(P);
   }
-  while (true) {
+  // One exit: the loop runs while coupled variables remain and no factor
+  // has vanished (a structured loop keeps the scalar control flow short).
+  int zf = 0;
+  for (;;) {
     // Variables with no coupling left are final: in this phase a variable's
     // coefficient and row change only through its coupling to the pivot, so
     // each sums out alone to 2 (a = 0), sqrt2 zeta^{+-1} (a odd) or 0 (a = 2),
     // all of them at once from ballots of their coefficients.
     u64 nz[W];
-    bool more = false, iso_any = false;
+    bool more = false;
 #pragma unroll
     for (int sl = 0; sl < W; ++sl) {
       bool rownz = false;
@@ -200,25 +203,20 @@ __device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int*
       for (int w = 0; w < W; ++w) rownz = rownz || (P.B[sl][w] != 0ull);
       nz[sl] = uni64(__ballot(rownz)) & P.rem[sl];
       more = more || (nz[sl] != 0ull);
-      iso_any = iso_any || ((P.rem[sl] & ~nz[sl]) != 0ull);
     }
-    if (iso_any) {
-      int n0 = 0, n1 = 0, n3 = 0;
-      bool z2 = false;
+    int n0 = 0, n1 = 0, n3 = 0;
 #pragma unroll
-      for (int sl = 0; sl < W; ++sl) {
-        const u64 iso = P.rem[sl] & ~nz[sl];
-        z2 = z2 || ((uni64(__ballot(P.a[sl] == 2)) & iso) != 0ull);
-        n0 += __builtin_popcountll(uni64(__ballot(P.a[sl] == 0)) & iso);
-        n1 += __builtin_popcountll(uni64(__ballot(P.a[sl] == 1)) & iso);
-        n3 += __builtin_popcountll(uni64(__ballot(P.a[sl] == 3)) & iso);
-        P.rem[sl] = nz[sl];
-      }
-      if (z2) { *zero = 1; return; }
-      P.mag -= 2 * n0 + n1 + n3;
-      P.c0 = (P.c0 + n1 + 7 * n3) & 7;
+    for (int sl = 0; sl < W; ++sl) {
+      const u64 iso = P.rem[sl] & ~nz[sl];
+      zf |= (uni64(__ballot(P.a[sl] == 2)) & iso) != 0ull;
+      n0 += __builtin_popcountll(uni64(__ballot(P.a[sl] == 0)) & iso);
+      n1 += __builtin_popcountll(uni64(__ballot(P.a[sl] == 1)) & iso);
+      n3 += __builtin_popcountll(uni64(__ballot(P.a[sl] == 3)) & iso);
+      P.rem[sl] = nz[sl];
     }
-    if (!more) break;
+    P.mag -= 2 * n0 + n1 + n3;
+    P.c0 = (P.c0 + n1 + 7 * n3) & 7;
+    if (zf || !more) break;
     const int p = uni(ffsv<W>(nz));
     const int pw = wordof<W>(p), pb = p & 63;
     clearbit<W>(P.rem, pw, pb);
@@ -253,7 +251,7 @@ __device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int*
     }
     uniform_state<W>(P);
   }
-  *zero = 0;
+  *zero = zf;
   *K = P.c0;
   *R = P.mag;
 }

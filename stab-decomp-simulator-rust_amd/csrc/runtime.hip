@@ -206,10 +206,31 @@ static void d2d(void* d, const void* s, size_t b) {
 }
 
 // kernel timing (HIP events on the library stream)
+// Per-kernel timing with HIP events on the library stream.  Scopes record an
+// event pair and do not wait; the pairs are read (and recycled) when the
+// totals are asked for, so profiled runs keep the stream's own pacing.
 struct Prof {
   bool on = false;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
+  struct Rec {
+    std::string name;
+    hipEvent_t e0, e1;
+  };
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+  std::vector<Rec> pending;
   std::map<std::string, std::pair<double, int64_t>> acc;
+  // caller holds g_prof_mu
+  void collect() {
+    for (Rec& r : pending) {
+      hip_check(hipEventSynchronize(r.e1), "hipEventSynchronize");
+      float ms = 0.f;
+      hip_check(hipEventElapsedTime(&ms, r.e0, r.e1), "hipEventElapsedTime");
+      auto& a = acc[r.name];
+      a.first += ms;
+      a.second += 1;
+      pool.push_back({r.e0, r.e1});
+    }
+    pending.clear();
+  }
 };
 static std::mutex g_prof_mu;
 static Prof g_prof;
@@ -217,27 +238,29 @@ static Prof g_prof;
 struct ProfScope {
   const char* name;
   bool on;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   explicit ProfScope(const char* nm) : name(nm), on(false) {
     std::lock_guard<std::mutex> lk(g_prof_mu);
     if (!g_prof.on) return;
     on = true;
-    if (!g_prof.e0) {
-      hip_check(hipEventCreate(&g_prof.e0), "hipEventCreate");
-      hip_check(hipEventCreate(&g_prof.e1), "hipEventCreate");
+    if (g_prof.pool.empty()) {
+      hipEvent_t a, b;
+      hip_check(hipEventCreate(&a), "hipEventCreate");
+      hip_check(hipEventCreate(&b), "hipEventCreate");
+      g_prof.pool.push_back({a, b});
     }
-    hip_check(hipEventRecord(g_prof.e0, stream()), "hipEventRecord");
+    e0 = g_prof.pool.back().first;
+    e1 = g_prof.pool.back().second;
+    g_prof.pool.pop_back();
+    hip_check(hipEventRecord(e0, stream()), "hipEventRecord");
   }
   void end() {
     if (!on) return;
     on = false;
     std::lock_guard<std::mutex> lk(g_prof_mu);
-    hip_check(hipEventRecord(g_prof.e1, stream()), "hipEventRecord");
-    hip_check(hipEventSynchronize(g_prof.e1), "hipEventSynchronize");
-    float ms = 0.f;
-    hip_check(hipEventElapsedTime(&ms, g_prof.e0, g_prof.e1), "hipEventElapsedTime");
-    auto& a = g_prof.acc[name];
-    a.first += ms;
-    a.second += 1;
+    hip_check(hipEventRecord(e1, stream()), "hipEventRecord");
+    g_prof.pending.push_back({name, e0, e1});
+    if (g_prof.pending.size() >= 4096) g_prof.collect();
   }
   ~ProfScope() {
     if (on) {
@@ -1984,6 +2007,7 @@ int nx_profile_enable(int on) {
   NX_TRY({
     (void)ctx();
     std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof.collect();
     g_prof.on = on != 0;
     g_prof.acc.clear();
   })
@@ -1992,6 +2016,7 @@ int nx_profile_enable(int on) {
 int nx_profile_get(const char* kernel, double* total_ms, int64_t* count) {
   NX_TRY({
     std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof.collect();
     auto it = g_prof.acc.find(kernel ? kernel : "");
     *total_ms = it == g_prof.acc.end() ? 0.0 : it->second.first;
     *count = it == g_prof.acc.end() ? 0 : it->second.second;
