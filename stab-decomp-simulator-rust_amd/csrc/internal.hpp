@@ -48,20 +48,22 @@ struct QFormDev {
   uint8_t* a;                // [chi][R]
   uint64_t *alo, *ahi, *rhs; // [chi][W]
   int32_t *nc, *k;           // [chi]
-  uint8_t* p0;               // [chi]
+  int32_t* p0;               // [chi] (int32: read with scalar loads)
   int64_t chi;
   int n, W;
 };
 
-// Coefficients: exact Scalars (zero, k mod 8, r) when scalar != 0, else the
+// Coefficients: exact Scalars when scalar != 0, packed one int32 per term
+// (cs = -1 for Scalar::Zero, else k mod 8 | (r + CS_ROFF) << 3), else the
 // complex128 values c.
+enum : int { CS_ROFF = 1024 };
 struct CoefDev {
   int scalar;
-  const uint8_t* cz;
-  const uint8_t* ck;
-  const int32_t* cr;
+  const int32_t* cs;
   const double2* c;
 };
+__host__ __device__ inline int cs_k(int32_t v) { return v & 7; }
+__host__ __device__ inline int cs_r(int32_t v) { return (v >> 3) - CS_ROFF; }
 
 // Ket-side variants of a term set, bit-sliced: variant k of term t is the
 // term's quadratic form with

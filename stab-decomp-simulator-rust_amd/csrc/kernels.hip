@@ -296,7 +296,7 @@ __global__ __launch_bounds__(256) void k_qform(FormsDev in, QFormDev q) {
     for (int w = 0; w < W; ++w) k += __builtin_popcountll(f.v[w]);
     q.nc[t] = nc;
     q.k[t] = k;
-    q.p0[t] = (uint8_t)f.ph;
+    q.p0[t] = f.ph;
   }
 }
 
@@ -588,11 +588,11 @@ __global__ __launch_bounds__(256) void k_pairs(QFormDev X, QFormDev Y, const int
         // (c_bra.conj() * c_ket).into()
         double cr_, ci_;
         if (cx.scalar) {
-          const int zb = cx.cz[bra], zk = cy.cz[ket];
-          if (zb || zk) { cr_ = 0.0; ci_ = 0.0; }
+          const int32_t sb = cx.cs[bra], sk = cy.cs[ket];
+          if ((sb | sk) < 0) { cr_ = 0.0; ci_ = 0.0; }
           else {
-            const int kk = ((8 - cx.ck[bra]) % 8 + cy.ck[ket]) % 8;
-            const int rr = cx.cr[bra] + cy.cr[ket];
+            const int kk = (cs_k(sk) - cs_k(sb)) & 7;
+            const int rr = cs_r(sb) + cs_r(sk);
             const double nn = tabs[TAB_POW + rr];
             cr_ = tabs[TAB_COS + kk] * nn;
             ci_ = tabs[TAB_SIN + kk] * nn;

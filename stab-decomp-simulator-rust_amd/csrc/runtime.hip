@@ -319,7 +319,7 @@ struct QForms {
     rhs.alloc(chi * W * 8);
     nc.alloc(chi * 4);
     k.alloc(chi * 4);
-    p0.alloc(chi);
+    p0.alloc(chi * 4);
   }
   // only the fields the pair kernels read (projected forms)
   void alloc_pair(int64_t chi_, int n_) {
@@ -333,14 +333,14 @@ struct QForms {
     rhs.alloc(chi * W * 8);
     nc.alloc(chi * 4);
     k.alloc(chi * 4);
-    p0.alloc(chi);
+    p0.alloc(chi * 4);
   }
   QFormDev desc() const {
     QFormDev d;
     d.Q = Q.as<uint64_t>(); d.B = B.as<uint64_t>(); d.C = C.as<uint64_t>(); d.CT = CT.as<uint64_t>();
     d.a = a.as<uint8_t>();
     d.alo = alo.as<uint64_t>(); d.ahi = ahi.as<uint64_t>(); d.rhs = rhs.as<uint64_t>();
-    d.nc = nc.as<int32_t>(); d.k = k.as<int32_t>(); d.p0 = p0.as<uint8_t>();
+    d.nc = nc.as<int32_t>(); d.k = k.as<int32_t>(); d.p0 = p0.as<int32_t>();
     d.chi = chi; d.n = n; d.W = W;
     return d;
   }
@@ -488,7 +488,7 @@ struct nx_state {
   mutable bool q_ok = false;
   mutable nx::QForms q;
   mutable bool c_ok = false;
-  mutable nx::DevBuf d_c, d_cz, d_ck, d_cr;
+  mutable nx::DevBuf d_c, d_cs;
 
   int64_t chi() const { return forms.chi; }
   void invalidate() { q_ok = false; c_ok = false; }
@@ -517,21 +517,14 @@ static CoefDev coef_of(const nx_state* s) {
     s->d_c.alloc(chi * 16);
     h2d(s->d_c.p, c.data(), chi * 16);
     if (!s->is_complex) {
-      std::vector<uint8_t> z(chi), k(chi);
-      std::vector<int32_t> r(chi);
+      std::vector<int32_t> cs(chi);
       for (int64_t i = 0; i < chi; ++i) {
-        z[i] = s->sc[i].z;
-        k[i] = s->sc[i].k;
         if (s->sc[i].r < -TAB_RMAX / 2 || s->sc[i].r > TAB_RMAX / 2)
           fail(NX_E_INTERNAL, "coefficient exponent %lld outside the supported range", (long long)s->sc[i].r);
-        r[i] = (int32_t)s->sc[i].r;
+        cs[i] = s->sc[i].z ? -1 : (int32_t)((s->sc[i].k & 7) | ((s->sc[i].r + CS_ROFF) << 3));
       }
-      s->d_cz.alloc(chi);
-      s->d_ck.alloc(chi);
-      s->d_cr.alloc(chi * 4);
-      h2d(s->d_cz.p, z.data(), chi);
-      h2d(s->d_ck.p, k.data(), chi);
-      h2d(s->d_cr.p, r.data(), chi * 4);
+      s->d_cs.alloc(chi * 4);
+      h2d(s->d_cs.p, cs.data(), chi * 4);
       sync();
     }
     sync();
@@ -539,9 +532,7 @@ static CoefDev coef_of(const nx_state* s) {
   }
   CoefDev d;
   d.scalar = s->is_complex ? 0 : 1;
-  d.cz = s->d_cz.as<uint8_t>();
-  d.ck = s->d_ck.as<uint8_t>();
-  d.cr = s->d_cr.as<int32_t>();
+  d.cs = s->d_cs.as<int32_t>();
   d.c = s->d_c.as<double2>();
   return d;
 }

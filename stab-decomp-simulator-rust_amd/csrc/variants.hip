@@ -365,7 +365,7 @@ __global__ __launch_bounds__(256) void k_qproject(QFormDev in, QFormDev out, con
     P.rhsb[w] = 0ull;
     P.rem[w] = 0ull;
   }
-  P.c0 = uni((int)in.p0[t]);
+  P.c0 = uni(in.p0[t]);
   P.mag = 0;
   u64 r0[W];
 #pragma unroll
@@ -387,7 +387,7 @@ __global__ __launch_bounds__(256) void k_qproject(QFormDev in, QFormDev out, con
   if (lane == 0) {
 #pragma unroll
     for (int w = 0; w < W; ++w) out.rhs[(size_t)t * W + w] = P.rhsa[w];
-    out.p0[t] = (uint8_t)P.c0;
+    out.p0[t] = P.c0;
     out.nc[t] = in.nc[t];
     out.k[t] = in.k[t];
   }
@@ -890,9 +890,10 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
       double cr_, ci_;
       int kc = 0, rc = 0, czero = 0;  // exact coefficient product zeta^kc 2^{-rc/2}
       if (exact) {
-        czero = jd.cx.cz[o] | jd.cy.cz[fx];
-        kc = ((8 - jd.cx.ck[o]) % 8 + jd.cy.ck[fx]) % 8;
-        rc = jd.cx.cr[o] + jd.cy.cr[fx];
+        const int32_t sb = jd.cx.cs[o], sk = jd.cy.cs[fx];
+        czero = (sb | sk) < 0;
+        kc = (cs_k(sk) - cs_k(sb)) & 7;
+        rc = cs_r(sb) + cs_r(sk);
         cr_ = 0.0;
         ci_ = 0.0;
       } else {
