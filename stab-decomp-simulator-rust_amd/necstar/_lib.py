@@ -63,6 +63,14 @@ def lib():
     missing: there is no CPU fallback behind the necstar module."""
     global _lib
     if _lib is None:
+        # PyTorch-ROCm carries its own HIP runtime.  When torch is installed
+        # it is imported first, so that this library binds to that runtime
+        # instead of loading a second one (with two runtimes in one process,
+        # a torch imported afterwards finds no GPU).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
@@ -73,6 +81,12 @@ def lib():
             fn.restype = res
             fn.argtypes = args
         _lib = L
+    return _lib
+
+
+def loaded():
+    """The library if it is already loaded, else None (for destructors, which
+    may run during interpreter shutdown and must not load anything)."""
     return _lib
 
 

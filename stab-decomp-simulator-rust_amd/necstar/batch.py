@@ -17,7 +17,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import check, lib
+from ._lib import check, lib, loaded
 
 NX_BATCH_AUTOTUNE = 1
 NX_BATCH_GRAPH = 2
@@ -46,8 +46,13 @@ class Batch:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().nx_batch_free(h)
             self._h = None
+            try:
+                L = loaded()
+            except Exception:  # interpreter shutdown: module state is gone
+                return
+            if L is not None:
+                L.nx_batch_free(h)
 
     @property
     def shape(self):

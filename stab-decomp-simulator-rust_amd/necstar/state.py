@@ -5,7 +5,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import check, lib
+from ._lib import check, lib, loaded
 from .gate import QuantumGate, _usize
 from .pauli_string import PauliString
 
@@ -60,8 +60,13 @@ class QuantumState:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().nx_state_free(h)
             self._h = None
+            try:
+                L = loaded()
+            except Exception:  # interpreter shutdown: module state is gone
+                return
+            if L is not None:
+                L.nx_state_free(h)
 
     # -- construction --------------------------------------------------------
     @staticmethod
