@@ -71,15 +71,18 @@ def cpu_baseline(cfg, seconds):
                 sample=f"{na} strings x {chi} terms amplitudes + {npairs} pairs, cfg1 state")
 
 
-def read_traffic(kernel):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+def read_counters(kernel):
+    """Per-launch figures from the committed rocprofv3 --pmc summary of the
+    kernel (HBM bytes, VALU / SALU wave instructions per unit)."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_cfg1_traffic.txt")
+    out = {}
     if not os.path.exists(path):
-        return None
+        return out
     for line in open(path):
-        if line.startswith("hbm_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
+        key, _, val = line.partition(":")
+        if key in ("hbm_bytes_per_launch", "valu_wave_instr_per_unit", "salu_wave_instr_per_unit"):
+            out[key] = float(val)
+    return out
 
 
 def main():
@@ -192,10 +195,26 @@ def main():
     else:
         kern, per_unit, units, kms = "k_amplitude", ops_amp(n), amp_units, amp_ms
     achieved = per_unit * units / (kms / 1e3) / 1e12
+    ctr = read_counters(kern)
     roofline = {"bound": "valu", "kernel": kern, "achieved": achieved, "peak": PEAK_VALU_TOPS,
                 "unit": "TOP/s (int32 lane-ops)", "frac": achieved / PEAK_VALU_TOPS,
                 "ops_per_unit": per_unit, "units_per_launch": units, "avg_launch_ms": kms,
-                "traffic": read_traffic(kern)}
+                "ops_basis": ("6 n^2 ceil(n/32) lane-ops per pair: GF(2) elimination of the composed n x n "
+                              "matrices" if kern == "k_pairs" else
+                              "12 ceil(n/2) ceil(n/32) + 4 ceil(n/32) lane-ops per evaluation: X applied bit "
+                              "by bit"),
+                "traffic": ctr.get("hbm_bytes_per_launch")}
+    # what the kernel executes (rocprofv3 counters of the same launch): the
+    # exponential-sum elimination needs fewer operations than ops_basis
+    # counts, so frac can exceed 1; these fractions are of the issue peaks
+    if "valu_wave_instr_per_unit" in ctr:
+        valu = ctr["valu_wave_instr_per_unit"] * 64.0
+        salu = ctr.get("salu_wave_instr_per_unit", 0.0)
+        rate = units / (kms / 1e3)
+        roofline["executed"] = {
+            "valu_lane_ops_per_unit": valu, "salu_instr_per_unit": salu,
+            "valu_frac": valu * rate / (PEAK_VALU_TOPS * 1e12),
+            "salu_issue_frac": salu * rate / (256 * 2.4e9)}
 
     cpu = None
     if args.cpu_seconds > 0:

@@ -991,8 +991,17 @@ __global__ __launch_bounds__(256) void k_reduce_seg(const PairJobDev* __restrict
   const int64_t wend = jd.wave0 + jd.nwaves;
   const int64_t w1 = (w0 + RED_SEG < wend) ? w0 + RED_SEG : wend;
   for (int t = threadIdx.x; t < NV2; t += 256) {
+    // loads issued 8 at a time, added in wave order
     double a = 0.0;
-    for (int64_t w = w0; w < w1; ++w) a = a + partial[(size_t)w * NV2 + t];
+    int64_t w = w0;
+    for (; w + 8 <= w1; w += 8) {
+      double v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v[k] = partial[(size_t)(w + k) * NV2 + t];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) a = a + v[k];
+    }
+    for (; w < w1; ++w) a = a + partial[(size_t)w * NV2 + t];
     seg_part[(size_t)b * NV2 + t] = a;
   }
 }
@@ -1006,8 +1015,24 @@ __global__ __launch_bounds__(256) void k_reduce_var(const PairJobDev* __restrict
   constexpr double SQRT2 = 1.4142135623730951;  // nearest double to sqrt(2)
   const PairJobDev& jd = jobs[blockIdx.x];
   for (int v = threadIdx.x; v < NVS; v += 256) {
+    // loads issued 8 segments at a time, added in segment order
     double ta = 0.0, tb = 0.0;
-    for (int64_t s = jd.seg0; s < jd.seg0 + jd.nseg; ++s) {
+    const int64_t s1 = jd.seg0 + jd.nseg;
+    int64_t s = jd.seg0;
+    for (; s + 8 <= s1; s += 8) {
+      double va[8], vb[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        va[k] = seg_part[(size_t)(s + k) * 2 * NVS + v];
+        vb[k] = seg_part[(size_t)(s + k) * 2 * NVS + NVS + v];
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        ta = ta + va[k];
+        tb = tb + vb[k];
+      }
+    }
+    for (; s < s1; ++s) {
       ta = ta + seg_part[(size_t)s * 2 * NVS + v];
       tb = tb + seg_part[(size_t)s * 2 * NVS + NVS + v];
     }
