@@ -147,6 +147,11 @@ int nx_state_projection_norms(const nx_state *s, int32_t nfixed, const int32_t *
  * counterpart (sampling.rs:80-189 evaluates every node's norms). */
 int nx_state_uniform_prefix(const nx_state *s, int32_t nq, const int32_t *qargs, int32_t *out);
 
+/* The same for the marginal conditioned on outcome fixed_b[i] of qubit
+ * fixed_q[i] (the pairs of the projected terms).  No reference counterpart. */
+int nx_state_uniform_prefix_given(const nx_state *s, int32_t nfixed, const int32_t *fixed_q, const int32_t *fixed_b,
+                                  int32_t nq, const int32_t *qargs, int32_t *out);
+
 /* Exact pair scalars <phi_i|phi_j> for all i, j < chi (chi*chi uint32:
  * 0xFFFFFFFF zero, else k | r << 8).  Parity aid; the reference computes
  * the same values in StabilizerCHForm::inner_product (inner_product.rs:14-48). */

@@ -166,9 +166,10 @@ hipError_t launch_reduce_var(const PairJobDev* jobs, int njobs, int VW, const do
 // atomicMin into *dmin (entering as ncoord + 1) of the smallest D such that
 // some pair value <X_o| Z^t |X_f> (o >= f) with t != 0 supported on the
 // first D coordinates is nonzero; coord[r] = coordinate of qubit r or -1;
+// removed (W words, or null) = variables substituted already;
 // VW = ceil(ncoord / 64); offsets and J as for launch_pairs (triangle)
-hipError_t launch_pairs_sym(const QFormDev& X, const int64_t* offsets, int J, int VW, const int32_t* coord, int* dmin,
-                            int nblocks, hipStream_t st);
+hipError_t launch_pairs_sym(const QFormDev& X, const int64_t* offsets, int J, int VW, const int32_t* coord,
+                            const uint64_t* removed, int* dmin, int nblocks, hipStream_t st);
 // out = in with x_{uq[u]} = ub[u] substituted into every term's form, u <
 // nunit (the fields the pair kernels read: B, C, a, rhs, p0, nc, k)
 hipError_t launch_qproject(const QFormDev& in, const QFormDev& out, const int32_t* uq, const int32_t* ub, int nunit,

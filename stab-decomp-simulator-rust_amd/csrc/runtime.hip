@@ -1340,35 +1340,43 @@ struct Window {
 // is provably uniform: no pair value <phi_o| Z^t |phi_f> with t != 0
 // supported on them is nonzero, so <psi| Z^t |psi> = 0 for all such t and
 // ||Pi_b psi||^2 = 2^-D ||psi||^2 for every outcome b on them.  A sufficient
-// condition (cancellation between pairs is not used); at most 128.
-static int uniform_prefix(const nx_state* s, const std::vector<int>& order) {
-  const int64_t chi = s->chi();
+// condition (cancellation between pairs is not used); at most 128.  With
+// `fixed` qubits, q holds the forms projected onto their outcomes (the
+// conditional marginal given those outcomes).
+static int uniform_depth(const QForms& q, int n, const std::vector<int>& order, const std::vector<int>& fixed) {
+  const int64_t chi = q.chi;
   const int ncoord = std::min<int>((int)order.size(), 128);
   if (chi == 0 || ncoord == 0) return 0;
-  const QForms& q = qform_of(s);
   const int J = 16;
   std::vector<int64_t> off(chi + 1);
   off[0] = 0;
   for (int64_t f = 0; f < chi; ++f) off[f + 1] = off[f] + (chi - f + J - 1) / J;
-  std::vector<int32_t> coord(s->n, -1);
+  std::vector<int32_t> coord(n, -1);
   for (int k = 0; k < ncoord; ++k) coord[order[k]] = k;
+  uint64_t removed[4] = {0, 0, 0, 0};
+  for (int qq : fixed) removed[qq >> 6] |= 1ull << (qq & 63);
   const int32_t init = ncoord + 1;
-  DevBuf doff(off.size() * 8), dcoord(coord.size() * 4), dmin(4);
+  DevBuf doff(off.size() * 8), dcoord(coord.size() * 4), dmin(4), drem(32);
   h2d(doff.p, off.data(), off.size() * 8);
   h2d(dcoord.p, coord.data(), coord.size() * 4);
   h2d(dmin.p, &init, 4);
+  h2d(drem.p, removed, 32);
   const int64_t units = off[chi];
   int nblocks = (int)std::min<int64_t>((units + 3) / 4, 4096);
   if (nblocks < 1) nblocks = 1;
   ProfScope ps("k_pairs_sym");
   hip_check(launch_pairs_sym(q.desc(), doff.as<int64_t>(), J, ncoord > 64 ? 2 : 1, dcoord.as<int32_t>(),
-                             dmin.as<int>(), nblocks, stream()),
+                             fixed.empty() ? nullptr : drem.as<uint64_t>(), dmin.as<int>(), nblocks, stream()),
             "k_pairs_sym");
   ps.end();
   int32_t d = 0;
   d2h(&d, dmin.p, 4);
   sync();
   return d - 1;
+}
+
+static int uniform_prefix(const nx_state* s, const std::vector<int>& order) {
+  return uniform_depth(qform_of(s), s->n, order, {});
 }
 
 struct SampleCtx {
@@ -1733,6 +1741,33 @@ int nx_state_uniform_prefix(const nx_state* s, int32_t nq, const int32_t* qargs,
   NX_TRY({
     validate_qargs(s, nq, qargs);
     *out = uniform_prefix(s, std::vector<int>(qargs, qargs + nq));
+  })
+}
+
+int nx_state_uniform_prefix_given(const nx_state* s, int32_t nfixed, const int32_t* fixed_q, const int32_t* fixed_b,
+                                  int32_t nq, const int32_t* qargs, int32_t* out) {
+  NX_TRY({
+    if (nfixed < 0) fail(NX_E_VALUE, "negative qubit count");
+    std::vector<int32_t> all(fixed_q, fixed_q + nfixed);
+    all.insert(all.end(), qargs, qargs + nq);
+    validate_qargs(s, (int32_t)all.size(), all.data());
+    const std::vector<int> order(qargs, qargs + nq);
+    if (nfixed == 0) {
+      *out = uniform_prefix(s, order);
+    } else if (s->chi() == 0) {
+      *out = 0;
+    } else {
+      QForms proj;
+      proj.alloc_pair(s->chi(), s->n);
+      DevBuf dq((size_t)nfixed * 4), db((size_t)nfixed * 4);
+      std::vector<int32_t> b(fixed_b, fixed_b + nfixed);
+      for (int32_t& x : b) x = x ? 1 : 0;
+      h2d(dq.p, fixed_q, (size_t)nfixed * 4);
+      h2d(db.p, b.data(), (size_t)nfixed * 4);
+      hip_check(launch_qproject(qform_of(s).desc(), proj.desc(), dq.as<int32_t>(), db.as<int32_t>(), nfixed, stream()),
+                "k_qproject");
+      *out = uniform_depth(proj, s->n, order, std::vector<int>(fixed_q, fixed_q + nfixed));
+    }
   })
 }
 

@@ -401,11 +401,12 @@ __global__ __launch_bounds__(256) void k_qproject(QFormDev in, QFormDev out, con
 // is c ^ <H_p, t>, a rank-1 update adds H_p), so after the same elimination
 // the row masks are the linear forms themselves.  Z^t never moves the
 // support, so the constraint phase is the same for every t; after it,
-// <bra| Z^t |ket> vanishes exactly when an isolated variable with even
+// <bra| Z^t |ket> vanishes exactly when an uncoupled variable with even
 // coefficient a_p + 2 <H_p, t> = 2 (mod 4), i.e. <H_p, t> != a_p / 2.  The
 // pair is therefore nonzero exactly for the t with <H_p, t> = a_p / 2 at
-// every isolated even pivot p: those equations are collected, one per lane
-// (lane e % 64 of slot e / 64 holds equation e), constants in em.
+// every uncoupled even variable p.  An uncoupled variable's row never
+// changes again, so its lane keeps its equation: eqm marks those lanes
+// (per slot), em the ones whose constant a_p / 2 is 1.
 // Returns 1 if the pair vanishes for every t (an inconsistent constraint).
 template <int W, int VW, bool USEB>
 __device__ __forceinline__ int take_constraint_sym(PairRegs<W>& P, VarRegs<W, VW>& V, int c) {
@@ -423,8 +424,8 @@ __device__ __forceinline__ int take_constraint_sym(PairRegs<W>& P, VarRegs<W, VW
 }
 
 template <int W, int VW>
-__device__ __forceinline__ int pair_eval_sym(PairRegs<W>& P, VarRegs<W, VW>& V, int nca, int ncb,
-                                             u64 (&EQ)[W][VW], u64 (&em)[W], int& neq) {
+__device__ __forceinline__ int pair_eval_sym(PairRegs<W>& P, VarRegs<W, VW>& V, int nca, int ncb, u64 (&eqm)[W],
+                                             u64 (&em)[W]) {
   uniform_state<W>(P);
   for (int c = 0; c < nca; ++c) {
     if (uni(take_constraint_sym<W, VW, false>(P, V, c))) return 1;
@@ -434,11 +435,26 @@ __device__ __forceinline__ int pair_eval_sym(PairRegs<W>& P, VarRegs<W, VW>& V, 
     if (uni(take_constraint_sym<W, VW, true>(P, V, c))) return 1;
     uniform_state<W>(P);
   }
-  neq = 0;
 #pragma unroll
-  for (int s = 0; s < W; ++s) em[s] = 0ull;
-  while (anyv<W>(P.rem)) {
-    const int p = uni(ffsv<W>(P.rem));
+  for (int s = 0; s < W; ++s) eqm[s] = em[s] = 0ull;
+  for (;;) {
+    u64 nz[W];
+    bool more = false;
+#pragma unroll
+    for (int sl = 0; sl < W; ++sl) {
+      bool rownz = false;
+#pragma unroll
+      for (int w = 0; w < W; ++w) rownz = rownz || (P.B[sl][w] != 0ull);
+      nz[sl] = uni64(__ballot(rownz)) & P.rem[sl];
+      more = more || (nz[sl] != 0ull);
+      const u64 iso = P.rem[sl] & ~nz[sl];
+      const u64 odd = uni64(__ballot(P.a[sl] & 1));
+      eqm[sl] |= iso & ~odd;
+      em[sl] |= iso & uni64(__ballot(P.a[sl] == 2));
+      P.rem[sl] = nz[sl];
+    }
+    if (!more) break;
+    const int p = uni(ffsv<W>(nz));
     const int pw = wordof<W>(p), pb = p & 63;
     clearbit<W>(P.rem, pw, pb);
     u64 Bp[W];
@@ -446,18 +462,7 @@ __device__ __forceinline__ int pair_eval_sym(PairRegs<W>& P, VarRegs<W, VW>& V, 
     const int ap = bcast_int<W>(P.a, p);
     u64 H[VW];
     bcast_v<W, VW>(V.hi, p, H);
-    if (!anyv<W>(Bp)) {
-      if (!(ap & 1)) {
-        const int slot = neq >> 6, ln = neq & 63;
-#pragma unroll
-        for (int s = 0; s < W; ++s) {
-#pragma unroll
-          for (int v = 0; v < VW; ++v) EQ[s][v] = (s == slot && P.lane == ln) ? H[v] : EQ[s][v];
-          if (s == slot && (ap & 2)) em[s] |= 1ull << ln;
-        }
-        ++neq;
-      }
-    } else if (ap & 1) {
+    if (ap & 1) {
       const int add = (ap == 1) ? 3 : 1;
 #pragma unroll
       for (int sl = 0; sl < W; ++sl) {
@@ -481,28 +486,25 @@ __device__ __forceinline__ int pair_eval_sym(PairRegs<W>& P, VarRegs<W, VW>& V, 
       subst_v<W, VW, CONS_NONE, false, false>(P, V, qv, Bp, ap >> 1, H);
     }
     uniform_state<W>(P);
-    neq = uni(neq);
-#pragma unroll
-    for (int s = 0; s < W; ++s) em[s] = uni64(em[s]);
   }
   return 0;
 }
 
-// Smallest D <= lim such that the equations have a solution t != 0
-// supported on coordinates [0, D); -1 if there is none.  Columns
-// are taken in order by a lane-parallel elimination over the equation rows:
-// with all constants zero, the first column that is not a pivot (it lies in
-// the span of the earlier columns) gives D; otherwise D is the first column
-// after which every non-pivot row has a zero constant (the constants lie in
-// the span of the columns so far).
+// Smallest D <= lim such that the equations (rows EQ on the lanes of valid,
+// one per lane and slot, constants em) have a solution t != 0 supported on
+// coordinates [0, D); -1 if there is none.  Columns are taken in order by a
+// lane-parallel elimination over the rows: with all constants zero, the
+// first column that is not a pivot (it lies in the span of the earlier
+// columns) gives D; otherwise D is the first column after which every
+// non-pivot row has a zero constant (the constants lie in the span of the
+// columns so far).  EQ and em are overwritten.
 template <int W, int VW>
-__device__ __forceinline__ int first_solution(u64 (&EQ)[W][VW], u64 (&em)[W], int neq, int lim, int lane) {
-  u64 valid[W], used[W];
+__device__ __forceinline__ int first_solution(u64 (&EQ)[W][VW], const u64 (&valid)[W], u64 (&em)[W], int lim,
+                                              int lane) {
+  u64 used[W];
   u64 anye = 0ull;
 #pragma unroll
   for (int s = 0; s < W; ++s) {
-    const int cnt = neq - 64 * s;
-    valid[s] = cnt >= 64 ? ~0ull : (cnt <= 0 ? 0ull : ((1ull << cnt) - 1ull));
     used[s] = 0ull;
     em[s] &= valid[s];
     anye |= em[s];
@@ -567,12 +569,13 @@ __device__ __forceinline__ int first_solution(u64 (&EQ)[W][VW], u64 (&em)[W], in
 // Over all pairs (bra o >= ket f) of one term set, the smallest D for which
 // some <phi_o| Z^t |phi_f> with t != 0 supported on the first D coordinates
 // is nonzero: atomicMin into *dmin (which enters as ncoord + 1).  coord[r]
-// is the coordinate of qubit r (-1 when it is not tracked).  Units and
-// offsets as in k_pairs (triangle, J bras per unit).
+// is the coordinate of qubit r (-1 when it is not tracked); variables in
+// removed (W words, may be null) were substituted already (projected
+// forms).  Units and offsets as in k_pairs (triangle, J bras per unit).
 template <int W, int VW>
 __global__ __launch_bounds__(256) void k_pairs_sym(QFormDev X, const int64_t* __restrict__ offsets, int64_t chi,
                                                    int J, const int32_t* __restrict__ coord,
-                                                   int* __restrict__ dmin) {
+                                                   const u64* __restrict__ removed, int* __restrict__ dmin) {
   const int lane = threadIdx.x & 63;
   const int64_t wave = (int64_t)blockIdx.x * 4 + uni((int)(threadIdx.x >> 6));  // uniform in the wave
   const int64_t nwaves = (int64_t)gridDim.x * 4;
@@ -584,6 +587,7 @@ __global__ __launch_bounds__(256) void k_pairs_sym(QFormDev X, const int64_t* __
   for (int w = 0; w < W; ++w) {
     const int hi = n - 64 * w;
     valid[w] = hi >= 64 ? ~0ull : (hi <= 0 ? 0ull : ((1ull << hi) - 1ull));
+    if (removed) valid[w] &= ~uni64(removed[w]);
     const int r = 64 * w + lane;
     const int cr = r < n ? coord[r] : -1;
 #pragma unroll
@@ -639,15 +643,10 @@ __global__ __launch_bounds__(256) void k_pairs_sym(QFormDev X, const int64_t* __
       V.bz = 0;
       P.c0 = 0;
       P.mag = 0;
-      u64 EQ[W][VW], em[W];
-#pragma unroll
-      for (int s = 0; s < W; ++s)
-#pragma unroll
-        for (int v = 0; v < VW; ++v) EQ[s][v] = 0ull;
-      int neq = 0;
-      if (pair_eval_sym<W, VW>(P, V, nco, ncf, EQ, em, neq)) continue;
+      u64 eqm[W], em[W];
+      if (pair_eval_sym<W, VW>(P, V, nco, ncf, eqm, em)) continue;
       // only D < cap can lower the minimum
-      const int d = first_solution<W, VW>(EQ, em, neq, cap - 1, lane);
+      const int d = first_solution<W, VW>(V.hi, eqm, em, cap - 1, lane);
       if (d > 0 && lane == 0) atomicMin(dmin, d);
     }
   }
@@ -1128,11 +1127,12 @@ hipError_t launch_qproject(const QFormDev& in, const QFormDev& out, const int32_
   return hipGetLastError();
 }
 
-hipError_t launch_pairs_sym(const QFormDev& X, const int64_t* offsets, int J, int VW, const int32_t* coord, int* dmin,
-                            int nblocks, hipStream_t st) {
+hipError_t launch_pairs_sym(const QFormDev& X, const int64_t* offsets, int J, int VW, const int32_t* coord,
+                            const uint64_t* removed, int* dmin, int nblocks, hipStream_t st) {
   if (X.chi == 0) return hipSuccess;
   NX_DISPATCH_WV(X.W, VW,
-                 k_pairs_sym<WW, VV><<<dim3(nblocks), dim3(256), 0, st>>>(X, offsets, X.chi, J, coord, dmin));
+                 k_pairs_sym<WW, VV><<<dim3(nblocks), dim3(256), 0, st>>>(X, offsets, X.chi, J, coord, removed,
+                                                                          dmin));
   return hipGetLastError();
 }
 

@@ -48,6 +48,7 @@ SIGNATURES = {
     "nx_state_drop_cache": (C.c_int, [_P]),
     "nx_state_projection_norms": (C.c_int, [_P, _I32, _I32P, _I32P, _I32, _I32P, _DP]),
     "nx_state_uniform_prefix": (C.c_int, [_P, _I32, _I32P, _I32P]),
+    "nx_state_uniform_prefix_given": (C.c_int, [_P, _I32, _I32P, _I32P, _I32, _I32P, _I32P]),
     "nx_batch_create": (C.c_int, [_I32, C.POINTER(_P), _I32P, _I32P, _I32, C.POINTER(_P)]),
     "nx_batch_num_outputs": (C.c_int, [_P, _I64P]),
     "nx_batch_shape": (C.c_int, [_P, _I32P, _I64P, _I32P]),

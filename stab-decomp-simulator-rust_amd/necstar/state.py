@@ -166,12 +166,19 @@ class QuantumState:
                                               out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 
-    def uniform_prefix(self, qargs) -> int:
+    def uniform_prefix(self, qargs, fixed=()) -> int:
         """The largest D such that the Z-basis marginal of qargs[:D] is
-        provably uniform (API addition; sample() skips those levels)."""
+        provably uniform, given outcome b of qubit q for each (q, b) in
+        `fixed` (API addition; sample() skips those levels)."""
         arr, ptr = _i32_array(qargs)
         out = C.c_int32()
-        check(lib().nx_state_uniform_prefix(self._h, len(qargs), ptr, C.byref(out)))
+        if not fixed:
+            check(lib().nx_state_uniform_prefix(self._h, len(qargs), ptr, C.byref(out)))
+        else:
+            fq, fqp = _i32_array([q for q, _ in fixed])
+            fb, fbp = _i32_array([1 if b else 0 for _, b in fixed])
+            check(lib().nx_state_uniform_prefix_given(self._h, len(fixed), fqp, fbp, len(qargs), ptr,
+                                                      C.byref(out)))
         return out.value
 
     def exp_values(self, paulis) -> np.ndarray:

@@ -133,3 +133,54 @@ def test_sample_skipping_uniform_levels_same_stream(nx, seed, n, nq, shots):
     res_g = st.sample(qargs, shots, seed=seed)
     res_o = {"".join("1" if b else "0" for b in bits): c for bits, c in o.sample(qargs, shots, seed)}
     assert res_g == res_o
+
+
+def _masked_terms(o, fixed):
+    T = np.array([np.asarray(o.term(i).statevector()) for i in range(o.chi)])
+    idx = np.arange(T.shape[1])
+    keep = np.ones(len(idx), bool)
+    for q, b in fixed:
+        keep &= ((idx >> q) & 1) == b
+    return T * keep
+
+
+@pytest.mark.parametrize("seed,n,nt", [(731, 6, 3), (732, 7, 4), (733, 8, 2), (734, 9, 3)])
+def test_uniform_prefix_given_vs_brute_force(nx, seed, n, nt):
+    """Conditional version: pairs of the terms projected onto the fixed
+    outcomes (checked on the masked term vectors), and the conditional
+    marginal of the dense state."""
+    rng = np.random.default_rng(seed)
+    for trial in range(3):
+        gates = WL.random_circuit(rng, n, 6 * n, nt, gate_set=tuple(range(11)))
+        st = nx.QuantumState.from_circuit(WL.to_circuit(n, gates))
+        o = O.State.compile(n, gates)
+        perm = [int(q) for q in rng.permutation(n)]
+        nf = int(rng.integers(1, n - 1))
+        fixed = [(q, int(rng.integers(2))) for q in perm[:nf]]
+        qargs = perm[nf:]
+        d = st.uniform_prefix(qargs, fixed)
+        T = _masked_terms(o, fixed)
+        idx = np.arange(T.shape[1])
+        ref = len(qargs)
+        for D in range(1, len(qargs) + 1):
+            hit = False
+            for mask in range(1 << (D - 1)):
+                tm = 1 << qargs[D - 1]
+                for k in range(D - 1):
+                    if (mask >> k) & 1:
+                        tm |= 1 << qargs[k]
+                sign = 1.0 - 2.0 * (np.array([bin(x).count("1") for x in (idx & tm)]) & 1)
+                if np.any(np.abs(T.conj() @ (T * sign).T) > 1e-9):
+                    hit = True
+                    break
+            if hit:
+                ref = D - 1
+                break
+        sv = np.asarray(o.statevector())
+        keep = np.ones(len(sv), bool)
+        for q, b in fixed:
+            keep &= ((np.arange(len(sv)) >> q) & 1) == b
+        true = marginal_uniform_depth(sv * keep, qargs) if np.abs(sv[keep]).sum() > 0 else len(qargs)
+        say(f"uniform prefix given seed={seed} trial={trial} n={n} fixed={fixed} qargs={qargs} d={d} ref={ref} true={true}")
+        assert d == ref
+        assert d <= true
