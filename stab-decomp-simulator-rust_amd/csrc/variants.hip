@@ -47,6 +47,17 @@ __device__ __forceinline__ void add_exact(int k, int s, double w, double& A, dou
   else A = A + ldexp(sg, -(t >> 1));
 }
 
+// OR of x over the 64 lanes, in every lane
+__device__ __forceinline__ u64 wave_or64(u64 x) {
+  unsigned lo = (unsigned)x, hi = (unsigned)(x >> 32);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    lo |= (unsigned)__shfl_xor((int)lo, off);
+    hi |= (unsigned)__shfl_xor((int)hi, off);
+  }
+  return uni64(((u64)hi << 32) | lo);
+}
+
 template <int W, int VW>
 struct VarRegs {
   u64 hi[W][VW];               // row r: variants whose a_r carries an extra 2
@@ -267,14 +278,13 @@ __device__ __forceinline__ void pair_eval_v(PairRegs<W>& P, VarRegs<W, VW>& V, i
         ne += __builtin_popcountll(me);
         n1 += __builtin_popcountll(m1);
         n3 += __builtin_popcountll(m3);
-        // a_p,k = 2: 1 + i^2 = 0; a_p,k = 0: factor 2
-        for (u64 m = me; m; m &= m - 1) {
-          const int q = 64 * sl + __builtin_ctzll(m);
-          u64 H[VW];
-          bcast_v<W, VW>(V.hi, q, H);
-          const bool two = (m2 >> (q & 63)) & 1ull;
+        // a_p,k = 2: 1 + i^2 = 0; a_p,k = 0: factor 2.  Each such lane holds
+        // its own row's mask: one OR over the wave covers all of them.
+        if (me) {
+          const bool in_e = (me >> P.lane) & 1ull, two = (m2 >> P.lane) & 1ull;
 #pragma unroll
-          for (int v = 0; v < VW; ++v) V.z[v] |= two ? ~H[v] : H[v];
+          for (int v = 0; v < VW; ++v)
+            V.z[v] |= wave_or64(in_e ? (two ? ~V.hi[sl][v] : V.hi[sl][v]) : 0ull);
         }
         // 1 + i^a: sqrt2 zeta (a = 1) or sqrt2 zeta^7 (a = 3); a variant
         // whose a_p differs from the base by 2 gets the other one: offset
