@@ -139,6 +139,16 @@ hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const uint64
                             uint8_t* per_term, int enumerate, double gre, double gim, hipStream_t st);
 hipError_t launch_amp_reduce(const double2* partial, int64_t nb, int nchunks, double2* out, double gre, double gim,
                              hipStream_t st);
+// exact amplitudes with Scalar coefficients cs: xout[b * 4 + m] = component
+// m (basis 1, zeta, zeta^2, zeta^3) of sum_t c_t <x_b|phi_t>; xout holds
+// nchunks * nb * 4 doubles (chunks are summed into the first)
+// sampling tails: the 2^d completions of base on qubits qs (W words each),
+// and the leaf norms A[k], B[k] = ab[k], ab[nv + k] of exact components al
+hipError_t launch_tail_strings(const uint64_t* base, const int32_t* qs, int d, int W, int64_t nv, uint64_t* out,
+                               hipStream_t st);
+hipError_t launch_tail_leaves(const double* al, int64_t nv, double* ab, hipStream_t st);
+hipError_t launch_amplitude_exact(const QFormDev& q, const int32_t* cs, const uint64_t* bits, int64_t nb,
+                                  int64_t chunk_terms, int nchunks, const double* tabs, double* xout, hipStream_t st);
 hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, const int64_t* offsets, int64_t nfixed,
                         int64_t nother, int triangle, int J, int mode, const CoefDev& cx, const CoefDev& cy,
                         const double* tabs, double2* partial, int nblocks, uint32_t* exact_out, int64_t exact_ld,

@@ -369,12 +369,39 @@ __device__ __forceinline__ void amp_build(const u64* raw, u64* tab) {
   }
 }
 
-template <int W, int XB>
+// Exact amplitudes (EX): with Scalar coefficients every term adds
+// zeta^K 2^{-R/2} to the amplitude.  For even R that is +-2^{-R/2} on one of
+// the components of the basis 1, zeta, zeta^2, zeta^3 (zeta^{j+4} = -zeta^j);
+// for odd R, sqrt2 = zeta + zeta^-1 splits it into 2^{-(R+1)/2} on zeta^{K+1}
+// and zeta^{K-1}.  Each addend is a power of two, so the four components are
+// exact sums while their exponent span fits a double.
+__device__ __forceinline__ void add_zeta(int K, int R, double (&al)[4]) {
+  int i1, i2;
+  double v1, v2;
+  if (!(R & 1)) {
+    i1 = K & 3;
+    v1 = ldexp((K & 4) ? -1.0 : 1.0, -(R >> 1));
+    i2 = 0;
+    v2 = 0.0;
+  } else {
+    const int e = -((R + 1) >> 1);
+    const int ka = (K + 1) & 7, kb = (K + 7) & 7;
+    i1 = ka & 3;
+    v1 = ldexp((ka & 4) ? -1.0 : 1.0, e);
+    i2 = kb & 3;
+    v2 = ldexp((kb & 4) ? -1.0 : 1.0, e);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) al[j] = al[j] + ((j == i1) ? v1 : 0.0) + ((j == i2) ? v2 : 0.0);
+}
+
+template <int W, int XB, bool EX>
 __global__ __launch_bounds__(256) void k_amplitude(QFormDev q, const double2* __restrict__ coef,
                                                    const u64* __restrict__ bits, int64_t nb, int64_t chunk_terms,
                                                    const double* __restrict__ tabs, double2* __restrict__ out,
                                                    uint8_t* __restrict__ per_term, int enumerate, int final_g,
-                                                   double gre, double gim) {
+                                                   double gre, double gim, const int32_t* __restrict__ cs,
+                                                   double* __restrict__ xout) {
   using L = AmpLds<W>;
   constexpr int NB = L::NB, NPRE = (L::RAW + 255) / 256;
   extern __shared__ u64 amp_lds[];
@@ -401,8 +428,13 @@ __global__ __launch_bounds__(256) void k_amplitude(QFormDev q, const double2* __
     }
   }
   double are[XB], aim[XB];
+  double al[EX ? XB : 1][4];
 #pragma unroll
   for (int j = 0; j < XB; ++j) { are[j] = 0.0; aim[j] = 0.0; }
+#pragma unroll
+  for (int j = 0; j < (EX ? XB : 1); ++j)
+#pragma unroll
+    for (int m = 0; m < 4; ++m) al[j][m] = 0.0;
   u64 pre[NPRE];
   if (t0 < t1) {
     amp_load_raw<W>(q, t0, pre);
@@ -425,7 +457,8 @@ __global__ __launch_bounds__(256) void k_amplitude(QFormDev q, const double2* __
       rhs[w] = uni64(q.rhs[(size_t)t * W + w]);
     }
     const double nrm = tabs[TAB_POW + k];
-    const double2 c = coef[t];
+    const double2 c = EX ? make_double2(0.0, 0.0) : coef[t];
+    const int32_t cst = EX ? cs[t] : 0;
     const uint4* tab4 = (const uint4*)tab;  // entry = W uint4: u64 words Q[0..W) then CT[0..W)
 #pragma unroll
     for (int j = 0; j < XB; ++j) {
@@ -463,11 +496,15 @@ __global__ __launch_bounds__(256) void k_amplitude(QFormDev q, const double2* __
       // Scalar::to_complex; the reference then multiplies by omega = 1,
       // which leaves every value reachable here unchanged (no table entry
       // is +-0 except sin(0) = +0), so that product is not formed
-      const double zr = bad ? 0.0 : s_cos[ph] * nrm;
-      const double zi = bad ? 0.0 : s_sin[ph] * nrm;
-      const double tr = zr * c.x - zi * c.y, ti = zr * c.y + zi * c.x;
-      are[j] = are[j] + tr;
-      aim[j] = aim[j] + ti;
+      if constexpr (EX) {
+        if (!bad && cst >= 0) add_zeta((ph + cs_k(cst)) & 7, k + cs_r(cst), al[j]);
+      } else {
+        const double zr = bad ? 0.0 : s_cos[ph] * nrm;
+        const double zi = bad ? 0.0 : s_sin[ph] * nrm;
+        const double tr = zr * c.x - zi * c.y, ti = zr * c.y + zi * c.x;
+        are[j] = are[j] + tr;
+        aim[j] = aim[j] + ti;
+      }
     }
     if (more) amp_store_raw<W>(raw, pre);
     __syncthreads();
@@ -477,11 +514,84 @@ __global__ __launch_bounds__(256) void k_amplitude(QFormDev q, const double2* __
 #pragma unroll
   for (int j = 0; j < XB; ++j) {
     if (!active[j]) continue;
-    if (final_g)  // * global_factor (statevector.rs:23)
+    if constexpr (EX) {
+      // four components per string, per chunk (the global factor is applied
+      // by the caller)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) xout[((size_t)blockIdx.y * nb + bidx[j]) * 4 + m] = al[j][m];
+    } else if (final_g) {  // * global_factor (statevector.rs:23)
       out[bidx[j]] = make_double2(are[j] * gre - aim[j] * gim, are[j] * gim + aim[j] * gre);
-    else
+    } else {
       out[(size_t)blockIdx.y * nb + bidx[j]] = make_double2(are[j], aim[j]);
+    }
   }
+}
+
+// The 2^d completions of a sampling path: string k = base with bit i of k on
+// qubit qs[i] (W words per string)
+__global__ void k_tail_strings(const u64* __restrict__ base, const int32_t* __restrict__ qs, int d, int W, int64_t nv,
+                               u64* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nv) return;
+  for (int w = 0; w < W; ++w) {
+    u64 x = base[w];
+    for (int i = 0; i < d; ++i)
+      if (((k >> i) & 1) && (qs[i] >> 6) == w) x |= 1ull << (qs[i] & 63);
+    out[(size_t)k * W + w] = x;
+  }
+}
+
+// |psi|^2 = A + sqrt2 B from the exact components a_0..a_3 (basis 1, zeta,
+// zeta^2, zeta^3): A = sum a_j^2, B = a0 a1 + a1 a2 + a2 a3 - a0 a3, each
+// accumulated in double-double from exact (FMA) products and rounded once
+struct DDev {
+  double hi = 0.0, lo = 0.0;
+  __device__ void add_prod(double a, double b) {
+    const double p = a * b, pe = fma(a, b, -p);
+    const double s = hi + p, bb = s - hi;
+    double e = (hi - (s - bb)) + (p - bb);
+    e = e + (lo + pe);
+    hi = s + e;
+    lo = e - (hi - s);
+  }
+  __device__ double value() const { return hi + lo; }
+};
+
+__global__ void k_tail_leaves(const double* __restrict__ al, int64_t nv, double* __restrict__ ab) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nv) return;
+  const double a0 = al[k * 4], a1 = al[k * 4 + 1], a2 = al[k * 4 + 2], a3 = al[k * 4 + 3];
+  DDev A, B;
+  A.add_prod(a0, a0);
+  A.add_prod(a1, a1);
+  A.add_prod(a2, a2);
+  A.add_prod(a3, a3);
+  B.add_prod(a0, a1);
+  B.add_prod(a1, a2);
+  B.add_prod(a2, a3);
+  B.add_prod(-a0, a3);
+  ab[k] = A.value();
+  ab[nv + k] = B.value();
+}
+
+hipError_t launch_tail_strings(const uint64_t* base, const int32_t* qs, int d, int W, int64_t nv, uint64_t* out,
+                               hipStream_t st) {
+  k_tail_strings<<<dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st>>>(base, qs, d, W, nv, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_tail_leaves(const double* al, int64_t nv, double* ab, hipStream_t st) {
+  k_tail_leaves<<<dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st>>>(al, nv, ab);
+  return hipGetLastError();
+}
+
+// exact components: sum of the chunks in chunk order into chunk 0's slots
+__global__ void k_amp_reduce_exact(double* __restrict__ xout, int64_t nb, int nchunks) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb * 4) return;
+  double a = xout[i];
+  for (int c = 1; c < nchunks; ++c) a = a + xout[(size_t)c * nb * 4 + i];
+  xout[i] = a;
 }
 
 // sum of the per-chunk partial amplitudes in chunk order, then * global_factor
@@ -696,12 +806,12 @@ hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const u64* b
   {                                                                                                               \
     const size_t sh = AmpLds<WW>::BYTES;                                                                          \
     if (sh > 64 * 1024) {                                                                                         \
-      hipError_t e = hipFuncSetAttribute((const void*)k_amplitude<WW, XBV>,                                      \
+      hipError_t e = hipFuncSetAttribute((const void*)k_amplitude<WW, XBV, false>,                               \
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);                    \
       if (e != hipSuccess) return e;                                                                              \
     }                                                                                                             \
-    k_amplitude<WW, XBV><<<grid, dim3(256), sh, st>>>(q, coef, bits, nb, chunk_terms, tabs, dst, per_term,        \
-                                                       enumerate, final_g, gre, gim);                             \
+    k_amplitude<WW, XBV, false><<<grid, dim3(256), sh, st>>>(q, coef, bits, nb, chunk_terms, tabs, dst, per_term, \
+                                                              enumerate, final_g, gre, gim, nullptr, nullptr);    \
   }
   switch (q.W) {
     case 1: NX_AMP_LAUNCH(1, 4) break;
@@ -711,6 +821,36 @@ hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const u64* b
     default: return hipErrorInvalidValue;
   }
 #undef NX_AMP_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t launch_amplitude_exact(const QFormDev& q, const int32_t* cs, const u64* bits, int64_t nb, int64_t chunk_terms,
+                                  int nchunks, const double* tabs, double* xout, hipStream_t st) {
+  if (nb == 0) return hipSuccess;
+  const int xb = amp_strings_per_thread(q.W);
+  const dim3 grid(blocks_for(nb, 256 * xb), nchunks);
+#define NX_AMPX_LAUNCH(WW, XBV)                                                                                  \
+  {                                                                                                               \
+    const size_t sh = AmpLds<WW>::BYTES;                                                                          \
+    if (sh > 64 * 1024) {                                                                                         \
+      hipError_t e = hipFuncSetAttribute((const void*)k_amplitude<WW, XBV, true>,                                \
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);                    \
+      if (e != hipSuccess) return e;                                                                              \
+    }                                                                                                             \
+    k_amplitude<WW, XBV, true><<<grid, dim3(256), sh, st>>>(q, nullptr, bits, nb, chunk_terms, tabs, nullptr,      \
+                                                             nullptr, 0, 0, 0.0, 0.0, cs, xout);                  \
+  }
+  switch (q.W) {
+    case 1: NX_AMPX_LAUNCH(1, 4) break;
+    case 2: NX_AMPX_LAUNCH(2, 4) break;
+    case 3: NX_AMPX_LAUNCH(3, 2) break;
+    case 4: NX_AMPX_LAUNCH(4, 2) break;
+    default: return hipErrorInvalidValue;
+  }
+#undef NX_AMPX_LAUNCH
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || nchunks <= 1) return e;
+  k_amp_reduce_exact<<<dim3(blocks_for(nb * 4, 256)), dim3(256), 0, st>>>(xout, nb, nchunks);
   return hipGetLastError();
 }
 

@@ -1379,15 +1379,69 @@ static int uniform_prefix(const nx_state* s, const std::vector<int>& order) {
   return uniform_depth(qform_of(s), s->n, order, {});
 }
 
+// Leaf norms of a sampling tail (Scalar coefficients): for the 2^d
+// completions of the path `base` on qubits qs, |psi(x)|^2 / |g|^2 = A + sqrt2 B
+// from exact amplitudes (components in the basis 1, zeta, zeta^2, zeta^3).
+// Device buffers and the page-locked result block are kept per state size.
+struct TailBufs {
+  int64_t nv = 0;
+  DevBuf strings, al, ab, meta;
+  PinnedBuf host;
+};
+
+static void tail_leaves(const nx_state* s, TailBufs& tb, const std::vector<uint64_t>& base,
+                        const std::vector<int32_t>& qs, std::vector<double>& A, std::vector<double>& B) {
+  const QForms& q = qform_of(s);
+  const CoefDev c = coef_of(s);
+  const int d = (int)qs.size(), W = q.W;
+  const int64_t nv = (int64_t)1 << d, chi = s->chi();
+  const int64_t per_wave = 64 * amp_strings_per_thread(W);
+  const int64_t waves_x = (nv + per_wave - 1) / per_wave;
+  int64_t nchunks = waves_x >= 8192 ? 1 : (8192 + waves_x - 1) / waves_x;
+  nchunks = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(nchunks, chi), 64));
+  const int64_t chunk_terms = (chi + nchunks - 1) / nchunks;
+  nchunks = (chi + chunk_terms - 1) / chunk_terms;
+  if (tb.nv < nv) {
+    tb.nv = nv;
+    tb.strings.alloc((size_t)nv * W * 8);
+    tb.al.alloc((size_t)64 * nv * 4 * 8);
+    tb.ab.alloc((size_t)nv * 2 * 8);
+    tb.meta.alloc(64 * 4 + 32);
+    tb.host.alloc(std::max<size_t>((size_t)nv * 2 * 8, 512));
+  }
+  // base words then qubit list through one staged upload
+  std::memcpy(tb.host.p, base.data(), (size_t)W * 8);
+  std::memcpy(tb.host.as<char>(32), qs.data(), (size_t)d * 4);
+  h2d(tb.meta.p, tb.host.p, 32 + (size_t)d * 4);
+  const uint64_t* d_base = tb.meta.as<uint64_t>();
+  const int32_t* d_qs = (const int32_t*)((const char*)tb.meta.p + 32);
+  {
+    ProfScope ps("k_amplitude_exact");
+    hip_check(launch_tail_strings(d_base, d_qs, d, W, nv, tb.strings.as<uint64_t>(), stream()), "k_tail_strings");
+    hip_check(launch_amplitude_exact(q.desc(), c.cs, tb.strings.as<uint64_t>(), nv, chunk_terms, (int)nchunks,
+                                     ctx()->tabs, tb.al.as<double>(), stream()),
+              "k_amplitude_exact");
+    hip_check(launch_tail_leaves(tb.al.as<double>(), nv, tb.ab.as<double>(), stream()), "k_tail_leaves");
+    ps.end();
+  }
+  d2h(tb.host.p, tb.ab.p, (size_t)nv * 16);
+  sync();
+  A.assign(tb.host.as<double>(), tb.host.as<double>() + nv);
+  B.assign(tb.host.as<double>() + nv, tb.host.as<double>() + 2 * nv);
+}
+
 struct SampleCtx {
   int uniform = 0;  // the first `uniform` qubits of q have a uniform marginal
+  int tail_m = 0;   // windows of at most this many levels come from exact amplitudes (0: never)
+  const nx_state* s = nullptr;
+  TailBufs tb;
   std::vector<int> q, bit;
   std::vector<uint64_t> bits, counts;
   std::vector<int> path;  // outcomes of the processed qubits so far
   ChaCha12Rng* rng;
   ProjPlan pp;
   int look = 6;
-  int64_t passes = 0;
+  int64_t passes = 0, tails = 0;
 };
 
 static Window open_window(SampleCtx& cx, int root) {
@@ -1404,6 +1458,36 @@ static Window open_window(SampleCtx& cx, int root) {
   w.lb[w.d].assign(ab.begin() + nv, ab.begin() + 2 * nv);
   ++cx.passes;
   // inner nodes: exact sums of their leaves (powers of two in A and B)
+  for (int j = w.d; j >= 2; --j) {
+    const size_t m = (size_t)1 << (j - 1);
+    w.la[j - 1].assign(m, 0.0);
+    w.lb[j - 1].assign(m, 0.0);
+    for (size_t k = 0; k < m; ++k) {
+      w.la[j - 1][k] = w.la[j][k] + w.la[j][k | m];
+      w.lb[j - 1][k] = w.lb[j][k] + w.lb[j][k | m];
+    }
+  }
+  return w;
+}
+
+// The last levels of a path (every qubit sampled, Scalar coefficients): the
+// leaf norms are |psi(x)|^2 of the 2^d completions x of the path, from exact
+// amplitudes: with psi = sum_j a_j zeta^j, |psi|^2 = sum_j a_j^2 +
+// sqrt2 (a0 a1 + a1 a2 + a2 a3 - a0 a3), A and B formed with exact products
+// and rounded once -- the values the pair sums give whenever those are exact.
+static Window open_tail(SampleCtx& cx, int root) {
+  Window w;
+  w.root = root;
+  w.d = (int)cx.q.size() - root;
+  const int W = words_for(cx.s->n);
+  std::vector<uint64_t> base(W, 0ull);
+  for (int i = 0; i < root; ++i)
+    if (cx.path[i]) base[cx.q[i] >> 6] |= 1ull << (cx.q[i] & 63);
+  std::vector<int32_t> qs(cx.q.begin() + root, cx.q.end());
+  w.la.resize(w.d + 1);
+  w.lb.resize(w.d + 1);
+  tail_leaves(cx.s, cx.tb, base, qs, w.la[w.d], w.lb[w.d]);
+  ++cx.tails;
   for (int j = w.d; j >= 2; --j) {
     const size_t m = (size_t)1 << (j - 1);
     w.la[j - 1].assign(m, 0.0);
@@ -1435,7 +1519,7 @@ static void recursive_sample(SampleCtx& cx, size_t idx, uint64_t shots, uint64_t
   int j = 0;        // children's relative depth in the window
   if ((int)idx >= cx.uniform) {
     if (!w || (int)idx + 1 > w->root + w->d) {
-      own = open_window(cx, (int)idx);
+      own = ((int)cx.q.size() - (int)idx <= cx.tail_m) ? open_tail(cx, (int)idx) : open_window(cx, (int)idx);
       w = &own;
       rel = 0;
     }
@@ -1722,6 +1806,11 @@ int nx_state_sample(const nx_state* s, int32_t nq, const int32_t* qargs, uint64_
     if (shots > 0 && nq > 0) {
       // at least the last level keeps its window: it also reports a zero state
       cx.uniform = std::min(uniform_prefix(s, cx.q), nq - 1);
+      cx.s = s;
+      // exact tails need every qubit sampled (leaf = one amplitude) and
+      // Scalar coefficients; 2^14 completions cost chi 2^14 amplitude
+      // evaluations against ceil(14/6) windows of chi^2 / 2 pairs
+      cx.tail_m = (!s->is_complex && nq == s->n) ? 14 : 0;
       cx.pp.init(s, nq, cx.look);
       recursive_sample(cx, 0, shots, 0, 0, nullptr, 0);
     } else if (shots > 0) {
