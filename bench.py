@@ -97,8 +97,8 @@ def main():
     ap.add_argument("--circuits", type=int, default=0, help="cfg2: use only the first N circuits")
     ap.add_argument("--plain-paulis", type=int, default=2,
                     help="cfg2: Pauli strings per circuit timed on the plain path (then scaled to 64)")
-    ap.add_argument("--sample-shots", type=int, default=0,
-                    help="also time QuantumState.sample over all 64 qubits with this many shots (cfg1)")
+    ap.add_argument("--sample-shots", type=int, default=1024,
+                    help="also time QuantumState.sample over all 64 qubits with this many shots (cfg1; 0: skip)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 1 GiB device copy (PMC calibration)")
     args = ap.parse_args()

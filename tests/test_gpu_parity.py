@@ -230,6 +230,7 @@ def test_sample_same_stream_as_oracle(nx):
     (611, 14, 14, 2000),   # three 6-qubit look-ahead windows
     (612, 20, 13, 300),    # subset of qubits, unsorted
     (613, 70, 10, 64),     # two-word rows
+    (614, 66, 66, 40),     # every qubit sampled: certified levels, then an exact-amplitude tail (two words)
 ])
 def test_sample_clifford_windows_same_stream(nx, seed, n, nq, shots):
     """Clifford states (probabilities 0, 1/2^k, 1): seeded outcomes equal
