@@ -88,8 +88,8 @@ def read_counters(kernel):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=16.0)
     ap.add_argument("--config", type=int, default=1, choices=(1, 2, 3, 4),
                     help="1: amplitude sweep + norm (the metric); 2: batched vs plain norm + 64 Pauli "
@@ -179,9 +179,15 @@ def main():
         sms, scnt = C.c_double(), C.c_int64()
         _lib.check(L.nx_profile_get(b"k_pairs_sym", C.byref(sms), C.byref(scnt)))
         _lib.check(L.nx_profile_enable(0))
+        # DFS nodes with shots (distinct outcome prefixes in the processing
+        # order, qubit n-1 first): the reference evaluates 3 norms of the
+        # projected state at each (sampling.rs:101-153)
+        keys = [k[::-1] for k in counts]
+        nodes = sum(len({k[:d] for k in keys}) for d in range(n))
         sampling = {"shots": args.sample_shots, "qubits": n, "seconds": te, "distinct_outcomes": len(counts),
                     "uniform_levels": st.uniform_prefix(list(range(n - 1, -1, -1))),
-                    "uniform_pass_ms": sms.value, "projection_passes": pcnt.value, "pass_kernel_ms": pms.value}
+                    "uniform_pass_ms": sms.value, "projection_passes": pcnt.value, "pass_kernel_ms": pms.value,
+                    "dfs_nodes": nodes}
 
     ms_step = (t1 - t0) * 1e3 / args.steps
     amp_units = chi * B
@@ -224,6 +230,11 @@ def main():
                "kind": "port", "sample": cb["sample"],
                "amplitude_evals_per_s": 1.0 / cb["t_amp"], "inner_products_per_s": 1.0 / cb["t_ip"],
                "compile_s": cb["compile_s"]}
+        if sampling is not None:
+            # extrapolated, not run: 3 norms of chi(chi+1)/2 pairs per DFS node
+            # at the measured oracle pair rate
+            cpu["sampling_estimate_s"] = sampling["dfs_nodes"] * 3 * pair_units * cb["t_ip"]
+            cpu["sampling_estimate"] = "dfs_nodes x 3 norms x chi(chi+1)/2 pairs / oracle inner products per s"
 
     line = {
         "metric": METRIC, "value": value, "unit": "CH-form evals/s (amplitude evals + inner products)",
