@@ -88,8 +88,8 @@ def read_counters(kernel):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default per config: 20, 1, 5, 2)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default per config: 3, 1, 1, 1)")
     ap.add_argument("--cpu-seconds", type=float, default=16.0)
     ap.add_argument("--config", type=int, default=1, choices=(1, 2, 3, 4),
                     help="1: amplitude sweep + norm (the metric); 2: batched vs plain norm + 64 Pauli "
@@ -102,6 +102,11 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 1 GiB device copy (PMC calibration)")
     args = ap.parse_args()
+    dsteps, dwarm = {1: (20, 3), 2: (1, 1), 3: (5, 1), 4: (2, 1)}[args.config]
+    if args.steps is None:
+        args.steps = dsteps
+    if args.warmup is None:
+        args.warmup = dwarm
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported",
               file=sys.stderr)
