@@ -195,7 +195,7 @@ __device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int*
     // each sums out alone to 2 (a = 0), sqrt2 zeta^{+-1} (a odd) or 0 (a = 2),
     // all of them at once from ballots of their coefficients.
     u64 nz[W];
-    bool more = false;
+    bool more = false, iso_any = false;
 #pragma unroll
     for (int sl = 0; sl < W; ++sl) {
       bool rownz = false;
@@ -203,19 +203,22 @@ __device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int*
       for (int w = 0; w < W; ++w) rownz = rownz || (P.B[sl][w] != 0ull);
       nz[sl] = uni64(__ballot(rownz)) & P.rem[sl];
       more = more || (nz[sl] != 0ull);
+      iso_any = iso_any || (nz[sl] != P.rem[sl]);
     }
-    int n0 = 0, n1 = 0, n3 = 0;
+    if (iso_any) {
+      int n0 = 0, n1 = 0, n3 = 0;
 #pragma unroll
-    for (int sl = 0; sl < W; ++sl) {
-      const u64 iso = P.rem[sl] & ~nz[sl];
-      zf |= (uni64(__ballot(P.a[sl] == 2)) & iso) != 0ull;
-      n0 += __builtin_popcountll(uni64(__ballot(P.a[sl] == 0)) & iso);
-      n1 += __builtin_popcountll(uni64(__ballot(P.a[sl] == 1)) & iso);
-      n3 += __builtin_popcountll(uni64(__ballot(P.a[sl] == 3)) & iso);
-      P.rem[sl] = nz[sl];
+      for (int sl = 0; sl < W; ++sl) {
+        const u64 iso = P.rem[sl] & ~nz[sl];
+        zf |= (uni64(__ballot(P.a[sl] == 2)) & iso) != 0ull;
+        n0 += __builtin_popcountll(uni64(__ballot(P.a[sl] == 0)) & iso);
+        n1 += __builtin_popcountll(uni64(__ballot(P.a[sl] == 1)) & iso);
+        n3 += __builtin_popcountll(uni64(__ballot(P.a[sl] == 3)) & iso);
+        P.rem[sl] = nz[sl];
+      }
+      P.mag -= 2 * n0 + n1 + n3;
+      P.c0 = (P.c0 + n1 + 7 * n3) & 7;
     }
-    P.mag -= 2 * n0 + n1 + n3;
-    P.c0 = (P.c0 + n1 + 7 * n3) & 7;
     if (zf || !more) break;
     const int p = uni(ffsv<W>(nz));
     const int pw = wordof<W>(p), pb = p & 63;
