@@ -489,6 +489,9 @@ struct nx_state {
   mutable nx::QForms q;
   mutable bool c_ok = false;
   mutable nx::DevBuf d_c, d_cs;
+  // work-unit offsets of the triangle of chi x chi pairs (J = 16), for norms
+  mutable int64_t tri_chi = -1, tri_units = 0;
+  mutable nx::DevBuf d_tri;
 
   int64_t chi() const { return forms.chi; }
   void invalidate() { q_ok = false; c_ok = false; }
@@ -537,25 +540,41 @@ static CoefDev coef_of(const nx_state* s) {
   return d;
 }
 
-// sum over pairs; triangle: o in [f, nother) else [0, nother)
-static cplx run_pairs(const QForms& X, const QForms& Y, int64_t nfixed, int64_t nother, bool triangle, int mode,
-                      const CoefDev& cx, const CoefDev& cy, uint32_t* d_exact = nullptr, int64_t ld = 0) {
-  if (nfixed == 0 || nother == 0) return cplx(0.0, 0.0);
-  const int J = 16;
+static const int PAIR_J = 16;
+
+
+// work-unit offsets (PAIR_J pairs per unit): row f owns units off[f] ..
+static std::vector<int64_t> pair_offsets(int64_t nfixed, int64_t nother, bool triangle) {
   std::vector<int64_t> off(nfixed + 1);
   off[0] = 0;
   for (int64_t f = 0; f < nfixed; ++f) {
     const int64_t cnt = triangle ? nother - f : nother;
-    off[f + 1] = off[f] + (cnt + J - 1) / J;
+    off[f + 1] = off[f] + (cnt + PAIR_J - 1) / PAIR_J;
   }
-  const int64_t units = off[nfixed];
-  DevBuf doff(off.size() * 8);
-  h2d(doff.p, off.data(), off.size() * 8);
+  return off;
+}
+
+// sum over pairs; triangle: o in [f, nother) else [0, nother).  d_off: the
+// offsets already on the device (or null: built here)
+static cplx run_pairs(const QForms& X, const QForms& Y, int64_t nfixed, int64_t nother, bool triangle, int mode,
+                      const CoefDev& cx, const CoefDev& cy, uint32_t* d_exact = nullptr, int64_t ld = 0,
+                      const int64_t* d_off = nullptr, int64_t units_known = 0) {
+  if (nfixed == 0 || nother == 0) return cplx(0.0, 0.0);
+  const int J = PAIR_J;
+  DevBuf doff;
+  int64_t units = units_known;
+  if (!d_off) {
+    const std::vector<int64_t> off = pair_offsets(nfixed, nother, triangle);
+    units = off[nfixed];
+    doff.alloc(off.size() * 8);
+    h2d(doff.p, off.data(), off.size() * 8);
+    d_off = doff.as<int64_t>();
+  }
   int nblocks = (int)std::min<int64_t>((units + 3) / 4, 4096);
   if (nblocks < 1) nblocks = 1;
   DevBuf partial((size_t)nblocks * 4 * 16), out(16);
   ProfScope ps("k_pairs");
-  hip_check(launch_pairs(X.desc(), Y.desc(), doff.as<int64_t>(), nfixed, nother, triangle ? 1 : 0, J, mode, cx, cy,
+  hip_check(launch_pairs(X.desc(), Y.desc(), d_off, nfixed, nother, triangle ? 1 : 0, J, mode, cx, cy,
                          ctx()->tabs, partial.as<double2>(), nblocks, d_exact, ld, stream()),
             "k_pairs");
   ps.end();
@@ -571,7 +590,15 @@ static double norm_squared(const nx_state* s) {
   if (s->chi() == 0) return 0.0 * std::norm(s->g);
   const QForms& q = qform_of(s);
   CoefDev c = coef_of(s);
-  cplx sum = run_pairs(q, q, s->chi(), s->chi(), true, /*fixed bra, herm*/ 2, c, c);
+  if (s->tri_chi != s->chi()) {
+    const std::vector<int64_t> off = pair_offsets(s->chi(), s->chi(), true);
+    s->d_tri.alloc(off.size() * 8);
+    h2d(s->d_tri.p, off.data(), off.size() * 8);
+    s->tri_chi = s->chi();
+    s->tri_units = off.back();
+  }
+  cplx sum = run_pairs(q, q, s->chi(), s->chi(), true, /*fixed bra, herm*/ 2, c, c, nullptr, 0,
+                       s->d_tri.as<int64_t>(), s->tri_units);
   return sum.real() * (s->g.real() * s->g.real() + s->g.imag() * s->g.imag());
 }
 
@@ -2155,9 +2182,10 @@ int nx_random_clifford(int32_t n, const uint8_t* seed32, int64_t max_gates, int3
 
 int nx_state_drop_cache(nx_state* s) {
   NX_TRY({
+    // the quadratic forms are derived per call (the coefficient copies are
+    // inputs: they stay resident until the coefficients change)
     s->q = QForms();
     s->q_ok = false;
-    s->c_ok = false;
   })
 }
 
