@@ -131,12 +131,17 @@ hipError_t launch_program(const FormsDev& in, const FormsDev& out, const ProgOp*
 hipError_t launch_kron(const FormsDev& A, const FormsDev& B, const FormsDev& O, hipStream_t st);
 hipError_t launch_gather(const FormsDev& in, const FormsDev& O, const int64_t* idx, hipStream_t st);
 hipError_t launch_qform(const FormsDev& in, const QFormDev& q, hipStream_t st);
-// bitstrings per thread of the amplitude kernel (a block holds 256 threads)
+// bitstrings per thread of the amplitude kernel
 inline int amp_strings_per_thread(int W) { return W <= 2 ? 4 : 2; }
-// nchunks > 1: partial must hold nchunks * nb values
-hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const uint64_t* bits, int64_t nb,
-                            int64_t chunk_terms, int nchunks, const double* tabs, double2* out, double2* partial,
-                            uint8_t* per_term, int enumerate, double gre, double gim, hipStream_t st);
+// number of term chunks (blockIdx.y) of an amplitude launch over nb strings
+// and chi terms: the launch fills whole rounds of the resident blocks, with
+// at least AMP_MIN_TERMS terms per chunk and at most 64 chunks
+int amp_chunks(int W, int exact, int64_t nb, int64_t chi);
+// chunk c covers terms [c chi / nchunks, (c + 1) chi / nchunks); nchunks > 1:
+// partial must hold nchunks * nb values
+hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const uint64_t* bits, int64_t nb, int nchunks,
+                            const double* tabs, double2* out, double2* partial, uint8_t* per_term, int enumerate,
+                            double gre, double gim, hipStream_t st);
 hipError_t launch_amp_reduce(const double2* partial, int64_t nb, int nchunks, double2* out, double gre, double gim,
                              hipStream_t st);
 // exact amplitudes with Scalar coefficients cs: xout[b * 4 + m] = component
@@ -147,8 +152,8 @@ hipError_t launch_amp_reduce(const double2* partial, int64_t nb, int nchunks, do
 hipError_t launch_tail_strings(const uint64_t* base, const int32_t* qs, int d, int W, int64_t nv, uint64_t* out,
                                hipStream_t st);
 hipError_t launch_tail_leaves(const double* al, int64_t nv, double* ab, hipStream_t st);
-hipError_t launch_amplitude_exact(const QFormDev& q, const int32_t* cs, const uint64_t* bits, int64_t nb,
-                                  int64_t chunk_terms, int nchunks, const double* tabs, double* xout, hipStream_t st);
+hipError_t launch_amplitude_exact(const QFormDev& q, const int32_t* cs, const uint64_t* bits, int64_t nb, int nchunks,
+                                  const double* tabs, double* xout, hipStream_t st);
 hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, const int64_t* offsets, int64_t nfixed,
                         int64_t nother, int triangle, int J, int mode, const CoefDev& cx, const CoefDev& cy,
                         const double* tabs, double2* partial, int nblocks, uint32_t* exact_out, int64_t exact_ld,

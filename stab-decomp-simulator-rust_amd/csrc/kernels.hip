@@ -16,6 +16,9 @@
 //              sums per wavefront
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
+
 #include "device_form.hpp"
 #include "internal.hpp"
 #include "pair_core.hpp"
@@ -302,69 +305,155 @@ __global__ __launch_bounds__(256) void k_qform(FormsDev in, QFormDev q) {
 
 // ---------------------------------------------------------------------------
 // Amplitudes.  Each thread evaluates XB bitstrings; blockIdx.y selects a
-// chunk of terms [t0, t1).  Per term and string x: the support test
-// C x = rhs, the linear part a.x mod 4 from two bit planes, and the
-// quadratic part sum_{l<m} Q_lm x_l x_m = par(x & XOR_{l in x} Q_l) (Q_l
-// holds only bits m > l).  Both XOR_{l in x} Q_l and C x = XOR_{l in x} CT_l
-// are read from per-term nibble tables in LDS: for nibble position b
-// (variables 4b .. 4b+3) and value v,
-//   T[b][v] = (XOR_{k : v_k = 1} Q_{4b+k}, XOR_{k : v_k = 1} CT_{4b+k}),
-// so a string costs one lookup per nibble.  The block stages the next
-// term's rows while it evaluates the current one (two table buffers).  The
-// FP64 arithmetic per term is the reference's to_statevector arithmetic
-// (Scalar::to_complex, * omega, * coefficient, +=) in term order within
-// the chunk; chunks are summed in chunk order by k_amp_reduce, which also
-// applies the global factor.
+// chunk of terms [t0, t1).  Per term and string x:
+//   support  C x = rhs, tested row by row (parity of C_k & x) when the term
+//            has at most AMP_NCD constraints, else from a table of CT rows;
+//   phase    p0 + 2 a.x + 4 sum_{l<m} Q_lm x_l x_m (mod 8) with a = alo + 2 ahi
+//            = p0 + 2 popc(x & alo) + 4 par(x & (ahi ^ XOR_{l in x} Q_l)),
+//            since 4 popc(x & ahi) = 4 par(x & ahi) (mod 8) and Q_l holds
+//            only the bits m > l.
+// XOR_{l in x} Q_l (and C x = XOR_{l in x} CT_l) are read from per-term
+// nibble tables in LDS: for nibble position b (variables 4b .. 4b+3) and
+// value v, TQ[b][v] = XOR_{k : v_k = 1} Q_{4b+k}, so a string costs one
+// lookup per nibble.  An entry is W words; for W = 1 the 16 entries of one
+// position fill the 32 LDS banks once, so the lookups of a wave at one
+// position never conflict.  The nibble offsets of a thread's strings do not
+// change over the term loop: for W = 1 they are computed once and the two
+// table buffers are addressed with immediate offsets (the term loop is
+// unrolled by two).  The block stages the next term's rows while it
+// evaluates the current one.
+// FP64: per term the table P[ph] = Scalar(ph, k).to_complex() * coefficient
+// (the reference's to_statevector arithmetic: (cos, sin) * 2^{-k/2}, then the
+// complex product) is formed once; P[8] = 0 is selected outside the support.
+// Each string adds P[ph] in term order within the chunk; chunks are summed
+// in chunk order by k_amp_reduce, which also applies the global factor.
+constexpr int AMP_NCD = 8;  // constraints tested row by row
+constexpr int AMP_NT = 256;  // threads per block
+
+// Q-table layout.  Q_l holds only bits m > l, so the entries of nibble
+// position b (variables 4b .. 4b+3) are zero below bit 4b + 1, which lies in
+// dword b >> 3.  An entry keeps the dwords from there on, the first rounded
+// down to an even dword (8-byte aligned entries) except in the last word,
+// whose upper half alone is kept for its last 8 positions.  For W = 1 that
+// is 8 bytes for positions 0..7 and 4 bytes for 8..15; the 16 entries of a
+// position then span 32 or 16 LDS banks once and a wave's lookups at one
+// position never conflict.
 template <int W>
-struct AmpLds {
-  static constexpr int EW = 2 * W;           // words per table entry (Q part, CT part)
-  static constexpr int NB = 16 * W;          // nibble positions
-  static constexpr int RAW = 64 * W * EW;    // raw rows of one term
-  static constexpr int TAB = NB * 16 * EW;   // one table
-  static constexpr size_t BYTES = (size_t)(RAW + 2 * TAB) * 8;
+struct QLay {
+  static constexpr __host__ __device__ int d0(int b) {
+    return ((b >> 3) == 2 * W - 1) ? (b >> 3) : ((b >> 3) & ~1);
+  }
+  static constexpr __host__ __device__ int nd(int b) { return 2 * W - d0(b); }
+  // byte offset of the 16 entries of position b
+  static constexpr __host__ __device__ int off(int b) {
+    int o = 0;
+    for (int g = 0; g < (b >> 3); ++g) o += 8 * 16 * 4 * nd(8 * g);
+    return o + (b & 7) * 16 * 4 * nd(b);
+  }
+  static constexpr int BYTES = off(16 * W);
 };
 
 template <int W>
-__device__ __forceinline__ void amp_load_raw(const QFormDev& q, int64_t t, u64 (&pre)[(AmpLds<W>::RAW + 255) / 256]) {
-  constexpr int EW = AmpLds<W>::EW, RAW = AmpLds<W>::RAW;
-  const int R = 64 * W;
+struct AmpLds {
+  static constexpr int NB = 16 * W;  // nibble positions
+  static constexpr int NE = NB * 16; // entries of one table
+  static constexpr int NCD = AMP_NCD;
+  // staging of one term (words): Q rows, CT rows ([64W][W] each), C rows [NCD][W]
+  static constexpr int R_CT = 64 * W * W, R_C = 2 * 64 * W * W, RAW = R_C + NCD * W;
+  // one table buffer (bytes): TQ (packed), TC [NE][W] words, C rows [NCD][W]
+  // words, P[0..8] (double2)
+  static constexpr int B_CT = (QLay<W>::BYTES + 15) & ~15, B_C = B_CT + NE * W * 8, B_PH = B_C + NCD * W * 8,
+                       BUF = B_PH + 9 * 16;
+  static constexpr int NPRE = (RAW + AMP_NT - 1) / AMP_NT;
+  static constexpr size_t BYTES = (size_t)RAW * 8 + 2 * (size_t)BUF;
+};
+
+template <int W>
+__device__ __forceinline__ void amp_load_raw(const QFormDev& q, int64_t t, u64 (&pre)[AmpLds<W>::NPRE]) {
+  using L = AmpLds<W>;
+  const size_t base = (size_t)t * (64 * W) * W;  // row 0 of term t in [chi][R][W]
 #pragma unroll
-  for (int i = 0; i < (RAW + 255) / 256; ++i) {
-    const int e = threadIdx.x + 256 * i;
+  for (int i = 0; i < L::NPRE; ++i) {
+    const int e = threadIdx.x + AMP_NT * i;
     pre[i] = 0ull;
-    if (e < RAW) {
-      const int row = e / EW, ww = e % EW;
-      pre[i] = ww < W ? q.Q[rowidx(t, R, row, W, ww)] : q.CT[rowidx(t, R, row, W, ww - W)];
-    }
+    if (e < L::R_CT) pre[i] = q.Q[base + e];
+    else if (e < L::R_C) pre[i] = q.CT[base + (e - L::R_CT)];
+    else if (e < L::RAW) pre[i] = q.C[base + (e - L::R_C)];
   }
 }
 
 template <int W>
-__device__ __forceinline__ void amp_store_raw(u64* raw, const u64 (&pre)[(AmpLds<W>::RAW + 255) / 256]) {
-  constexpr int RAW = AmpLds<W>::RAW;
+__device__ __forceinline__ void amp_store_raw(u64* raw, const u64 (&pre)[AmpLds<W>::NPRE]) {
+  using L = AmpLds<W>;
 #pragma unroll
-  for (int i = 0; i < (RAW + 255) / 256; ++i) {
-    const int e = threadIdx.x + 256 * i;
-    if (e < RAW) raw[e] = pre[i];
+  for (int i = 0; i < L::NPRE; ++i) {
+    const int e = threadIdx.x + AMP_NT * i;
+    if (e < L::RAW) raw[e] = pre[i];
   }
 }
 
-template <int W>
-__device__ __forceinline__ void amp_build(const u64* raw, u64* tab) {
-  constexpr int EW = AmpLds<W>::EW, NE = AmpLds<W>::NB * 16;
+// tables of one term from its staged rows; c and nrm = 2^{-k/2} of the term
+template <int W, bool EX>
+__device__ __forceinline__ void amp_build(const u64* raw, char* buf, bool bigc, double2 c, double nrm,
+                                          const double* s_cos, const double* s_sin) {
+  using L = AmpLds<W>;
+  using Y = QLay<W>;
 #pragma unroll
-  for (int i = 0; i < (NE + 255) / 256; ++i) {
-    const int e = threadIdx.x + 256 * i;
-    if (e < NE) {
+  for (int i = 0; i < (L::NE + AMP_NT - 1) / AMP_NT; ++i) {
+    const int e = threadIdx.x + AMP_NT * i;
+    if (e < L::NE) {
       const int b = e >> 4, v = e & 15;
+      u64 qv[W], cv[W];
 #pragma unroll
-      for (int ww = 0; ww < EW; ++ww) {
-        u64 val = 0ull;
+      for (int ww = 0; ww < W; ++ww) {
+        qv[ww] = 0ull;
+        cv[ww] = 0ull;
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-          if ((v >> k) & 1) val ^= raw[(4 * b + k) * EW + ww];
-        tab[e * EW + ww] = val;
+          if ((v >> k) & 1) {
+            qv[ww] ^= raw[(4 * b + k) * W + ww];
+            cv[ww] ^= raw[L::R_CT + (4 * b + k) * W + ww];
+          }
       }
+      const int d0 = Y::d0(b), nd = 2 * W - d0;
+      uint32_t* dq = (uint32_t*)(buf + Y::off(b)) + v * nd;
+#pragma unroll
+      for (int d = 0; d < 2 * W; ++d)
+        if (d >= d0) dq[d - d0] = (uint32_t)(qv[d >> 1] >> (32 * (d & 1)));
+      if (bigc) {
+        u64* dc = (u64*)(buf + L::B_CT) + e * W;
+#pragma unroll
+        for (int ww = 0; ww < W; ++ww) dc[ww] = cv[ww];
+      }
+    }
+  }
+  if (threadIdx.x < L::NCD * W) ((u64*)(buf + L::B_C))[threadIdx.x] = raw[L::R_C + threadIdx.x];
+  if (!EX && threadIdx.x < 9) {
+    double tr = 0.0, ti = 0.0;
+    if (threadIdx.x < 8) {
+      const double zr = s_cos[threadIdx.x] * nrm, zi = s_sin[threadIdx.x] * nrm;
+      tr = zr * c.x - zi * c.y;
+      ti = zr * c.y + zi * c.x;
+    }
+    ((double2*)(buf + L::B_PH))[threadIdx.x] = make_double2(tr, ti);
+  }
+}
+
+// one packed Q entry of ND dwords
+template <int ND>
+__device__ __forceinline__ void ld_entry(const char* p, uint32_t (&e)[2 * 4]) {
+  if constexpr (ND == 1) {
+    e[0] = *(const uint32_t*)p;
+  } else if constexpr (ND == 2) {
+    const uint2 a = *(const uint2*)p;
+    e[0] = a.x;
+    e[1] = a.y;
+  } else {
+#pragma unroll
+    for (int i = 0; i < ND; i += 2) {
+      const uint2 a = ((const uint2*)p)[i >> 1];
+      e[i] = a.x;
+      e[i + 1] = a.y;
     }
   }
 }
@@ -395,25 +484,32 @@ __device__ __forceinline__ void add_zeta(int K, int R, double (&al)[4]) {
   for (int j = 0; j < 4; ++j) al[j] = al[j] + ((j == i1) ? v1 : 0.0) + ((j == i2) ? v2 : 0.0);
 }
 
+template <int V>
+struct IConst {
+  static constexpr int value = V;
+};
+
 template <int W, int XB, bool EX>
-__global__ __launch_bounds__(256) void k_amplitude(QFormDev q, const double2* __restrict__ coef,
-                                                   const u64* __restrict__ bits, int64_t nb, int64_t chunk_terms,
+__global__ __launch_bounds__(AMP_NT) void k_amplitude(QFormDev q, const double2* __restrict__ coef,
+                                                   const u64* __restrict__ bits, int64_t nb, int64_t nchunks,
                                                    const double* __restrict__ tabs, double2* __restrict__ out,
                                                    uint8_t* __restrict__ per_term, int enumerate, int final_g,
                                                    double gre, double gim, const int32_t* __restrict__ cs,
                                                    double* __restrict__ xout) {
   using L = AmpLds<W>;
-  constexpr int NB = L::NB, NPRE = (L::RAW + 255) / 256;
-  extern __shared__ u64 amp_lds[];
+  using Y = QLay<W>;
+  constexpr int NB = L::NB;
+  constexpr bool PREADDR = (W == 1);
+  extern __shared__ __align__(16) u64 amp_lds[];
   u64* raw = amp_lds;
-  u64* tab_base = amp_lds + L::RAW;  // two tables of L::TAB words
+  char* bufs = (char*)(amp_lds + L::RAW);  // two table buffers of L::BUF bytes
   __shared__ double s_cos[8], s_sin[8];
   if (threadIdx.x < 8) {
     s_cos[threadIdx.x] = tabs[TAB_COS + threadIdx.x];
     s_sin[threadIdx.x] = tabs[TAB_SIN + threadIdx.x];
   }
-  const int64_t t0 = (int64_t)blockIdx.y * chunk_terms;
-  const int64_t t1 = (t0 + chunk_terms < q.chi) ? t0 + chunk_terms : q.chi;
+  const int64_t t0 = (int64_t)blockIdx.y * q.chi / nchunks;
+  const int64_t t1 = ((int64_t)blockIdx.y + 1) * q.chi / nchunks;
   int64_t bidx[XB];
   bool active[XB];
   u64 x[XB][W];
@@ -427,6 +523,23 @@ __global__ __launch_bounds__(256) void k_amplitude(QFormDev q, const double2* __
       else x[j][w] = active[j] ? bits[(size_t)bidx[j] * W + w] : 0ull;
     }
   }
+  auto nib = [&](int j, int b) { return (uint32_t)((x[j][b >> 4] >> (4 * (b & 15))) & 15ull); };
+  // the same from x_j ^ z with z = 0 varying per term (z = t >> 62 for a
+  // term index t < 2^62): offsets derived from it are recomputed per term
+  // instead of being hoisted out of the term loop into registers
+  auto nib_local = [&](const u64(&xl)[W], int b) { return (uint32_t)((xl[b >> 4] >> (4 * (b & 15))) & 15ull); };
+  auto opaque_x = [&](int j, u64 z, u64(&xl)[W]) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) xl[w] = x[j][w] ^ z;
+  };
+  // byte offset of string j's Q entry at nibble position b (W == 1)
+  uint32_t off[PREADDR ? XB : 1][PREADDR ? NB : 1];
+  if constexpr (PREADDR) {
+#pragma unroll
+    for (int j = 0; j < XB; ++j)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) off[j][b] = nib(j, b) * (uint32_t)(4 * Y::nd(b));
+  }
   double are[XB], aim[XB];
   double al[EX ? XB : 1][4];
 #pragma unroll
@@ -435,81 +548,180 @@ __global__ __launch_bounds__(256) void k_amplitude(QFormDev q, const double2* __
   for (int j = 0; j < (EX ? XB : 1); ++j)
 #pragma unroll
     for (int m = 0; m < 4; ++m) al[j][m] = 0.0;
-  u64 pre[NPRE];
+  u64 pre[L::NPRE];
   if (t0 < t1) {
     amp_load_raw<W>(q, t0, pre);
     amp_store_raw<W>(raw, pre);
   }
   __syncthreads();
-  if (t0 < t1) amp_build<W>(raw, tab_base);
+  if (t0 < t1) {
+    const double2 c = EX ? make_double2(0.0, 0.0) : coef[t0];
+    amp_build<W, EX>(raw, bufs, uni(q.nc[t0]) > L::NCD, c, tabs[TAB_POW + uni(q.k[t0])], s_cos, s_sin);
+  }
   __syncthreads();
-  for (int64_t t = t0; t < t1; ++t) {
-    const bool more = t + 1 < t1;
-    if (more) amp_load_raw<W>(q, t + 1, pre);  // in flight during the evaluation below
-    const u64* tab = tab_base + ((t - t0) & 1) * L::TAB;
-    const int k = uni(q.k[t]);
-    const int p0 = uni(q.p0[t]);
+
+  // acc (dwords) = init ^ XOR_{l in x_j} Q_l from the packed Q table of
+  // buffer BI, two positions at a time
+  auto qlook = [&](auto bic, int j, u64 z, uint32_t(&acc)[2 * W]) {
+    constexpr int BI = decltype(bic)::value;
+    const char* tb = bufs + BI * L::BUF;
+    u64 xl[W];
+    if constexpr (!PREADDR) opaque_x(j, z, xl);
+#pragma unroll
+    for (int b = 0; b < NB; b += 2) {
+      const int nd = Y::nd(b), d0 = Y::d0(b);
+      uint32_t o0, o1;
+      if constexpr (PREADDR) {
+        o0 = off[j][b];
+        o1 = off[j][b + 1];
+      } else {
+        o0 = nib_local(xl, b) * (uint32_t)(4 * nd);
+        o1 = nib_local(xl, b + 1) * (uint32_t)(4 * nd);
+      }
+      const char* p0 = tb + Y::off(b) + o0;
+      const char* p1 = tb + Y::off(b + 1) + o1;
+      uint32_t e0[8], e1[8];
+      if (nd == 1) {
+        ld_entry<1>(p0, e0);
+        ld_entry<1>(p1, e1);
+      } else if (nd == 2) {
+        ld_entry<2>(p0, e0);
+        ld_entry<2>(p1, e1);
+      } else if (nd == 4) {
+        ld_entry<4>(p0, e0);
+        ld_entry<4>(p1, e1);
+      } else if (nd == 6) {
+        ld_entry<6>(p0, e0);
+        ld_entry<6>(p1, e1);
+      } else {
+        ld_entry<8>(p0, e0);
+        ld_entry<8>(p1, e1);
+      }
+#pragma unroll
+      for (int d = 0; d < 2 * W; ++d)
+        if (d >= d0) acc[d] = xor3(acc[d], e0[(d - d0) & 7], e1[(d - d0) & 7]);
+    }
+  };
+  // y = C x_j from the full CT table of buffer BI (terms with many constraints)
+  auto ctlook = [&](auto bic, int j, u64 z, u64(&y)[W]) {
+    constexpr int BI = decltype(bic)::value;
+    const u64* tc = (const u64*)(bufs + BI * L::BUF + L::B_CT);
+    u64 xl[W];
+    opaque_x(j, z, xl);
+#pragma unroll
+    for (int w = 0; w < W; ++w) y[w] = 0ull;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const u64* e = tc + ((size_t)b * 16 + nib_local(xl, b)) * W;
+#pragma unroll
+      for (int w = 0; w < W; ++w) y[w] ^= e[w];
+    }
+  };
+
+  // wave-uniform values of a term (scalar registers), read one term ahead
+  struct TermU {
+    int k, p0, nc;
+    int32_t cst;
     u64 alo[W], ahi[W], rhs[W];
+  };
+  auto load_u = [&](int64_t t, TermU& u) {
+    u.k = uni(q.k[t]);
+    u.p0 = uni(q.p0[t]);
+    u.nc = uni(q.nc[t]);
+    u.cst = EX ? uni(cs[t]) : 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-      alo[w] = uni64(q.alo[(size_t)t * W + w]);
-      ahi[w] = uni64(q.ahi[(size_t)t * W + w]);
-      rhs[w] = uni64(q.rhs[(size_t)t * W + w]);
+      u.alo[w] = uni64(q.alo[(size_t)t * W + w]);
+      u.ahi[w] = uni64(q.ahi[(size_t)t * W + w]);
+      u.rhs[w] = uni64(q.rhs[(size_t)t * W + w]);
     }
-    const double nrm = tabs[TAB_POW + k];
-    const double2 c = EX ? make_double2(0.0, 0.0) : coef[t];
-    const int32_t cst = EX ? cs[t] : 0;
-    const uint4* tab4 = (const uint4*)tab;  // entry = W uint4: u64 words Q[0..W) then CT[0..W)
+  };
+  TermU cur;
+  if (t0 < t1) load_u(t0, cur);
+
+  // evaluate term t from buffer BI, stage term t + 1 into the other buffer
+  auto run = [&](auto bic, int64_t t) {
+    constexpr int BI = decltype(bic)::value;
+    const bool more = t + 1 < t1;
+    double2 cn = make_double2(0.0, 0.0);
+    TermU nxt = cur;
+    if (more) {  // in flight during the evaluation below
+      amp_load_raw<W>(q, t + 1, pre);
+      if (!EX) cn = coef[t + 1];
+      load_u(t + 1, nxt);
+    }
+    const char* buf = bufs + BI * L::BUF;
+    const int k = cur.k, p0 = cur.p0, nc = cur.nc;
+    const int32_t cst = cur.cst;
+    const bool bigc = nc > L::NCD;
+    const u64 z = (u64)(t >> 62);
+    // support test: bad[j] & 1 is set outside the support
+    uint32_t bad[XB];
 #pragma unroll
-    for (int j = 0; j < XB; ++j) {
-      u64 acc[W], y[W];
+    for (int j = 0; j < XB; ++j) bad[j] = 0u;
+    if (!bigc) {
+      const u64* crow = (const u64*)(buf + L::B_C);
+      for (int kk = 0; kk < nc; ++kk) {
+        u64 cr[W];
 #pragma unroll
-      for (int w = 0; w < W; ++w) { acc[w] = 0ull; y[w] = 0ull; }
-      // all 16 W nibbles (those above n are 0 and select the zero entry);
-      // pairs of lookups are combined before they are added in
+        for (int w = 0; w < W; ++w) cr[w] = crow[kk * W + w];
+        const uint32_t rk = (uint32_t)(cur.rhs[0] >> kk) & 1u;
 #pragma unroll
-      for (int b = 0; b < NB; b += 2) {
-        const int v0 = (int)((x[j][b >> 4] >> (4 * (b & 15))) & 15ull);
-        const int v1 = (int)((x[j][(b + 1) >> 4] >> (4 * ((b + 1) & 15))) & 15ull);
-        const uint4* e0 = tab4 + (b * 16 + v0) * W;
-        const uint4* e1 = tab4 + ((b + 1) * 16 + v1) * W;
+        for (int j = 0; j < XB; ++j) {
+          uint32_t p = rk;
 #pragma unroll
-        for (int h = 0; h < W; ++h) {
-          // uint4 h of an entry holds u64 words 2h, 2h + 1 of (Q[0..W), CT[0..W))
-          const uint4 a0 = e0[h], a1 = e1[h];
-          const int wlo = 2 * h, whi = 2 * h + 1;
-          u64& dlo = wlo < W ? acc[wlo < W ? wlo : 0] : y[(wlo - W) < W ? wlo - W : 0];
-          u64& dhi = whi < W ? acc[whi < W ? whi : 0] : y[(whi - W) < W ? whi - W : 0];
-          dlo = ((u64)xor3(hi32(dlo), a0.y, a1.y) << 32) | xor3(lo32(dlo), a0.x, a1.x);
-          dhi = ((u64)xor3(hi32(dhi), a0.w, a1.w) << 32) | xor3(lo32(dhi), a0.z, a1.z);
+          for (int w = 0; w < W; ++w) {
+            p += (uint32_t)__builtin_popcount(lo32(x[j][w]) & lo32(cr[w]));
+            p += (uint32_t)__builtin_popcount(hi32(x[j][w]) & hi32(cr[w]));
+          }
+          bad[j] |= p;
         }
       }
-      int ax = 0, bad = 0;
+    }
+#pragma unroll
+    for (int j = 0; j < XB; ++j) {
+      if (bigc) {
+        u64 y[W];
+        ctlook(bic, j, z, y);
+#pragma unroll
+        for (int w = 0; w < W; ++w) bad[j] |= (y[w] != cur.rhs[w]) ? 1u : 0u;
+      }
+      // acc = ahi ^ XOR_{l in x} Q_l
+      uint32_t acc[2 * W];
 #pragma unroll
       for (int w = 0; w < W; ++w) {
-        ax += __builtin_popcountll(x[j][w] & alo[w]) + 2 * __builtin_popcountll(x[j][w] & ahi[w]);
-        bad |= (y[w] != rhs[w]);
+        acc[2 * w] = lo32(cur.ahi[w]);
+        acc[2 * w + 1] = hi32(cur.ahi[w]);
       }
-      const int quad = parand<W>(acc, x[j]);
-      const int ph = (p0 + 2 * ax + 4 * quad) & 7;
-      if (per_term && active[j]) per_term[(size_t)t * nb + bidx[j]] = bad ? (uint8_t)0xFF : (uint8_t)ph;
-      // Scalar::to_complex; the reference then multiplies by omega = 1,
-      // which leaves every value reachable here unchanged (no table entry
-      // is +-0 except sin(0) = +0), so that product is not formed
+      qlook(bic, j, z, acc);
+      uint32_t pa = 0u, pq = 0u;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        pa += (uint32_t)__builtin_popcountll(x[j][w] & cur.alo[w]);
+        pq += (uint32_t)__builtin_popcount(lo32(x[j][w]) & acc[2 * w]);
+        pq += (uint32_t)__builtin_popcount(hi32(x[j][w]) & acc[2 * w + 1]);
+      }
+      const int ph = (int)(((uint32_t)p0 + 2u * pa + 4u * pq) & 7u);
+      const bool outside = (bad[j] & 1u) != 0u;
+      if (per_term && active[j]) per_term[(size_t)t * nb + bidx[j]] = outside ? (uint8_t)0xFF : (uint8_t)ph;
       if constexpr (EX) {
-        if (!bad && cst >= 0) add_zeta((ph + cs_k(cst)) & 7, k + cs_r(cst), al[j]);
+        if (!outside && cst >= 0) add_zeta((ph + cs_k(cst)) & 7, k + cs_r(cst), al[j]);
       } else {
-        const double zr = bad ? 0.0 : s_cos[ph] * nrm;
-        const double zi = bad ? 0.0 : s_sin[ph] * nrm;
-        const double tr = zr * c.x - zi * c.y, ti = zr * c.y + zi * c.x;
-        are[j] = are[j] + tr;
-        aim[j] = aim[j] + ti;
+        const double2 d = ((const double2*)(buf + L::B_PH))[outside ? 8 : ph];
+        are[j] = are[j] + d.x;
+        aim[j] = aim[j] + d.y;
       }
     }
     if (more) amp_store_raw<W>(raw, pre);
     __syncthreads();
-    if (more) amp_build<W>(raw, tab_base + ((t + 1 - t0) & 1) * L::TAB);
+    if (more) amp_build<W, EX>(raw, bufs + (1 - BI) * L::BUF, nxt.nc > L::NCD, cn, tabs[TAB_POW + nxt.k], s_cos, s_sin);
     __syncthreads();
+    cur = nxt;
+  };
+
+  for (int64_t t = t0; t < t1; t += 2) {
+    run(IConst<0>{}, t);
+    if (t + 1 < t1) run(IConst<1>{}, t + 1);
   }
 #pragma unroll
   for (int j = 0; j < XB; ++j) {
@@ -794,14 +1006,57 @@ hipError_t launch_qform(const FormsDev& in, const QFormDev& q, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const u64* bits, int64_t nb, int64_t chunk_terms,
-                            int nchunks, const double* tabs, double2* out, double2* partial, uint8_t* per_term,
-                            int enumerate, double gre, double gim, hipStream_t st) {
+// resident blocks of k_amplitude<W, XB, EX> on the device (occupancy x CUs),
+// queried once per instance
+template <int W, int XB, bool EX>
+static int amp_resident() {
+  static std::atomic<int> cached{0};
+  int v = cached.load(std::memory_order_relaxed);
+  if (v > 0) return v;
+  const void* f = (const void*)k_amplitude<W, XB, EX>;
+  const size_t sh = AmpLds<W>::BYTES;
+  if (sh > 64 * 1024) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    cus = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, AMP_NT, sh) != hipSuccess) per = 1;
+  v = std::max(per, 1) * std::max(cus, 1);
+  cached.store(v, std::memory_order_relaxed);
+  return v;
+}
+
+constexpr int AMP_ROUNDS = 2;      // rounds of resident blocks a chunked launch fills
+constexpr int AMP_MIN_TERMS = 8;   // fewest terms per chunk
+
+int amp_chunks(int W, int exact, int64_t nb, int64_t chi) {
+  if (nb <= 0 || chi <= 1) return 1;
+  int cap = 1;
+#define NX_AMP_CAP(WW, XBV) cap = exact ? amp_resident<WW, XBV, true>() : amp_resident<WW, XBV, false>();
+  switch (W) {
+    case 1: NX_AMP_CAP(1, 4) break;
+    case 2: NX_AMP_CAP(2, 4) break;
+    case 3: NX_AMP_CAP(3, 2) break;
+    default: NX_AMP_CAP(4, 2) break;
+  }
+#undef NX_AMP_CAP
+  const int64_t per_block = (int64_t)AMP_NT * amp_strings_per_thread(W);
+  const int64_t cols = (nb + per_block - 1) / per_block;
+  for (int m = AMP_ROUNDS; m >= 1; --m) {
+    const int64_t nch = (int64_t)m * cap / cols;
+    if (nch >= 2 && chi / nch >= AMP_MIN_TERMS) return (int)std::min<int64_t>(std::min<int64_t>(nch, chi), 64);
+  }
+  return 1;
+}
+
+hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const u64* bits, int64_t nb, int nchunks,
+                            const double* tabs, double2* out, double2* partial, uint8_t* per_term, int enumerate,
+                            double gre, double gim, hipStream_t st) {
   if (nb == 0) return hipSuccess;
   const int final_g = nchunks == 1;
   double2* dst = final_g ? out : partial;
   const int xb = amp_strings_per_thread(q.W);
-  const dim3 grid(blocks_for(nb, 256 * xb), nchunks);
+  const dim3 grid(blocks_for(nb, AMP_NT * xb), nchunks);
 #define NX_AMP_LAUNCH(WW, XBV)                                                                                   \
   {                                                                                                               \
     const size_t sh = AmpLds<WW>::BYTES;                                                                          \
@@ -810,8 +1065,8 @@ hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const u64* b
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);                    \
       if (e != hipSuccess) return e;                                                                              \
     }                                                                                                             \
-    k_amplitude<WW, XBV, false><<<grid, dim3(256), sh, st>>>(q, coef, bits, nb, chunk_terms, tabs, dst, per_term, \
-                                                              enumerate, final_g, gre, gim, nullptr, nullptr);    \
+    k_amplitude<WW, XBV, false><<<grid, dim3(AMP_NT), sh, st>>>(q, coef, bits, nb, nchunks, tabs, dst, per_term,  \
+                                                                 enumerate, final_g, gre, gim, nullptr, nullptr); \
   }
   switch (q.W) {
     case 1: NX_AMP_LAUNCH(1, 4) break;
@@ -824,11 +1079,11 @@ hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const u64* b
   return hipGetLastError();
 }
 
-hipError_t launch_amplitude_exact(const QFormDev& q, const int32_t* cs, const u64* bits, int64_t nb, int64_t chunk_terms,
-                                  int nchunks, const double* tabs, double* xout, hipStream_t st) {
+hipError_t launch_amplitude_exact(const QFormDev& q, const int32_t* cs, const u64* bits, int64_t nb, int nchunks,
+                                  const double* tabs, double* xout, hipStream_t st) {
   if (nb == 0) return hipSuccess;
   const int xb = amp_strings_per_thread(q.W);
-  const dim3 grid(blocks_for(nb, 256 * xb), nchunks);
+  const dim3 grid(blocks_for(nb, AMP_NT * xb), nchunks);
 #define NX_AMPX_LAUNCH(WW, XBV)                                                                                  \
   {                                                                                                               \
     const size_t sh = AmpLds<WW>::BYTES;                                                                          \
@@ -837,8 +1092,8 @@ hipError_t launch_amplitude_exact(const QFormDev& q, const int32_t* cs, const u6
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);                    \
       if (e != hipSuccess) return e;                                                                              \
     }                                                                                                             \
-    k_amplitude<WW, XBV, true><<<grid, dim3(256), sh, st>>>(q, nullptr, bits, nb, chunk_terms, tabs, nullptr,      \
-                                                             nullptr, 0, 0, 0.0, 0.0, cs, xout);                  \
+    k_amplitude<WW, XBV, true><<<grid, dim3(AMP_NT), sh, st>>>(q, nullptr, bits, nb, nchunks, tabs, nullptr,       \
+                                                                nullptr, 0, 0, 0.0, 0.0, cs, xout);               \
   }
   switch (q.W) {
     case 1: NX_AMPX_LAUNCH(1, 4) break;

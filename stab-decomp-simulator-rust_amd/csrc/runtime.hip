@@ -1422,12 +1422,7 @@ static void tail_leaves(const nx_state* s, TailBufs& tb, const std::vector<uint6
   const CoefDev c = coef_of(s);
   const int d = (int)qs.size(), W = q.W;
   const int64_t nv = (int64_t)1 << d, chi = s->chi();
-  const int64_t per_wave = 64 * amp_strings_per_thread(W);
-  const int64_t waves_x = (nv + per_wave - 1) / per_wave;
-  int64_t nchunks = waves_x >= 8192 ? 1 : (8192 + waves_x - 1) / waves_x;
-  nchunks = std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(nchunks, chi), 64));
-  const int64_t chunk_terms = (chi + nchunks - 1) / nchunks;
-  nchunks = (chi + chunk_terms - 1) / chunk_terms;
+  const int nchunks = amp_chunks(W, 1, nv, chi);
   if (tb.nv < nv) {
     tb.nv = nv;
     tb.strings.alloc((size_t)nv * W * 8);
@@ -1445,8 +1440,8 @@ static void tail_leaves(const nx_state* s, TailBufs& tb, const std::vector<uint6
   {
     ProfScope ps("k_amplitude_exact");
     hip_check(launch_tail_strings(d_base, d_qs, d, W, nv, tb.strings.as<uint64_t>(), stream()), "k_tail_strings");
-    hip_check(launch_amplitude_exact(q.desc(), c.cs, tb.strings.as<uint64_t>(), nv, chunk_terms, (int)nchunks,
-                                     ctx()->tabs, tb.al.as<double>(), stream()),
+    hip_check(launch_amplitude_exact(q.desc(), c.cs, tb.strings.as<uint64_t>(), nv, nchunks, ctx()->tabs,
+                                     tb.al.as<double>(), stream()),
               "k_amplitude_exact");
     hip_check(launch_tail_leaves(tb.al.as<double>(), nv, tb.ab.as<double>(), stream()), "k_tail_leaves");
     ps.end();
@@ -1703,19 +1698,12 @@ static void amplitudes_impl(const nx_state* s, int64_t nbits, const uint64_t* d_
                             uint8_t* d_per_term, int enumerate) {
   const QForms& q = qform_of(s);
   CoefDev c = coef_of(s);
-  // enough waves to cover the GPU several times: split the term loop into
-  // chunks when the strings alone give too few waves
-  const int64_t chi = s->chi();
-  const int64_t per_wave = 64 * amp_strings_per_thread(q.W);
-  const int64_t waves_x = (nbits + per_wave - 1) / per_wave;
-  int64_t nchunks = waves_x >= 8192 ? 1 : (8192 + waves_x - 1) / waves_x;
-  nchunks = std::min<int64_t>(std::min<int64_t>(nchunks, chi), 64);
-  if (nchunks < 1) nchunks = 1;
-  const int64_t chunk_terms = (chi + nchunks - 1) / nchunks;
-  nchunks = (chi + chunk_terms - 1) / chunk_terms;
+  // the term loop is split into chunks when the strings alone give too
+  // few blocks to fill the GPU
+  const int nchunks = amp_chunks(q.W, 0, nbits, s->chi());
   DevBuf partial(nchunks > 1 ? (size_t)nchunks * nbits * 16 : 0);
   ProfScope ps("k_amplitude");
-  hip_check(launch_amplitude(q.desc(), c.c, d_bits, nbits, chunk_terms, (int)nchunks, ctx()->tabs, (double2*)d_out,
+  hip_check(launch_amplitude(q.desc(), c.c, d_bits, nbits, nchunks, ctx()->tabs, (double2*)d_out,
                              partial.as<double2>(), d_per_term, enumerate, s->g.real(), s->g.imag(), stream()),
             "k_amplitude");
   ps.end();
