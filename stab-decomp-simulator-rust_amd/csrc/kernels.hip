@@ -827,6 +827,13 @@ __global__ void k_amp_reduce(const double2* __restrict__ part, int64_t nb, int n
 // host).  mode bit 0: fixed index is the ket (else the bra); mode bit 1:
 // Hermitian real sum (diagonal once, off-diagonal twice, norm.rs:16-27 and
 // exp_value.rs:38-58) else the complex sum (inner_product.rs:12-17).
+// 2^{-r/2}: the host table entry pow(2, -r/2), formed exactly as
+// ldexp(pow(2, -1/2) or 1, -floor(r/2)) (equal to the table for every |r| <=
+// TAB_RMAX; a test compares the two)
+__device__ __forceinline__ double pow_half(int r) {
+  return ldexp((r & 1) ? 0x1.6a09e667f3bcdp-1 : 1.0, -(r >> 1));
+}
+
 template <int W>
 __global__ __launch_bounds__(256) void k_pairs(QFormDev X, QFormDev Y, const int64_t* __restrict__ offsets,
                                                int64_t nfixed, int64_t nother, int triangle, int J, int mode,
@@ -846,6 +853,11 @@ __global__ __launch_bounds__(256) void k_pairs(QFormDev X, QFormDev Y, const int
   }
   const bool fixed_is_ket = mode & 1;
   const bool herm = (mode >> 1) & 1;
+  // cos, sin of k pi / 4 (the host tables) in LDS: read with a lane address
+  // instead of a scalar load at the end of each pair
+  __shared__ double s_cs[16];
+  if (threadIdx.x < 16) s_cs[threadIdx.x] = tabs[TAB_COS + threadIdx.x];
+  __syncthreads();
   double acc_re = 0.0, acc_im = 0.0;
   for (int64_t u = wave; u < nunits; u += nwaves) {
     // row f with offsets[f] <= u < offsets[f+1]
@@ -865,11 +877,18 @@ __global__ __launch_bounds__(256) void k_pairs(QFormDev X, QFormDev Y, const int
     int af[W];
     load_qrows<W>(qf, fx, lane, Bf, af, Cf, rf);
     const int ncf = uni(qf.nc[fx]), kf = uni(qf.k[fx]), pf = uni(qf.p0[fx]);
+    const CoefDev& cf = fixed_is_ket ? cy : cx;
+    const CoefDev& co = fixed_is_ket ? cx : cy;
+    const int32_t csf = cf.scalar ? uni(cf.cs[fx]) : 0;
+    const double2 cvf = cf.scalar ? make_double2(0.0, 0.0) : cf.c[fx];
     for (int64_t o = o_lo; o < o_hi; ++o) {
       u64 Bo[W][W], Co[W][W], ro[W];
       int ao[W];
       load_qrows<W>(qo, o, lane, Bo, ao, Co, ro);
       const int nco = uni(qo.nc[o]), ko = uni(qo.k[o]), po = uni(qo.p0[o]);
+      // the other term's coefficient, read now and used after the elimination
+      const int32_t cso = co.scalar ? uni(co.cs[o]) : 0;
+      const double2 cvo = co.scalar ? make_double2(0.0, 0.0) : co.c[o];
       const int64_t bra = fixed_is_ket ? o : fx;
       const int64_t ket = fixed_is_ket ? fx : o;
       PairRegs<W> P;
@@ -905,22 +924,21 @@ __global__ __launch_bounds__(256) void k_pairs(QFormDev X, QFormDev Y, const int
         exact_out[(size_t)bra * exact_ld + ket] = zero ? 0xFFFFFFFFu : ((uint32_t)K | ((uint32_t)Rr << 8));
       if (!zero) {
         // ip = omega_a * omega_b * Scalar::to_complex (inner_product.rs:44-47)
-        const double nrm = tabs[TAB_POW + Rr];
-        const double ir = tabs[TAB_COS + K] * nrm, ii = tabs[TAB_SIN + K] * nrm;
+        const double nrm = pow_half(Rr);
+        const double ir = s_cs[K] * nrm, ii = s_cs[8 + K] * nrm;
         // (c_bra.conj() * c_ket).into()
         double cr_, ci_;
         if (cx.scalar) {
-          const int32_t sb = cx.cs[bra], sk = cy.cs[ket];
+          const int32_t sb = fixed_is_ket ? cso : csf, sk = fixed_is_ket ? csf : cso;
           if ((sb | sk) < 0) { cr_ = 0.0; ci_ = 0.0; }
           else {
             const int kk = (cs_k(sk) - cs_k(sb)) & 7;
-            const int rr = cs_r(sb) + cs_r(sk);
-            const double nn = tabs[TAB_POW + rr];
-            cr_ = tabs[TAB_COS + kk] * nn;
-            ci_ = tabs[TAB_SIN + kk] * nn;
+            const double nn = pow_half(cs_r(sb) + cs_r(sk));
+            cr_ = s_cs[kk] * nn;
+            ci_ = s_cs[8 + kk] * nn;
           }
         } else {
-          const double2 b0 = cx.c[bra], k0 = cy.c[ket];
+          const double2 b0 = fixed_is_ket ? cvo : cvf, k0 = fixed_is_ket ? cvf : cvo;
           const double br = b0.x, bi = -b0.y;
           cr_ = br * k0.x - bi * k0.y;
           ci_ = br * k0.y + bi * k0.x;
@@ -938,18 +956,32 @@ __global__ __launch_bounds__(256) void k_pairs(QFormDev X, QFormDev Y, const int
   if (lane == 0) partial[wave] = make_double2(acc_re, acc_im);
 }
 
-// single-block ordered reduction of the per-wave partial sums
-__global__ void k_reduce(const double2* __restrict__ partial, int64_t n, double2* __restrict__ out) {
-  __shared__ double sr[256], si[256];
+// single-block reduction of the per-wave partial sums in a fixed order:
+// thread i sums partial[i], partial[i + 1024], ... in index order (eight
+// loads in flight per step), then a fixed tree over the threads
+__global__ __launch_bounds__(1024) void k_reduce(const double2* __restrict__ partial, int64_t n,
+                                                 double2* __restrict__ out) {
+  __shared__ double sr[1024], si[1024];
   double r = 0.0, i = 0.0;
-  for (int64_t k = threadIdx.x; k < n; k += blockDim.x) {
+  int64_t k = threadIdx.x;
+  for (; k + 7 * 1024 < n; k += 8 * 1024) {
+    double2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = partial[k + u * 1024];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      r += v[u].x;
+      i += v[u].y;
+    }
+  }
+  for (; k < n; k += 1024) {
     r += partial[k].x;
     i += partial[k].y;
   }
   sr[threadIdx.x] = r;
   si[threadIdx.x] = i;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+  for (int s = 512; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) {
       sr[threadIdx.x] += sr[threadIdx.x + s];
       si[threadIdx.x] += si[threadIdx.x + s];
@@ -1127,7 +1159,7 @@ hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, const int64_t* off
 }
 
 hipError_t launch_reduce(const double2* partial, int64_t n, double2* out, hipStream_t st) {
-  k_reduce<<<dim3(1), dim3(256), 0, st>>>(partial, n, out);
+  k_reduce<<<dim3(1), dim3(1024), 0, st>>>(partial, n, out);
   return hipGetLastError();
 }
 

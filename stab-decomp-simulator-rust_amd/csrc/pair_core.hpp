@@ -259,22 +259,25 @@ __device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int*
   *R = P.mag;
 }
 
+// rows of term t (word offsets fit 32 bits: chi * 64W * W < 2^32 is checked
+// where the forms are allocated)
 template <int W>
 __device__ __forceinline__ void load_qrows(const QFormDev& q, int64_t t, int lane, u64 (&B)[W][W], int (&a)[W],
                                            u64 (&C)[W][W], u64 (&rhs)[W]) {
-  const int R = 64 * W;
+  const uint32_t R = 64 * W;
+  const uint32_t tw = (uint32_t)t * (R * W);
 #pragma unroll
   for (int sl = 0; sl < W; ++sl) {
-    const int r = 64 * sl + lane;
+    const uint32_t r = 64 * sl + lane;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-      B[sl][w] = q.B[rowidx(t, R, r, W, w)];
-      C[sl][w] = q.C[rowidx(t, R, r, W, w)];
+      B[sl][w] = q.B[tw + r * W + w];
+      C[sl][w] = q.C[tw + r * W + w];
     }
-    a[sl] = q.a[(size_t)t * R + r];
+    a[sl] = q.a[(uint32_t)t * R + r];
   }
 #pragma unroll
-  for (int w = 0; w < W; ++w) rhs[w] = uni64(q.rhs[(size_t)t * W + w]);
+  for (int w = 0; w < W; ++w) rhs[w] = uni64(q.rhs[(uint32_t)t * W + w]);
 }
 
 }  // namespace nx

@@ -304,10 +304,17 @@ struct QForms {
   DevBuf Q, B, C, CT, a, alo, ahi, rhs, nc, k, p0;
   int64_t chi = 0;
   int n = 0, W = 1;
+  // the pair kernels index rows with 32-bit word offsets
+  static void check_size(int64_t chi_, int W_) {
+    if ((uint64_t)chi_ * 64u * (uint64_t)W_ * (uint64_t)W_ >= (1ull << 32))
+      fail(NX_E_VALUE, "stabilizer rank %lld at %d words per row exceeds the device table limit",
+           (long long)chi_, W_);
+  }
   void alloc(int64_t chi_, int n_) {
     chi = chi_;
     n = n_;
     W = words_for(n_);
+    check_size(chi, W);
     const size_t R = 64 * (size_t)W;
     Q.alloc(chi * R * W * 8);
     B.alloc(chi * R * W * 8);
@@ -326,6 +333,7 @@ struct QForms {
     chi = chi_;
     n = n_;
     W = words_for(n_);
+    check_size(chi, W);
     const size_t R = 64 * (size_t)W;
     B.alloc(chi * R * W * 8);
     C.alloc(chi * R * W * 8);

@@ -124,3 +124,14 @@ def test_seed_parsing():
         parse_seed(1 << 300)
     with pytest.raises(OverflowError):
         parse_seed(-1)
+
+
+def test_pow_half_identity():
+    """The pair kernel forms 2^{-r/2} as ldexp(pow(2, -1/2) or 1, -floor(r/2))
+    in place of the host table entry pow(2, -r/2): the two agree bit for bit
+    for every |r| <= 1024 (the table range), and the constant it uses is
+    pow(2, -1/2)."""
+    import math
+    assert math.pow(2.0, -0.5) == float.fromhex("0x1.6a09e667f3bcdp-1")
+    for r in range(-1024, 1025):
+        assert math.pow(2.0, -r / 2.0) == math.ldexp(math.pow(2.0, -0.5) if r & 1 else 1.0, -(r >> 1))
