@@ -134,6 +134,80 @@ def test_amplitudes_exact(nx, seed, n, nc, nt, gs):
         assert vals[b] == val_o or abs(vals[b] - val_o) <= TOL
 
 
+def _support_circuit(rng, n, k, nt):
+    """H on k qubits, then CX / CZ / S / X gates and nt T gates: the support
+    of the state is {c + sum_i h_i v_i} for the columns v_i and constant c
+    returned (bit j of a column = qubit j)."""
+    hq = [int(q) for q in rng.choice(n, k, replace=False)]
+    gates = [(O.H, q, 0) for q in hq]
+    cols = np.zeros((k, n), np.uint8)
+    for i, q in enumerate(hq):
+        cols[i, q] = 1
+    const = np.zeros(n, np.uint8)
+    tpos = set(int(p) for p in rng.choice(3 * n, nt, replace=False))
+    for step in range(3 * n):
+        a, b = (int(v) for v in rng.choice(n, 2, replace=False))
+        r = rng.random()
+        if r < 0.5:
+            gates.append((O.CX, a, b))
+            cols[:, b] ^= cols[:, a]
+            const[b] ^= const[a]
+        elif r < 0.7:
+            gates.append((O.CZ, a, b))
+        elif r < 0.85:
+            gates.append((O.S, a, 0))
+        else:
+            gates.append((O.X, a, 0))
+            const[a] ^= 1
+        if step in tpos:
+            gates.append((O.T, a, 0))
+    return gates, cols, const
+
+
+def _pack(xbits, n):
+    words = (n + 63) // 64
+    out = np.zeros(words, np.uint64)
+    for q in np.nonzero(xbits)[0]:
+        out[q // 64] |= np.uint64(1) << np.uint64(q % 64)
+    return out
+
+
+@pytest.mark.parametrize("seed,n,k", [(901, 64, 3), (902, 64, 57), (903, 64, 58), (904, 64, 60),
+                                      (905, 100, 10), (906, 100, 93), (907, 100, 94), (908, 130, 40)])
+def test_amplitudes_constraint_paths(nx, seed, n, k):
+    """Terms with many support constraints (tested against a table of C^T
+    rows when there are more than 8) and with few (tested row by row),
+    including exactly 8 and 9, evaluated at strings inside the support and
+    at random strings; per-term phases and zero flags bit-exact against the
+    oracle, sums within 1e-12."""
+    rng = np.random.default_rng(seed)
+    gates, cols, const = _support_circuit(rng, n, k, 4)
+    g, o = compile_both(n, gates)
+    ncs = sorted({n - int(o.term(i).tables()["v"].sum()) for i in range(o.chi)})
+    say(f"amplitudes constraints seed={seed} n={n} k={k} chi={o.chi} constraint counts={ncs}")
+    inside = [(const ^ (rng.integers(0, 2, k).astype(np.uint8) @ cols % 2).astype(np.uint8)) for _ in range(40)]
+    rand = [rng.integers(0, 2, n).astype(np.uint8) for _ in range(16)]
+    xs = inside + rand
+    bits = np.array([_pack(x, n) for x in xs], np.uint64)
+    vals, pt, rr = g.amplitudes(bits, per_term=True)
+    nonzero = 0
+    for b, x in enumerate(xs):
+        val_o, (zo, ko, ro) = o.amplitude(x, per_term=True)
+        for i in range(o.chi):
+            if zo[i]:
+                assert pt[i, b] == 255, (b, i)
+            else:
+                nonzero += 1
+                assert pt[i, b] == ko[i] and rr[i] == ro[i], (b, i, pt[i, b], ko[i], rr[i], ro[i])
+        assert abs(vals[b] - val_o) <= TOL, (b, vals[b], val_o)
+    assert nonzero > 0
+    # both support tests are exercised where the case is meant to
+    if n - k > 8:
+        assert max(ncs) > 8
+    if n - k <= 8:
+        assert min(ncs) <= 8
+
+
 def test_statevector_vs_dense(nx):
     rng = np.random.default_rng(77)
     for trial in range(6):
