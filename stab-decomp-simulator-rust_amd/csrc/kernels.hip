@@ -893,6 +893,7 @@ __global__ __launch_bounds__(256) void k_pairs(QFormDev X, QFormDev Y, const int
       const int64_t ket = fixed_is_ket ? fx : o;
       PairRegs<W> P;
       P.lane = lane;
+      P.nodiag = ~(1ull << lane);
 #pragma unroll
       for (int sl = 0; sl < W; ++sl) {
 #pragma unroll

@@ -73,6 +73,7 @@ struct PairRegs {
   u64 rem[W];
   int c0, mag;
   int lane;
+  u64 nodiag;  // ~(1 << lane)
 };
 
 template <int W>
@@ -110,36 +111,47 @@ __device__ __forceinline__ void subst(PairRegs<W>& P, int p, const u64 (&r)[W], 
   // a_l += ap (1 - 2c) r_l + 2 (c + r_l) B_pl ;  B_l ^= r_l ((ap odd) r ^ Bp) ^ B_pl r
   const int lin = c ? (4 - ap) & 3 : ap;
   const int pw = wordof<W>(p), pb = p & 63;
+  const u32 cm = c ? ~0u : 0u;
 #pragma unroll
   for (int sl = 0; sl < W; ++sl) {
-    const int rl = (int)((r[sl] >> P.lane) & 1ull);
-    const int bpl = (int)((wsel<W>(P.B[sl], pw) >> pb) & 1ull);
-    P.a[sl] = (P.a[sl] + (rl ? lin : 0) + ((c ^ rl) & bpl ? 2 : 0)) & 3;
-    const u64 mr = 0ull - (u64)rl, mb = 0ull - (u64)bpl;
+    // lane masks: mr where r_l, mb where B_pl
+    const u32 mr = (u32)-(int)((r[sl] >> P.lane) & 1ull);
+    const u32 mb = (u32)-(int)((wsel<W>(P.B[sl], pw) >> pb) & 1ull);
+    // (c ^ r_l) & B_pl ? 2 : 0
+    const u32 two = __builtin_amdgcn_bitop3_b32(mr ^ cm, mb, 2u, 0x80);
+    P.a[sl] = (int)(((u32)P.a[sl] + ((u32)lin & mr) + two) & 3u);
+    // B_l ^= (t & mr) ^ (r & mb), t = ((ap odd) r) ^ B_p; then clear this
+    // lane's diagonal bit and column p
 #pragma unroll
     for (int w = 0; w < W; ++w) {
       const u64 t = ((ap & 1) ? r[w] : 0ull) ^ Bp[w];
-      P.B[sl][w] ^= (t & mr) ^ (r[w] & mb);
+      const u64 keep = ((w == sl) ? P.nodiag : ~0ull) & ((w == pw) ? ~(1ull << pb) : ~0ull);
+      const u32 lo = __builtin_amdgcn_bitop3_b32(
+          __builtin_amdgcn_bitop3_b32(lo32(P.B[sl][w]), lo32(t), mr, 0x78), lo32(r[w]), mb, 0x78);
+      const u32 hi = __builtin_amdgcn_bitop3_b32(
+          __builtin_amdgcn_bitop3_b32(hi32(P.B[sl][w]), hi32(t), mr, 0x78), hi32(r[w]), mb, 0x78);
+      P.B[sl][w] = (((u64)hi << 32) | lo) & keep;
     }
-    // clear the diagonal bit of this lane's row and column p
-#pragma unroll
-    for (int w = 0; w < W; ++w)
-      if (w == sl) P.B[sl][w] &= ~(1ull << P.lane);
-    clearbit<W>(P.B[sl], pw, pb);
     if (CONS == CONS_AB) {
       const int ha = (int)((wsel<W>(P.ca[sl], pw) >> pb) & 1ull);
-      const u64 ma = 0ull - (u64)ha;
+      const u32 ma = (u32)-ha;
 #pragma unroll
-      for (int w = 0; w < W; ++w) P.ca[sl][w] ^= r[w] & ma;
-      clearbit<W>(P.ca[sl], pw, pb);
+      for (int w = 0; w < W; ++w) {
+        const u32 lo = __builtin_amdgcn_bitop3_b32(lo32(P.ca[sl][w]), lo32(r[w]), ma, 0x78);
+        const u32 hi = __builtin_amdgcn_bitop3_b32(hi32(P.ca[sl][w]), hi32(r[w]), ma, 0x78);
+        P.ca[sl][w] = (((u64)hi << 32) | lo) & ((w == pw) ? ~(1ull << pb) : ~0ull);
+      }
       if (c) P.rhsa[sl] ^= __ballot(ha);
     }
     if (CONS != CONS_NONE) {
       const int hb = (int)((wsel<W>(P.cb[sl], pw) >> pb) & 1ull);
-      const u64 mb2 = 0ull - (u64)hb;
+      const u32 mb2 = (u32)-hb;
 #pragma unroll
-      for (int w = 0; w < W; ++w) P.cb[sl][w] ^= r[w] & mb2;
-      clearbit<W>(P.cb[sl], pw, pb);
+      for (int w = 0; w < W; ++w) {
+        const u32 lo = __builtin_amdgcn_bitop3_b32(lo32(P.cb[sl][w]), lo32(r[w]), mb2, 0x78);
+        const u32 hi = __builtin_amdgcn_bitop3_b32(hi32(P.cb[sl][w]), hi32(r[w]), mb2, 0x78);
+        P.cb[sl][w] = (((u64)hi << 32) | lo) & ((w == pw) ? ~(1ull << pb) : ~0ull);
+      }
       if (c) P.rhsb[sl] ^= __ballot(hb);
     }
   }
@@ -233,16 +245,20 @@ __device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int*
       P.c0 = (P.c0 + (ap == 1 ? 1 : 7)) & 7;
 #pragma unroll
       for (int sl = 0; sl < W; ++sl) {
-        const int bpl = (int)((wsel<W>(P.B[sl], pw) >> pb) & 1ull);
-        const u64 mb = 0ull - (u64)bpl;
-        P.a[sl] = (P.a[sl] + (bpl ? add : 0)) & 3;
+        // mb = all ones where B_pl (bit p of this lane's row)
+        const u32 mb = (u32)-(int)((wsel<W>(P.B[sl], pw) >> pb) & 1ull);
+        P.a[sl] = (P.a[sl] + (int)((u32)add & mb)) & 3;
+        // B_l = (B_l ^ (B_p & mb)) & ~e_l & ~e_p, two three-input operations per dword
 #pragma unroll
-        for (int w = 0; w < W; ++w) P.B[sl][w] ^= Bp[w] & mb;
-#pragma unroll
-        for (int w = 0; w < W; ++w)
-          if (w == sl) P.B[sl][w] &= ~(1ull << P.lane);
-        clearbit<W>(P.B[sl], pw, pb);
+        for (int w = 0; w < W; ++w) {
+          const u64 keep = ((w == sl) ? P.nodiag : ~0ull) & ((w == pw) ? ~(1ull << pb) : ~0ull);
+          const u32 lo = __builtin_amdgcn_bitop3_b32(lo32(P.B[sl][w]), lo32(Bp[w]), mb, 0x78);
+          const u32 hi = __builtin_amdgcn_bitop3_b32(hi32(P.B[sl][w]), hi32(Bp[w]), mb, 0x78);
+          P.B[sl][w] = ((u64)hi << 32) | lo;
+          P.B[sl][w] &= keep;
+        }
       }
+      uniform_state<W>(P);
     } else {
       // sum over x_p: 2 [lambda = a_p / 2]; then substitute for x_q
       P.mag -= 2;
@@ -251,8 +267,8 @@ __device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int*
       const int qv = ffsv<W>(Bp);
       flipbit<W>(Bp, qv, 1);
       subst<W, CONS_NONE>(P, qv, Bp, ap >> 1);
+      uniform_state<W>(P);
     }
-    uniform_state<W>(P);
   }
   *zero = zf;
   *K = P.c0;

@@ -365,6 +365,7 @@ __global__ __launch_bounds__(256) void k_qproject(QFormDev in, QFormDev out, con
   const int R = 64 * W;
   PairRegs<W> P;
   P.lane = lane;
+  P.nodiag = ~(1ull << lane);
   load_qrows<W>(in, t, lane, P.B, P.a, P.ca, P.rhsa);
 #pragma unroll
   for (int sl = 0; sl < W; ++sl)
@@ -627,6 +628,7 @@ __global__ __launch_bounds__(256) void k_pairs_sym(QFormDev X, const int64_t* __
       PairRegs<W> P;
       VarRegs<W, VW> V;
       P.lane = lane;
+      P.nodiag = ~(1ull << lane);
 #pragma unroll
       for (int sl = 0; sl < W; ++sl) {
 #pragma unroll
@@ -861,6 +863,7 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
       PairRegs<W> P;
       VarRegs<W, VW> V;
       P.lane = lane;
+      P.nodiag = ~(1ull << lane);
 #pragma unroll
       for (int sl = 0; sl < W; ++sl) {
 #pragma unroll
