@@ -42,6 +42,16 @@ __device__ __forceinline__ u32 lo32(u64 x) { return (u32)x; }
 __device__ __forceinline__ u32 hi32(u64 x) { return (u32)(x >> 32); }
 // a ^ b ^ c in one gfx950 three-input bitwise operation (truth table 0x96)
 __device__ __forceinline__ u32 xor3(u32 a, u32 b, u32 c) { return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96); }
+// x ^ (y & m) for a lane mask m (all ones or zero), one three-input
+// operation (truth table 0x78) per dword
+// (the builtin returns int: each half is taken as u32 before widening)
+__device__ __forceinline__ u64 xor_and(u64 x, u64 y, u32 m) {
+  const u32 lo = (u32)__builtin_amdgcn_bitop3_b32((u32)x, (u32)y, m, 0x78);
+  const u32 hi = (u32)__builtin_amdgcn_bitop3_b32((u32)(x >> 32), (u32)(y >> 32), m, 0x78);
+  return ((u64)hi << 32) | lo;
+}
+// -(bit b of x) as a lane mask
+__device__ __forceinline__ u32 bitmask(u64 x, int b) { return (u32) - (int)((x >> b) & 1ull); }
 __device__ __forceinline__ int ffs64(u64 x) { return __builtin_ctzll(x); }
 
 // word of variable c in a row of W words (always word 0 when W = 1, which

@@ -115,8 +115,8 @@ __device__ __forceinline__ void subst(PairRegs<W>& P, int p, const u64 (&r)[W], 
 #pragma unroll
   for (int sl = 0; sl < W; ++sl) {
     // lane masks: mr where r_l, mb where B_pl
-    const u32 mr = (u32)-(int)((r[sl] >> P.lane) & 1ull);
-    const u32 mb = (u32)-(int)((wsel<W>(P.B[sl], pw) >> pb) & 1ull);
+    const u32 mr = bitmask(r[sl], P.lane);
+    const u32 mb = bitmask(wsel<W>(P.B[sl], pw), pb);
     // (c ^ r_l) & B_pl ? 2 : 0
     const u32 two = __builtin_amdgcn_bitop3_b32(mr ^ cm, mb, 2u, 0x80);
     P.a[sl] = (int)(((u32)P.a[sl] + ((u32)lin & mr) + two) & 3u);
@@ -126,33 +126,19 @@ __device__ __forceinline__ void subst(PairRegs<W>& P, int p, const u64 (&r)[W], 
     for (int w = 0; w < W; ++w) {
       const u64 t = ((ap & 1) ? r[w] : 0ull) ^ Bp[w];
       const u64 keep = ((w == sl) ? P.nodiag : ~0ull) & ((w == pw) ? ~(1ull << pb) : ~0ull);
-      const u32 lo = __builtin_amdgcn_bitop3_b32(
-          __builtin_amdgcn_bitop3_b32(lo32(P.B[sl][w]), lo32(t), mr, 0x78), lo32(r[w]), mb, 0x78);
-      const u32 hi = __builtin_amdgcn_bitop3_b32(
-          __builtin_amdgcn_bitop3_b32(hi32(P.B[sl][w]), hi32(t), mr, 0x78), hi32(r[w]), mb, 0x78);
-      P.B[sl][w] = (((u64)hi << 32) | lo) & keep;
+      P.B[sl][w] = xor_and(xor_and(P.B[sl][w], t, mr), r[w], mb) & keep;
     }
     if (CONS == CONS_AB) {
-      const int ha = (int)((wsel<W>(P.ca[sl], pw) >> pb) & 1ull);
-      const u32 ma = (u32)-ha;
+      const u32 ma = bitmask(wsel<W>(P.ca[sl], pw), pb);
 #pragma unroll
-      for (int w = 0; w < W; ++w) {
-        const u32 lo = __builtin_amdgcn_bitop3_b32(lo32(P.ca[sl][w]), lo32(r[w]), ma, 0x78);
-        const u32 hi = __builtin_amdgcn_bitop3_b32(hi32(P.ca[sl][w]), hi32(r[w]), ma, 0x78);
-        P.ca[sl][w] = (((u64)hi << 32) | lo) & ((w == pw) ? ~(1ull << pb) : ~0ull);
-      }
-      if (c) P.rhsa[sl] ^= __ballot(ha);
+      for (int w = 0; w < W; ++w) P.ca[sl][w] = xor_and(P.ca[sl][w], r[w], ma) & ((w == pw) ? ~(1ull << pb) : ~0ull);
+      if (c) P.rhsa[sl] ^= __ballot(ma != 0u);
     }
     if (CONS != CONS_NONE) {
-      const int hb = (int)((wsel<W>(P.cb[sl], pw) >> pb) & 1ull);
-      const u32 mb2 = (u32)-hb;
+      const u32 mb2 = bitmask(wsel<W>(P.cb[sl], pw), pb);
 #pragma unroll
-      for (int w = 0; w < W; ++w) {
-        const u32 lo = __builtin_amdgcn_bitop3_b32(lo32(P.cb[sl][w]), lo32(r[w]), mb2, 0x78);
-        const u32 hi = __builtin_amdgcn_bitop3_b32(hi32(P.cb[sl][w]), hi32(r[w]), mb2, 0x78);
-        P.cb[sl][w] = (((u64)hi << 32) | lo) & ((w == pw) ? ~(1ull << pb) : ~0ull);
-      }
-      if (c) P.rhsb[sl] ^= __ballot(hb);
+      for (int w = 0; w < W; ++w) P.cb[sl][w] = xor_and(P.cb[sl][w], r[w], mb2) & ((w == pw) ? ~(1ull << pb) : ~0ull);
+      if (c) P.rhsb[sl] ^= __ballot(mb2 != 0u);
     }
   }
   clearbit<W>(P.rem, pw, pb);
@@ -246,16 +232,13 @@ __device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int*
 #pragma unroll
       for (int sl = 0; sl < W; ++sl) {
         // mb = all ones where B_pl (bit p of this lane's row)
-        const u32 mb = (u32)-(int)((wsel<W>(P.B[sl], pw) >> pb) & 1ull);
+        const u32 mb = bitmask(wsel<W>(P.B[sl], pw), pb);
         P.a[sl] = (P.a[sl] + (int)((u32)add & mb)) & 3;
-        // B_l = (B_l ^ (B_p & mb)) & ~e_l & ~e_p, two three-input operations per dword
+        // B_l = (B_l ^ (B_p & mb)) & ~e_l & ~e_p
 #pragma unroll
         for (int w = 0; w < W; ++w) {
           const u64 keep = ((w == sl) ? P.nodiag : ~0ull) & ((w == pw) ? ~(1ull << pb) : ~0ull);
-          const u32 lo = __builtin_amdgcn_bitop3_b32(lo32(P.B[sl][w]), lo32(Bp[w]), mb, 0x78);
-          const u32 hi = __builtin_amdgcn_bitop3_b32(hi32(P.B[sl][w]), hi32(Bp[w]), mb, 0x78);
-          P.B[sl][w] = ((u64)hi << 32) | lo;
-          P.B[sl][w] &= keep;
+          P.B[sl][w] = xor_and(P.B[sl][w], Bp[w], mb) & keep;
         }
       }
       uniform_state<W>(P);

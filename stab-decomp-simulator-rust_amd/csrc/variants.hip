@@ -144,45 +144,42 @@ __device__ __forceinline__ void subst_v(PairRegs<W>& P, VarRegs<W, VW>& V, int p
   }
   const int lin = c ? (4 - ap) & 3 : ap;
   const int pw = wordof<W>(p), pb = p & 63;
+  const u32 cm = c ? ~0u : 0u;
 #pragma unroll
   for (int sl = 0; sl < W; ++sl) {
-    const int rl = (int)((r[sl] >> P.lane) & 1ull);
-    const int bpl = (int)((wsel<W>(P.B[sl], pw) >> pb) & 1ull);
-    P.a[sl] = (P.a[sl] + (rl ? lin : 0) + ((c ^ rl) & bpl ? 2 : 0)) & 3;
+    // lane masks: mr where r_l, mb where B_pl
+    const u32 mr = bitmask(r[sl], P.lane);
+    const u32 mb = bitmask(wsel<W>(P.B[sl], pw), pb);
+    const u32 two = __builtin_amdgcn_bitop3_b32(mr ^ cm, mb, 2u, 0x80);  // (c ^ r_l) & B_pl ? 2 : 0
+    P.a[sl] = (int)(((u32)P.a[sl] + ((u32)lin & mr) + two) & 3u);
 #pragma unroll
-    for (int v = 0; v < VW; ++v) V.hi[sl][v] ^= (rl ? L[v] : 0ull) ^ (bpl ? E[v] : 0ull);
-    const u64 mr = 0ull - (u64)rl, mb = 0ull - (u64)bpl;
+    for (int v = 0; v < VW; ++v) V.hi[sl][v] = xor_and(xor_and(V.hi[sl][v], L[v], mr), E[v], mb);
 #pragma unroll
     for (int w = 0; w < W; ++w) {
       const u64 t = ((ap & 1) ? r[w] : 0ull) ^ Bp[w];
-      P.B[sl][w] ^= (t & mr) ^ (r[w] & mb);
+      const u64 keep = ((w == sl) ? P.nodiag : ~0ull) & ((w == pw) ? ~(1ull << pb) : ~0ull);
+      P.B[sl][w] = xor_and(xor_and(P.B[sl][w], t, mr), r[w], mb) & keep;
     }
-#pragma unroll
-    for (int w = 0; w < W; ++w)
-      if (w == sl) P.B[sl][w] &= ~(1ull << P.lane);
-    clearbit<W>(P.B[sl], pw, pb);
     if (CONS == CONS_AB) {
       const int ha = (int)((wsel<W>(P.ca[sl], pw) >> pb) & 1ull);
-      const u64 ma = 0ull - (u64)ha;
+      const u32 ma = (u32)-ha;
 #pragma unroll
-      for (int w = 0; w < W; ++w) P.ca[sl][w] ^= r[w] & ma;
-      clearbit<W>(P.ca[sl], pw, pb);
+      for (int w = 0; w < W; ++w) P.ca[sl][w] = xor_and(P.ca[sl][w], r[w], ma) & ((w == pw) ? ~(1ull << pb) : ~0ull);
       if (c) P.rhsa[sl] ^= __ballot(ha);
       if (UNITS) {
 #pragma unroll
-        for (int v = 0; v < VW; ++v) V.ra[sl][v] ^= ha ? E[v] : 0ull;
+        for (int v = 0; v < VW; ++v) V.ra[sl][v] = xor_and(V.ra[sl][v], E[v], ma);
       }
     }
     if (CONS != CONS_NONE) {
       const int hb = (int)((wsel<W>(P.cb[sl], pw) >> pb) & 1ull);
-      const u64 mb2 = 0ull - (u64)hb;
+      const u32 mb2 = (u32)-hb;
 #pragma unroll
-      for (int w = 0; w < W; ++w) P.cb[sl][w] ^= r[w] & mb2;
-      clearbit<W>(P.cb[sl], pw, pb);
+      for (int w = 0; w < W; ++w) P.cb[sl][w] = xor_and(P.cb[sl][w], r[w], mb2) & ((w == pw) ? ~(1ull << pb) : ~0ull);
       if (c) P.rhsb[sl] ^= __ballot(hb);
       if (CONS == CONS_B || UNITS) {
 #pragma unroll
-        for (int v = 0; v < VW; ++v) V.rb[sl][v] ^= hb ? E[v] : 0ull;
+        for (int v = 0; v < VW; ++v) V.rb[sl][v] = xor_and(V.rb[sl][v], E[v], mb2);
       }
     }
   }
@@ -321,18 +318,17 @@ __device__ __forceinline__ void pair_eval_v(PairRegs<W>& P, VarRegs<W, VW>& V, i
       for (int v = 0; v < VW; ++v) z8_add(V.d0[v], V.d1[v], V.d2[v], 0ull, H[v], ap == 1 ? H[v] : 0ull);
 #pragma unroll
       for (int sl = 0; sl < W; ++sl) {
-        const int bpl = (int)((wsel<W>(P.B[sl], pw) >> pb) & 1ull);
-        const u64 mb = 0ull - (u64)bpl;
-        P.a[sl] = (P.a[sl] + (bpl ? add : 0)) & 3;
+        const u32 mb = bitmask(wsel<W>(P.B[sl], pw), pb);
+        P.a[sl] = (P.a[sl] + (int)((u32)add & mb)) & 3;
 #pragma unroll
-        for (int v = 0; v < VW; ++v) V.hi[sl][v] ^= H[v] & mb;
+        for (int v = 0; v < VW; ++v) V.hi[sl][v] = xor_and(V.hi[sl][v], H[v], mb);
 #pragma unroll
-        for (int w = 0; w < W; ++w) P.B[sl][w] ^= Bp[w] & mb;
-#pragma unroll
-        for (int w = 0; w < W; ++w)
-          if (w == sl) P.B[sl][w] &= ~(1ull << P.lane);
-        clearbit<W>(P.B[sl], pw, pb);
+        for (int w = 0; w < W; ++w) {
+          const u64 keep = ((w == sl) ? P.nodiag : ~0ull) & ((w == pw) ? ~(1ull << pb) : ~0ull);
+          P.B[sl][w] = xor_and(P.B[sl][w], Bp[w], mb) & keep;
+        }
       }
+      uniform_state<W, VW>(P, V);
     } else {
       P.mag -= 2;
 #pragma unroll
@@ -340,8 +336,8 @@ __device__ __forceinline__ void pair_eval_v(PairRegs<W>& P, VarRegs<W, VW>& V, i
       const int qv = ffsv<W>(Bp);
       flipbit<W>(Bp, qv, 1);
       subst_v<W, VW, CONS_NONE, UNITS>(P, V, qv, Bp, ap >> 1, H);
+      uniform_state<W, VW>(P, V);
     }
-    uniform_state<W, VW>(P, V);
   }
 }
 
