@@ -834,8 +834,11 @@ __device__ __forceinline__ double pow_half(int r) {
   return ldexp((r & 1) ? 0x1.6a09e667f3bcdp-1 : 1.0, -(r >> 1));
 }
 
-template <int W>
-__global__ __launch_bounds__(256) void k_pairs(QFormDev X, QFormDev Y, const int64_t* __restrict__ offsets,
+// SAME: X and Y (and cx and cy) are the same state's forms and coefficients
+// (norm, exp_value): one set of table pointers is kept in scalar registers,
+// which leaves room for 8 waves per SIMD.
+template <int W, bool SAME>
+__global__ __launch_bounds__(256, SAME ? 8 : 1) void k_pairs(QFormDev X, QFormDev Y, const int64_t* __restrict__ offsets,
                                                int64_t nfixed, int64_t nother, int triangle, int J, int mode,
                                                CoefDev cx, CoefDev cy, const double* __restrict__ tabs,
                                                double2* __restrict__ partial, uint32_t* __restrict__ exact_out,
@@ -871,14 +874,14 @@ __global__ __launch_bounds__(256) void k_pairs(QFormDev X, QFormDev Y, const int
     const int64_t base = triangle ? fx : 0;
     const int64_t o_lo = base + (u - offsets[fx]) * J;
     const int64_t o_hi = (o_lo + J < nother) ? o_lo + J : nother;
-    const QFormDev& qf = fixed_is_ket ? Y : X;
-    const QFormDev& qo = fixed_is_ket ? X : Y;
+    const QFormDev& qf = (SAME || !fixed_is_ket) ? X : Y;
+    const QFormDev& qo = (SAME || fixed_is_ket) ? X : Y;
     u64 Bf[W][W], Cf[W][W], rf[W];
     int af[W];
     load_qrows<W>(qf, fx, lane, Bf, af, Cf, rf);
     const int ncf = uni(qf.nc[fx]), kf = uni(qf.k[fx]), pf = uni(qf.p0[fx]);
-    const CoefDev& cf = fixed_is_ket ? cy : cx;
-    const CoefDev& co = fixed_is_ket ? cx : cy;
+    const CoefDev& cf = (SAME || !fixed_is_ket) ? cx : cy;
+    const CoefDev& co = (SAME || fixed_is_ket) ? cx : cy;
     const int32_t csf = cf.scalar ? uni(cf.cs[fx]) : 0;
     const double2 cvf = cf.scalar ? make_double2(0.0, 0.0) : cf.c[fx];
     for (int64_t o = o_lo; o < o_hi; ++o) {
@@ -1153,9 +1156,17 @@ hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, const int64_t* off
                         int64_t nother, int triangle, int J, int mode, const CoefDev& cx, const CoefDev& cy,
                         const double* tabs, double2* partial, int nblocks, uint32_t* exact_out, int64_t exact_ld,
                         hipStream_t st) {
-  NX_DISPATCH_W(X.W, k_pairs<WW><<<dim3(nblocks), dim3(256), 0, st>>>(X, Y, offsets, nfixed, nother, triangle, J,
-                                                                      mode, cx, cy, tabs, partial, exact_out,
-                                                                      exact_ld));
+  const bool same = X.B == Y.B && X.C == Y.C && X.a == Y.a && cx.scalar == cy.scalar && cx.cs == cy.cs &&
+                    cx.c == cy.c;
+  if (same) {
+    NX_DISPATCH_W(X.W, k_pairs<WW, true><<<dim3(nblocks), dim3(256), 0, st>>>(
+                           X, X, offsets, nfixed, nother, triangle, J, mode, cx, cx, tabs, partial, exact_out,
+                           exact_ld));
+  } else {
+    NX_DISPATCH_W(X.W, k_pairs<WW, false><<<dim3(nblocks), dim3(256), 0, st>>>(
+                           X, Y, offsets, nfixed, nother, triangle, J, mode, cx, cy, tabs, partial, exact_out,
+                           exact_ld));
+  }
   return hipGetLastError();
 }
 
