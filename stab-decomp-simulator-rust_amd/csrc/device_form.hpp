@@ -59,13 +59,24 @@ __device__ __forceinline__ int ffs64(u64 x) { return __builtin_ctzll(x); }
 template <int W>
 __device__ __forceinline__ int wordof(int c) { return W == 1 ? 0 : (c >> 6); }
 
+// Element w of a small register array.  For W >= 3 a chain of selects over
+// the elements is folded by the compiler into a load at a run-time index,
+// which sends the whole array (and the CH-form holding it) to scratch
+// memory; masks keep every element in registers.
 template <int W>
 __device__ __forceinline__ u64 wsel(const u64 (&x)[W], int w) {
-  u64 r = x[0];
+  if constexpr (W >= 3) {
+    u64 r = 0ull;
 #pragma unroll
-  for (int k = 1; k < W; ++k)
-    if (w == k) r = x[k];
-  return r;
+    for (int k = 0; k < W; ++k) r |= x[k] & (0ull - (u64)(w == k));
+    return r;
+  } else {
+    u64 r = x[0];
+#pragma unroll
+    for (int k = 1; k < W; ++k)
+      if (w == k) r = x[k];
+    return r;
+  }
 }
 template <int W>
 __device__ __forceinline__ int getbit(const u64 (&x)[W], int c) {
@@ -128,20 +139,34 @@ __device__ __forceinline__ void bcast(const u64 (&X)[W][W], int r, u64 (&out)[W]
   const int sl = wordof<W>(r), ln = r & 63;
 #pragma unroll
   for (int w = 0; w < W; ++w) {
-    u64 val = X[0][w];
+    u64 val;
+    if constexpr (W >= 3) {  // masks, as in wsel
+      val = 0ull;
 #pragma unroll
-    for (int k = 1; k < W; ++k)
-      if (sl == k) val = X[k][w];
+      for (int k = 0; k < W; ++k) val |= X[k][w] & (0ull - (u64)(sl == k));
+    } else {
+      val = X[0][w];
+#pragma unroll
+      for (int k = 1; k < W; ++k)
+        if (sl == k) val = X[k][w];
+    }
     out[w] = rl64(val, ln);
   }
 }
 template <int W>
 __device__ __forceinline__ int bcast_g(const Form<W>& f, int r) {
   const int sl = wordof<W>(r);
-  int val = f.g[0];
+  int val;
+  if constexpr (W >= 3) {  // masks, as in wsel
+    val = 0;
 #pragma unroll
-  for (int k = 1; k < W; ++k)
-    if (sl == k) val = f.g[k];
+    for (int k = 0; k < W; ++k) val |= f.g[k] & -(int)(sl == k);
+  } else {
+    val = f.g[0];
+#pragma unroll
+    for (int k = 1; k < W; ++k)
+      if (sl == k) val = f.g[k];
+  }
   return rl32(val, r & 63);
 }
 template <int W>

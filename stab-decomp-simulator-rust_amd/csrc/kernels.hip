@@ -836,9 +836,9 @@ __device__ __forceinline__ double pow_half(int r) {
 
 // SAME: X and Y (and cx and cy) are the same state's forms and coefficients
 // (norm, exp_value): one set of table pointers is kept in scalar registers,
-// which leaves room for 8 waves per SIMD.
+// which leaves room for 8 waves per SIMD at W = 1.
 template <int W, bool SAME>
-__global__ __launch_bounds__(256, SAME ? 8 : 1) void k_pairs(QFormDev X, QFormDev Y, const int64_t* __restrict__ offsets,
+__global__ __launch_bounds__(256, (SAME && W == 1) ? 8 : 1) void k_pairs(QFormDev X, QFormDev Y, const int64_t* __restrict__ offsets,
                                                int64_t nfixed, int64_t nother, int triangle, int J, int mode,
                                                CoefDev cx, CoefDev cy, const double* __restrict__ tabs,
                                                double2* __restrict__ partial, uint32_t* __restrict__ exact_out,

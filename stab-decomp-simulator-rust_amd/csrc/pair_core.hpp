@@ -86,10 +86,17 @@ __device__ __forceinline__ void clearbit(u64 (&x)[W], int pw, int pb) {
 template <int W>
 __device__ __forceinline__ int bcast_int(const int (&x)[W], int r) {
   const int sl = wordof<W>(r);
-  int val = x[0];
+  int val;
+  if constexpr (W >= 3) {  // masks, as in wsel
+    val = 0;
 #pragma unroll
-  for (int k = 1; k < W; ++k)
-    if (sl == k) val = x[k];
+    for (int k = 0; k < W; ++k) val |= x[k] & -(int)(sl == k);
+  } else {
+    val = x[0];
+#pragma unroll
+    for (int k = 1; k < W; ++k)
+      if (sl == k) val = x[k];
+  }
   return rl32(val, r & 63);
 }
 

@@ -73,10 +73,17 @@ __device__ __forceinline__ void bcast_v(const u64 (&X)[W][VW], int r, u64 (&out)
   const int sl = wordof<W>(r), ln = r & 63;
 #pragma unroll
   for (int v = 0; v < VW; ++v) {
-    u64 val = X[0][v];
+    u64 val;
+    if constexpr (W >= 3) {  // masks, as in wsel
+      val = 0ull;
 #pragma unroll
-    for (int k = 1; k < W; ++k)
-      if (sl == k) val = X[k][v];
+      for (int k = 0; k < W; ++k) val |= X[k][v] & (0ull - (u64)(sl == k));
+    } else {
+      val = X[0][v];
+#pragma unroll
+      for (int k = 1; k < W; ++k)
+        if (sl == k) val = X[k][v];
+    }
     out[v] = rl64(val, ln);
   }
 }
