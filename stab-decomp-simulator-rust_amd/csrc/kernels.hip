@@ -837,9 +837,11 @@ __device__ __forceinline__ double pow_half(int r) {
 // SAME: X and Y (and cx and cy) are the same state's forms and coefficients
 // (norm, exp_value): one set of table pointers is kept in scalar registers,
 // which leaves room for 8 waves per SIMD at W = 1.
-template <int W, bool SAME>
+// MODE: the mode argument when it is 0..3 at compile time (no per-pair role
+// selects), or -1 to read it at run time.
+template <int W, bool SAME, int MODE>
 __global__ __launch_bounds__(256, (SAME && W == 1) ? 8 : 1) void k_pairs(QFormDev X, QFormDev Y, const int64_t* __restrict__ offsets,
-                                               int64_t nfixed, int64_t nother, int triangle, int J, int mode,
+                                               int64_t nfixed, int64_t nother, int triangle, int J, int mode_rt,
                                                CoefDev cx, CoefDev cy, const double* __restrict__ tabs,
                                                double2* __restrict__ partial, uint32_t* __restrict__ exact_out,
                                                int64_t exact_ld) {
@@ -854,6 +856,7 @@ __global__ __launch_bounds__(256, (SAME && W == 1) ? 8 : 1) void k_pairs(QFormDe
     const int hi = n - 64 * w;
     valid[w] = hi >= 64 ? ~0ull : (hi <= 0 ? 0ull : ((1ull << hi) - 1ull));
   }
+  const int mode = MODE >= 0 ? MODE : mode_rt;
   const bool fixed_is_ket = mode & 1;
   const bool herm = (mode >> 1) & 1;
   // cos, sin of k pi / 4 (the host tables) in LDS: read with a lane address
@@ -1158,15 +1161,27 @@ hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, const int64_t* off
                         hipStream_t st) {
   const bool same = X.B == Y.B && X.C == Y.C && X.a == Y.a && cx.scalar == cy.scalar && cx.cs == cy.cs &&
                     cx.c == cy.c;
-  if (same) {
-    NX_DISPATCH_W(X.W, k_pairs<WW, true><<<dim3(nblocks), dim3(256), 0, st>>>(
-                           X, X, offsets, nfixed, nother, triangle, J, mode, cx, cx, tabs, partial, exact_out,
-                           exact_ld));
+  // the combinations the runtime uses: norm (same, fixed bra, Hermitian),
+  // pair tables (same, complex), exp_value (fixed ket, Hermitian), inner
+  // products (complex); any other mode is read at run time
+#define NX_PAIRS(SM, MD)                                                                                      \
+  NX_DISPATCH_W(X.W, k_pairs<WW, SM, MD><<<dim3(nblocks), dim3(256), 0, st>>>(                                \
+                         X, SM ? X : Y, offsets, nfixed, nother, triangle, J, mode, cx, SM ? cx : cy, tabs, partial, \
+                         exact_out, exact_ld))
+  if (same && mode == 2) {
+    NX_PAIRS(true, 2);
+  } else if (same && mode == 0) {
+    NX_PAIRS(true, 0);
+  } else if (same) {
+    NX_PAIRS(true, -1);
+  } else if (mode == 3) {
+    NX_PAIRS(false, 3);
+  } else if (mode == 0) {
+    NX_PAIRS(false, 0);
   } else {
-    NX_DISPATCH_W(X.W, k_pairs<WW, false><<<dim3(nblocks), dim3(256), 0, st>>>(
-                           X, Y, offsets, nfixed, nother, triangle, J, mode, cx, cy, tabs, partial, exact_out,
-                           exact_ld));
+    NX_PAIRS(false, -1);
   }
+#undef NX_PAIRS
   return hipGetLastError();
 }
 
