@@ -873,10 +873,12 @@ __global__ __launch_bounds__(256, (SAME && W == 1) ? 8 : 1) void k_pairs(QFormDe
       if (offsets[mid] <= u) lo_r = mid;
       else hi_r = mid;
     }
-    const int64_t fx = lo_r;
-    const int64_t base = triangle ? fx : 0;
-    const int64_t o_lo = base + (u - offsets[fx]) * J;
-    const int64_t o_hi = (o_lo + J < nother) ? o_lo + J : nother;
+    // term indices fit 32 bits (the forms' 32-bit row offsets already
+    // require chi * 64W * W < 2^32)
+    const int fx = (int)lo_r;
+    const int base = triangle ? fx : 0;
+    const int o_lo = base + (int)(u - offsets[fx]) * J;
+    const int o_hi = (o_lo + J < (int)nother) ? o_lo + J : (int)nother;
     const QFormDev& qf = (SAME || !fixed_is_ket) ? X : Y;
     const QFormDev& qo = (SAME || fixed_is_ket) ? X : Y;
     u64 Bf[W][W], Cf[W][W], rf[W];
@@ -887,7 +889,7 @@ __global__ __launch_bounds__(256, (SAME && W == 1) ? 8 : 1) void k_pairs(QFormDe
     const CoefDev& co = (SAME || fixed_is_ket) ? cx : cy;
     const int32_t csf = cf.scalar ? uni(cf.cs[fx]) : 0;
     const double2 cvf = cf.scalar ? make_double2(0.0, 0.0) : cf.c[fx];
-    for (int64_t o = o_lo; o < o_hi; ++o) {
+    for (int o = o_lo; o < o_hi; ++o) {
       u64 Bo[W][W], Co[W][W], ro[W];
       int ao[W];
       load_qrows<W>(qo, o, lane, Bo, ao, Co, ro);
@@ -895,8 +897,8 @@ __global__ __launch_bounds__(256, (SAME && W == 1) ? 8 : 1) void k_pairs(QFormDe
       // the other term's coefficient, read now and used after the elimination
       const int32_t cso = co.scalar ? uni(co.cs[o]) : 0;
       const double2 cvo = co.scalar ? make_double2(0.0, 0.0) : co.c[o];
-      const int64_t bra = fixed_is_ket ? o : fx;
-      const int64_t ket = fixed_is_ket ? fx : o;
+      const int bra = fixed_is_ket ? o : fx;
+      const int ket = fixed_is_ket ? fx : o;
       PairRegs<W> P;
       P.lane = lane;
       P.nodiag = ~(1ull << lane);
