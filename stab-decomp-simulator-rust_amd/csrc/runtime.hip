@@ -1088,6 +1088,7 @@ static VarPlan make_plan(const std::vector<VarJob>& in, int VW, int J, int64_t t
     d.nunit = 0;
     d.uq = nullptr;
     d.ub = nullptr;
+    d.termvar = 0;
     d.split_ab = j.split_ab;
     for (int w = 0; w < 4; ++w) d.removed[w] = 0ull;
     d.wave0 = wave;
@@ -1263,7 +1264,9 @@ static std::vector<double> exp_values_var(const nx_state* s, const PauliSet& ps,
 struct ProjPlan {
   const nx_state* s = nullptr;
   VarPlan plan;
-  QForms proj;  // the forms projected onto the fixed outcomes of a window
+  QForms proj;   // the forms projected onto the fixed outcomes of a window
+  QForms projv;  // and onto each look-ahead outcome (base form + tables)
+  DevBuf vhi, vrhs, vph;
   std::vector<PairJobDev> hj;
   int VW = 1, maxunit = 0;
   double g2 = 1.0;
@@ -1298,6 +1301,11 @@ struct ProjPlan {
     stage_h.alloc(stage_bytes);
     res_h.alloc((size_t)2 * 64 * VW * 8);
     proj.alloc_pair(st->chi(), st->n);
+    projv.alloc_pair(st->chi(), st->n);
+    const size_t rows = (size_t)st->chi() * 64 * projv.W;
+    vhi.alloc(std::max<size_t>(rows * VW * 8, 8));
+    vrhs.alloc(std::max<size_t>(rows * VW * 8, 8));
+    vph.alloc(std::max<size_t>((size_t)st->chi() * 3 * VW * 8, 8));
     sync();
     g2 = st->g.real() * st->g.real() + st->g.imag() * st->g.imag();
   }
@@ -1307,8 +1315,8 @@ struct ProjPlan {
     const int d = (int)look.size(), nv = 1 << d;
     if ((int)(fq.size() + look.size()) > maxunit) fail(NX_E_INTERNAL, "too many projected qubits");
     if (nv > 64 * VW) fail(NX_E_INTERNAL, "too many look-ahead qubits");
-    // the fixed outcomes are substituted once per term (k_qproject); the
-    // look-ahead qubits stay unit constraints with one outcome per variant
+    // the fixed outcomes are substituted once per term (k_qproject), then
+    // the look-ahead outcomes, one per variant (k_qproject_var)
     const QForms& q0 = qform_of(s);
     char* dbase = (char*)stage_d.p;
     PairJobDev& job = hj[0];
@@ -1340,13 +1348,24 @@ struct ProjPlan {
     job.uq = (const int32_t*)(dbase + o_uq);
     job.ub = (const uint64_t*)(dbase + o_ub);
     job.nvar = nv;
+    // the look-ahead outcomes are substituted once per term as well
+    // (k_qproject_var): the pairs start from the projected forms and tables
+    const VarDev vt{vhi.as<uint64_t>(), vrhs.as<uint64_t>(), vph.as<uint64_t>()};
+    job.X = job.Y = projv.desc();
+    job.V = vt;
+    job.termvar = 1;
+    for (int i = 0; i < d; ++i) job.removed[look[i] >> 6] |= 1ull << (look[i] & 63);
     std::memcpy(stage_h.p, &job, sizeof(PairJobDev));
     h2d(stage_d.p, stage_h.p, stage_bytes);
-    if (!fq.empty()) {
+    {
       ProfScope ps("k_qproject");
-      hip_check(launch_qproject(q0.desc(), proj.desc(), (const int32_t*)(dbase + o_fq),
-                                (const int32_t*)(dbase + o_fb), (int)fq.size(), stream()),
-                "k_qproject");
+      if (!fq.empty())
+        hip_check(launch_qproject(q0.desc(), proj.desc(), (const int32_t*)(dbase + o_fq),
+                                  (const int32_t*)(dbase + o_fb), (int)fq.size(), stream()),
+                  "k_qproject");
+      hip_check(launch_qproject_var(fq.empty() ? q0.desc() : proj.desc(), projv.desc(), vt, VW,
+                                    (const int32_t*)(dbase + o_uq), (const uint64_t*)(dbase + o_ub), d, stream()),
+                "k_qproject_var");
       ps.end();
     }
     {

@@ -408,6 +408,93 @@ __global__ __launch_bounds__(256) void k_qproject(QFormDev in, QFormDev out, con
 }
 
 // ---------------------------------------------------------------------------
+// Projection of every term onto the look-ahead outcomes of a sampling window,
+// one variant per outcome: x_{uq[u]} = ub[u * (VW + 1)] for the base and the
+// other value for the variants in the masks that follow, u < nunit in order,
+// substituted into the term's own form as in pair_eval_v's unit
+// constraints.  The result is the base form (out) and, per term, the
+// variants' differences from it (vout): high bits of the linear
+// coefficients (hi, per row), right-hand-side flips of the term's support
+// rows (rhs, per row) and Z8 phase planes (ph, 3 x VW words).  A
+// substitution is linear in the form, so the pair <Pi_k phi_o | Pi_k phi_f>
+// of two projected terms is the elimination of the difference of the two,
+// with variant differences hi_f ^ hi_o, the ket's and the bra's rhs flips and
+// phase d_f - d_o: each pair of the window pass starts after the unit
+// substitutions (Pi_k is a projector, so this is <phi_o| Pi_k |phi_f>).
+template <int W, int VW>
+__global__ __launch_bounds__(256) void k_qproject_var(QFormDev in, QFormDev out, VarDev vout,
+                                                      const int32_t* __restrict__ uq,
+                                                      const uint64_t* __restrict__ ub, int nunit) {
+  const int64_t t = (int64_t)blockIdx.x * 4 + uni((int)(threadIdx.x >> 6));  // uniform in the wave
+  const int lane = threadIdx.x & 63;
+  if (t >= in.chi) return;
+  const int R = 64 * W;
+  PairRegs<W> P;
+  VarRegs<W, VW> V;
+  P.lane = lane;
+  P.nodiag = ~(1ull << lane);
+  load_qrows<W>(in, t, lane, P.B, P.a, P.ca, P.rhsa);
+#pragma unroll
+  for (int sl = 0; sl < W; ++sl)
+#pragma unroll
+    for (int w = 0; w < W; ++w) P.cb[sl][w] = 0ull;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    P.rhsb[w] = 0ull;
+    P.rem[w] = 0ull;
+  }
+  P.c0 = uni(in.p0[t]);
+  P.mag = 0;
+#pragma unroll
+  for (int sl = 0; sl < W; ++sl)
+#pragma unroll
+    for (int v = 0; v < VW; ++v) V.hi[sl][v] = V.ra[sl][v] = V.rb[sl][v] = 0ull;
+#pragma unroll
+  for (int v = 0; v < VW; ++v) V.d0[v] = V.d1[v] = V.d2[v] = V.z[v] = 0ull;
+  V.bz = 0;
+  u64 r0[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) r0[w] = 0ull;
+  for (int u = 0; u < nunit; ++u) {
+    const int q = uni(uq[u]);
+    const int b = (int)uni64(ub[(size_t)u * (VW + 1)]);
+    u64 E[VW];
+#pragma unroll
+    for (int v = 0; v < VW; ++v) E[v] = uni64(ub[(size_t)u * (VW + 1) + 1 + v]);
+    subst_v<W, VW, CONS_AB, true>(P, V, q, r0, b, E);
+    uniform_state<W, VW>(P, V);
+  }
+#pragma unroll
+  for (int sl = 0; sl < W; ++sl) {
+    const int r = 64 * sl + lane;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      out.B[rowidx(t, R, r, W, w)] = P.B[sl][w];
+      out.C[rowidx(t, R, r, W, w)] = P.ca[sl][w];
+    }
+    out.a[(size_t)t * R + r] = (uint8_t)P.a[sl];
+#pragma unroll
+    for (int v = 0; v < VW; ++v) {
+      ((u64*)vout.hi)[((size_t)t * R + r) * VW + v] = V.hi[sl][v];
+      ((u64*)vout.rhs)[((size_t)t * R + r) * VW + v] = V.ra[sl][v];
+    }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) out.rhs[(size_t)t * W + w] = P.rhsa[w];
+    out.p0[t] = P.c0;
+    out.nc[t] = in.nc[t];
+    out.k[t] = in.k[t];
+#pragma unroll
+    for (int v = 0; v < VW; ++v) {
+      ((u64*)vout.ph)[((size_t)t * 3 + 0) * VW + v] = V.d0[v];
+      ((u64*)vout.ph)[((size_t)t * 3 + 1) * VW + v] = V.d1[v];
+      ((u64*)vout.ph)[((size_t)t * 3 + 2) * VW + v] = V.d2[v];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Uniform Z-marginals.  Read the ket variants as symbols: variant k is Z on
 // coordinate k of a qubit order, and bit k of a row's mask is the
 // coefficient of t_k in a linear form over GF(2).  Every update of the
@@ -847,12 +934,16 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
     const int64_t o_lo = base + (u - jd.row_off[fx]) * J;
     const int64_t o_hi = (o_lo + J < nother) ? o_lo + J : nother;
     const int ncf = uni(Y.nc[fx]), kf = uni(Y.k[fx]), pf = uni(Y.p0[fx]);
+    // ket variant tables: Pauli strings (!UNITS), or window outcomes on both
+    // sides (UNITS with termvar: k_qproject_var tables for X = Y)
+    const bool tv = UNITS && jd.termvar;
+    const bool ket_tab = !UNITS || tv;
     u64 vp0[VW], vp1[VW], vp2[VW];
 #pragma unroll
     for (int v = 0; v < VW; ++v) {
-      vp0[v] = UNITS ? 0ull : uni64(jd.V.ph[((size_t)fx * 3 + 0) * VW + v]);
-      vp1[v] = UNITS ? 0ull : uni64(jd.V.ph[((size_t)fx * 3 + 1) * VW + v]);
-      vp2[v] = UNITS ? 0ull : uni64(jd.V.ph[((size_t)fx * 3 + 2) * VW + v]);
+      vp0[v] = ket_tab ? uni64(jd.V.ph[((size_t)fx * 3 + 0) * VW + v]) : 0ull;
+      vp1[v] = ket_tab ? uni64(jd.V.ph[((size_t)fx * 3 + 1) * VW + v]) : 0ull;
+      vp2[v] = ket_tab ? uni64(jd.V.ph[((size_t)fx * 3 + 2) * VW + v]) : 0ull;
     }
     for (int64_t o = o_lo; o < o_hi; ++o) {
       // ket rows are re-read per pair (cache hits) to keep registers free
@@ -877,11 +968,16 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
         }
         P.a[sl] = (af[sl] - ao[sl]) & 3;
         const size_t r = (size_t)fx * R + 64 * sl + lane;
+        const size_t ro_ = (size_t)o * R + 64 * sl + lane;
 #pragma unroll
         for (int v = 0; v < VW; ++v) {
-          V.hi[sl][v] = UNITS ? 0ull : jd.V.hi[r * VW + v];
-          V.rb[sl][v] = UNITS ? 0ull : jd.V.rhs[r * VW + v];
+          V.hi[sl][v] = ket_tab ? jd.V.hi[r * VW + v] : 0ull;
+          V.rb[sl][v] = ket_tab ? jd.V.rhs[r * VW + v] : 0ull;
           V.ra[sl][v] = 0ull;
+          if (tv) {
+            V.hi[sl][v] ^= jd.V.hi[ro_ * VW + v];
+            V.ra[sl][v] = jd.V.rhs[ro_ * VW + v];
+          }
         }
       }
 #pragma unroll
@@ -895,12 +991,17 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
         V.d0[v] = vp0[v];
         V.d1[v] = vp1[v];
         V.d2[v] = vp2[v];
+        if (tv) {  // minus the bra's phase planes: + ~d + 1 (mod 8)
+          z8_add(V.d0[v], V.d1[v], V.d2[v], ~uni64(jd.V.ph[((size_t)o * 3 + 0) * VW + v]),
+                 ~uni64(jd.V.ph[((size_t)o * 3 + 1) * VW + v]), ~uni64(jd.V.ph[((size_t)o * 3 + 2) * VW + v]));
+          z8_add(V.d0[v], V.d1[v], V.d2[v], ~0ull, 0ull, 0ull);
+        }
         V.z[v] = invalid[v];
       }
       V.bz = !want_base;
       P.c0 = (pf - po) & 7;
       P.mag = kf + ko;
-      pair_eval_v<W, VW, UNITS>(P, V, nco, ncf, jd.nunit, jd.uq, jd.ub);
+      pair_eval_v<W, VW, UNITS>(P, V, nco, ncf, tv ? 0 : jd.nunit, jd.uq, jd.ub);
       // (c_bra.conj() * c_ket).into(), shared by the variants
       double cr_, ci_;
       int kc = 0, rc = 0, czero = 0;  // exact coefficient product zeta^kc 2^{-rc/2}
@@ -1126,6 +1227,26 @@ hipError_t launch_pairs_var(int W, int VW, int units, const PairJobDev* jobs, co
   if (nwaves == 0) return hipSuccess;
   if (units) NX_LAUNCH_PAIRS_VAR(true)
   else NX_LAUNCH_PAIRS_VAR(false)
+  return hipGetLastError();
+}
+
+hipError_t launch_qproject_var(const QFormDev& in, const QFormDev& out, const VarDev& vout, int VW,
+                               const int32_t* uq, const uint64_t* ub, int nunit, hipStream_t st) {
+  if (in.chi == 0) return hipSuccess;
+  const dim3 g(nblk(in.chi, 4)), b(256);
+#define NX_QPV(WW, VV) k_qproject_var<WW, VV><<<g, b, 0, st>>>(in, out, vout, uq, ub, nunit)
+  switch (in.W * 4 + VW) {
+    case 1 * 4 + 1: NX_QPV(1, 1); break;
+    case 1 * 4 + 2: NX_QPV(1, 2); break;
+    case 2 * 4 + 1: NX_QPV(2, 1); break;
+    case 2 * 4 + 2: NX_QPV(2, 2); break;
+    case 3 * 4 + 1: NX_QPV(3, 1); break;
+    case 3 * 4 + 2: NX_QPV(3, 2); break;
+    case 4 * 4 + 1: NX_QPV(4, 1); break;
+    case 4 * 4 + 2: NX_QPV(4, 2); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef NX_QPV
   return hipGetLastError();
 }
 
