@@ -191,10 +191,12 @@ hipError_t launch_pairs_sym(const QFormDev& X, const int64_t* offsets, int J, in
                             const uint64_t* removed, int* dmin, int nblocks, hipStream_t st);
 // out = in with x_{uq[u]} = ub[u] substituted into every term's form, u <
 // nunit (the fields the pair kernels read: B, C, a, rhs, p0, nc, k)
-// window projection with one variant per outcome (tables in vout: hi and rhs
+// projection onto the fixed outcomes fq/fb (as launch_qproject), then onto
+// a window with one variant per outcome (tables in vout: hi and rhs
 // [chi][64W][VW], ph [chi][3][VW]); uq, ub as the unit constraints of a job
 hipError_t launch_qproject_var(const QFormDev& in, const QFormDev& out, const VarDev& vout, int VW,
-                               const int32_t* uq, const uint64_t* ub, int nunit, hipStream_t st);
+                               const int32_t* fq, const int32_t* fb, int nfix, const int32_t* uq,
+                               const uint64_t* ub, int nunit, hipStream_t st);
 hipError_t launch_qproject(const QFormDev& in, const QFormDev& out, const int32_t* uq, const int32_t* ub, int nunit,
                            hipStream_t st);
 // y = G c for the Hermitian G held in full in Gw[j*ld + i]

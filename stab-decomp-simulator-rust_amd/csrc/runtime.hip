@@ -1264,8 +1264,7 @@ static std::vector<double> exp_values_var(const nx_state* s, const PauliSet& ps,
 struct ProjPlan {
   const nx_state* s = nullptr;
   VarPlan plan;
-  QForms proj;   // the forms projected onto the fixed outcomes of a window
-  QForms projv;  // and onto each look-ahead outcome (base form + tables)
+  QForms projv;  // the forms projected onto the fixed and look-ahead outcomes (base + tables)
   DevBuf vhi, vrhs, vph;
   std::vector<PairJobDev> hj;
   int VW = 1, maxunit = 0;
@@ -1300,7 +1299,6 @@ struct ProjPlan {
     stage_d.alloc(stage_bytes);
     stage_h.alloc(stage_bytes);
     res_h.alloc((size_t)2 * 64 * VW * 8);
-    proj.alloc_pair(st->chi(), st->n);
     projv.alloc_pair(st->chi(), st->n);
     const size_t rows = (size_t)st->chi() * 64 * projv.W;
     vhi.alloc(std::max<size_t>(rows * VW * 8, 8));
@@ -1315,21 +1313,16 @@ struct ProjPlan {
     const int d = (int)look.size(), nv = 1 << d;
     if ((int)(fq.size() + look.size()) > maxunit) fail(NX_E_INTERNAL, "too many projected qubits");
     if (nv > 64 * VW) fail(NX_E_INTERNAL, "too many look-ahead qubits");
-    // the fixed outcomes are substituted once per term (k_qproject), then
-    // the look-ahead outcomes, one per variant (k_qproject_var)
+    // the fixed outcomes and then the look-ahead outcomes, one per variant,
+    // are substituted once per term (k_qproject_var)
     const QForms& q0 = qform_of(s);
     char* dbase = (char*)stage_d.p;
     PairJobDev& job = hj[0];
     for (int w = 0; w < 4; ++w) job.removed[w] = 0ull;
-    if (!fq.empty()) {
-      for (size_t i = 0; i < fq.size(); ++i) {
-        stage_h.as<int32_t>(o_fq)[i] = fq[i];
-        stage_h.as<int32_t>(o_fb)[i] = fb[i];
-        job.removed[fq[i] >> 6] |= 1ull << (fq[i] & 63);
-      }
-      job.X = job.Y = proj.desc();
-    } else {
-      job.X = job.Y = q0.desc();
+    for (size_t i = 0; i < fq.size(); ++i) {
+      stage_h.as<int32_t>(o_fq)[i] = fq[i];
+      stage_h.as<int32_t>(o_fb)[i] = fb[i];
+      job.removed[fq[i] >> 6] |= 1ull << (fq[i] & 63);
     }
     int32_t* uq = stage_h.as<int32_t>(o_uq);
     uint64_t* ub = stage_h.as<uint64_t>(o_ub);
@@ -1358,12 +1351,9 @@ struct ProjPlan {
     std::memcpy(stage_h.p, &job, sizeof(PairJobDev));
     h2d(stage_d.p, stage_h.p, stage_bytes);
     {
-      ProfScope ps("k_qproject");
-      if (!fq.empty())
-        hip_check(launch_qproject(q0.desc(), proj.desc(), (const int32_t*)(dbase + o_fq),
-                                  (const int32_t*)(dbase + o_fb), (int)fq.size(), stream()),
-                  "k_qproject");
-      hip_check(launch_qproject_var(fq.empty() ? q0.desc() : proj.desc(), projv.desc(), vt, VW,
+      ProfScope ps("k_qproject_var");
+      hip_check(launch_qproject_var(q0.desc(), projv.desc(), vt, VW, (const int32_t*)(dbase + o_fq),
+                                    (const int32_t*)(dbase + o_fb), (int)fq.size(),
                                     (const int32_t*)(dbase + o_uq), (const uint64_t*)(dbase + o_ub), d, stream()),
                 "k_qproject_var");
       ps.end();

@@ -408,7 +408,8 @@ __global__ __launch_bounds__(256) void k_qproject(QFormDev in, QFormDev out, con
 }
 
 // ---------------------------------------------------------------------------
-// Projection of every term onto the look-ahead outcomes of a sampling window,
+// Projection of every term onto the fixed outcomes x_{fq[u]} = fb[u], u <
+// nfix, of a sampling path, then onto the look-ahead outcomes of its window,
 // one variant per outcome: x_{uq[u]} = ub[u * (VW + 1)] for the base and the
 // other value for the variants in the masks that follow, u < nunit in order,
 // substituted into the term's own form as in pair_eval_v's unit
@@ -423,6 +424,8 @@ __global__ __launch_bounds__(256) void k_qproject(QFormDev in, QFormDev out, con
 // substitutions (Pi_k is a projector, so this is <phi_o| Pi_k |phi_f>).
 template <int W, int VW>
 __global__ __launch_bounds__(256) void k_qproject_var(QFormDev in, QFormDev out, VarDev vout,
+                                                      const int32_t* __restrict__ fq,
+                                                      const int32_t* __restrict__ fb, int nfix,
                                                       const int32_t* __restrict__ uq,
                                                       const uint64_t* __restrict__ ub, int nunit) {
   const int64_t t = (int64_t)blockIdx.x * 4 + uni((int)(threadIdx.x >> 6));  // uniform in the wave
@@ -455,6 +458,12 @@ __global__ __launch_bounds__(256) void k_qproject_var(QFormDev in, QFormDev out,
   u64 r0[W];
 #pragma unroll
   for (int w = 0; w < W; ++w) r0[w] = 0ull;
+  // the fixed outcomes first (k_qproject's substitutions: the variant
+  // differences are still zero and stay so)
+  for (int u = 0; u < nfix; ++u) {
+    subst<W, CONS_AB>(P, uni(fq[u]), r0, uni(fb[u]));
+    uniform_state<W>(P);
+  }
   for (int u = 0; u < nunit; ++u) {
     const int q = uni(uq[u]);
     const int b = (int)uni64(ub[(size_t)u * (VW + 1)]);
@@ -1231,10 +1240,11 @@ hipError_t launch_pairs_var(int W, int VW, int units, const PairJobDev* jobs, co
 }
 
 hipError_t launch_qproject_var(const QFormDev& in, const QFormDev& out, const VarDev& vout, int VW,
-                               const int32_t* uq, const uint64_t* ub, int nunit, hipStream_t st) {
+                               const int32_t* fq, const int32_t* fb, int nfix, const int32_t* uq,
+                               const uint64_t* ub, int nunit, hipStream_t st) {
   if (in.chi == 0) return hipSuccess;
   const dim3 g(nblk(in.chi, 4)), b(256);
-#define NX_QPV(WW, VV) k_qproject_var<WW, VV><<<g, b, 0, st>>>(in, out, vout, uq, ub, nunit)
+#define NX_QPV(WW, VV) k_qproject_var<WW, VV><<<g, b, 0, st>>>(in, out, vout, fq, fb, nfix, uq, ub, nunit)
   switch (in.W * 4 + VW) {
     case 1 * 4 + 1: NX_QPV(1, 1); break;
     case 1 * 4 + 2: NX_QPV(1, 2); break;
