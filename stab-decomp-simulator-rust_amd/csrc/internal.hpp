@@ -115,7 +115,8 @@ struct PairJobDev {
   // out[out_base + k] and sqrt(2) part B at out[out_base + nvar + k]
   int32_t split_ab;
   // variables already substituted in X and Y (forms projected by
-  // launch_qproject): they take no part in the elimination
+  // launch_qproject or launch_qproject_var): they take no part in the
+  // elimination
   uint64_t removed[4];
   // reduction segments seg0 .. seg0 + nseg - 1 (RED_SEG waves each)
   int64_t seg0;
