@@ -183,6 +183,30 @@ __global__ void k_gather(FormsDev in, FormsDev O, const int64_t* __restrict__ id
 // 2^{-|v|/2} with a_j = gamma_j/2 + 2 (par(s & w_j) ^ c_j) mod 4 and
 // Q_lm = par(u_l & w_m) for l < m.  The support constraints are the columns
 // k of F with v_k = 0: par(F[:,k] & x) = s_k.
+// LDS written by the lanes of a wave, then read by other lanes of it
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Transpose of a 64 x 64 bit matrix held one row per lane (bit k of lane
+// i's word = entry (i, k)): six rounds of swapping the off-diagonal blocks
+// of halving size with the partner lane.
+__device__ __forceinline__ u64 transpose64(u64 x, int lane) {
+  const u64 masks[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                        0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    const int j = 32 >> s;
+    const u64 m = masks[s];
+    const u32 ylo = (u32)__shfl_xor((int)lo32(x), j), yhi = (u32)__shfl_xor((int)hi32(x), j);
+    const u64 y = ((u64)yhi << 32) | ylo;
+    x = (lane & j) ? ((x & ~m) | ((y & ~m) >> j)) : ((x & m) | ((y & m) << j));
+  }
+  return x;
+}
+
 template <int W>
 __global__ __launch_bounds__(256) void k_qform(FormsDev in, QFormDev q) {
   const int64_t t = (int64_t)blockIdx.x * 4 + uni((int)(threadIdx.x >> 6));  // uniform in the wave
@@ -207,29 +231,57 @@ __global__ __launch_bounds__(256) void k_qform(FormsDev in, QFormDev q) {
     const int L = parand<W>(f.s, Wm[sl]) ^ c;
     a[sl] = (f.g[sl] + 2 * L) & 3;
   }
-  for (int m = 0; m < n; ++m) {
-    u64 wm[W];
-    bcast<W>(Wm, m, wm);
-#pragma unroll
-    for (int sl = 0; sl < W; ++sl) {
-      const int row = 64 * sl + lane;
-      const int bit = (row < m) ? parand<W>(U[sl], wm) : 0;
-      flipbit<W>(Q[sl], m, bit);
-    }
-  }
-  // B = Q ^ Q^T
   u64 Bm[W][W];
+  if constexpr (W == 1) {
+    // Q_l = (XOR_{k in u_l} WT_k) restricted to m > l, WT = W transposed:
+    // WT rows in LDS, combined four at a time into per-wave nibble tables
+    // (16 lookups per lane instead of 64 parities)
+    __shared__ u64 s_wt[4][64], s_tab[4][16 * 16];
+    const int wv = threadIdx.x >> 6;
+    s_wt[wv][lane] = transpose64(Wm[0][0], lane);  // row k: bit m = bit k of w_m
+    wave_lds_sync();
+    // s_tab[16 b + v] = XOR of WT_{4b + i} for the set bits i of v
 #pragma unroll
-  for (int sl = 0; sl < W; ++sl)
+    for (int e = lane; e < 256; e += 64) {
+      const int b = e >> 4, v = e & 15;
+      u64 acc = 0ull;
 #pragma unroll
-    for (int w = 0; w < W; ++w) Bm[sl][w] = Q[sl][w];
-  for (int l = 0; l < n; ++l) {
+      for (int i = 0; i < 4; ++i)
+        if ((v >> i) & 1) acc ^= s_wt[wv][4 * b + i];
+      s_tab[wv][e] = acc;
+    }
+    wave_lds_sync();
+    u64 p = 0ull;
 #pragma unroll
-    for (int sw = 0; sw < W; ++sw) {
-      const u64 col = __ballot(getbit<W>(Q[sw], l));
+    for (int b = 0; b < 16; ++b) p ^= s_tab[wv][16 * b + (int)((U[0][0] >> (4 * b)) & 15ull)];
+    // strict upper triangle (m > l); rows at or above n have u = 0
+    Q[0][0] = (lane < 63) ? (p & (~0ull << (lane + 1))) : 0ull;
+    // B = Q ^ Q^T
+    Bm[0][0] = Q[0][0] ^ transpose64(Q[0][0], lane);
+  } else {
+    for (int m = 0; m < n; ++m) {
+      u64 wm[W];
+      bcast<W>(Wm, m, wm);
 #pragma unroll
-      for (int sl = 0; sl < W; ++sl)
-        if (owns<W>(f, l, sl)) Bm[sl][sw] |= col;
+      for (int sl = 0; sl < W; ++sl) {
+        const int row = 64 * sl + lane;
+        const int bit = (row < m) ? parand<W>(U[sl], wm) : 0;
+        flipbit<W>(Q[sl], m, bit);
+      }
+    }
+    // B = Q ^ Q^T
+#pragma unroll
+    for (int sl = 0; sl < W; ++sl)
+#pragma unroll
+      for (int w = 0; w < W; ++w) Bm[sl][w] = Q[sl][w];
+    for (int l = 0; l < n; ++l) {
+#pragma unroll
+      for (int sw = 0; sw < W; ++sw) {
+        const u64 col = __ballot(getbit<W>(Q[sw], l));
+#pragma unroll
+        for (int sl = 0; sl < W; ++sl)
+          if (owns<W>(f, l, sl)) Bm[sl][sw] |= col;
+      }
     }
   }
   // constraints
@@ -262,13 +314,17 @@ __global__ __launch_bounds__(256) void k_qform(FormsDev in, QFormDev q) {
   for (int sl = 0; sl < W; ++sl)
 #pragma unroll
     for (int w = 0; w < W; ++w) CT[sl][w] = 0ull;
-  for (int j = 0; j < n; ++j) {
+  if constexpr (W == 1) {
+    CT[0][0] = transpose64(C[0][0], lane);
+  } else {
+    for (int j = 0; j < n; ++j) {
 #pragma unroll
-    for (int sw = 0; sw < W; ++sw) {
-      const u64 col = __ballot(getbit<W>(C[sw], j));
+      for (int sw = 0; sw < W; ++sw) {
+        const u64 col = __ballot(getbit<W>(C[sw], j));
 #pragma unroll
-      for (int sl = 0; sl < W; ++sl)
-        if (owns<W>(f, j, sl)) CT[sl][sw] = col;
+        for (int sl = 0; sl < W; ++sl)
+          if (owns<W>(f, j, sl)) CT[sl][sw] = col;
+      }
     }
   }
   // store
