@@ -580,16 +580,19 @@ static cplx run_pairs(const QForms& X, const QForms& Y, int64_t nfixed, int64_t 
   }
   int nblocks = (int)std::min<int64_t>((units + 3) / 4, 4096);
   if (nblocks < 1) nblocks = 1;
-  DevBuf partial((size_t)nblocks * 4 * 16), out(16);
+  DevBuf partial((size_t)nblocks * 4 * 16);
+  // the reduction writes the sum straight into page-locked host memory (one
+  // buffer per calling thread): no copy after it
+  static thread_local PinnedBuf t_res;
+  if (!t_res.p) t_res.alloc(64);
   ProfScope ps("k_pairs");
   hip_check(launch_pairs(X.desc(), Y.desc(), d_off, nfixed, nother, triangle ? 1 : 0, J, mode, cx, cy,
                          ctx()->tabs, partial.as<double2>(), nblocks, d_exact, ld, stream()),
             "k_pairs");
   ps.end();
-  hip_check(launch_reduce(partial.as<double2>(), (int64_t)nblocks * 4, out.as<double2>(), stream()), "k_reduce");
-  double h[2];
-  d2h(h, out.p, 16);
+  hip_check(launch_reduce(partial.as<double2>(), (int64_t)nblocks * 4, t_res.as<double2>(), stream()), "k_reduce");
   sync();
+  const double* h = t_res.as<double>();
   return cplx(h[0], h[1]);
 }
 
@@ -1014,14 +1017,16 @@ struct VarPlan {
   int64_t nseg = 0;
   DevBuf jobs, wave_job, row_off, partial, out, seg_job, seg_part;
   // jobs_dev: the job table to read (default: this plan's own)
-  void enqueue(hipStream_t st, const PairJobDev* jobs_dev = nullptr) const {
+  // out_override: where the reduction writes instead of out (page-locked
+  // host memory for results the host reads right after)
+  void enqueue(hipStream_t st, const PairJobDev* jobs_dev = nullptr, double* out_override = nullptr) const {
     const PairJobDev* jd = jobs_dev ? jobs_dev : jobs.as<PairJobDev>();
     for (const Launch& L : launches)
       hip_check(launch_pairs_var(L.W, VW, units, jd, wave_job.as<int32_t>(), L.wave_base, L.nwaves, J, ctx()->tabs,
                                  partial.as<double>(), st),
                 "k_pairs_var");
     hip_check(launch_reduce_var(jd, njobs, VW, partial.as<double>(), seg_job.as<int32_t>(), nseg,
-                                seg_part.as<double>(), out.as<double>(), st),
+                                seg_part.as<double>(), out_override ? out_override : out.as<double>(), st),
               "k_reduce_var");
   }
 };
@@ -1360,10 +1365,9 @@ struct ProjPlan {
     }
     {
       ProfScope ps("k_pairs_var");
-      plan.enqueue(stream(), (const PairJobDev*)dbase);
+      plan.enqueue(stream(), (const PairJobDev*)dbase, res_h.as<double>());
       ps.end();
     }
-    d2h(res_h.p, plan.out.p, (size_t)nv * 16);
     sync();
     return std::vector<double>(res_h.as<double>(), res_h.as<double>() + 2 * nv);
   }
