@@ -68,6 +68,7 @@ VAR_CASES = [  # (seed, n, cliffords, t, number of strings, strings checked agai
     (44, 64, 150, 8, 65, 24),
     (45, 70, 180, 9, 33, 12),
     (46, 130, 300, 5, 20, 6),
+    (47, 240, 300, 12, 20, 6),
 ]
 
 

@@ -62,7 +62,11 @@ CASES = [  # (seed, n, cliffords, t, gate set)
     (7, 64, 500, 8, tuple(range(11))),
     (8, 70, 500, 9, tuple(range(11))),
     (9, 130, 700, 5, WL.CLIFFORD_SET),
+    (10, 200, 600, 12, WL.CLIFFORD_SET),   # four-word rows
+    (11, 240, 700, 16, tuple(range(11))),  # 240 + 16 ancillas: the 256-qubit maximum while compiling
 ]
+# pair sums at one to four row words (the 70- and 130-qubit cases are covered by the amplitude and table tests)
+PAIR_CASES = CASES[:7] + CASES[9:]
 
 
 @pytest.mark.parametrize("seed,n,nc,nt,gs", CASES)
@@ -74,7 +78,7 @@ def test_compile_tables_bit_exact(nx, seed, n, nc, nt, gs):
     assert_tables_equal(g, o)
 
 
-@pytest.mark.parametrize("seed,n,nc,nt,gs", CASES[:7])
+@pytest.mark.parametrize("seed,n,nc,nt,gs", PAIR_CASES)
 def test_pair_scalars_exact(nx, seed, n, nc, nt, gs):
     rng = np.random.default_rng(100 + seed)
     gates = WL.random_circuit(rng, n, nc, nt, gate_set=gs)
@@ -95,7 +99,7 @@ def test_pair_scalars_exact(nx, seed, n, nc, nt, gs):
             assert v == (k | (r << 8)), (i, j, v, k, r)
 
 
-@pytest.mark.parametrize("seed,n,nc,nt,gs", CASES[:7])
+@pytest.mark.parametrize("seed,n,nc,nt,gs", PAIR_CASES)
 def test_norm_exp_value_inner_product(nx, seed, n, nc, nt, gs):
     rng = np.random.default_rng(200 + seed)
     gates = WL.random_circuit(rng, n, nc, nt, gate_set=gs)
@@ -305,6 +309,8 @@ def test_sample_same_stream_as_oracle(nx):
     (612, 20, 13, 300),    # subset of qubits, unsorted
     (613, 70, 10, 64),     # two-word rows
     (614, 66, 66, 40),     # every qubit sampled: certified levels, then an exact-amplitude tail (two words)
+    (615, 200, 20, 40),    # four-word rows
+    (616, 196, 128, 6),    # the 128-qubit sampling maximum, four-word rows
 ])
 def test_sample_clifford_windows_same_stream(nx, seed, n, nq, shots):
     """Clifford states (probabilities 0, 1/2^k, 1): seeded outcomes equal
@@ -325,7 +331,7 @@ def test_projection_norms_vs_oracle(nx):
     norm_squared (projection.rs:10-31, norm.rs:8-30) per outcome."""
     rng = np.random.default_rng(701)
     for trial, (n, nfix, nlook) in enumerate([(3, 0, 1), (4, 1, 0), (5, 0, 2), (6, 2, 3), (9, 3, 6), (12, 4, 7),
-                                              (70, 5, 4)]):
+                                              (70, 5, 4), (200, 3, 5)]):
         gates = WL.random_circuit(rng, n, 12 * n, 5, gate_set=tuple(range(11)))
         g, o = compile_both(n, gates)
         qs = [int(q) for q in rng.choice(n, nfix + nlook, replace=False)]
@@ -404,6 +410,25 @@ def test_errors_match_oracle(nx):
     big = QuantumState.from_circuit(QuantumCircuit(3))
     with pytest.raises(ValueError, match="The qubit counts of the two states must match"):
         st.inner_product(big)
+    wide = QuantumState.from_circuit(QuantumCircuit(130))
+    with pytest.raises(ValueError, match="Sampling more than 128 qubits is not supported."):
+        wide.sample(list(range(129)), 10)
+    with pytest.raises(O.OracleError, match="Sampling more than 128 qubits"):
+        O.State.compile(130, []).sample(list(range(129)), 10, 0)
+
+
+def test_qubit_limit(nx):
+    """The device tables hold at most 256 qubits (four row words) while
+    compiling, ancillas of the T gadgets included; a circuit above that is
+    rejected on the host before any launch, and one exactly at it compiles."""
+    from necstar import QuantumState
+    with pytest.raises(ValueError, match="above 256 qubits"):
+        QuantumState.from_circuit(WL.to_circuit(250, [(0, 0)] + [(11, q) for q in range(7)]))
+    with pytest.raises(ValueError, match="above 256 qubits"):
+        QuantumState.from_circuit(WL.to_circuit(257, [(0, 0)]))
+    g, o = compile_both(250, [(0, q) for q in range(6)] + [(11, q) for q in range(6)])
+    assert_tables_equal(g, o)
+    assert abs(g.norm() - o.norm()) <= TOL
 
 
 def test_config0_workload(nx):
