@@ -38,24 +38,34 @@ __device__ __forceinline__ void z8_add(u64& b0, u64& b1, u64& b2, u64 x0, u64 x1
 // is added to A when t is even and as +-w 2^{-(t+1)/2} to B when t is odd,
 // so that the sum is A + sqrt(2) B.  Each addend is a power of two, so A and
 // B are exact while their exponent span fits a double's 53 bits.
+// Written without branches (lanes differ in k and s): the addend not taken
+// is +0.0, and x + 0.0 == x for every x the sums reach (they start at +0.0
+// and no addend is a zero, so neither becomes -0.0).
 __device__ __forceinline__ void add_exact(int k, int s, double w, double& A, double& B) {
   k &= 7;
-  if (k == 2 || k == 6) return;
-  const double sg = (k == 0 || k == 1 || k == 7) ? w : -w;
+  const double sg = (k == 2 || k == 6) ? 0.0 : ((k == 0 || k == 1 || k == 7) ? w : -w);
   const int t = (k & 1) ? s + 1 : s;
-  if (t & 1) B = B + ldexp(sg, -((t + 1) >> 1));
-  else A = A + ldexp(sg, -(t >> 1));
+  const double val = ldexp(sg, -((t + 1) >> 1));  // (t + 1) >> 1 == t / 2 for even t
+  A = A + ((t & 1) ? 0.0 : val);
+  B = B + ((t & 1) ? val : 0.0);
 }
 
-// OR of x over the 64 lanes, in every lane
+// OR of x over the 64 lanes (all active), as a wave-uniform value.  DPP
+// steps, no LDS: quad permutations [1,0,3,2] and [2,3,0,1], the half-row and
+// row mirrors leave every lane of a 16-lane row with the row's OR; lane 15's
+// value broadcast into rows 1 and 3, then lane 31's into rows 2 and 3 (the
+// rows not written receive 0) leave the total in lane 63.
+__device__ __forceinline__ u32 wave_or32(u32 x) {
+  x |= (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+  x |= (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+  x |= (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, false);  // row_half_mirror
+  x |= (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, false);  // row_mirror
+  x |= (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);  // row_bcast:15 -> rows 1, 3
+  x |= (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);  // row_bcast:31 -> rows 2, 3
+  return (u32)__builtin_amdgcn_readlane((int)x, 63);
+}
 __device__ __forceinline__ u64 wave_or64(u64 x) {
-  unsigned lo = (unsigned)x, hi = (unsigned)(x >> 32);
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    lo |= (unsigned)__shfl_xor((int)lo, off);
-    hi |= (unsigned)__shfl_xor((int)hi, off);
-  }
-  return uni64(((u64)hi << 32) | lo);
+  return ((u64)wave_or32((u32)(x >> 32)) << 32) | wave_or32((u32)x);
 }
 
 template <int W, int VW>
@@ -1041,6 +1051,9 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
         }
       }
       double gr = 0.0, gi = 0.0;
+      // the FP64 value of zeta^K 2^{-mag/2} is needed by complex
+      // coefficients and by the weighted Gram row, not by exact sums
+      const bool need_f = !exact || jd.gw;
 #pragma unroll
       for (int v = 0; v < VW; ++v) {
         const int zk = (int)((V.z[v] >> lane) & 1ull);
@@ -1048,15 +1061,16 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
           const int D = (int)((V.d0[v] >> lane) & 1ull) | ((int)((V.d1[v] >> lane) & 1ull) << 1) |
                         ((int)((V.d2[v] >> lane) & 1ull) << 2);
           const int K = (P.c0 + D) & 7;
-          const double ir = tabs[TAB_COS + K] * nrm, ii = tabs[TAB_SIN + K] * nrm;
-          if (exact) {
-            if (!czero) add_exact(kc + K, rc + P.mag, w2, acc[v], accB[v]);
-          } else {
-            const double tr = cr_ * ir - ci_ * ii;
-            acc[v] = diag ? acc[v] + tr : acc[v] + (tr + tr);
+          if (exact && !czero) add_exact(kc + K, rc + P.mag, w2, acc[v], accB[v]);
+          if (need_f) {
+            const double ir = tabs[TAB_COS + K] * nrm, ii = tabs[TAB_SIN + K] * nrm;
+            if (!exact) {
+              const double tr = cr_ * ir - ci_ * ii;
+              acc[v] = diag ? acc[v] + tr : acc[v] + (tr + tr);
+            }
+            gr = gr + wk[v] * ir;
+            gi = gi + wk[v] * ii;
           }
-          gr = gr + wk[v] * ir;
-          gi = gi + wk[v] * ii;
         }
       }
       if (jd.gw) {
