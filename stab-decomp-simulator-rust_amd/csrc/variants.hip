@@ -73,7 +73,8 @@ struct VarRegs {
   u64 hi[W][VW];               // row r: variants whose a_r carries an extra 2
   u64 ra[W][VW];               // bra constraint row c: variants whose rhs is flipped (unit mode)
   u64 rb[W][VW];               // ket constraint row c: variants whose rhs is flipped
-  u64 d0[VW], d1[VW], d2[VW];  // uniform: Z8 phase offset of each variant
+  u64 d0[VW], d1[VW], d2[VW];  // uniform: Z8 phase offset of each variant, plus dv
+  int dv;                      // uniform: Z8 offset shared by all variants (kept out of the planes)
   u64 z[VW];                   // uniform: variants whose sum is already zero
   int bz;                      // uniform: the base (identity) sum is zero or not wanted
 };
@@ -118,6 +119,7 @@ __device__ __forceinline__ void uniform_state(PairRegs<W>& P, VarRegs<W, VW>& V)
   P.c0 = uni(P.c0);
   P.mag = uni(P.mag);
   V.bz = uni(V.bz);
+  V.dv = uni(V.dv);
 #pragma unroll
   for (int v = 0; v < VW; ++v) {
     V.z[v] = uni64(V.z[v]);
@@ -134,7 +136,7 @@ __device__ __forceinline__ void uniform_state(PairRegs<W>& P, VarRegs<W, VW>& V)
 // B_pl), which differs from the base change by 2 (r_l (h_p,k ^ (a_p odd &
 // e_k)) ^ (B_pl & e_k)); the phase gains 2 (a_p,k c_k mod 4) per variant and
 // 2 (a_p c mod 4) for the base.  Bra constraints carry no variant flips
-// (E = 0 while they are taken).  PH = false skips the phase planes.
+// (E = 0 while they are taken).  PH = false skips the phase planes and dv.
 template <int W, int VW, int CONS, bool UNITS, bool PH = true>
 __device__ __forceinline__ void subst_v(PairRegs<W>& P, VarRegs<W, VW>& V, int p, const u64 (&r)[W], int c,
                                         const u64 (&E)[VW]) {
@@ -149,16 +151,15 @@ __device__ __forceinline__ void subst_v(PairRegs<W>& P, VarRegs<W, VW>& V, int p
   for (int v = 0; v < VW; ++v) {
     if (PH) {
       // variant phase minus base phase, 2 (a_p,k c_k - a_p c) mod 8, as the
-      // sum of the full variant term and 8 - 2 a_p c (both bit-sliced)
+      // full variant term (bit-sliced) plus 8 - 2 a_p c (in dv, below)
       const u64 cm = c ? ~E[v] : E[v];                  // variants with c_k = 1
       const u64 lo = (ap & 1) ? cm : 0ull;              // (a_p,k c_k) mod 2
       const u64 hi = cm & ((ap & 2) ? ~H[v] : H[v]);    // high bit of a_p,k c_k
       z8_add(V.d0[v], V.d1[v], V.d2[v], 0ull, lo, hi);
-      const int nb = (8 - 2 * ap * c) & 7;
-      z8_add(V.d0[v], V.d1[v], V.d2[v], 0ull, (nb & 2) ? ~0ull : 0ull, (nb & 4) ? ~0ull : 0ull);
     }
     L[v] = H[v] ^ ((ap & 1) ? E[v] : 0ull);
   }
+  if (PH) V.dv = (V.dv + 8 - 2 * ap * c) & 7;
   const int lin = c ? (4 - ap) & 3 : ap;
   const int pw = wordof<W>(p), pb = p & 63;
   const u32 cm = c ? ~0u : 0u;
@@ -465,6 +466,7 @@ __global__ __launch_bounds__(256) void k_qproject_var(QFormDev in, QFormDev out,
 #pragma unroll
   for (int v = 0; v < VW; ++v) V.d0[v] = V.d1[v] = V.d2[v] = V.z[v] = 0ull;
   V.bz = 0;
+  V.dv = 0;
   u64 r0[W];
 #pragma unroll
   for (int w = 0; w < W; ++w) r0[w] = 0ull;
@@ -506,6 +508,9 @@ __global__ __launch_bounds__(256) void k_qproject_var(QFormDev in, QFormDev out,
     out.k[t] = in.k[t];
 #pragma unroll
     for (int v = 0; v < VW; ++v) {
+      // the stored planes hold each variant's full offset: dv folded in
+      z8_add(V.d0[v], V.d1[v], V.d2[v], (V.dv & 1) ? ~0ull : 0ull, (V.dv & 2) ? ~0ull : 0ull,
+             (V.dv & 4) ? ~0ull : 0ull);
       ((u64*)vout.ph)[((size_t)t * 3 + 0) * VW + v] = V.d0[v];
       ((u64*)vout.ph)[((size_t)t * 3 + 1) * VW + v] = V.d1[v];
       ((u64*)vout.ph)[((size_t)t * 3 + 2) * VW + v] = V.d2[v];
@@ -762,6 +767,7 @@ __global__ __launch_bounds__(256) void k_pairs_sym(QFormDev X, const int64_t* __
 #pragma unroll
       for (int v = 0; v < VW; ++v) V.d0[v] = V.d1[v] = V.d2[v] = V.z[v] = 0ull;
       V.bz = 0;
+      V.dv = 0;
       P.c0 = 0;
       P.mag = 0;
       u64 eqm[W], em[W];
@@ -1018,6 +1024,7 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
         V.z[v] = invalid[v];
       }
       V.bz = !want_base;
+      V.dv = 0;
       P.c0 = (pf - po) & 7;
       P.mag = kf + ko;
       pair_eval_v<W, VW, UNITS>(P, V, nco, ncf, tv ? 0 : jd.nunit, jd.uq, jd.ub);
@@ -1060,7 +1067,7 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
         if (!zk) {
           const int D = (int)((V.d0[v] >> lane) & 1ull) | ((int)((V.d1[v] >> lane) & 1ull) << 1) |
                         ((int)((V.d2[v] >> lane) & 1ull) << 2);
-          const int K = (P.c0 + D) & 7;
+          const int K = (P.c0 + V.dv + D) & 7;
           if (exact && !czero) add_exact(kc + K, rc + P.mag, w2, acc[v], accB[v]);
           if (need_f) {
             const double ir = tabs[TAB_COS + K] * nrm, ii = tabs[TAB_SIN + K] * nrm;
