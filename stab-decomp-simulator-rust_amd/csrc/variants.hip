@@ -1016,15 +1016,13 @@ __global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict_
         V.d0[v] = vp0[v];
         V.d1[v] = vp1[v];
         V.d2[v] = vp2[v];
-        if (tv) {  // minus the bra's phase planes: + ~d + 1 (mod 8)
+        if (tv)  // minus the bra's phase planes: + ~d here, the + 1 (mod 8) in dv
           z8_add(V.d0[v], V.d1[v], V.d2[v], ~uni64(jd.V.ph[((size_t)o * 3 + 0) * VW + v]),
                  ~uni64(jd.V.ph[((size_t)o * 3 + 1) * VW + v]), ~uni64(jd.V.ph[((size_t)o * 3 + 2) * VW + v]));
-          z8_add(V.d0[v], V.d1[v], V.d2[v], ~0ull, 0ull, 0ull);
-        }
         V.z[v] = invalid[v];
       }
       V.bz = !want_base;
-      V.dv = 0;
+      V.dv = tv ? 1 : 0;
       P.c0 = (pf - po) & 7;
       P.mag = kf + ko;
       pair_eval_v<W, VW, UNITS>(P, V, nco, ncf, tv ? 0 : jd.nunit, jd.uq, jd.ub);
