@@ -908,7 +908,7 @@ __global__ __launch_bounds__(256) void k_pauli_var(FormsDev in, const int32_t* _
 // which is exp_value.rs:38-58 for Y_f^(k) = P_k phi_f (norm.rs:16-27 for
 // the base).  Per wave, partial holds 64*VW variant sums then the base.
 template <int W, int VW, bool UNITS>
-__global__ __launch_bounds__(256) void k_pairs_var(const PairJobDev* __restrict__ jobs,
+__global__ __launch_bounds__(256, (W == 1 && VW == 1) ? 8 : 1) void k_pairs_var(const PairJobDev* __restrict__ jobs,
                                                    const int32_t* __restrict__ wave_job, int64_t wave_base,
                                                    int64_t nwaves_launch, int J, const double* __restrict__ tabs,
                                                    double* __restrict__ partial) {
