@@ -1717,6 +1717,7 @@ int nx_state_inner_product(const nx_state* a, const nx_state* b, double* re, dou
 
 static void amplitudes_impl(const nx_state* s, int64_t nbits, const uint64_t* d_bits, double* d_out,
                             uint8_t* d_per_term, int enumerate) {
+  if (nbits <= 0) return;  // no strings: no launch (a grid of zero blocks is not a valid launch)
   const QForms& q = qform_of(s);
   CoefDev c = coef_of(s);
   // the term loop is split into chunks when the strings alone give too
@@ -1739,6 +1740,7 @@ int nx_state_amplitudes(const nx_state* s, int64_t nbits, const uint64_t* bits, 
   NX_TRY({
     const int W = words_for(s->n);
     if (words < W) fail(NX_E_VALUE, "bitstrings need %d words for %d qubits", W, s->n);
+    if (nbits < 0) fail(NX_E_VALUE, "negative bitstring count %lld", (long long)nbits);
     std::vector<uint64_t> packed((size_t)nbits * W);
     for (int64_t b = 0; b < nbits; ++b)
       for (int w = 0; w < W; ++w) {
@@ -1771,6 +1773,8 @@ int nx_state_amplitudes_device(const nx_state* s, int64_t nbits, const uint64_t*
                                double* d_out) {
   NX_TRY({
     if (words != words_for(s->n)) fail(NX_E_VALUE, "device bitstrings must use %d words", words_for(s->n));
+    if (nbits < 0) fail(NX_E_VALUE, "negative bitstring count %lld", (long long)nbits);
+    if (nbits == 0) return NX_OK;
     if (s->chi() == 0) {
       hip_check(hipMemsetAsync(d_out, 0, (size_t)nbits * 16, stream()), "hipMemsetAsync");
     } else {
@@ -1999,6 +2003,8 @@ int nx_state_pair_table(const nx_state* s, uint32_t* out) {
 
 int nx_state_exp_values(const nx_state* s, int32_t npauli, const int32_t* ops, double* out) {
   NX_TRY({
+    if (npauli < 0) fail(NX_E_VALUE, "negative Pauli count %d", npauli);
+    if (npauli == 0) return NX_OK;
     PauliSet ps;
     add_dense_paulis(s, npauli, ops, ps);
     const std::vector<double> r = exp_values_var(s, ps);

@@ -159,3 +159,26 @@ def test_state_project_unnormalized(nc):
     st.project_unnormalized(0, False)
     with pytest.raises(ValueError):
         make_toffoli_state(nc).project_unnormalized(3, True)
+
+
+def test_empty_inputs(nc):
+    """Zero strings, zero Pauli strings, zero shots and no projected qubits:
+    empty results (or the norm), no launch, as the reference's loops give
+    for empty input (sampling.rs:91-94 returns at zero shots)."""
+    import numpy as np
+    from necstar import Batch, QuantumState
+    qc = nc.QuantumCircuit(3)
+    qc.apply_h(0)
+    qc.apply_t(0)
+    qc.apply_cx(0, 1)
+    st = QuantumState.from_circuit(qc)
+    assert st.amplitudes(np.zeros((0, 1), np.uint64)).shape == (0,)
+    vals, pt, rr = st.amplitudes(np.zeros((0, 1), np.uint64), per_term=True)
+    assert vals.shape == (0,) and pt.shape == (st.stabilizer_rank, 0) and rr.shape == (st.stabilizer_rank,)
+    assert st.exp_values([]).shape == (0,)
+    assert st.sample([0, 1], 0, seed=1) == {}
+    assert abs(st.projection_norms([], [])[0] - st.norm() ** 2) <= 1e-12
+    loss, grad = st.grad_exp_values([], [])
+    assert loss == 0.0 and not np.any(grad)
+    norms, values = Batch([st], [[]]).run()
+    assert abs(norms[0] - st.norm()) <= 1e-12 and len(values[0]) == 0
