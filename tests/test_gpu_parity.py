@@ -510,3 +510,19 @@ def test_config4_full_size(nx):
         ip = basis.inner_product(st)
         scale = max(abs(ip), abs(amp[b]), 1e-300)
         assert abs(ip - amp[b]) <= 1e-12 * scale, (b, ip, amp[b])
+
+
+def test_amplitudes_ignore_bits_beyond_n(nx):
+    """Ragged input: strings given with more words than the state needs, and
+    set bits above qubit n - 1, read as the string of their first n bits
+    (the reference's bitstrings have exactly n entries)."""
+    rng = np.random.default_rng(1201)
+    n = 70
+    gates = WL.random_circuit(rng, n, 300, 6, gate_set=tuple(range(11)))
+    g, o = compile_both(n, gates)
+    bits = WL.random_bitstrings(rng, n, 32)
+    wide = np.concatenate([bits, rng.integers(0, 2**63, (32, 1), dtype=np.uint64)], axis=1)
+    wide[:, 1] |= rng.integers(0, 2**57, 32, dtype=np.uint64) << np.uint64(6)
+    assert np.array_equal(g.amplitudes(wide), g.amplitudes(bits))
+    x = np.array([(int(bits[0, q // 64]) >> (q % 64)) & 1 for q in range(n)], np.uint8)
+    assert abs(g.amplitudes(wide)[0] - o.amplitude(x)) <= TOL
