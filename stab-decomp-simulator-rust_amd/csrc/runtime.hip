@@ -1167,33 +1167,40 @@ static std::pair<int, int64_t> tuned_shape(const std::vector<VarJob>& jobs, int 
 }
 
 // Each candidate is timed on a shortened plan (about 2^22 pairs: the same
-// jobs with a fraction of their ket rows).
+// jobs with a fraction of their ket rows); the two fastest are timed again
+// on a plan four times as long, since a short plan has few units per wave,
+// which favours short chunks.
 static std::pair<int, int64_t> autotune(const std::vector<VarJob>& jobs, int VW) {
   static const int Js[] = {8, 16, 32};
   static const int64_t Ws[] = {8192, 16384, 32768};
   double pairs = 0.0;
   for (const VarJob& j : jobs) pairs += 0.5 * (double)j.X->chi * (double)(j.X->chi + 1);
-  const double frac = std::min(1.0, 4194304.0 / std::max(pairs, 1.0));
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "hipEventCreate");
   hip_check(hipEventCreate(&e1), "hipEventCreate");
-  float best = 1e30f;
-  std::pair<int, int64_t> arg{16, DEFAULT_WAVES};
+  auto time_shape = [&](int J, int64_t nw, double frac) {
+    VarPlan pl = make_plan(jobs, VW, J, nw, frac);
+    pl.enqueue(stream());  // warm-up
+    hip_check(hipEventRecord(e0, stream()), "hipEventRecord");
+    pl.enqueue(stream());
+    hip_check(hipEventRecord(e1, stream()), "hipEventRecord");
+    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    return ms;
+  };
+  const double frac = std::min(1.0, 4194304.0 / std::max(pairs, 1.0));
+  std::vector<std::pair<float, std::pair<int, int64_t>>> t;
   for (int J : Js)
-    for (int64_t nw : Ws) {
-      VarPlan pl = make_plan(jobs, VW, J, nw, frac);
-      pl.enqueue(stream());  // warm-up
-      hip_check(hipEventRecord(e0, stream()), "hipEventRecord");
-      pl.enqueue(stream());
-      hip_check(hipEventRecord(e1, stream()), "hipEventRecord");
-      hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
-      float ms = 0.f;
-      hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-      if (ms < best) {
-        best = ms;
-        arg = {J, nw};
-      }
-    }
+    for (int64_t nw : Ws) t.push_back({time_shape(J, nw, frac), {J, nw}});
+  std::stable_sort(t.begin(), t.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+  std::pair<int, int64_t> arg = t[0].second;
+  if (frac < 1.0) {
+    const double frac4 = std::min(1.0, 4.0 * frac);
+    const float a = time_shape(t[0].second.first, t[0].second.second, frac4);
+    const float b = time_shape(t[1].second.first, t[1].second.second, frac4);
+    if (b < a) arg = t[1].second;
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   std::lock_guard<std::mutex> lk(g_tune_mu);
