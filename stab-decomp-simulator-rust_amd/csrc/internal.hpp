@@ -177,6 +177,9 @@ hipError_t launch_pauli_var(const FormsDev& in, const int32_t* voff, const int32
 hipError_t launch_pairs_var(int W, int VW, int units, const PairJobDev* jobs, const int32_t* wave_job,
                             int64_t wave_base, int64_t nwaves, int J, const double* tabs, double* partial,
                             hipStream_t st);
+// waves of k_pairs_var<W, VW, units> resident on the current device at once
+// (occupancy x CUs x 4 waves per 256-thread block); 0 for an unknown shape
+int64_t pairs_var_resident_waves(int W, int VW, int units);
 // out[jobs[j].out_base + k] = sum over the job's waves of partial (k < nvar),
 // and the base sum to out[jobs[j].base_out] when that is >= 0: first per
 // segment of RED_SEG waves (seg_job[s] = job of segment s) into seg_part,

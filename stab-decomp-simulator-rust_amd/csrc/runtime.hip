@@ -1297,8 +1297,23 @@ struct ProjPlan {
     j.cy = j.cx;
     j.nvar = 1 << max_look;
     j.split_ab = 1;
-    const auto shape = tuned_shape({j}, VW);
-    plan = make_plan({j}, VW, shape.first, shape.second);
+    // chunks of 8 pairs and three rounds of resident waves: one window pass
+    // of a single state is short, and fine work units keep its waves even
+    // (1,024 shots of a 64-qubit, 729-term state on all qubits: 1.26 s with
+    // 16-pair chunks in 16k waves, 1.19 s so)
+    static std::mutex res_mu;
+    static std::map<std::tuple<int, int, int>, int64_t> resident;  // (device, W, VW) -> waves
+    int64_t rw = 0;
+    {
+      int dev = 0;
+      hip_check(hipGetDevice(&dev), "hipGetDevice");
+      const auto key = std::make_tuple(dev, j.X->W, VW);
+      std::lock_guard<std::mutex> lk(res_mu);
+      auto it = resident.find(key);
+      if (it == resident.end()) it = resident.emplace(key, pairs_var_resident_waves(j.X->W, VW, 1)).first;
+      rw = it->second;
+    }
+    plan = make_plan({j}, VW, 8, std::max<int64_t>(3 * rw, 1024));
     hj.resize(1);
     d2h(hj.data(), plan.jobs.p, sizeof(PairJobDev));
     const size_t mu = (size_t)std::max(max_units, 1);

@@ -17,6 +17,8 @@
 //  k_gramc      y = G c for the weighted FP64 pair matrix (both triangles)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device_form.hpp"
 #include "internal.hpp"
 #include "pair_core.hpp"
@@ -1247,6 +1249,21 @@ hipError_t launch_pauli_var(const FormsDev& in, const int32_t* voff, const int32
                  k_pauli_var<WW, VV><<<dim3(nblk(in.chi, 4)), dim3(256), 0, st>>>(in, voff, vops, nvar, vhi, vrhs,
                                                                                    vph));
   return hipGetLastError();
+}
+
+int64_t pairs_var_resident_waves(int W, int VW, int units) {
+  if (W < 1 || W > 4 || VW < 1 || VW > 2) return 0;
+  const void* f = nullptr;
+#define NX_PV_FUNC(UU) NX_DISPATCH_WV(W, VW, f = (const void*)k_pairs_var<WW, VV, UU>)
+  if (units) NX_PV_FUNC(true)
+  else NX_PV_FUNC(false)
+#undef NX_PV_FUNC
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    cus = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, 256, 0) != hipSuccess) per = 1;
+  return (int64_t)std::max(per, 1) * std::max(cus, 1) * 4;
 }
 
 hipError_t launch_pairs_var(int W, int VW, int units, const PairJobDev* jobs, const int32_t* wave_job,
