@@ -103,7 +103,9 @@ int nx_state_measure(nx_state *s, int32_t nq, const int32_t *qargs, const uint8_
 
 /* sample (state/mod.rs:247-284, sampling.rs:13-189): distinct outcomes in
  * DFS order, outcome k as bits little-endian in 2 u64 words (bit i <->
- * qargs[i]) and its count.  Capacity max_out; *nout receives the count. */
+ * qargs[i]) and its count.  Capacity max_out; *nout receives the count,
+ * also when the call fails because max_out is too small (call again with
+ * that capacity and the same seed). */
 int nx_state_sample(const nx_state *s, int32_t nq, const int32_t *qargs, uint64_t shots, const uint8_t *seed32,
                     int64_t max_out, uint64_t *out_bits, uint64_t *out_counts, int64_t *nout);
 
@@ -156,6 +158,10 @@ int nx_state_uniform_prefix_given(const nx_state *s, int32_t nfixed, const int32
  * 0xFFFFFFFF zero, else k | r << 8).  Parity aid; the reference computes
  * the same values in StabilizerCHForm::inner_product (inner_product.rs:14-48). */
 int nx_state_pair_table(const nx_state *s, uint32_t *out);
+/* The same for the terms of two states: out[i * chi_b + j] = <a_i|b_j>, the
+ * pairs of QuantumState::inner_product (stabilizer_decomposed_state/
+ * inner_product.rs:9-19).  Parity aid. */
+int nx_state_cross_pair_table(const nx_state *a, const nx_state *b, uint32_t *out);
 
 /* exp_value for npauli dense Pauli strings ops[npauli][n] (little-endian,
  * 0 I, 1 X, 2 Y, 3 Z): out[k] as exp_value.rs:7-61 computes it. */
@@ -200,8 +206,9 @@ void nx_batch_free(nx_batch *b);
 int nx_profile_enable(int on);
 int nx_profile_get(const char *kernel, double *total_ms, int64_t *count);
 
-/* Drops the derived per-term device data (quadratic forms, coefficient
- * copies) so the next call recomputes it.  No reference counterpart. */
+/* Drops the derived per-term device data (the quadratic forms) so the
+ * next call recomputes it; the device copies of the coefficients stay
+ * resident until the coefficients change.  No reference counterpart. */
 int nx_state_drop_cache(nx_state *s);
 
 #ifdef __cplusplus

@@ -1884,6 +1884,8 @@ int nx_state_sample(const nx_state* s, int32_t nq, const int32_t* qargs, uint64_
       cx.counts.push_back(shots);
     }
     const int64_t k = (int64_t)cx.counts.size();
+    // the caller learns the capacity it needs even when this call fails
+    *nout = k;
     if (k > max_out) fail(NX_E_VALUE, "sample output buffer too small (%lld outcomes)", (long long)k);
     std::memcpy(out_bits, cx.bits.data(), k * 16);
     std::memcpy(out_counts, cx.counts.data(), k * 8);
@@ -2022,6 +2024,22 @@ int nx_state_pair_table(const nx_state* s, uint32_t* out) {
     DevBuf ex((size_t)chi * chi * 4);
     run_pairs(q, q, chi, chi, false, 0, c, c, ex.as<uint32_t>(), chi);
     d2h(out, ex.p, (size_t)chi * chi * 4);
+    sync();
+  })
+}
+
+int nx_state_cross_pair_table(const nx_state* a, const nx_state* b, uint32_t* out) {
+  NX_TRY({
+    if (a->n != b->n)
+      fail(NX_E_VALUE, "The qubit counts of the two states must match for calculating inner product, got %d and %d.",
+           a->n, b->n);
+    if (a->chi() == 0 || b->chi() == 0) return NX_OK;
+    const QForms& qa = qform_of(a);
+    const QForms& qb = qform_of(b);
+    CoefDev ca = coef_of(a), cb = coef_of(b);
+    DevBuf ex((size_t)a->chi() * b->chi() * 4);
+    run_pairs(qa, qb, a->chi(), b->chi(), false, 0, ca, cb, ex.as<uint32_t>(), b->chi());
+    d2h(out, ex.p, (size_t)a->chi() * b->chi() * 4);
     sync();
   })
 }

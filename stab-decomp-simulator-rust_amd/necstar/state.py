@@ -2,6 +2,7 @@
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -251,13 +252,24 @@ class QuantumState:
         shots = _usize(shots)
         arr, p = _i32_array(qargs)
         _buf, sp = _seed_ptr(seed)
-        cap = min(shots, 1 << min(len(qargs), 24)) + 1
-        bits = np.zeros(2 * cap, np.uint64)
-        counts = np.zeros(cap, np.uint64)
+        if sp is None:  # one entropy seed for both attempts below
+            _buf, sp = _seed_ptr(os.urandom(32))
+        # a first capacity that fits most calls; when more distinct outcomes
+        # occur, the library reports how many and the call is repeated with
+        # the same seed (the reference's ShotCount has no size limit)
+        cap = min(shots, 1 << min(len(qargs), 20)) + 1
         nout = C.c_int64()
-        check(lib().nx_state_sample(self._h, len(qargs), p, shots, sp, cap,
-                                    bits.ctypes.data_as(C.POINTER(C.c_uint64)),
-                                    counts.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(nout)))
+        for _attempt in range(2):
+            bits = np.zeros(2 * cap, np.uint64)
+            counts = np.zeros(cap, np.uint64)
+            nout.value = 0
+            code = lib().nx_state_sample(self._h, len(qargs), p, shots, sp, cap,
+                                         bits.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                         counts.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(nout))
+            if code == 0 or nout.value <= cap:
+                break
+            cap = nout.value
+        check(code)
         res = {}
         k = len(qargs)
         for i in range(nout.value):
@@ -348,3 +360,10 @@ class QuantumState:
         out = np.zeros((max(chi, 1), max(chi, 1)), np.uint32)
         check(lib().nx_state_pair_table(self._h, out.ctypes.data_as(C.POINTER(C.c_uint32))))
         return out[:chi, :chi]
+
+    def cross_pair_table(self, other: "QuantumState") -> np.ndarray:
+        """Exact <self_i|other_j> (uint32: 0xFFFFFFFF zero, else k | r << 8)."""
+        ca, cb = self.stabilizer_rank, other.stabilizer_rank
+        out = np.zeros((max(ca, 1), max(cb, 1)), np.uint32)
+        check(lib().nx_state_cross_pair_table(self._h, other._h, out.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return out.reshape(-1)[: ca * cb].reshape(ca, cb)

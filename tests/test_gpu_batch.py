@@ -5,20 +5,19 @@ pair in one elimination (k_pairs_var); necstar.Batch does that for many
 states of uneven n and chi in one launch plan (optionally autotuned and
 captured into a HIP graph).  Both are checked against the oracle's
 exp_value / norm (the reference's exp_value.rs:7-61 and norm.rs:8-35
-restated on the CPU) and against the single-string GPU path, with
-|gpu - ref| <= 1e-12 * max(1, |ref|).  Graph replays are checked one by one,
-with eager library work between them."""
+restated on the CPU) and against the single-string GPU path, within
+1e-12 relative (fp64.check).  Graph replays are checked one by one, with
+eager library work between them."""
 import sys
 
 import numpy as np
 import pytest
 
+import fp64
 import workloads as WL
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
-
-TOL = 1e-12
 
 
 def say(*a):
@@ -34,9 +33,9 @@ def nx():
     return necstar
 
 
-def close(a, b):
-    a, b = np.asarray(a, float), np.asarray(b, float)
-    return np.all(np.abs(a - b) <= TOL * np.maximum(1.0, np.abs(b)))
+def scales(st, ops_list):
+    """Term scales of st.exp_value for each dense op list."""
+    return np.array([fp64.exp_scale(st, ops) for ops in ops_list])
 
 
 def pauli_str(ops):
@@ -83,15 +82,17 @@ def test_exp_values_variants_vs_oracle(nx, seed, n, nc, nt, npauli, ncheck):
     paulis = [nx.PauliString.from_str(pauli_str(p)) for p in ops]
     say(f"variants seed={seed} n={n} chi={o.chi} strings={npauli} informative={len(good)}")
     ev = st.exp_values(paulis)
+    sc = scales(st, ops)
     # identity string = norm^2
-    assert close(ev[-1], o.norm_squared())
+    fp64.check(ev[-1], o.norm_squared(), sc[-1], "identity string = norm^2")
     idx = list(range(ncheck - 1)) + [npauli - 1]
     ref = np.array([o.exp_value(("dense", ops[k])) for k in idx])
-    assert close(ev[idx], ref)
+    fp64.check(ev[idx], ref, sc[idx], "variants vs oracle")
     assert np.sum(np.abs(ref[:-1]) > 1e-3) >= 3, "the check needs strings with nonzero expectation"
     # every string against the single-string GPU path
     single = [st.exp_value(p) for p in paulis]
-    assert close(ev, single)
+    fp64.check(ev, single, sc, "variants vs single-string path")
+    say(f"  worst relative error so far {fp64.worst}")
 
 
 def test_exp_values_complex_coefficients(nx):
@@ -106,7 +107,7 @@ def test_exp_values_complex_coefficients(nx):
     o.set_complex_coeffs(c)
     ops = random_ops(rng, n, 40)
     ev = st.exp_values([nx.PauliString.from_str(pauli_str(p)) for p in ops])
-    assert close(ev, [o.exp_value(("dense", p)) for p in ops])
+    fp64.check(ev, [o.exp_value(("dense", p)) for p in ops], scales(st, ops), "complex coefficients")
 
 
 def _mixed_states(nx, rng):
@@ -123,24 +124,28 @@ def test_batch_matches_plain_calls_and_graph_replays(nx):
     rng = np.random.default_rng(48)
     states, olist = _mixed_states(nx, rng)
     counts = [0, 1, 5, 63, 64, 65, 130]
-    paulis = []
+    paulis, opl = [], []
     for st, k in zip(states, counts):
         ops = WL.random_paulis(rng, st.num_qubits, k, 2, 6)
         paulis.append([nx.PauliString.from_str(pauli_str(p)) for p in ops])
+        opl.append(ops)
     # a state with no terms left (impossible projection)
     z = nx.QuantumState.from_circuit(nx.QuantumCircuit(3))
     z.project_unnormalized(0, True)
     assert z.stabilizer_rank == 0
     states.append(z)
     paulis.append([nx.PauliString.from_str("XIZ")])
+    opl.append([[1, 0, 3]])
     say("batch plain reference")
     ref_norm = [st.norm() for st in states]
     ref_vals = [np.array([st.exp_value(p) for p in pl]) for st, pl in zip(states, paulis)]
+    norm_sc = [np.sqrt(fp64.state_scale(st, st)) if st.stabilizer_rank else 0.0 for st in states]
+    val_sc = [scales(st, ops) for st, ops in zip(states, opl)]
     # oracle on the small states
     for i in (0, 1, 6):
         n, gates = olist[i]
         o = O.State.compile(n, gates)
-        assert close(ref_norm[i], o.norm())
+        fp64.check(ref_norm[i], o.norm(), norm_sc[i], f"plain norm {i} vs oracle")
     for autotune, graph in ((False, False), (True, False), (False, True), (True, True)):
         say(f"batch autotune={autotune} graph={graph}")
         b = nx.Batch(states, paulis, autotune=autotune, graph=graph)
@@ -148,9 +153,11 @@ def test_batch_matches_plain_calls_and_graph_replays(nx):
         assert J in (8, 16, 32) and waves >= 1 and vw == 2
         for rep in range(3):
             norms, vals = b.run()
-            assert close(norms, ref_norm), (autotune, graph, rep)
-            for v, r in zip(vals, ref_vals):
-                assert v.shape == r.shape and close(v, r), (autotune, graph, rep)
+            fp64.check(norms, ref_norm, norm_sc, f"batch norms {autotune} {graph} {rep}")
+            for i, (v, r) in enumerate(zip(vals, ref_vals)):
+                assert v.shape == r.shape, (autotune, graph, rep, i)
+                if len(r):
+                    fp64.check(v, r, val_sc[i], f"batch values {autotune} {graph} {rep} state {i}")
             states[2].norm()  # eager work on the same stream between replays
         del b
 
@@ -161,11 +168,12 @@ def test_batch_copies_states(nx):
     st = nx.QuantumState.from_circuit(WL.to_circuit(n, WL.random_circuit(rng, n, 100, 5)))
     p = [nx.PauliString.from_str("Z0"), nx.PauliString.from_str("X1 Y2")]
     ref = [st.exp_value(q) for q in p]
+    sc = scales(st, [[3] + [0] * (n - 1), [0, 1, 2] + [0] * (n - 3)])
     b = nx.Batch([st], [p], autotune=False, graph=True)
     st.apply_h(0)
     st.apply_cx(0, 1)
     norms, vals = b.run()
-    assert close(vals[0], ref)
+    fp64.check(vals[0], ref, sc, "batch holds a copy")
 
 
 def test_batch_errors(nx):

@@ -3,8 +3,8 @@
 Integer results are compared bit for bit: per-term CH-form tables (G, F,
 M, gamma, v, s, phase), exact coefficient scalars, exact pair scalars
 <phi_i|phi_j>, exact per-term amplitude phases.  FP64 sums are compared with
-|gpu - oracle| <= 1e-12 * max(1, |oracle|) (the states here have unit norm,
-so this is the 1e-12 relative bound).
+the relative bound of fp64.check: |gpu - oracle| <= 1e-12 |oracle| wherever
+the value is at least 1e-3 of the sum of its terms' magnitudes.
 """
 import sys
 
@@ -12,12 +12,12 @@ import numpy as np
 import pytest
 
 import dense as D
+import fp64
 import workloads as WL
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-TOL = 1e-12
 
 
 def say(*a):
@@ -65,8 +65,8 @@ CASES = [  # (seed, n, cliffords, t, gate set)
     (10, 200, 600, 12, WL.CLIFFORD_SET),   # four-word rows
     (11, 240, 700, 16, tuple(range(11))),  # 240 + 16 ancillas: the 256-qubit maximum while compiling
 ]
-# pair sums at one to four row words (the 70- and 130-qubit cases are covered by the amplitude and table tests)
-PAIR_CASES = CASES[:7] + CASES[9:]
+# pair sums at one to four row words (the 70-qubit case is covered by the amplitude and table tests)
+PAIR_CASES = CASES[:7] + CASES[8:]
 
 
 @pytest.mark.parametrize("seed,n,nc,nt,gs", CASES)
@@ -105,17 +105,19 @@ def test_norm_exp_value_inner_product(nx, seed, n, nc, nt, gs):
     gates = WL.random_circuit(rng, n, nc, nt, gate_set=gs)
     g, o = compile_both(n, gates)
     say(f"norm/exp seed={seed} n={n} chi={o.chi}")
-    assert abs(g.norm() - o.norm()) <= TOL
+    fp64.check(g.norm() ** 2, o.norm_squared(), fp64.state_scale(g, g), "norm^2")
+    fp64.check(g.norm(), o.norm(), np.sqrt(fp64.state_scale(g, g)), "norm")
     from necstar import PauliString
     for ops in WL.random_paulis(rng, n, 4, 1, min(4, n)):
         s = "".join("IXYZ"[x] for x in reversed(ops))
         ev_g = g.exp_value(PauliString.from_str(s))
         ev_o = o.exp_value(("dense", ops))
-        assert abs(ev_g - ev_o) <= TOL, (s, ev_g, ev_o)
+        fp64.check(ev_g, ev_o, fp64.exp_scale(g, ops), f"exp_value {s}")
     gates2 = WL.random_circuit(rng, n, nc, nt, gate_set=gs)
     g2, o2 = compile_both(n, gates2)
     ip_g, ip_o = g.inner_product(g2), o.inner_product(o2)
-    assert abs(ip_g - ip_o) <= TOL, (ip_g, ip_o)
+    fp64.check(ip_g, ip_o, fp64.state_scale(g, g2), "inner_product")
+    say(f"  worst relative error so far {fp64.worst}")
 
 
 @pytest.mark.parametrize("seed,n,nc,nt,gs", CASES)
@@ -135,7 +137,7 @@ def test_amplitudes_exact(nx, seed, n, nc, nt, gs):
                 assert pt[i, b] == 255, (b, i)
             else:
                 assert pt[i, b] == ko[i] and rr[i] == ro[i], (b, i, pt[i, b], ko[i], rr[i], ro[i])
-        assert vals[b] == val_o or abs(vals[b] - val_o) <= TOL
+        fp64.check(vals[b], val_o, fp64.amp_scale(o, (zo, ko, ro)), f"amplitude {b}")
 
 
 def _support_circuit(rng, n, k, nt):
@@ -203,7 +205,7 @@ def test_amplitudes_constraint_paths(nx, seed, n, k):
             else:
                 nonzero += 1
                 assert pt[i, b] == ko[i] and rr[i] == ro[i], (b, i, pt[i, b], ko[i], rr[i], ro[i])
-        assert abs(vals[b] - val_o) <= TOL, (b, vals[b], val_o)
+        fp64.check(vals[b], val_o, fp64.amp_scale(o, (zo, ko, ro)), f"amplitude {b}")
     assert nonzero > 0
     # both support tests are exercised where the case is meant to
     if n - k > 8:
@@ -223,7 +225,8 @@ def test_statevector_vs_dense(nx):
         g = QuantumState.from_circuit(WL.to_circuit(n, gates))
         o = O.State.compile(n, gates)
         sv = g.statevector_array()
-        assert np.abs(sv - o.statevector()).max() <= TOL
+        # |<x|phi_i>| <= 1 bounds every entry's term scale by |g| sum |c_i|
+        fp64.check(sv, o.statevector(), abs(o.global_factor) * np.abs(o.coeffs()).sum(), "statevector")
         assert np.abs(sv - D.run(n, gates)).max() < 1e-9
 
 
@@ -245,7 +248,7 @@ def test_projection_discard_gates_parity(nx):
         o.project_unnormalized(q, outc)
         assert_tables_equal(g, o)
         if o.chi:
-            assert abs(g.norm() - o.norm()) <= TOL
+            fp64.check(g.norm() ** 2, o.norm_squared(), fp64.state_scale(g, g), "projected norm^2")
         g.project_unnormalized(q, False)
         o.project_unnormalized(q, False)
         if o.chi:
@@ -348,7 +351,8 @@ def test_projection_norms_vs_oracle(nx):
                 c.project_unnormalized(q, (k >> i) & 1)
             ref.append(c.norm_squared())
         ref = np.array(ref)
-        assert np.all(np.abs(got - ref) <= TOL), (trial, got, ref)
+        # pair magnitudes are at most 1: (|g| sum |c_i|)^2 bounds the term scale
+        fp64.check(got, ref, (abs(o.global_factor) * np.abs(o.coeffs()).sum()) ** 2, f"projection norms {trial}")
         # exact sums: an outcome of probability zero gives exactly 0.0, as in
         # the oracle (sampling's binomial draws depend on it)
         assert np.all(got[ref == 0.0] == 0.0), (trial, got, ref)
@@ -428,7 +432,7 @@ def test_qubit_limit(nx):
         QuantumState.from_circuit(WL.to_circuit(257, [(0, 0)]))
     g, o = compile_both(250, [(0, q) for q in range(6)] + [(11, q) for q in range(6)])
     assert_tables_equal(g, o)
-    assert abs(g.norm() - o.norm()) <= TOL
+    fp64.check(g.norm(), o.norm(), np.sqrt(fp64.state_scale(g, g)), "norm at 256 compile qubits")
 
 
 def test_config0_workload(nx):
@@ -437,21 +441,26 @@ def test_config0_workload(nx):
     g, o = compile_both(n, cfg["gates"])
     say(f"cfg0 chi={o.chi}")
     assert_tables_equal(g, o)
-    assert abs(g.norm() - o.norm()) <= TOL and abs(g.norm() - 1.0) <= 1e-12
+    fp64.check(g.norm(), o.norm(), np.sqrt(fp64.state_scale(g, g)), "cfg0 norm")
+    assert abs(g.norm() - 1.0) <= 1e-12
     vals = g.amplitudes(cfg["bitstrings"])
     for b in range(len(cfg["bitstrings"])):
         x = np.array([(int(cfg["bitstrings"][b, 0]) >> q) & 1 for q in range(n)], np.uint8)
+        # 12 terms: one term chunk, so the sum runs in the reference's order
         assert vals[b] == o.amplitude(x)
     from necstar import PauliString
     for ops in cfg["paulis"]:
         s = "".join("IXYZ"[x] for x in reversed(ops))
-        assert abs(g.exp_value(PauliString.from_str(s)) - o.exp_value(("dense", ops))) <= TOL
+        fp64.check(g.exp_value(PauliString.from_str(s)), o.exp_value(("dense", ops)), fp64.exp_scale(g, ops),
+                   f"cfg0 exp_value {s}")
 
 
 def test_config1_full_size(nx):
     """cfg1 at its real shape (n=64, 2000 Cliffords, 24 T): chi and sampled
-    tables, pair scalars and amplitudes bit-exact against the oracle; the
-    norm of a compiled unitary circuit is 1 (size-independent property)."""
+    tables, pair scalars and per-term amplitudes bit-exact against the
+    oracle, 256 amplitude sums within 1e-12 relative (cfg1 amplitudes are
+    of order 2^-32 with terms cancelling by up to ~400x); the norm of a
+    compiled unitary circuit is 1 (size-independent property)."""
     cfg = WL.config(1)
     n = cfg["n"]
     g, o = compile_both(n, cfg["gates"])
@@ -466,20 +475,29 @@ def test_config1_full_size(nx):
             assert np.array_equal(tg[key], to[key]), (i, key)
         assert tg["phase"] == to["phase"]
         assert (bool(z[i]), int(k[i]), int(r[i])) == o.coeff_scalar(i)
+    # (the oracle's norm at cfg1, 266,085 pairs, takes minutes: the unit
+    # norm of a compiled unitary circuit is the check here)
     assert abs(g.norm() - 1.0) <= 1e-12
     tab = g.pair_table()
     for _ in range(60):
         i, j = (int(v) for v in rng.integers(0, o.chi, 2))
         (zz, kk, rr), _ = o.term(i).inner_product(o.term(j))
         assert int(tab[i, j]) == (0xFFFFFFFF if zz else (kk | (rr << 8))), (i, j)
-    bits = cfg["bitstrings"][:24]
+    # the first 256 strings of the workload, evaluated as part of all
+    # 65,536 (the launch shape, hence the term chunking, of the benchmark)
+    allv = g.amplitudes(cfg["bitstrings"])
+    bits = cfg["bitstrings"][:256]
     vals, pt, rr = g.amplitudes(bits, per_term=True)
+    rels = []
     for b in range(len(bits)):
         x = np.array([(int(bits[b, 0]) >> q) & 1 for q in range(n)], np.uint8)
         val_o, (zo, ko, ro) = o.amplitude(x, per_term=True)
         assert np.array_equal(pt[:, b] == 255, zo)
         assert np.array_equal(pt[~zo, b], ko[~zo]) and np.array_equal(rr[~zo], ro[~zo])
-        assert abs(vals[b] - val_o) <= TOL
+        sc = fp64.amp_scale(o, (zo, ko, ro))
+        rels.append(fp64.check(vals[b], val_o, sc, f"cfg1 amplitude {b}"))
+        rels.append(fp64.check(allv[b], val_o, sc, f"cfg1 amplitude {b} (65,536-string launch)"))
+    say(f"cfg1 amplitudes: largest relative error {max(rels):.3g} over {len(bits)} strings")
 
 
 def test_config4_full_size(nx):
@@ -525,4 +543,5 @@ def test_amplitudes_ignore_bits_beyond_n(nx):
     wide[:, 1] |= rng.integers(0, 2**57, 32, dtype=np.uint64) << np.uint64(6)
     assert np.array_equal(g.amplitudes(wide), g.amplitudes(bits))
     x = np.array([(int(bits[0, q // 64]) >> (q % 64)) & 1 for q in range(n)], np.uint8)
-    assert abs(g.amplitudes(wide)[0] - o.amplitude(x)) <= TOL
+    val_o, pt = o.amplitude(x, per_term=True)
+    fp64.check(g.amplitudes(wide)[0], val_o, fp64.amp_scale(o, pt), "ragged amplitude")
