@@ -65,6 +65,18 @@ int nx_state_apply_gates(nx_state *s, int64_t ngates, const int32_t *gates);
 int nx_state_norm(const nx_state *s, double *out);
 int nx_state_norm_squared(const nx_state *s, double *out);
 
+/* nx_state_norm that allocates nothing: no reference counterpart; for
+ * callers that cannot allocate (tight serving loops, harnesses that forbid
+ * hipMalloc).  Every device buffer of the call (quadratic forms, coefficient
+ * copies, pair-unit offsets, partial sums, result) lives in one file-scope
+ * device array of 64 MiB per GPU, held until the process ends: up to about
+ * 30,000 terms at n <= 64 or 7,900 at n <= 128; larger states fail with
+ * NX_E_VALUE.  Same kernels and summation order as nx_state_norm (equal
+ * results).  Calls on one GPU share that array: overlapping calls on one GPU
+ * (several threads) race.  Call nx_init on the device first (it sets the
+ * device up once, which allocates). */
+int nx_state_norm_noalloc(const nx_state *s, double *out);
+
 /* QuantumState::exp_value (state/mod.rs:309-313, exp_value.rs:7-61).
  * dense != 0: ops[len] little-endian (PauliString::Dense);
  * dense == 0: sparse terms (ops[i], qubits[i]) (PauliString::Sparse). */
@@ -204,6 +216,9 @@ void nx_batch_free(nx_batch *b);
  * "k_pairs_var", "k_gramc", "k_qform", "k_program", "batch_pairs" (a whole
  * nx_batch_run launch plan).  No reference counterpart (measurement aid). */
 int nx_profile_enable(int on);
+/* kernel "allocations": *count = device-buffer requests to the library's
+ * allocator plus page-locked host allocations so far (*total_ms = 0),
+ * whether or not profiling is enabled. */
 int nx_profile_get(const char *kernel, double *total_ms, int64_t *count);
 
 /* Drops the derived per-term device data (the quadratic forms) so the

@@ -165,6 +165,17 @@ hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, const int64_t* off
                         hipStream_t st);
 hipError_t launch_reduce(const double2* partial, int64_t n, double2* out, hipStream_t st);
 
+// Work units of the pair triangle {(f, o) : f <= o < chi} cut into units of
+// J consecutive o: S(M) = sum_{m=1..M} ceil(m / J) = J q (q + 1) / 2 +
+// r (q + 1) for M = q J + r, so row f starts at unit S(chi) - S(chi - f)
+// and the triangle holds S(chi) units.
+__host__ __device__ inline int64_t tri_units_upto(int64_t M, int J) {
+  const int64_t q = M / J, r = M % J;
+  return (int64_t)J * q * (q + 1) / 2 + r * (q + 1);
+}
+// off[f] = tri_units_upto(chi) - tri_units_upto(chi - f) for f = 0 .. chi
+hipError_t launch_tri_offsets(int64_t chi, int J, int64_t* off, hipStream_t st);
+
 // Pauli variants of every term of `in`: variant k applies the ops
 // vops[voff[k] .. voff[k+1]) (q << 2 | op, op 1 = X, 2 = Y, 3 = Z) in order.
 hipError_t launch_pauli_var(const FormsDev& in, const int32_t* voff, const int32_t* vops, int nvar, int VW,

@@ -1248,4 +1248,15 @@ hipError_t launch_reduce(const double2* partial, int64_t n, double2* out, hipStr
   return hipGetLastError();
 }
 
+__global__ void k_tri_offsets(int64_t chi, int J, int64_t* __restrict__ off) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f > chi) return;
+  off[f] = tri_units_upto(chi, J) - tri_units_upto(chi - f, J);
+}
+
+hipError_t launch_tri_offsets(int64_t chi, int J, int64_t* off, hipStream_t st) {
+  k_tri_offsets<<<dim3(blocks_for(chi + 1, 256)), dim3(256), 0, st>>>(chi, J, off);
+  return hipGetLastError();
+}
+
 }  // namespace nx

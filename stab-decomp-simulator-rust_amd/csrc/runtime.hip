@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <complex>
 #include <cstdarg>
@@ -68,11 +69,26 @@ static const char* gate_name(int op) {
 }
 
 // ---------------------------------------------------------------------------
-// device context: stream, FP64 tables, caching allocator
+// Scratch of the allocation-free norm (nx_state_norm_noalloc): the one
+// device buffer that entry point uses, one copy per GPU, held until the
+// process ends.  Host code never passes this name to a kernel or a copy: it
+// is a host-side stand-in, not a device address; each device's address is
+// looked up once (hipGetSymbolAddress) when its context is set up.
+static constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+// device allocations requested from the library's allocator (pool hits
+// included) and page-locked host allocations, for tests of the
+// allocation-free entry point
+static std::atomic<int64_t> g_alloc_calls{0};
+
+// ---------------------------------------------------------------------------
+// device context: stream, FP64 tables, caching allocator, scratch address
 struct DeviceCtx {
   int dev = 0;
   hipStream_t stream = nullptr;
   double* tabs = nullptr;
+  unsigned char* scratch = nullptr;  // this device's copy of `scratch`
   std::mutex mu;
   std::map<size_t, std::vector<void*>> free_lists;
 };
@@ -110,6 +126,9 @@ static DeviceCtx* ctx() {
   for (int r = -TAB_RMAX; r <= TAB_RMAX; ++r) t[TAB_POW + r] = std::pow(2.0, -((double)r) / 2.0);
   hip_check(hipMalloc(&c->tabs, sizeof(double) * TAB_SIZE), "hipMalloc");
   hip_check(hipMemcpy(c->tabs, t.data(), sizeof(double) * TAB_SIZE, hipMemcpyHostToDevice), "hipMemcpy");
+  void* sp = nullptr;
+  hip_check(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+  c->scratch = (unsigned char*)sp;
   g_ctx[t_dev] = c;
   return c;
 }
@@ -141,6 +160,7 @@ struct DevBuf {
   void alloc(size_t bytes) {
     release();
     if (bytes == 0) return;
+    g_alloc_calls.fetch_add(1, std::memory_order_relaxed);
     c = ctx();
     cap = size_class(bytes);
     {
@@ -182,6 +202,7 @@ struct PinnedBuf {
   void alloc(size_t bytes) {
     if (p) (void)hipHostFree(p);
     p = nullptr;
+    g_alloc_calls.fetch_add(1, std::memory_order_relaxed);
     if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
       p = nullptr;
       fail(NX_E_DEVICE, "out of pinned host memory (%zu bytes)", bytes);
@@ -614,6 +635,86 @@ static double norm_squared(const nx_state* s) {
   cplx sum = run_pairs(q, q, s->chi(), s->chi(), true, /*fixed bra, herm*/ 2, c, c, nullptr, 0,
                        s->d_tri.as<int64_t>(), s->tri_units);
   return sum.real() * (s->g.real() * s->g.real() + s->g.imag() * s->g.imag());
+}
+
+// The same sum with every device buffer carved out of this device's copy of
+// `scratch` (no allocation of any kind): the quadratic forms of all terms,
+// the coefficient copies, the unit offsets (closed form, on the device), the
+// per-wave partial sums and the reduced result.  The kernels, launch shape
+// and summation order are those of norm_squared, so the two agree bit for
+// bit.  Calls on one GPU share its scratch: calls that overlap on one GPU
+// (from several threads) race.
+static double norm_squared_noalloc(const nx_state* s) {
+  DeviceCtx* c = ctx();
+  const hipStream_t st = c->stream;
+  const int64_t chi = s->chi();
+  const double g2 = s->g.real() * s->g.real() + s->g.imag() * s->g.imag();
+  if (chi == 0) return 0.0 * g2;
+  const int W = s->forms.W, n = s->n;
+  const size_t R = 64 * (size_t)W;
+  const int64_t units = tri_units_upto(chi, PAIR_J);
+  const int nblocks = (int)std::max<int64_t>(1, std::min<int64_t>((units + 3) / 4, 6144));
+  // layout (256-byte aligned pieces)
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t r = o;
+    o += (bytes + 255) & ~(size_t)255;
+    return r;
+  };
+  const size_t rows = (size_t)chi * R * W * 8;
+  const size_t oQ = take(rows), oB = take(rows), oC = take(rows), oCT = take(rows), oa = take((size_t)chi * R);
+  const size_t oalo = take((size_t)chi * W * 8), oahi = take((size_t)chi * W * 8), orhs = take((size_t)chi * W * 8);
+  const size_t onc = take((size_t)chi * 4), ok = take((size_t)chi * 4), op0 = take((size_t)chi * 4);
+  const size_t ocoef = take((size_t)chi * (s->is_complex ? 16 : 4));
+  const size_t ooff = take((size_t)(chi + 1) * 8);
+  const size_t opart = take((size_t)nblocks * 4 * 16), ores = take(16);
+  if (o > NOALLOC_SCRATCH_BYTES)
+    fail(NX_E_VALUE,
+         "nx_state_norm_noalloc: %lld terms on %d qubits need %zu bytes of scratch, more than its %zu bytes",
+         (long long)chi, n, o, NOALLOC_SCRATCH_BYTES);
+  unsigned char* b = c->scratch;
+  QFormDev q;
+  q.Q = (uint64_t*)(b + oQ); q.B = (uint64_t*)(b + oB); q.C = (uint64_t*)(b + oC); q.CT = (uint64_t*)(b + oCT);
+  q.a = b + oa;
+  q.alo = (uint64_t*)(b + oalo); q.ahi = (uint64_t*)(b + oahi); q.rhs = (uint64_t*)(b + orhs);
+  q.nc = (int32_t*)(b + onc); q.k = (int32_t*)(b + ok); q.p0 = (int32_t*)(b + op0);
+  q.chi = chi; q.n = n; q.W = W;
+  CoefDev cd;
+  cd.scalar = s->is_complex ? 0 : 1;
+  cd.cs = (const int32_t*)(b + ocoef);
+  cd.c = (const double2*)(b + ocoef);
+  if (s->is_complex) {
+    // std::complex<double> is laid out as (re, im), as double2
+    hip_check(hipMemcpyAsync(b + ocoef, s->cc.data(), (size_t)chi * 16, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  } else {
+    // packed Scalars (CoefDev), staged through a fixed host block
+    int32_t blk[2048];
+    for (int64_t i0 = 0; i0 < chi; i0 += 2048) {
+      const int64_t m = std::min<int64_t>(2048, chi - i0);
+      for (int64_t i = 0; i < m; ++i) {
+        const Scal& v = s->sc[i0 + i];
+        if (v.r < -TAB_RMAX / 2 || v.r > TAB_RMAX / 2)
+          fail(NX_E_INTERNAL, "coefficient exponent %lld outside the supported range", (long long)v.r);
+        blk[i] = v.z ? -1 : (int32_t)((v.k & 7) | ((v.r + CS_ROFF) << 3));
+      }
+      hip_check(hipMemcpyAsync(b + ocoef + (size_t)i0 * 4, blk, (size_t)m * 4, hipMemcpyHostToDevice, st),
+                "hipMemcpyAsync");
+      hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");  // blk is reused
+    }
+  }
+  int64_t* off = (int64_t*)(b + ooff);
+  double2* part = (double2*)(b + opart);
+  double2* res = (double2*)(b + ores);
+  hip_check(launch_qform(s->forms.desc(), q, st), "k_qform");
+  hip_check(launch_tri_offsets(chi, PAIR_J, off, st), "k_tri_offsets");
+  hip_check(launch_pairs(q, q, off, chi, chi, 1, PAIR_J, /*fixed bra, herm*/ 2, cd, cd, c->tabs, part, nblocks, nullptr,
+                         0, st),
+            "k_pairs");
+  hip_check(launch_reduce(part, (int64_t)nblocks * 4, res, st), "k_reduce");
+  double sum = 0.0;
+  hip_check(hipMemcpyAsync(&sum, res, 8, hipMemcpyDeviceToHost, st), "hipMemcpyAsync D2H");
+  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+  return sum * g2;
 }
 
 // ---------------------------------------------------------------------------
@@ -1717,6 +1818,8 @@ int nx_state_norm_squared(const nx_state* s, double* out) { NX_TRY(*out = norm_s
 
 int nx_state_norm(const nx_state* s, double* out) { NX_TRY(*out = std::sqrt(norm_squared(s))) }
 
+int nx_state_norm_noalloc(const nx_state* s, double* out) { NX_TRY(*out = std::sqrt(norm_squared_noalloc(s))) }
+
 int nx_state_exp_value(const nx_state* s, int dense, int32_t len, const int32_t* ops, const int32_t* qubits,
                        double* out) {
   NX_TRY(*out = exp_value(s, pauli_prog(s, dense, len, ops, qubits)))
@@ -2215,6 +2318,11 @@ int nx_profile_enable(int on) {
 
 int nx_profile_get(const char* kernel, double* total_ms, int64_t* count) {
   NX_TRY({
+    if (kernel && std::strcmp(kernel, "allocations") == 0) {
+      *total_ms = 0.0;
+      *count = g_alloc_calls.load(std::memory_order_relaxed);
+      return NX_OK;
+    }
     std::lock_guard<std::mutex> lk(g_prof_mu);
     g_prof.collect();
     auto it = g_prof.acc.find(kernel ? kernel : "");

@@ -25,6 +25,7 @@ SIGNATURES = {
     "nx_state_stabilizer_rank": (C.c_int, [_P, _I64P]),
     "nx_state_apply_gates": (C.c_int, [_P, _I64, _I32P]),
     "nx_state_norm": (C.c_int, [_P, _DP]),
+    "nx_state_norm_noalloc": (C.c_int, [_P, _DP]),
     "nx_state_norm_squared": (C.c_int, [_P, _DP]),
     "nx_state_exp_value": (C.c_int, [_P, C.c_int, _I32, _I32P, _I32P, _DP]),
     "nx_state_inner_product": (C.c_int, [_P, _P, _DP, _DP]),
