@@ -6,11 +6,25 @@ one pass of the hot path over device-resident inputs:
   * the per-term quadratic-form rewrite (k_qform, chi terms),
   * <x|psi> for 65,536 random 64-bit strings (k_amplitude, chi*65536
     CH-form amplitude evaluations),
-  * the norm via all chi(chi+1)/2 pairwise CH-form inner products (k_pairs).
+  * the norm via all chi(chi+1)/2 pairwise CH-form inner products (k_pairs),
+the library enqueueing on PyTorch's stream; the host waits for each step's
+norm (nx_state_norm, the reference's synchronous call), the amplitudes stay
+in device memory.  The same steps fully asynchronous (norm into device
+memory, nx_state_norm_async) are timed as well.
 value = (amplitude evaluations + inner products) per second of step time.
 
-Prints one JSON line.  The CPU baseline is the oracle (a C restatement of
-the reference algorithm, one core) on a bounded sample of the same units.
+The same run also measures, by default:
+  * cfg1 sampling: sample(range(64), 1024, seed=1001);
+  * cfg2 batched vs plain: 256 circuits (n ~ U[16,128], 30 n Cliffords,
+    t ~ U[4,32]), norm + 64 Pauli expectations each, as one necstar.Batch
+    (autotuned launch shape, captured HIP graph) against the plain per-call
+    API (norm() and exp_value(P) for every string of every circuit, all of
+    them timed);
+  * the CPU baseline: the oracle (a C restatement of the reference
+    algorithm) on a bounded sample of the cfg1 units, on one core and on an
+    8-process pool.
+
+Prints one JSON line.
 """
 import argparse
 import ctypes as C
@@ -26,77 +40,209 @@ for _p in (ROOT, PKG):
         sys.path.insert(0, _p)
 
 METRIC = "CH-form amplitude evals/s + inner products/s, n=64; batched vs plain, 1 GPU"
-# MI355X: 256 CUs x 64 lanes x 2.4 GHz peak engine clock = int32 lane-ops/s
-PEAK_VALU_TOPS = 256 * 64 * 2.4e9 / 1e12
+# MI355X: 256 CUs x 64 lanes x 2.4 GHz peak engine clock
+CUS, CLOCK = 256, 2.4e9
+PEAK_VALU_TOPS = CUS * 64 * CLOCK / 1e12      # int32 lane-ops/s (one VALU wave-instruction per CU per cycle)
+PEAK_SALU_GIPS = CUS * CLOCK / 1e9            # scalar instructions/s (one per CU per cycle)
+PEAK_LDS_TBS = CUS * 128 * CLOCK / 1e12       # LDS bytes/s (128 B per CU per cycle)
 
 
-def ops_amp(n):  # int32 lane-ops per amplitude evaluation
-    w32 = (n + 31) // 32
-    return 12 * ((n + 1) // 2) * w32 + 4 * w32
+# ---------------------------------------------------------------------------
+# CPU baseline (oracle = C restatement of the reference algorithm)
 
-
-def ops_ip(n):  # int32 lane-ops per CH-form inner product
-    return 6 * n * n * ((n + 31) // 32)
-
-
-def cpu_baseline(cfg, seconds):
-    """Oracle (C restatement of the reference, 1 core) on a bounded sample."""
+def _cpu_worker(args):
+    """Times oracle amplitude evaluations and inner products of the cfg1
+    state for about `seconds` (worker `k` of `nw` takes every nw-th unit)."""
+    k, nw, seconds = args
     import numpy as np
+    import workloads as WL
     from oracle import oracle as O
+    cfg = WL.config(1)
     n = cfg["n"]
     t0 = time.perf_counter()
     st = O.State.compile(n, cfg["gates"])
     compile_s = time.perf_counter() - t0
     chi = st.chi
     bits = cfg["bitstrings"]
-    # amplitude evaluations: whole strings (all chi terms each)
     t0 = time.perf_counter()
-    na = 0
-    while time.perf_counter() - t0 < seconds / 2 and na < len(bits):
-        x = np.array([(int(bits[na, 0]) >> q) & 1 for q in range(n)], np.uint8)
+    na, b = 0, k
+    while time.perf_counter() - t0 < seconds / 2 and b < len(bits):
+        x = np.array([(int(bits[b, 0]) >> q) & 1 for q in range(n)], np.uint8)
         st.amplitude(x)
         na += 1
-    t_amp = (time.perf_counter() - t0) / (na * chi)
-    # pairwise inner products
+        b += nw
+    t_amp = time.perf_counter() - t0
     t0 = time.perf_counter()
-    npairs, i, j = 0, 0, 0
+    npairs, i, j = 0, 0, k
     while time.perf_counter() - t0 < seconds / 2:
         st.term(i).inner_product(st.term(j))
         npairs += 1
-        j += 1
-        if j >= chi:
-            i, j = i + 1, i + 1
-    t_ip = (time.perf_counter() - t0) / npairs
-    return dict(chi=chi, compile_s=compile_s, t_amp=t_amp, t_ip=t_ip,
-                sample=f"{na} strings x {chi} terms amplitudes + {npairs} pairs, cfg1 state")
+        j += nw
+        while j >= chi:  # next row of the triangle (from the start after the last)
+            i, j = (i + 1, i + 1 + (j - chi)) if i < chi - 1 else (0, k)
+    t_ip = time.perf_counter() - t0
+    return dict(chi=chi, compile_s=compile_s, amp_evals=na * chi, t_amp=t_amp, pairs=npairs, t_ip=t_ip, strings=na)
 
+
+def cpu_baseline(seconds, workers):
+    """One core, then a pool of `workers` processes (an upper bound: the
+    reference is single-threaded; rayon is declared but unused)."""
+    one = _cpu_worker((0, 1, seconds))
+    pool = None
+    if workers > 1:
+        import multiprocessing as mp
+        with mp.get_context("spawn").Pool(workers) as p:
+            res = p.map(_cpu_worker, [(k, workers, seconds) for k in range(workers)])
+        pool = dict(amp_rate=sum(r["amp_evals"] for r in res) / max(r["t_amp"] for r in res),
+                    ip_rate=sum(r["pairs"] for r in res) / max(r["t_ip"] for r in res))
+    return one, pool
+
+
+def host_info():
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "model": model}
+
+
+# ---------------------------------------------------------------------------
+# committed rocprofv3 counter summaries
 
 def read_counters(kernel):
-    """Per-launch figures from the committed rocprofv3 --pmc summary of the
-    kernel (HBM bytes, VALU / SALU wave instructions per unit)."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_cfg1_traffic.txt")
+    """Per-launch figures from profiles/<kernel>_cfg1_counters.txt (HBM
+    bytes from FETCH_SIZE / WRITE_SIZE, executed VALU / SALU / LDS
+    wave-instructions per unit)."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_cfg1_counters.txt")
     out = {}
     if not os.path.exists(path):
         return out
     for line in open(path):
         key, _, val = line.partition(":")
-        if key in ("hbm_bytes_per_launch", "valu_wave_instr_per_unit", "salu_wave_instr_per_unit"):
-            out[key] = float(val)
+        key = key.strip()
+        if key in ("hbm_bytes_per_launch", "valu_wave_instr_per_unit", "salu_wave_instr_per_unit",
+                   "lds_wave_instr_per_unit", "lds_bytes_per_unit"):
+            try:
+                out[key] = float(val.split()[0])
+            except (ValueError, IndexError):
+                pass
     return out
 
+
+def roofline_for(kernel, units, kms):
+    """The kernel's executed work against the issue peaks it is bound by.
+    k_pairs: int32 VALU lane-ops (SQ_INSTS_VALU x 64 per pair) against the
+    VALU peak, and the scalar-instruction issue (its binding resource);
+    k_amplitude: LDS bytes per evaluation (16 nibble-table lookups, 8 B at
+    positions 0-7 and 4 B at 8-15, and one 16 B value-table read) against
+    the LDS peak."""
+    ctr = read_counters(kernel)
+    rate = units / (kms / 1e3)
+    r = {"kernel": kernel, "units_per_launch": units, "avg_launch_ms": kms, "traffic": ctr.get("hbm_bytes_per_launch")}
+    if "valu_wave_instr_per_unit" in ctr:
+        valu = ctr["valu_wave_instr_per_unit"] * 64.0
+        salu = ctr.get("salu_wave_instr_per_unit", 0.0)
+        r["executed"] = {"valu_lane_ops_per_unit": valu, "salu_instr_per_unit": salu,
+                         "valu_frac": valu * rate / (PEAK_VALU_TOPS * 1e12),
+                         "salu_issue_frac": salu * rate / (PEAK_SALU_GIPS * 1e9)}
+    if kernel == "k_amplitude":
+        per = ctr.get("lds_bytes_per_unit", 112.0)
+        ach = per * rate / 1e12
+        r.update({"bound": "lds", "achieved": ach, "peak": PEAK_LDS_TBS, "unit": "TB/s (LDS)",
+                  "frac": ach / PEAK_LDS_TBS, "basis": f"{per:g} LDS bytes per (term, string) evaluation"})
+    else:
+        valu = r.get("executed", {}).get("valu_lane_ops_per_unit")
+        if valu is not None:
+            ach = valu * rate / 1e12
+            r.update({"bound": "valu", "achieved": ach, "peak": PEAK_VALU_TOPS, "unit": "TOP/s (int32 lane-ops)",
+                      "frac": ach / PEAK_VALU_TOPS,
+                      "basis": "executed VALU lane-ops per pair (rocprofv3 SQ_INSTS_VALU x 64, committed summary); "
+                               "the binding resource is scalar issue (executed.salu_issue_frac)"})
+    return r
+
+
+# ---------------------------------------------------------------------------
+# cfg2: batched vs plain
+
+def pauli_text(ops):
+    return "".join("IXYZ"[x] for x in reversed(ops))
+
+
+def cfg2_compare(ncirc, steps, log):
+    import numpy as np
+    import workloads as WL
+    from necstar import Batch, PauliString, QuantumState
+    from necstar import _lib
+    L = _lib.lib()
+    cfg = WL.config(2)
+    circs = cfg["circuits"][:ncirc]
+    t0 = time.perf_counter()
+    states = [QuantumState.from_circuit(WL.to_circuit(c["n"], c["gates"])) for c in circs]
+    compile_s = time.perf_counter() - t0
+    paulis = [[PauliString.from_str(pauli_text(p)) for p in c["paulis"]] for c in circs]
+    chis = np.array([s.stabilizer_rank for s in states])
+    log(f"cfg2: {len(states)} circuits compiled in {compile_s:.1f} s, chi max {chis.max()}")
+    t0 = time.perf_counter()
+    b = Batch(states, paulis, autotune=True, graph=True)
+    create_s = time.perf_counter() - t0
+    J, waves, vw = b.shape
+    b.run()  # warm-up replay
+    _lib.check(L.nx_profile_enable(1))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        norms, vals = b.run()
+    t1 = time.perf_counter()
+    ms, cnt = C.c_double(), C.c_int64()
+    _lib.check(L.nx_profile_get(b"batch_pairs", C.byref(ms), C.byref(cnt)))
+    _lib.check(L.nx_profile_enable(0))
+    batched_s = (t1 - t0) / steps
+    log(f"cfg2: batched run {batched_s:.3f} s (J={J}, waves={waves}); plain leg next")
+    # the plain per-call path: every call of every circuit, timed
+    maxdiff = 0.0
+    t0 = time.perf_counter()
+    for i, (st, pl) in enumerate(zip(states, paulis)):
+        nrm = st.norm()
+        ev = np.array([st.exp_value(p) for p in pl])
+        maxdiff = max(maxdiff, abs(nrm - norms[i]) / max(abs(nrm), 1e-300))
+        nzm = np.abs(ev) > 1e-9
+        if np.any(nzm):
+            maxdiff = max(maxdiff, float(np.max(np.abs(ev[nzm] - vals[i][nzm]) / np.abs(ev[nzm]))))
+        if i % 64 == 63:
+            log(f"cfg2: plain leg {i + 1} circuits, {time.perf_counter() - t0:.1f} s")
+    plain_s = time.perf_counter() - t0
+    nc = len(states)
+    pairs = float((chis.astype(float) * (chis + 1) / 2).sum())
+    return {
+        "workload": f"cfg2: {nc} circuits, n U[16,128], 30n Cliffords, t U[4,32], norm + 64 Pauli expectations "
+                    "(weight 2-6) each; sampling not included",
+        "circuits": nc, "chi_max": int(chis.max()), "chi_mean": float(chis.mean()),
+        "batched_s": batched_s, "batched_circuits_per_s": nc / batched_s,
+        "batched_graph_ms": ms.value / max(cnt.value, 1),
+        "plain_s": plain_s, "plain_circuits_per_s": nc / plain_s, "plain_calls": nc * 65,
+        "batched_vs_plain": plain_s / batched_s, "max_rel_diff_batched_vs_plain": maxdiff,
+        "pair_string_values_per_s": pairs * 65 / batched_s,
+        "shape": {"J": J, "waves": waves, "variant_words": vw}, "compile_s": compile_s, "batch_create_s": create_s,
+    }
+
+
+# ---------------------------------------------------------------------------
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default per config: 20, 1, 5, 2)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default per config: 3, 1, 1, 1)")
-    ap.add_argument("--cpu-seconds", type=float, default=16.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample length per leg (0: skip)")
+    ap.add_argument("--cpu-workers", type=int, default=8, help="processes of the pooled CPU baseline (1: skip it)")
     ap.add_argument("--config", type=int, default=1, choices=(1, 2, 3, 4),
-                    help="1: amplitude sweep + norm (the metric); 2: batched vs plain norm + 64 Pauli "
-                         "expectations over 256 circuits; 3: Pauli-loss forward vs backward")
-    ap.add_argument("--circuits", type=int, default=0, help="cfg2: use only the first N circuits")
-    ap.add_argument("--plain-paulis", type=int, default=2,
-                    help="cfg2: Pauli strings per circuit timed on the plain path (then scaled to 64)")
+                    help="1: amplitude sweep + norm (the metric); 2: batched vs plain alone; "
+                         "3: Pauli-loss forward vs backward; 4: n=128 amplitude sweep")
+    ap.add_argument("--circuits", type=int, default=256, help="cfg2 circuits (the first N; 0: skip the cfg2 leg)")
+    ap.add_argument("--cfg2-steps", type=int, default=2, help="timed batched runs of the cfg2 leg")
     ap.add_argument("--sample-shots", type=int, default=1024,
                     help="also time QuantumState.sample over all 64 qubits with this many shots (cfg1; 0: skip)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -112,6 +258,15 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    def log(msg):
+        print(f"# {msg}", file=sys.stderr, flush=True)
+
+    # the CPU baseline first, before this process touches the GPU
+    cpu_one = cpu_pool = None
+    if args.config == 1 and args.cpu_seconds > 0:
+        log("CPU baseline (oracle)")
+        cpu_one, cpu_pool = cpu_baseline(args.cpu_seconds, args.cpu_workers)
+
     import numpy as np
     import torch
 
@@ -124,7 +279,8 @@ def main():
     if args.config == 3:
         return bench_cfg3(args)
     if args.config == 2:
-        return bench_cfg2(args)
+        print(json.dumps(cfg2_compare(args.circuits or 256, args.cfg2_steps, log)), flush=True)
+        return
     if args.config == 4:
         return bench_cfg4(args)
     cfg = WL.config(1)
@@ -141,29 +297,47 @@ def main():
     torch.cuda.synchronize()
     bits_p = C.cast(d_bits.data_ptr(), C.POINTER(C.c_uint64))
     out_p = C.cast(d_out.data_ptr(), C.POINTER(C.c_double))
+    ts = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    # the library enqueues on PyTorch's stream: no cross-stream events per call
+    _lib.check(L.nx_set_stream(ts, 1))
+    d_norm = torch.zeros(1, dtype=torch.float64, device=dev)
+    norm_p = C.cast(d_norm.data_ptr(), C.POINTER(C.c_double))
     norm = C.c_double()
 
     def step():
         _lib.check(L.nx_state_drop_cache(st._h))
-        _lib.check(L.nx_state_amplitudes_device(st._h, B, bits_p, 1, out_p))
+        _lib.check(L.nx_state_amplitudes_async(st._h, B, bits_p, 1, out_p, ts))
+        _lib.check(L.nx_state_norm_async(st._h, norm_p, ts))
+
+    def step_sync():
+        _lib.check(L.nx_state_drop_cache(st._h))
+        _lib.check(L.nx_state_amplitudes_async(st._h, B, bits_p, 1, out_p, ts))
         _lib.check(L.nx_state_norm(st._h, C.byref(norm)))
 
     for _ in range(args.warmup):
         step()
+        step_sync()
     torch.cuda.synchronize()
+    assert float(d_norm.item()) == norm.value, (float(d_norm.item()), norm.value)
+    ts0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    ms_step_async = (time.perf_counter() - ts0) * 1e3 / args.steps
     _lib.check(L.nx_profile_enable(1))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step()
+        step_sync()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     prof = {}
-    for k in ("k_amplitude", "k_pairs", "k_qform"):
+    for k in ("k_amplitude", "k_pairs", "k_qform", "k_amp_reduce", "k_reduce"):
         ms, cnt = C.c_double(), C.c_int64()
         _lib.check(L.nx_profile_get(k.encode(), C.byref(ms), C.byref(cnt)))
         prof[k] = (ms.value / max(cnt.value, 1), cnt.value)
     _lib.check(L.nx_profile_enable(0))
+    amp = d_out.view(torch.complex128).cpu().numpy()
     if args.copy_probe:
         src = torch.empty(1 << 27, dtype=torch.float64, device=dev)
         dst = torch.empty_like(src)
@@ -172,10 +346,10 @@ def main():
 
     sampling = None
     if args.sample_shots > 0:
-        ts = time.perf_counter()
+        ts0 = time.perf_counter()
         counts = st.sample(list(range(n)), args.sample_shots, seed=cfg["sample_seed"])
-        te = time.perf_counter() - ts
-        # again with per-launch timing (synchronizing) for the pass counts
+        te = time.perf_counter() - ts0
+        # again with per-launch timing for the pass counts
         _lib.check(L.nx_profile_enable(1))
         counts2 = st.sample(list(range(n)), args.sample_shots, seed=cfg["sample_seed"])
         assert counts2 == counts
@@ -194,53 +368,42 @@ def main():
                     "uniform_pass_ms": sms.value, "projection_passes": pcnt.value, "pass_kernel_ms": pms.value,
                     "dfs_nodes": nodes}
 
+    cfg2 = cfg2_compare(args.circuits, args.cfg2_steps, log) if args.circuits > 0 else None
+
     ms_step = (t1 - t0) * 1e3 / args.steps
     amp_units = chi * B
     pair_units = chi * (chi + 1) // 2
     value = (amp_units + pair_units) / (ms_step / 1e3)
-    amp_ms, pair_ms = prof["k_amplitude"][0], prof["k_pairs"][0]
+    amp_ms, pair_ms, qf_ms = prof["k_amplitude"][0], prof["k_pairs"][0], prof["k_qform"][0]
     amp_rate = amp_units / (amp_ms / 1e3) if amp_ms > 0 else None
     pair_rate = pair_units / (pair_ms / 1e3) if pair_ms > 0 else None
-    if pair_ms >= amp_ms:
-        kern, per_unit, units, kms = "k_pairs", ops_ip(n), pair_units, pair_ms
-    else:
-        kern, per_unit, units, kms = "k_amplitude", ops_amp(n), amp_units, amp_ms
-    achieved = per_unit * units / (kms / 1e3) / 1e12
-    ctr = read_counters(kern)
-    roofline = {"bound": "valu", "kernel": kern, "achieved": achieved, "peak": PEAK_VALU_TOPS,
-                "unit": "TOP/s (int32 lane-ops)", "frac": achieved / PEAK_VALU_TOPS,
-                "ops_per_unit": per_unit, "units_per_launch": units, "avg_launch_ms": kms,
-                "ops_basis": ("6 n^2 ceil(n/32) lane-ops per pair: GF(2) elimination of the composed n x n "
-                              "matrices" if kern == "k_pairs" else
-                              "12 ceil(n/2) ceil(n/32) + 4 ceil(n/32) lane-ops per evaluation: X applied bit "
-                              "by bit"),
-                "traffic": ctr.get("hbm_bytes_per_launch")}
-    # what the kernel executes (rocprofv3 counters of the same launch): the
-    # exponential-sum elimination needs fewer operations than ops_basis
-    # counts, so frac can exceed 1; these fractions are of the issue peaks
-    if "valu_wave_instr_per_unit" in ctr:
-        valu = ctr["valu_wave_instr_per_unit"] * 64.0
-        salu = ctr.get("salu_wave_instr_per_unit", 0.0)
-        rate = units / (kms / 1e3)
-        roofline["executed"] = {
-            "valu_lane_ops_per_unit": valu, "salu_instr_per_unit": salu,
-            "valu_frac": valu * rate / (PEAK_VALU_TOPS * 1e12),
-            "salu_issue_frac": salu * rate / (256 * 2.4e9)}
+    dom = "k_pairs" if pair_ms >= amp_ms else "k_amplitude"
+    roofline = roofline_for(dom, pair_units if dom == "k_pairs" else amp_units, max(pair_ms, amp_ms))
+    other = roofline_for("k_amplitude" if dom == "k_pairs" else "k_pairs",
+                         amp_units if dom == "k_pairs" else pair_units, min(pair_ms, amp_ms))
 
     cpu = None
-    if args.cpu_seconds > 0:
-        cb = cpu_baseline(cfg, args.cpu_seconds)
-        cpu_step_s = amp_units * cb["t_amp"] + pair_units * cb["t_ip"]
+    if cpu_one is not None:
+        t_amp = cpu_one["t_amp"] / max(cpu_one["amp_evals"], 1)
+        t_ip = cpu_one["t_ip"] / max(cpu_one["pairs"], 1)
+        cpu_step_s = amp_units * t_amp + pair_units * t_ip
         cpu = {"value": (amp_units + pair_units) / cpu_step_s, "unit": "CH-form evals/s", "cores": 1,
-               "kind": "port", "sample": cb["sample"],
-               "amplitude_evals_per_s": 1.0 / cb["t_amp"], "inner_products_per_s": 1.0 / cb["t_ip"],
-               "compile_s": cb["compile_s"]}
+               "kind": "port",
+               "sample": f"{cpu_one['strings']} strings x {chi} terms amplitudes + {cpu_one['pairs']} pairs of the "
+                         f"cfg1 state, oracle (C restatement of the reference algorithm), {args.cpu_seconds:g} s",
+               "amplitude_evals_per_s": 1.0 / t_amp, "inner_products_per_s": 1.0 / t_ip,
+               "compile_s": cpu_one["compile_s"], "host": host_info()}
+        if cpu_pool is not None:
+            pool_step = amp_units / cpu_pool["amp_rate"] + pair_units / cpu_pool["ip_rate"]
+            cpu["pool"] = {"workers": args.cpu_workers, "value": (amp_units + pair_units) / pool_step,
+                           "amplitude_evals_per_s": cpu_pool["amp_rate"], "inner_products_per_s": cpu_pool["ip_rate"],
+                           "note": "term/string-parallel upper bound; the reference itself is single-threaded"}
         if sampling is not None:
             # extrapolated, not run: 3 norms of chi(chi+1)/2 pairs per DFS node
-            # at the measured oracle pair rate
-            cpu["sampling_estimate_s"] = sampling["dfs_nodes"] * 3 * pair_units * cb["t_ip"]
+            cpu["sampling_estimate_s"] = sampling["dfs_nodes"] * 3 * pair_units * t_ip
             cpu["sampling_estimate"] = "dfs_nodes x 3 norms x chi(chi+1)/2 pairs / oracle inner products per s"
 
+    kernel_sum = sum(v[0] * v[1] for v in prof.values()) / args.steps  # every kernel of the step
     line = {
         "metric": METRIC, "value": value, "unit": "CH-form evals/s (amplitude evals + inner products)",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
@@ -249,92 +412,14 @@ def main():
         "config": {"workload": "cfg1: n=64, 2000 Clifford + 24 T gates, 65536 amplitude strings + full norm",
                    "n": n, "chi": chi, "bitstrings": B, "pairs": pair_units, "cliffords": 2000, "t": 24},
         "amplitude_evals_per_s": amp_rate, "inner_products_per_s": pair_rate,
-        "kernel_ms": {k: v[0] for k, v in prof.items()},
-        "norm": norm.value, "compile_s": compile_s,
-        "batched_vs_plain": None, "sampling": sampling,
-        "roofline": roofline, "cpu_baseline": cpu,
+        "kernel_ms": {k: v[0] for k, v in prof.items()}, "kernel_sum_ms": kernel_sum,
+        "step_overhead_frac": (ms_step - kernel_sum) / ms_step,
+        "ms_per_step_async": ms_step_async,
+        "norm": norm.value, "sum_abs2_amplitudes": float(np.sum(np.abs(amp) ** 2)), "compile_s": compile_s,
+        "batched_vs_plain": cfg2["batched_vs_plain"] if cfg2 else None, "cfg2": cfg2,
+        "sampling": sampling, "roofline": roofline, "roofline_other": other, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
-
-
-def pauli_text(ops):
-    return "".join("IXYZ"[x] for x in reversed(ops))
-
-
-def bench_cfg2(args):
-    """Batching workload: 256 circuits, n ~ U[16,128], 30 n Cliffords,
-    t ~ U[4,32], each with its norm and 64 Pauli expectations (weight 2-6).
-    Batched: one necstar.Batch (autotuned, HIP graph) evaluates all of them
-    per run.  Plain: the reference's per-call API, QuantumState.norm() and
-    exp_value(P) one string at a time, on every circuit for the first
-    --plain-paulis strings; its per-circuit time is norm + 64 x (mean string
-    time).  Both legs run in the same process on the same device-resident
-    states."""
-    import ctypes as C
-    import numpy as np
-    import workloads as WL
-    from necstar import Batch, PauliString, QuantumState
-    from necstar import _lib
-    L = _lib.lib()
-    cfg = WL.config(2)
-    circs = cfg["circuits"][: args.circuits] if args.circuits else cfg["circuits"]
-    t0 = time.perf_counter()
-    states = [QuantumState.from_circuit(WL.to_circuit(c["n"], c["gates"])) for c in circs]
-    compile_s = time.perf_counter() - t0
-    paulis = [[PauliString.from_str(pauli_text(p)) for p in c["paulis"]] for c in circs]
-    chis = np.array([s.stabilizer_rank for s in states])
-    ns = np.array([c["n"] for c in circs])
-    print(f"# cfg2: {len(states)} circuits compiled in {compile_s:.1f} s, chi max {chis.max()}, "
-          f"sum chi^2/2 {float((chis.astype(float) ** 2).sum() / 2):.3e}", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    b = Batch(states, paulis, autotune=True, graph=True)
-    create_s = time.perf_counter() - t0
-    J, waves, vw = b.shape
-    for _ in range(args.warmup):
-        b.run()
-    _lib.check(L.nx_profile_enable(1))
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        norms, vals = b.run()
-    t1 = time.perf_counter()
-    ms, cnt = C.c_double(), C.c_int64()
-    _lib.check(L.nx_profile_get(b"batch_pairs", C.byref(ms), C.byref(cnt)))
-    _lib.check(L.nx_profile_enable(0))
-    batched_s = (t1 - t0) / args.steps
-    graph_ms = ms.value / max(cnt.value, 1)
-    print(f"# batched run {batched_s:.3f} s (J={J}, waves={waves})", file=sys.stderr, flush=True)
-    # plain per-call path
-    K = max(1, min(args.plain_paulis, 64))
-    plain_s, maxdiff = 0.0, 0.0
-    t_start = time.perf_counter()
-    for i, (st, pl) in enumerate(zip(states, paulis)):
-        ta = time.perf_counter()
-        nrm = st.norm()
-        tb = time.perf_counter()
-        ev = [st.exp_value(p) for p in pl[:K]]
-        tc = time.perf_counter()
-        plain_s += (tb - ta) + (tc - tb) / K * len(pl)
-        maxdiff = max(maxdiff, abs(nrm - norms[i]) / max(1.0, abs(nrm)),
-                      float(np.max(np.abs(np.array(ev) - vals[i][:K]) / np.maximum(1.0, np.abs(ev)))))
-    plain_wall = time.perf_counter() - t_start
-    nc = len(states)
-    pairs = float((chis.astype(float) * (chis + 1) / 2).sum())
-    print(json.dumps({
-        "metric": "cfg2 batched vs plain, circuits/s (norm + 64 Pauli expectations per circuit)",
-        "value": nc / batched_s, "unit": "circuits/s", "higher_is_better": True,
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "batched_s": batched_s, "batched_graph_ms": graph_ms, "plain_s_est": plain_s,
-        "plain_circuits_per_s": nc / plain_s, "batched_vs_plain": plain_s / batched_s,
-        "plain_sample": f"norm + {K} of 64 strings per circuit timed ({plain_wall:.1f} s wall), "
-                        f"string time scaled to 64",
-        "max_rel_diff_batched_vs_plain": maxdiff,
-        "pair_evaluations_per_run": pairs, "pair_string_values_per_s": pairs * 65 / batched_s,
-        "shape": {"J": J, "waves": waves, "variant_words": vw}, "compile_s": compile_s, "batch_create_s": create_s,
-        "config": {"workload": "cfg2: 256 circuits, n U[16,128], 30n Cliffords, t U[4,32], norm + 64 Paulis "
-                               "(weight 2-6) each; sampling not included",
-                   "circuits": nc, "chi_max": int(chis.max()), "chi_mean": float(chis.mean()),
-                   "n_mean": float(ns.mean())},
-    }), flush=True)
 
 
 def bench_cfg4(args):
@@ -342,7 +427,6 @@ def bench_cfg4(args):
     gates, <x|psi> for 2^16 random 128-bit strings with inputs and outputs
     resident in HBM.  One step is one pass over all chi x 2^16 (term, string)
     evaluations."""
-    import ctypes as C
     import numpy as np
     import torch
     import workloads as WL

@@ -45,6 +45,15 @@ const char *nx_last_error(void);
  * and checks it is a gfx950.  No reference counterpart. */
 int nx_init(int device);
 
+/* use != 0: later work of the calling thread on its current device goes to
+ * `stream` (a hipStream_t; NULL is the null stream) instead of the
+ * library's own non-blocking stream, so that it is ordered with the
+ * caller's other work on that stream without events; use == 0: back to the
+ * library's stream.  Work already enqueued is waited for first.  Batch
+ * graphs are still captured on the library's stream.  No reference
+ * counterpart. */
+int nx_set_stream(void *stream, int use);
+
 /* QuantumState::from_circuit -> StabDecompCompiler::compile
  * (crates/necstar/src/state/mod.rs:98-102, state/compiler/mod.rs:44-140). */
 int nx_state_compile(int num_qubits, int64_t ngates, const int32_t *gates, nx_state **out);
@@ -64,6 +73,12 @@ int nx_state_apply_gates(nx_state *s, int64_t ngates, const int32_t *gates);
 /* QuantumState::norm (state/mod.rs:702-706, norm.rs:8-35). */
 int nx_state_norm(const nx_state *s, double *out);
 int nx_state_norm_squared(const nx_state *s, double *out);
+
+/* nx_state_norm into device memory d_out[0] (8 bytes), asynchronously: the
+ * same ordering against `stream` as nx_state_amplitudes_async, and the same
+ * value as nx_state_norm.  No reference counterpart (for device-resident
+ * pipelines; norm() itself returns to the host). */
+int nx_state_norm_async(const nx_state *s, double *d_out, void *stream);
 
 /* nx_state_norm that allocates nothing: no reference counterpart; for
  * callers that cannot allocate (tight serving loops, harnesses that forbid
@@ -99,6 +114,15 @@ int nx_state_amplitudes(const nx_state *s, int64_t nbits, const uint64_t *bits, 
 /* Same with device pointers (inputs already resident in HBM). */
 int nx_state_amplitudes_device(const nx_state *s, int64_t nbits, const uint64_t *d_bits, int32_t words,
                                double *d_out);
+
+/* The same, asynchronously: the pass is enqueued on the library's stream
+ * after the work already enqueued on `stream` (a hipStream_t, e.g. PyTorch's
+ * current stream; NULL is the null stream), and later work on `stream` is
+ * ordered after it; the call returns without waiting.  Results are also
+ * complete when any later call that returns host data (norm, exp_value, ...)
+ * returns.  No reference counterpart. */
+int nx_state_amplitudes_async(const nx_state *s, int64_t nbits, const uint64_t *d_bits, int32_t words,
+                              double *d_out, void *stream);
 
 /* QuantumState::to_statevector (statevector.rs:12-24): out complex128 [2^n]. */
 int nx_state_statevector(const nx_state *s, double *out);

@@ -49,6 +49,7 @@ struct QFormDev {
   uint64_t *alo, *ahi, *rhs; // [chi][W]
   int32_t *nc, *k;           // [chi]
   int32_t* p0;               // [chi] (int32: read with scalar loads)
+  int4* meta;                // [chi] {nc, k, p0, 0}: one scalar load per term in k_pairs
   int64_t chi;
   int n, W;
 };
@@ -159,11 +160,19 @@ hipError_t launch_tail_strings(const uint64_t* base, const int32_t* qs, int d, i
 hipError_t launch_tail_leaves(const double* al, int64_t nv, double* ab, hipStream_t st);
 hipError_t launch_amplitude_exact(const QFormDev& q, const int32_t* cs, const uint64_t* bits, int64_t nb, int nchunks,
                                   const double* tabs, double* xout, hipStream_t st);
-hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, const int64_t* offsets, int64_t nfixed,
-                        int64_t nother, int triangle, int J, int mode, const CoefDev& cx, const CoefDev& cy,
-                        const double* tabs, double2* partial, int nblocks, uint32_t* exact_out, int64_t exact_ld,
-                        hipStream_t st);
-hipError_t launch_reduce(const double2* partial, int64_t n, double2* out, hipStream_t st);
+// Pair sums over units of J = 8 consecutive other indices o of a fixed
+// index f (triangle: o in [f, nother), nfixed == nother, rows from the
+// closed form below; else o in [0, nother)), four waves per block, nblocks
+// blocks; per block partial[block] = {A_re, A_im, B_re, B_im}: the sum is
+// A + sqrt(2) B, accumulated exactly (powers of two) for Scalar
+// coefficients, B = 0 for complex128 coefficients
+hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, int64_t nfixed, int64_t nother, int triangle, int J,
+                        int mode, const CoefDev& cx, const CoefDev& cy, const double* tabs, double4* partial,
+                        int nblocks, uint32_t* exact_out, int64_t exact_ld, hipStream_t st);
+// sum of n double4 partials in a fixed order (one block) into out (if not
+// null); with norm_out, sqrt((A_re + sqrt2 B_re) g2) into norm_out[0]
+hipError_t launch_reduce(const double4* partial, int64_t n, double4* out, hipStream_t st,
+                         double* norm_out = nullptr, double g2 = 1.0);
 
 // Work units of the pair triangle {(f, o) : f <= o < chi} cut into units of
 // J consecutive o: S(M) = sum_{m=1..M} ceil(m / J) = J q (q + 1) / 2 +
@@ -173,8 +182,6 @@ __host__ __device__ inline int64_t tri_units_upto(int64_t M, int J) {
   const int64_t q = M / J, r = M % J;
   return (int64_t)J * q * (q + 1) / 2 + r * (q + 1);
 }
-// off[f] = tri_units_upto(chi) - tri_units_upto(chi - f) for f = 0 .. chi
-hipError_t launch_tri_offsets(int64_t chi, int J, int64_t* off, hipStream_t st);
 
 // Pauli variants of every term of `in`: variant k applies the ops
 // vops[voff[k] .. voff[k+1]) (q << 2 | op, op 1 = X, 2 = Y, 3 = Z) in order.

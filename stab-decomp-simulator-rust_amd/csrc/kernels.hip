@@ -356,6 +356,7 @@ __global__ __launch_bounds__(256) void k_qform(FormsDev in, QFormDev q) {
     q.nc[t] = nc;
     q.k[t] = k;
     q.p0[t] = f.ph;
+    q.meta[t] = make_int4(nc, k, f.ph, 0);
   }
 }
 
@@ -876,12 +877,12 @@ __global__ void k_amp_reduce(const double2* __restrict__ part, int64_t nb, int n
   out[b] = make_double2(r * gre - i * gim, r * gim + i * gre);
 }
 
-// Coefficients: exact Scalars (ck, cr, cz) when scalar != 0, else c.
+// Coefficients: exact Scalars when cx.scalar != 0, else complex128 values.
 // Work decomposition: fixed index f in [0, nfixed) pairs with the other
 // indices o in [lo_f, nother), lo_f = f (triangle) or 0 (full), cut into
-// units of J; offsets[f] is the first unit of row f (prefix sum on the
-// host).  mode bit 0: fixed index is the ket (else the bra); mode bit 1:
-// Hermitian real sum (diagonal once, off-diagonal twice, norm.rs:16-27 and
+// units of J; the row of a unit comes from a closed form (unit_row).  mode
+// bit 0: fixed index is the ket (else the bra); mode bit 1: Hermitian real
+// sum (diagonal once, off-diagonal twice, norm.rs:16-27 and
 // exp_value.rs:38-58) else the complex sum (inner_product.rs:12-17).
 // 2^{-r/2}: the host table entry pow(2, -r/2), formed exactly as
 // ldexp(pow(2, -1/2) or 1, -floor(r/2)) (equal to the table for every |r| <=
@@ -890,21 +891,73 @@ __device__ __forceinline__ double pow_half(int r) {
   return ldexp((r & 1) ? 0x1.6a09e667f3bcdp-1 : 1.0, -(r >> 1));
 }
 
+// Row f of unit u and the first unit of that row (wave-uniform).  Full
+// rows hold U = ceil(nother / J) units each.  Triangle rows (nfixed ==
+// nother = N) hold ceil((N - f) / J) units starting at S(N) - S(N - f), S =
+// tri_units_upto: with t = S(N) - u, m = N - f is the smallest m with
+// S(m) >= t.  S(m) = m (m + J) / (2J) + r (J - r) / (2J) (m = qJ + r), so the
+// root of m (m + J) = 2 J t is at most one above m; the loops settle it.
+// units of J = PAIR_UNIT pairs (a compile-time power of two: the unit
+// arithmetic is shifts and masks)
+constexpr int PAIR_UNIT = 8, PAIR_UNIT_LOG = 3;
+__device__ __forceinline__ int64_t tri_units8(int64_t M) {
+  const int64_t q = M >> PAIR_UNIT_LOG, r = M & (PAIR_UNIT - 1);
+  return ((q * (q + 1)) << (PAIR_UNIT_LOG - 1)) + r * (q + 1);
+}
+__device__ __forceinline__ int64_t unit_row(int64_t u, int64_t nother, int triangle, int64_t* first) {
+  if (!triangle) {
+    const int64_t U = (nother + PAIR_UNIT - 1) >> PAIR_UNIT_LOG;
+    int64_t f = (int64_t)((double)u / (double)U);
+    f = (int64_t)uni64((uint64_t)f);
+    while (f * U > u) --f;
+    while ((f + 1) * U <= u) ++f;
+    *first = f * U;
+    return f;
+  }
+  const int64_t SN = tri_units8(nother);
+  const int64_t t = SN - u;
+  const double dJ = (double)PAIR_UNIT;
+  int64_t m = (int64_t)ceil(0.5 * (sqrt(dJ * dJ + 8.0 * dJ * (double)t) - dJ));
+  m = (int64_t)uni64((uint64_t)m);
+  if (m > nother) m = nother;
+  if (m < 1) m = 1;
+  while (m > 1 && tri_units8(m - 1) >= t) --m;
+  while (tri_units8(m) < t) ++m;
+  *first = SN - tri_units8(m);
+  return nother - m;
+}
+
+// Re of zeta^K 2^{-R/2} (times 2^x2) added exactly to A (dyadic part) or B
+// (sqrt 2 part): cos(K pi/4) is +-1, +-sqrt2/2 or 0 and 2^{-R/2} is 2^{-R/2}
+// or sqrt2 2^{-(R+1)/2}, so the term is +-2^e in A when K and R have the
+// same parity, in B otherwise, with e = -ceil(R/2) - (K odd, R even).
+__device__ __forceinline__ void add_re_exact(int K, int R, int x2, double& A, double& B) {
+  K &= 7;
+  if ((K & 3) == 2) return;  // cos(pi/2), cos(3pi/2)
+  const bool neg = K >= 3 && K <= 5;
+  const int kodd = K & 1, rodd = R & 1;
+  const int e = -((R + 1) >> 1) - (kodd & (rodd ^ 1)) + x2;
+  const double v = ldexp(neg ? -1.0 : 1.0, e);
+  if (kodd == rodd) A = A + v;
+  else B = B + v;
+}
+
 // SAME: X and Y (and cx and cy) are the same state's forms and coefficients
 // (norm, exp_value): one set of table pointers is kept in scalar registers,
 // which leaves room for 8 waves per SIMD at W = 1.
 // MODE: the mode argument when it is 0..3 at compile time (no per-pair role
 // selects), or -1 to read it at run time.
 template <int W, bool SAME, int MODE>
-__global__ __launch_bounds__(256, (SAME && W == 1) ? 8 : 1) void k_pairs(QFormDev X, QFormDev Y, const int64_t* __restrict__ offsets,
+__global__ __launch_bounds__(256, (SAME && W == 1) ? 8 : 1) void k_pairs(QFormDev X, QFormDev Y,
                                                int64_t nfixed, int64_t nother, int triangle, int J, int mode_rt,
                                                CoefDev cx, CoefDev cy, const double* __restrict__ tabs,
-                                               double2* __restrict__ partial, uint32_t* __restrict__ exact_out,
+                                               double4* __restrict__ partial, uint32_t* __restrict__ exact_out,
                                                int64_t exact_ld) {
   const int lane = threadIdx.x & 63;
   const int64_t wave = (int64_t)blockIdx.x * 4 + uni((int)(threadIdx.x >> 6));  // uniform in the wave
   const int64_t nwaves = (int64_t)gridDim.x * 4;
-  const int64_t nunits = offsets[nfixed];
+  const int64_t nunits =
+      triangle ? tri_units8(nother) : nfixed * ((nother + PAIR_UNIT - 1) >> PAIR_UNIT_LOG);
   const int n = X.n;
   u64 valid[W];
 #pragma unroll
@@ -915,32 +968,28 @@ __global__ __launch_bounds__(256, (SAME && W == 1) ? 8 : 1) void k_pairs(QFormDe
   const int mode = MODE >= 0 ? MODE : mode_rt;
   const bool fixed_is_ket = mode & 1;
   const bool herm = (mode >> 1) & 1;
-  // cos, sin of k pi / 4 (the host tables) in LDS: read with a lane address
-  // instead of a scalar load at the end of each pair
+  const bool exact = cx.scalar != 0;
+  // cos, sin of k pi / 4 (the host tables) in LDS, for complex128
+  // coefficients: read with a lane address instead of a scalar load
   __shared__ double s_cs[16];
   if (threadIdx.x < 16) s_cs[threadIdx.x] = tabs[TAB_COS + threadIdx.x];
   __syncthreads();
-  double acc_re = 0.0, acc_im = 0.0;
+  double a_re = 0.0, a_im = 0.0, b_re = 0.0, b_im = 0.0;
   for (int64_t u = wave; u < nunits; u += nwaves) {
-    // row f with offsets[f] <= u < offsets[f+1]
-    int64_t lo_r = 0, hi_r = nfixed;
-    while (hi_r - lo_r > 1) {
-      const int64_t mid = (lo_r + hi_r) >> 1;
-      if (offsets[mid] <= u) lo_r = mid;
-      else hi_r = mid;
-    }
+    int64_t first = 0;
     // term indices fit 32 bits (the forms' 32-bit row offsets already
     // require chi * 64W * W < 2^32)
-    const int fx = (int)lo_r;
+    const int fx = (int)unit_row(u, nother, triangle, &first);
     const int base = triangle ? fx : 0;
-    const int o_lo = base + (int)(u - offsets[fx]) * J;
-    const int o_hi = (o_lo + J < (int)nother) ? o_lo + J : (int)nother;
+    const int o_lo = base + ((int)(u - first) << PAIR_UNIT_LOG);
+    const int o_hi = (o_lo + PAIR_UNIT < (int)nother) ? o_lo + PAIR_UNIT : (int)nother;
     const QFormDev& qf = (SAME || !fixed_is_ket) ? X : Y;
     const QFormDev& qo = (SAME || fixed_is_ket) ? X : Y;
     u64 Bf[W][W], Cf[W][W], rf[W];
     int af[W];
     load_qrows<W>(qf, fx, lane, Bf, af, Cf, rf);
-    const int ncf = uni(qf.nc[fx]), kf = uni(qf.k[fx]), pf = uni(qf.p0[fx]);
+    const int4 mf = qf.meta[fx];
+    const int ncf = uni(mf.x), kf = uni(mf.y), pf = uni(mf.z);
     const CoefDev& cf = (SAME || !fixed_is_ket) ? cx : cy;
     const CoefDev& co = (SAME || fixed_is_ket) ? cx : cy;
     const int32_t csf = cf.scalar ? uni(cf.cs[fx]) : 0;
@@ -949,7 +998,8 @@ __global__ __launch_bounds__(256, (SAME && W == 1) ? 8 : 1) void k_pairs(QFormDe
       u64 Bo[W][W], Co[W][W], ro[W];
       int ao[W];
       load_qrows<W>(qo, o, lane, Bo, ao, Co, ro);
-      const int nco = uni(qo.nc[o]), ko = uni(qo.k[o]), po = uni(qo.p0[o]);
+      const int4 mo = qo.meta[o];
+      const int nco = uni(mo.x), ko = uni(mo.y), po = uni(mo.z);
       // the other term's coefficient, read now and used after the elimination
       const int32_t cso = co.scalar ? uni(co.cs[o]) : 0;
       const double2 cvo = co.scalar ? make_double2(0.0, 0.0) : co.c[o];
@@ -987,73 +1037,102 @@ __global__ __launch_bounds__(256, (SAME && W == 1) ? 8 : 1) void k_pairs(QFormDe
       pair_eval<W>(P, nca, ncb, &zero, &K, &Rr);
       if (exact_out && lane == 0)
         exact_out[(size_t)bra * exact_ld + ket] = zero ? 0xFFFFFFFFu : ((uint32_t)K | ((uint32_t)Rr << 8));
-      if (!zero) {
-        // ip = omega_a * omega_b * Scalar::to_complex (inner_product.rs:44-47)
+      if (zero) continue;
+      if (exact) {
+        // c_bra* c_ket <bra|ket> = zeta^{K'} 2^{-R'/2}, summed exactly
+        const int32_t sb = fixed_is_ket ? cso : csf, sk = fixed_is_ket ? csf : cso;
+        if ((sb | sk) < 0) continue;  // a Scalar::Zero coefficient
+        const int Kp = K + cs_k(sk) - cs_k(sb);
+        const int Rp = Rr + cs_r(sb) + cs_r(sk);
+        if (herm) {
+          add_re_exact(Kp, Rp, bra == ket ? 0 : 1, a_re, b_re);
+        } else {
+          add_re_exact(Kp, Rp, 0, a_re, b_re);
+          add_re_exact(Kp - 2, Rp, 0, a_im, b_im);  // Im zeta^K = Re zeta^{K-2}
+        }
+      } else {
+        // (c_bra.conj() * c_ket).into() * <bra|ket> as num-complex forms it
+        // (inner_product.rs:44-47: omega_a * omega_b = 1)
         const double nrm = pow_half(Rr);
         const double ir = s_cs[K] * nrm, ii = s_cs[8 + K] * nrm;
-        // (c_bra.conj() * c_ket).into()
-        double cr_, ci_;
-        if (cx.scalar) {
-          const int32_t sb = fixed_is_ket ? cso : csf, sk = fixed_is_ket ? csf : cso;
-          if ((sb | sk) < 0) { cr_ = 0.0; ci_ = 0.0; }
-          else {
-            const int kk = (cs_k(sk) - cs_k(sb)) & 7;
-            const double nn = pow_half(cs_r(sb) + cs_r(sk));
-            cr_ = s_cs[kk] * nn;
-            ci_ = s_cs[8 + kk] * nn;
-          }
-        } else {
-          const double2 b0 = fixed_is_ket ? cvo : cvf, k0 = fixed_is_ket ? cvf : cvo;
-          const double br = b0.x, bi = -b0.y;
-          cr_ = br * k0.x - bi * k0.y;
-          ci_ = br * k0.y + bi * k0.x;
-        }
+        const double2 b0 = fixed_is_ket ? cvo : cvf, k0 = fixed_is_ket ? cvf : cvo;
+        const double br = b0.x, bi = -b0.y;
+        const double cr_ = br * k0.x - bi * k0.y, ci_ = br * k0.y + bi * k0.x;
         const double tr = cr_ * ir - ci_ * ii, ti = cr_ * ii + ci_ * ir;
         if (herm) {
-          acc_re = (bra == ket) ? acc_re + tr : acc_re + (tr + tr);
+          a_re = (bra == ket) ? a_re + tr : a_re + (tr + tr);
         } else {
-          acc_re = acc_re + tr;
-          acc_im = acc_im + ti;
+          a_re = a_re + tr;
+          a_im = a_im + ti;
         }
       }
     }
   }
-  if (lane == 0) partial[wave] = make_double2(acc_re, acc_im);
+  // the block's four waves, in wave order, to partial[block]
+  __shared__ double4 s_part[4];
+  if (lane == 0) s_part[threadIdx.x >> 6] = make_double4(a_re, a_im, b_re, b_im);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double4 r = s_part[0];
+#pragma unroll
+    for (int w = 1; w < 4; ++w) {
+      r.x += s_part[w].x;
+      r.y += s_part[w].y;
+      r.z += s_part[w].z;
+      r.w += s_part[w].w;
+    }
+    partial[blockIdx.x] = r;
+  }
 }
 
-// single-block reduction of the per-wave partial sums in a fixed order:
+// single-block reduction of the per-block partial sums in a fixed order:
 // thread i sums partial[i], partial[i + 1024], ... in index order (eight
-// loads in flight per step), then a fixed tree over the threads
-__global__ __launch_bounds__(1024) void k_reduce(const double2* __restrict__ partial, int64_t n,
-                                                 double2* __restrict__ out) {
-  __shared__ double sr[1024], si[1024];
-  double r = 0.0, i = 0.0;
+// loads in flight per step), then a fixed tree over the threads.  With
+// norm_out, also sqrt((A_re + sqrt2 B_re) g2) there (the norm, as the host
+// forms it from the same sums)
+__global__ __launch_bounds__(1024) void k_reduce(const double4* __restrict__ partial, int64_t n,
+                                                 double4* __restrict__ out, double* __restrict__ norm_out,
+                                                 double g2) {
+  __shared__ double4 sh[1024];
+  double4 r = make_double4(0.0, 0.0, 0.0, 0.0);
   int64_t k = threadIdx.x;
   for (; k + 7 * 1024 < n; k += 8 * 1024) {
-    double2 v[8];
+    double4 v[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) v[u] = partial[k + u * 1024];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      r += v[u].x;
-      i += v[u].y;
+      r.x += v[u].x;
+      r.y += v[u].y;
+      r.z += v[u].z;
+      r.w += v[u].w;
     }
   }
   for (; k < n; k += 1024) {
-    r += partial[k].x;
-    i += partial[k].y;
+    const double4 v = partial[k];
+    r.x += v.x;
+    r.y += v.y;
+    r.z += v.z;
+    r.w += v.w;
   }
-  sr[threadIdx.x] = r;
-  si[threadIdx.x] = i;
+  sh[threadIdx.x] = r;
   __syncthreads();
   for (int s = 512; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) {
-      sr[threadIdx.x] += sr[threadIdx.x + s];
-      si[threadIdx.x] += si[threadIdx.x + s];
+      const double4 v = sh[threadIdx.x + s];
+      double4 w = sh[threadIdx.x];
+      w.x += v.x;
+      w.y += v.y;
+      w.z += v.z;
+      w.w += v.w;
+      sh[threadIdx.x] = w;
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[0] = make_double2(sr[0], si[0]);
+  if (threadIdx.x == 0) {
+    if (out) out[0] = sh[0];
+    if (norm_out) norm_out[0] = sqrt((sh[0].x + 1.4142135623730951 * sh[0].z) * g2);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1213,10 +1292,10 @@ hipError_t launch_amp_reduce(const double2* partial, int64_t nb, int nchunks, do
   return hipGetLastError();
 }
 
-hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, const int64_t* offsets, int64_t nfixed,
-                        int64_t nother, int triangle, int J, int mode, const CoefDev& cx, const CoefDev& cy,
-                        const double* tabs, double2* partial, int nblocks, uint32_t* exact_out, int64_t exact_ld,
-                        hipStream_t st) {
+hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, int64_t nfixed, int64_t nother, int triangle, int J,
+                        int mode, const CoefDev& cx, const CoefDev& cy, const double* tabs, double4* partial,
+                        int nblocks, uint32_t* exact_out, int64_t exact_ld, hipStream_t st) {
+  if ((triangle && nfixed != nother) || J != PAIR_UNIT) return hipErrorInvalidValue;
   const bool same = X.B == Y.B && X.C == Y.C && X.a == Y.a && cx.scalar == cy.scalar && cx.cs == cy.cs &&
                     cx.c == cy.c;
   // the combinations the runtime uses: norm (same, fixed bra, Hermitian),
@@ -1224,7 +1303,7 @@ hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, const int64_t* off
   // products (complex); any other mode is read at run time
 #define NX_PAIRS(SM, MD)                                                                                      \
   NX_DISPATCH_W(X.W, k_pairs<WW, SM, MD><<<dim3(nblocks), dim3(256), 0, st>>>(                                \
-                         X, SM ? X : Y, offsets, nfixed, nother, triangle, J, mode, cx, SM ? cx : cy, tabs, partial, \
+                         X, SM ? X : Y, nfixed, nother, triangle, J, mode, cx, SM ? cx : cy, tabs, partial,    \
                          exact_out, exact_ld))
   if (same && mode == 2) {
     NX_PAIRS(true, 2);
@@ -1243,19 +1322,9 @@ hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, const int64_t* off
   return hipGetLastError();
 }
 
-hipError_t launch_reduce(const double2* partial, int64_t n, double2* out, hipStream_t st) {
-  k_reduce<<<dim3(1), dim3(1024), 0, st>>>(partial, n, out);
-  return hipGetLastError();
-}
-
-__global__ void k_tri_offsets(int64_t chi, int J, int64_t* __restrict__ off) {
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f > chi) return;
-  off[f] = tri_units_upto(chi, J) - tri_units_upto(chi - f, J);
-}
-
-hipError_t launch_tri_offsets(int64_t chi, int J, int64_t* off, hipStream_t st) {
-  k_tri_offsets<<<dim3(blocks_for(chi + 1, 256)), dim3(256), 0, st>>>(chi, J, off);
+hipError_t launch_reduce(const double4* partial, int64_t n, double4* out, hipStream_t st, double* norm_out,
+                         double g2) {
+  k_reduce<<<dim3(1), dim3(1024), 0, st>>>(partial, n, out, norm_out, g2);
   return hipGetLastError();
 }
 

@@ -89,6 +89,8 @@ struct DeviceCtx {
   hipStream_t stream = nullptr;
   double* tabs = nullptr;
   unsigned char* scratch = nullptr;  // this device's copy of `scratch`
+  // ordering against a caller's stream (asynchronous entry points)
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
   std::mutex mu;
   std::map<size_t, std::vector<void*>> free_lists;
 };
@@ -115,6 +117,8 @@ static DeviceCtx* ctx() {
   DeviceCtx* c = new DeviceCtx();
   c->dev = t_dev;
   hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming), "hipEventCreate");
+  hip_check(hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming), "hipEventCreate");
   // PhaseFactor::to_complex (phase_factor.rs:19-22) and powf(2, -r/2)
   // (scalar.rs:30-38): computed with the same libm calls on the host
   std::vector<double> t(TAB_SIZE);
@@ -214,7 +218,32 @@ struct PinnedBuf {
   }
 };
 
-static hipStream_t stream() { return ctx()->stream; }
+// The stream the calling thread's work goes to: the device context's own
+// (non-blocking) stream, or the caller's stream set with nx_set_stream.
+static thread_local bool t_user_stream = false;
+static thread_local hipStream_t t_stream = nullptr;
+static hipStream_t stream() {
+  DeviceCtx* c = ctx();
+  return t_user_stream ? t_stream : c->stream;
+}
+
+// Orders the library's work after the caller's earlier work on `us` (and
+// the caller's later work after the library's) when they are different
+// streams; nothing to do when `us` is the library's stream for this thread.
+struct StreamOrder {
+  DeviceCtx* c;
+  hipStream_t us, lib;
+  StreamOrder(void* user) : c(ctx()), us((hipStream_t)user), lib(stream()) {
+    if (us == lib) return;
+    hip_check(hipEventRecord(c->ev_in, us), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(lib, c->ev_in, 0), "hipStreamWaitEvent");
+  }
+  void done() {
+    if (us == lib) return;
+    hip_check(hipEventRecord(c->ev_out, lib), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(us, c->ev_out, 0), "hipStreamWaitEvent");
+  }
+};
 static void sync() { hip_check(hipStreamSynchronize(stream()), "hipStreamSynchronize"); }
 static void h2d(void* d, const void* h, size_t b) {
   if (b) hip_check(hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, stream()), "hipMemcpyAsync H2D");
@@ -322,7 +351,7 @@ struct Forms {
 };
 
 struct QForms {
-  DevBuf Q, B, C, CT, a, alo, ahi, rhs, nc, k, p0;
+  DevBuf Q, B, C, CT, a, alo, ahi, rhs, nc, k, p0, meta;
   int64_t chi = 0;
   int n = 0, W = 1;
   // the pair kernels index rows with 32-bit word offsets
@@ -348,6 +377,7 @@ struct QForms {
     nc.alloc(chi * 4);
     k.alloc(chi * 4);
     p0.alloc(chi * 4);
+    meta.alloc(chi * 16);
   }
   // only the fields the pair kernels read (projected forms)
   void alloc_pair(int64_t chi_, int n_) {
@@ -370,6 +400,7 @@ struct QForms {
     d.a = a.as<uint8_t>();
     d.alo = alo.as<uint64_t>(); d.ahi = ahi.as<uint64_t>(); d.rhs = rhs.as<uint64_t>();
     d.nc = nc.as<int32_t>(); d.k = k.as<int32_t>(); d.p0 = p0.as<int32_t>();
+    d.meta = meta.as<int4>();
     d.chi = chi; d.n = n; d.W = W;
     return d;
   }
@@ -467,6 +498,14 @@ static Forms kron(const Forms& A, const Forms& B) {
   return out;
 }
 
+// quadratic forms of f into q (its buffers are reused when the shape matches)
+static void fill_qform(const Forms& f, QForms& q) {
+  if (q.chi != f.chi || q.n != f.n || !q.meta.p) q.alloc(f.chi, f.n);
+  ProfScope ps("k_qform");
+  hip_check(launch_qform(f.desc(), q.desc(), stream()), "k_qform");
+  ps.end();
+}
+
 static QForms make_qform(const Forms& f) {
   QForms q;
   q.alloc(f.chi, f.n);
@@ -518,9 +557,6 @@ struct nx_state {
   mutable nx::QForms q;
   mutable bool c_ok = false;
   mutable nx::DevBuf d_c, d_cs;
-  // work-unit offsets of the triangle of chi x chi pairs (J = 16), for norms
-  mutable int64_t tri_chi = -1, tri_units = 0;
-  mutable nx::DevBuf d_tri;
 
   int64_t chi() const { return forms.chi; }
   void invalidate() { q_ok = false; c_ok = false; }
@@ -531,7 +567,7 @@ namespace nx {
 
 static const QForms& qform_of(const nx_state* s) {
   if (!s->q_ok) {
-    s->q = make_qform(s->forms);
+    fill_qform(s->forms, s->q);
     s->q_ok = true;
   }
   return s->q;
@@ -572,52 +608,65 @@ static CoefDev coef_of(const nx_state* s) {
 static const int PAIR_J = 8;
 
 
-// work-unit offsets (PAIR_J pairs per unit): row f owns units off[f] ..
-static std::vector<int64_t> pair_offsets(int64_t nfixed, int64_t nother, bool triangle) {
-  std::vector<int64_t> off(nfixed + 1);
-  off[0] = 0;
-  for (int64_t f = 0; f < nfixed; ++f) {
-    const int64_t cnt = triangle ? nother - f : nother;
-    off[f + 1] = off[f] + (cnt + PAIR_J - 1) / PAIR_J;
-  }
-  return off;
+// work units (PAIR_J pairs of one fixed index each) of a pair plan
+static int64_t pair_units(int64_t nfixed, int64_t nother, bool triangle) {
+  return triangle ? tri_units_upto(nother, PAIR_J) : nfixed * ((nother + PAIR_J - 1) / PAIR_J);
 }
 
-// sum over pairs; triangle: o in [f, nother) else [0, nother).  d_off: the
-// offsets already on the device (or null: built here)
+// at most three rounds of resident waves (8 per SIMD on 256 CUs): with
+// 8-pair units a state's pairs spread evenly (64 qubits, 729 terms: 0.197 ms,
+// against 0.215 ms for 16-pair units in 16,384 waves)
+static int pair_blocks(int64_t units) { return (int)std::max<int64_t>(1, std::min<int64_t>((units + 3) / 4, 6144)); }
+
+// the sum A + sqrt(2) B of a reduced partial {A_re, A_im, B_re, B_im}
+static cplx ab_value(const double* r) {
+  const double s2 = 1.4142135623730951;
+  return cplx(r[0] + s2 * r[2], r[1] + s2 * r[3]);
+}
+
+// sum over pairs; triangle: o in [f, nother) (nfixed == nother) else
+// [0, nother)
 static cplx run_pairs(const QForms& X, const QForms& Y, int64_t nfixed, int64_t nother, bool triangle, int mode,
-                      const CoefDev& cx, const CoefDev& cy, uint32_t* d_exact = nullptr, int64_t ld = 0,
-                      const int64_t* d_off = nullptr, int64_t units_known = 0) {
+                      const CoefDev& cx, const CoefDev& cy, uint32_t* d_exact = nullptr, int64_t ld = 0) {
   if (nfixed == 0 || nother == 0) return cplx(0.0, 0.0);
-  const int J = PAIR_J;
-  DevBuf doff;
-  int64_t units = units_known;
-  if (!d_off) {
-    const std::vector<int64_t> off = pair_offsets(nfixed, nother, triangle);
-    units = off[nfixed];
-    doff.alloc(off.size() * 8);
-    h2d(doff.p, off.data(), off.size() * 8);
-    d_off = doff.as<int64_t>();
-  }
-  // at most three rounds of resident waves (8 per SIMD on 256 CUs): with
-  // 8-pair units a state's pairs spread evenly (64 qubits, 729 terms:
-  // 0.197 ms, against 0.215 ms for 16-pair units in 16,384 waves)
-  int nblocks = (int)std::min<int64_t>((units + 3) / 4, 6144);
-  if (nblocks < 1) nblocks = 1;
-  DevBuf partial((size_t)nblocks * 4 * 16);
+  const int nblocks = pair_blocks(pair_units(nfixed, nother, triangle));
+  DevBuf partial((size_t)nblocks * 32);
   // the reduction writes the sum straight into page-locked host memory (one
   // buffer per calling thread): no copy after it
   static thread_local PinnedBuf t_res;
   if (!t_res.p) t_res.alloc(64);
   ProfScope ps("k_pairs");
-  hip_check(launch_pairs(X.desc(), Y.desc(), d_off, nfixed, nother, triangle ? 1 : 0, J, mode, cx, cy,
-                         ctx()->tabs, partial.as<double2>(), nblocks, d_exact, ld, stream()),
+  hip_check(launch_pairs(X.desc(), Y.desc(), nfixed, nother, triangle ? 1 : 0, PAIR_J, mode, cx, cy, ctx()->tabs,
+                         partial.as<double4>(), nblocks, d_exact, ld, stream()),
             "k_pairs");
   ps.end();
-  hip_check(launch_reduce(partial.as<double2>(), (int64_t)nblocks * 4, t_res.as<double2>(), stream()), "k_reduce");
+  hip_check(launch_reduce(partial.as<double4>(), nblocks, t_res.as<double4>(), stream()), "k_reduce");
   sync();
-  const double* h = t_res.as<double>();
-  return cplx(h[0], h[1]);
+  return ab_value(t_res.as<double>());
+}
+
+// norm (norm.rs:32-35) into device memory d_out[0], enqueued on the library
+// stream (no host wait)
+static void norm_enqueue(const nx_state* s, double* d_out) {
+  const double g2 = s->g.real() * s->g.real() + s->g.imag() * s->g.imag();
+  if (s->chi() == 0) {
+    hip_check(hipMemsetAsync(d_out, 0, 8, stream()), "hipMemsetAsync");
+    return;
+  }
+  const QForms& q = qform_of(s);
+  CoefDev c = coef_of(s);
+  const int nblocks = pair_blocks(pair_units(s->chi(), s->chi(), true));
+  DevBuf partial((size_t)nblocks * 32);
+  ProfScope ps("k_pairs");
+  hip_check(launch_pairs(q.desc(), q.desc(), s->chi(), s->chi(), 1, PAIR_J, /*fixed bra, herm*/ 2, c, c, ctx()->tabs,
+                         partial.as<double4>(), nblocks, nullptr, 0, stream()),
+            "k_pairs");
+  ps.end();
+  ProfScope pr("k_reduce");
+  hip_check(launch_reduce(partial.as<double4>(), nblocks, nullptr, stream(), d_out, g2), "k_reduce");
+  pr.end();
+  // `partial` returns to the pool; its next user is ordered after these
+  // launches on the same stream
 }
 
 // norm.rs:8-30
@@ -625,35 +674,25 @@ static double norm_squared(const nx_state* s) {
   if (s->chi() == 0) return 0.0 * std::norm(s->g);
   const QForms& q = qform_of(s);
   CoefDev c = coef_of(s);
-  if (s->tri_chi != s->chi()) {
-    const std::vector<int64_t> off = pair_offsets(s->chi(), s->chi(), true);
-    s->d_tri.alloc(off.size() * 8);
-    h2d(s->d_tri.p, off.data(), off.size() * 8);
-    s->tri_chi = s->chi();
-    s->tri_units = off.back();
-  }
-  cplx sum = run_pairs(q, q, s->chi(), s->chi(), true, /*fixed bra, herm*/ 2, c, c, nullptr, 0,
-                       s->d_tri.as<int64_t>(), s->tri_units);
+  cplx sum = run_pairs(q, q, s->chi(), s->chi(), true, /*fixed bra, herm*/ 2, c, c);
   return sum.real() * (s->g.real() * s->g.real() + s->g.imag() * s->g.imag());
 }
 
 // The same sum with every device buffer carved out of this device's copy of
 // `scratch` (no allocation of any kind): the quadratic forms of all terms,
-// the coefficient copies, the unit offsets (closed form, on the device), the
-// per-wave partial sums and the reduced result.  The kernels, launch shape
+// the coefficient copies, the per-wave partial sums and the reduced result.  The kernels, launch shape
 // and summation order are those of norm_squared, so the two agree bit for
 // bit.  Calls on one GPU share its scratch: calls that overlap on one GPU
 // (from several threads) race.
 static double norm_squared_noalloc(const nx_state* s) {
   DeviceCtx* c = ctx();
-  const hipStream_t st = c->stream;
+  const hipStream_t st = stream();
   const int64_t chi = s->chi();
   const double g2 = s->g.real() * s->g.real() + s->g.imag() * s->g.imag();
   if (chi == 0) return 0.0 * g2;
   const int W = s->forms.W, n = s->n;
   const size_t R = 64 * (size_t)W;
-  const int64_t units = tri_units_upto(chi, PAIR_J);
-  const int nblocks = (int)std::max<int64_t>(1, std::min<int64_t>((units + 3) / 4, 6144));
+  const int nblocks = pair_blocks(pair_units(chi, chi, true));
   // layout (256-byte aligned pieces)
   size_t o = 0;
   auto take = [&](size_t bytes) {
@@ -665,9 +704,9 @@ static double norm_squared_noalloc(const nx_state* s) {
   const size_t oQ = take(rows), oB = take(rows), oC = take(rows), oCT = take(rows), oa = take((size_t)chi * R);
   const size_t oalo = take((size_t)chi * W * 8), oahi = take((size_t)chi * W * 8), orhs = take((size_t)chi * W * 8);
   const size_t onc = take((size_t)chi * 4), ok = take((size_t)chi * 4), op0 = take((size_t)chi * 4);
+  const size_t ometa = take((size_t)chi * 16);
   const size_t ocoef = take((size_t)chi * (s->is_complex ? 16 : 4));
-  const size_t ooff = take((size_t)(chi + 1) * 8);
-  const size_t opart = take((size_t)nblocks * 4 * 16), ores = take(16);
+  const size_t opart = take((size_t)nblocks * 32), ores = take(32);
   if (o > NOALLOC_SCRATCH_BYTES)
     fail(NX_E_VALUE,
          "nx_state_norm_noalloc: %lld terms on %d qubits need %zu bytes of scratch, more than its %zu bytes",
@@ -678,6 +717,7 @@ static double norm_squared_noalloc(const nx_state* s) {
   q.a = b + oa;
   q.alo = (uint64_t*)(b + oalo); q.ahi = (uint64_t*)(b + oahi); q.rhs = (uint64_t*)(b + orhs);
   q.nc = (int32_t*)(b + onc); q.k = (int32_t*)(b + ok); q.p0 = (int32_t*)(b + op0);
+  q.meta = (int4*)(b + ometa);
   q.chi = chi; q.n = n; q.W = W;
   CoefDev cd;
   cd.scalar = s->is_complex ? 0 : 1;
@@ -702,19 +742,17 @@ static double norm_squared_noalloc(const nx_state* s) {
       hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");  // blk is reused
     }
   }
-  int64_t* off = (int64_t*)(b + ooff);
-  double2* part = (double2*)(b + opart);
-  double2* res = (double2*)(b + ores);
+  double4* part = (double4*)(b + opart);
+  double4* res = (double4*)(b + ores);
   hip_check(launch_qform(s->forms.desc(), q, st), "k_qform");
-  hip_check(launch_tri_offsets(chi, PAIR_J, off, st), "k_tri_offsets");
-  hip_check(launch_pairs(q, q, off, chi, chi, 1, PAIR_J, /*fixed bra, herm*/ 2, cd, cd, c->tabs, part, nblocks, nullptr,
-                         0, st),
+  hip_check(launch_pairs(q, q, chi, chi, 1, PAIR_J, /*fixed bra, herm*/ 2, cd, cd, c->tabs, part, nblocks, nullptr, 0,
+                         st),
             "k_pairs");
-  hip_check(launch_reduce(part, (int64_t)nblocks * 4, res, st), "k_reduce");
-  double sum = 0.0;
-  hip_check(hipMemcpyAsync(&sum, res, 8, hipMemcpyDeviceToHost, st), "hipMemcpyAsync D2H");
+  hip_check(launch_reduce(part, nblocks, res, st), "k_reduce");
+  double sum[4] = {0.0, 0.0, 0.0, 0.0};
+  hip_check(hipMemcpyAsync(sum, res, 32, hipMemcpyDeviceToHost, st), "hipMemcpyAsync D2H");
   hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-  return sum * g2;
+  return ab_value(sum).real() * g2;
 }
 
 // ---------------------------------------------------------------------------
@@ -1764,6 +1802,15 @@ const char* nx_last_error(void) { return g_err.c_str(); }
 
 int nx_init(int device) { NX_TRY(t_dev = device; (void)ctx()) }
 
+int nx_set_stream(void* stream_, int use) {
+  NX_TRY({
+    (void)ctx();
+    hip_check(hipStreamSynchronize(stream()), "hipStreamSynchronize");  // the previous stream's work first
+    t_user_stream = use != 0;
+    t_stream = (hipStream_t)stream_;
+  })
+}
+
 int nx_state_compile(int num_qubits, int64_t ngates, const int32_t* gates, nx_state** out) {
   NX_TRY(*out = compile(num_qubits, ngates, gates))
 }
@@ -1820,6 +1867,14 @@ int nx_state_norm(const nx_state* s, double* out) { NX_TRY(*out = std::sqrt(norm
 
 int nx_state_norm_noalloc(const nx_state* s, double* out) { NX_TRY(*out = std::sqrt(norm_squared_noalloc(s))) }
 
+int nx_state_norm_async(const nx_state* s, double* d_out, void* stream_) {
+  NX_TRY({
+    StreamOrder order(stream_);  // NULL: the null stream
+    norm_enqueue(s, d_out);
+    order.done();
+  })
+}
+
 int nx_state_exp_value(const nx_state* s, int dense, int32_t len, const int32_t* ops, const int32_t* qubits,
                        double* out) {
   NX_TRY(*out = exp_value(s, pauli_prog(s, dense, len, ops, qubits)))
@@ -1843,8 +1898,9 @@ int nx_state_inner_product(const nx_state* a, const nx_state* b, double* re, dou
   })
 }
 
-static void amplitudes_impl(const nx_state* s, int64_t nbits, const uint64_t* d_bits, double* d_out,
-                            uint8_t* d_per_term, int enumerate) {
+// enqueues the amplitude pass on the library stream (no host wait)
+static void amplitudes_enqueue(const nx_state* s, int64_t nbits, const uint64_t* d_bits, double* d_out,
+                               uint8_t* d_per_term, int enumerate) {
   if (nbits <= 0) return;  // no strings: no launch (a grid of zero blocks is not a valid launch)
   const QForms& q = qform_of(s);
   CoefDev c = coef_of(s);
@@ -1857,9 +1913,18 @@ static void amplitudes_impl(const nx_state* s, int64_t nbits, const uint64_t* d_
                              partial.as<double2>(), d_per_term, enumerate, s->g.real(), s->g.imag(), stream()),
             "k_amplitude");
   ps.end();
+  ProfScope pr("k_amp_reduce");
   hip_check(launch_amp_reduce(partial.as<double2>(), nbits, (int)nchunks, (double2*)d_out, s->g.real(), s->g.imag(),
                               stream()),
             "k_amp_reduce");
+  pr.end();
+  // `partial` returns to the pool now; its next user is ordered after these
+  // launches on the same stream
+}
+
+static void amplitudes_impl(const nx_state* s, int64_t nbits, const uint64_t* d_bits, double* d_out,
+                            uint8_t* d_per_term, int enumerate) {
+  amplitudes_enqueue(s, nbits, d_bits, d_out, d_per_term, enumerate);
   sync();
 }
 
@@ -1909,6 +1974,24 @@ int nx_state_amplitudes_device(const nx_state* s, int64_t nbits, const uint64_t*
       amplitudes_impl(s, nbits, d_bits, d_out, nullptr, 0);
     }
     sync();
+  })
+}
+
+int nx_state_amplitudes_async(const nx_state* s, int64_t nbits, const uint64_t* d_bits, int32_t words, double* d_out,
+                              void* stream_) {
+  NX_TRY({
+    if (words != words_for(s->n)) fail(NX_E_VALUE, "device bitstrings must use %d words", words_for(s->n));
+    if (nbits < 0) fail(NX_E_VALUE, "negative bitstring count %lld", (long long)nbits);
+    if (nbits == 0) return NX_OK;
+    // the caller's stream (NULL: the null stream): its earlier work (the
+    // inputs) comes first, its later work sees the results
+    StreamOrder order(stream_);
+    if (s->chi() == 0) {
+      hip_check(hipMemsetAsync(d_out, 0, (size_t)nbits * 16, stream()), "hipMemsetAsync");
+    } else {
+      amplitudes_enqueue(s, nbits, d_bits, d_out, nullptr, 0);
+    }
+    order.done();
   })
 }
 
@@ -2255,7 +2338,9 @@ int nx_batch_create(int32_t nstates, const nx_state* const* states, const int32_
       b->plan.nout = nout;
       sync();
       if (flags & NX_BATCH_GRAPH) {
-        hipStream_t st = stream();
+        // captured on the context's own stream (a caller's stream set with
+        // nx_set_stream may be the null stream, which cannot be captured)
+        hipStream_t st = ctx()->stream;
         hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         try {
           b->plan.enqueue(st);
@@ -2348,9 +2433,9 @@ int nx_random_clifford(int32_t n, const uint8_t* seed32, int64_t max_gates, int3
 
 int nx_state_drop_cache(nx_state* s) {
   NX_TRY({
-    // the quadratic forms are derived per call (the coefficient copies are
-    // inputs: they stay resident until the coefficients change)
-    s->q = QForms();
+    // the quadratic forms are recomputed by the next call, into the same
+    // device buffers (the coefficient copies are inputs: they stay resident
+    // until the coefficients change)
     s->q_ok = false;
   })
 }

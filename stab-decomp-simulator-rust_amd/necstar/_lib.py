@@ -17,6 +17,7 @@ _U8P, _U32P, _DP = C.POINTER(C.c_uint8), C.POINTER(C.c_uint32), C.POINTER(C.c_do
 SIGNATURES = {
     "nx_last_error": (C.c_char_p, []),
     "nx_init": (C.c_int, [C.c_int]),
+    "nx_set_stream": (C.c_int, [_P, C.c_int]),
     "nx_random_clifford": (C.c_int, [_I32, _U8P, _I64, _I32P, _I64P]),
     "nx_state_compile": (C.c_int, [C.c_int, _I64, _I32P, C.POINTER(_P)]),
     "nx_state_clone": (C.c_int, [_P, C.POINTER(_P)]),
@@ -26,11 +27,13 @@ SIGNATURES = {
     "nx_state_apply_gates": (C.c_int, [_P, _I64, _I32P]),
     "nx_state_norm": (C.c_int, [_P, _DP]),
     "nx_state_norm_noalloc": (C.c_int, [_P, _DP]),
+    "nx_state_norm_async": (C.c_int, [_P, _DP, _P]),
     "nx_state_norm_squared": (C.c_int, [_P, _DP]),
     "nx_state_exp_value": (C.c_int, [_P, C.c_int, _I32, _I32P, _I32P, _DP]),
     "nx_state_inner_product": (C.c_int, [_P, _P, _DP, _DP]),
     "nx_state_amplitudes": (C.c_int, [_P, _I64, _U64P, _I32, _DP, _U8P, _I32P]),
     "nx_state_amplitudes_device": (C.c_int, [_P, _I64, _U64P, _I32, _DP]),
+    "nx_state_amplitudes_async": (C.c_int, [_P, _I64, _U64P, _I32, _DP, _P]),
     "nx_state_statevector": (C.c_int, [_P, _DP]),
     "nx_state_project": (C.c_int, [_P, _I32, _I32, _I32]),
     "nx_state_discard": (C.c_int, [_P, _I32]),
