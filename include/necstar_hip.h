@@ -207,10 +207,19 @@ int nx_state_exp_values(const nx_state *s, int32_t npauli, const int32_t *ops, d
  * decomposition coefficients: grad = dL/d(conj c) * 2 = 2 |g|^2 sum_k w_k G_k c,
  * (G_k)_{ji} = <phi_j|P_k phi_i>.  paulis: npauli dense strings ops[npauli][n].
  * grad: complex128 [chi] (the PyTorch convention for a real loss of complex
- * leaves); *loss receives L.  Up to 32768 terms.  No reference counterpart (an addition for
+ * leaves); *loss receives L.  One pass over the pairs per 128 strings
+ * computes L and the gradient together (O(chi^2 / 16) device memory, no
+ * chi x chi matrix).  No reference counterpart (an addition for
  * differentiation; the forward is exp_value.rs:7-61). */
 int nx_state_grad_exp_values(const nx_state *s, int32_t npauli, const int32_t *ops, const double *weights,
                              double *loss, double *grad);
+/* The same with device memory: coefficient values d_coef (complex128
+ * [chi]; NULL: the state's own), L into d_loss[0], the gradient into
+ * d_grad (complex128 [chi]); weights stay host memory.  Ordered against
+ * `stream` like nx_state_amplitudes_async, no host wait.  No reference
+ * counterpart (used by the torch.autograd.Function). */
+int nx_state_grad_exp_values_device(const nx_state *s, int32_t npauli, const int32_t *ops, const double *weights,
+                                    const double *d_coef, double *d_loss, double *d_grad, void *stream);
 
 /* Batches (many circuits of uneven n and chi in one launch plan).  State b
  * (copied at creation; later changes to it are not seen) gets its norm

@@ -84,10 +84,7 @@ struct VarDev {
 // One pair job: the Hermitian sum over kets f in [0, nfixed) of Y and bras
 // o in [f, nother) (triangle) or [0, nother) of X, for every variant of
 // the ket; the job owns waves [wave0, wave0 + nwaves) of the launch.
-// row_off[f] is the first unit (J consecutive bras) of ket row f.  With gw
-// set, the weighted pair values sum_k wts[k] <X_o | Y_f variant k> are
-// written (accumulate = 0) or added (accumulate = 1) at gw[o * gw_ld + f],
-// and their conjugates at gw[f * gw_ld + o].
+// row_off[f] is the first unit (J consecutive bras) of ket row f.
 struct PairJobDev {
   QFormDev X, Y;
   VarDev V;
@@ -98,10 +95,6 @@ struct PairJobDev {
   int64_t wave0, nwaves;
   int64_t out_base;  // variant k -> out[out_base + k]
   int64_t base_out;  // >= 0: the base (ket itself, identity) -> out[base_out]
-  double2* gw;
-  int64_t gw_ld;
-  const double* wts;
-  int32_t accumulate;
   // unit constraints (sampling): x_{uq[u]} = ub[u*(VW+1)] for the base, the
   // variants in masks ub[u*(VW+1) + 1 .. VW] take the other value; the ket
   // then carries no variant tables (V unused)
@@ -221,7 +214,25 @@ hipError_t launch_qproject_var(const QFormDev& in, const QFormDev& out, const Va
                                const uint64_t* ub, int nunit, hipStream_t st);
 hipError_t launch_qproject(const QFormDev& in, const QFormDev& out, const int32_t* uq, const int32_t* ub, int nunit,
                            hipStream_t st);
-// y = G c for the Hermitian G held in full in Gw[j*ld + i]
-hipError_t launch_gramc(const double2* Gw, int64_t ld, int64_t chi, const double2* c, double2* y, hipStream_t st);
+// Coefficient backward (k_grad_var): for the Pauli variants V of the terms
+// of X (up to 64 VW strings, weights wts) and coefficient values c, per
+// tile of GT x GT pairs the row and column partials of y = G c, G =
+// sum_k w_k G_k; per wave (nblocks * 4 waves) the 64 VW per-string
+// Hermitian sums in partial.  tile_row and tile_col hold grad_tiles(chi) *
+// grad_tile_size() values each.
+int64_t grad_tiles(int64_t chi);
+int grad_tile_size();
+hipError_t launch_grad_var(const QFormDev& X, const VarDev& V, int VW, const double2* c, int nvar, const double* wts,
+                           const double* tabs, int nblocks, double2* tile_row, double2* tile_col, double* partial,
+                           hipStream_t st);
+// y (+)= scale * (G c) from the tile partials, per index in a fixed order
+hipError_t launch_grad_reduce(const double2* tile_row, const double2* tile_col, int64_t chi, double scale,
+                              int accumulate, double2* y, hipStream_t st);
+// vals[k] = scale * sum over the nwaves per-wave partials of string k
+// (partial[w * stride + k]), k < nvar
+hipError_t launch_grad_vals(const double* partial, int64_t nwaves, int stride, int nvar, double scale, double* vals,
+                            hipStream_t st);
+// *loss = sum_k wts[k] vals[k], k < n, in order
+hipError_t launch_grad_loss(const double* vals, const double* wts, int n, double* loss, hipStream_t st);
 
 }  // namespace nx

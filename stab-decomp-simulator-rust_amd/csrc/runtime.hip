@@ -1141,10 +1141,6 @@ struct VarJob {
   CoefDev cx{}, cy{};
   int64_t out_base = 0;
   int64_t base_out = -1;  // >= 0: also the base sum (norm^2 / |g|^2) there
-  double2* gw = nullptr;
-  int64_t gw_ld = 0;
-  const double* wts = nullptr;
-  int accumulate = 0;
 };
 
 struct VarPlan {
@@ -1242,10 +1238,6 @@ static VarPlan make_plan(const std::vector<VarJob>& in, int VW, int J, int64_t t
     d.nwaves = nw;
     d.out_base = j.out_base;
     d.base_out = j.base_out;
-    d.gw = j.gw;
-    d.gw_ld = j.gw_ld;
-    d.wts = j.wts;
-    d.accumulate = j.accumulate;
     for (int64_t w = 0; w < nw; ++w) wj.push_back(idx);
     wave += nw;
     pl.launches.back().nwaves += nw;
@@ -1356,9 +1348,8 @@ static void add_dense_paulis(const nx_state* s, int32_t npauli, const int32_t* o
 }
 
 // Unnormalised <P_k> for every string of ps (exp_value.rs:7-61), all
-// strings per pair in one pass; with gw, also the weighted pair matrix.
-static std::vector<double> exp_values_var(const nx_state* s, const PauliSet& ps, double2* gw = nullptr,
-                                          const double* d_wts = nullptr) {
+// strings per pair in one pass.
+static std::vector<double> exp_values_var(const nx_state* s, const PauliSet& ps) {
   const int np = ps.size();
   std::vector<double> res(np, 0.0);
   if (s->chi() == 0 || np == 0) return res;
@@ -1369,45 +1360,66 @@ static std::vector<double> exp_values_var(const nx_state* s, const PauliSet& ps,
   std::vector<VarSet> vs;
   vs.reserve((np + per - 1) / per);
   for (int first = 0; first < np; first += per) vs.push_back(pauli_vars(s->forms, ps, first, std::min(per, np - first), VW));
-  std::vector<double> outh;
-  // chunks that write gw run one after another (each adds to the matrix);
-  // without gw all chunks share one launch
-  const size_t nchunks = vs.size();
-  const size_t groups = gw ? nchunks : 1;
-  for (size_t gi = 0; gi < groups; ++gi) {
-    std::vector<VarJob> jobs;
-    for (size_t ci = (gw ? gi : 0); ci < (gw ? gi + 1 : nchunks); ++ci) {
-      VarJob j;
-      j.X = &q;
-      j.Y = &q;
-      j.V = &vs[ci];
-      j.cx = c;
-      j.cy = c;
-      j.out_base = (int64_t)ci * per;
-      if (gw) {
-        j.gw = gw;
-        j.gw_ld = s->chi();
-        j.wts = d_wts + ci * per;
-        j.accumulate = ci > 0;
-      }
-      jobs.push_back(j);
-    }
-    const auto shape = tuned_shape(jobs, VW);
-    VarPlan pl = make_plan(jobs, VW, shape.first, shape.second);
-    {
-      ProfScope ps_("k_pairs_var");
-      pl.enqueue(stream());
-      ps_.end();
-    }
-    std::vector<double> o(pl.nout);
-    d2h(o.data(), pl.out.p, pl.nout * 8);
-    sync();
-    for (size_t ci = (gw ? gi : 0); ci < (gw ? gi + 1 : nchunks); ++ci)
-      for (int k = 0; k < vs[ci].nvar; ++k) res[ci * per + k] = o[ci * per + k];
+  std::vector<VarJob> jobs;
+  for (size_t ci = 0; ci < vs.size(); ++ci) {
+    VarJob j;
+    j.X = &q;
+    j.Y = &q;
+    j.V = &vs[ci];
+    j.cx = c;
+    j.cy = c;
+    j.out_base = (int64_t)ci * per;
+    jobs.push_back(j);
   }
+  const auto shape = tuned_shape(jobs, VW);
+  VarPlan pl = make_plan(jobs, VW, shape.first, shape.second);
+  {
+    ProfScope ps_("k_pairs_var");
+    pl.enqueue(stream());
+    ps_.end();
+  }
+  std::vector<double> o(pl.nout);
+  d2h(o.data(), pl.out.p, pl.nout * 8);
+  sync();
   const double g2 = s->g.real() * s->g.real() + s->g.imag() * s->g.imag();
-  for (double& r : res) r *= g2;
+  for (int k = 0; k < np; ++k) res[k] = o[k] * g2;
   return res;
+}
+
+// Coefficient backward of L = sum_k w_k exp_value(P_k) (exp_value.rs:7-61)
+// for the strings of ps, device weights d_wts and coefficient values c
+// (device complex128 [chi]): d_loss[0] = L and d_grad = 2 |g|^2 G c, G =
+// sum_k w_k G_k, from one pass over the pairs per 64 VW strings (k_grad_var:
+// tiles of pairs, no chi x chi matrix), enqueued on the library stream.
+static void grad_enqueue(const nx_state* s, const PauliSet& ps, const double* d_wts, const double2* c, double* d_loss,
+                         double2* d_grad) {
+  const int np = ps.size();
+  const int64_t chi = s->chi();
+  const int VW = np <= 64 ? 1 : 2;
+  const int per = 64 * VW;
+  const QForms& q = qform_of(s);
+  const double g2 = s->g.real() * s->g.real() + s->g.imag() * s->g.imag();
+  const int64_t nt = grad_tiles(chi);
+  const size_t tile_vals = (size_t)nt * grad_tile_size();
+  // three rounds of resident waves (tiles of 16 x 16 pairs even out)
+  const int nblocks = (int)std::max<int64_t>(1, std::min<int64_t>((nt + 3) / 4, 6144));
+  const int64_t nwaves = (int64_t)nblocks * 4;
+  DevBuf trow(tile_vals * 16), tcol(tile_vals * 16), part((size_t)nwaves * per * 8), vals((size_t)np * 8);
+  for (int first = 0; first < np; first += per) {
+    const int cnt = std::min(per, np - first);
+    VarSet vs = pauli_vars(s->forms, ps, first, cnt, VW);
+    ProfScope pg("k_grad_var");
+    hip_check(launch_grad_var(q.desc(), vs.desc(), VW, c, cnt, d_wts + first, ctx()->tabs, nblocks, trow.as<double2>(),
+                              tcol.as<double2>(), part.as<double>(), stream()),
+              "k_grad_var");
+    pg.end();
+    hip_check(launch_grad_reduce(trow.as<double2>(), tcol.as<double2>(), chi, 2.0 * g2, first > 0, d_grad, stream()),
+              "k_grad_reduce");
+    hip_check(launch_grad_vals(part.as<double>(), nwaves, per, cnt, g2, vals.as<double>() + first, stream()),
+              "k_grad_vals");
+    // vs returns to the pool; its next user is ordered after these launches
+  }
+  hip_check(launch_grad_loss(vals.as<double>(), d_wts, np, d_loss, stream()), "k_grad_loss");
 }
 
 // One state's Z-basis projection norms, 2^d outcomes per pass.  The plan
@@ -2250,25 +2262,32 @@ int nx_state_grad_exp_values(const nx_state* s, int32_t npauli, const int32_t* o
     for (int64_t j = 0; j < 2 * chi; ++j) grad[j] = 0.0;
     *loss = 0.0;
     if (chi == 0 || npauli == 0) return NX_OK;
-    if (chi > 32768) fail(NX_E_VALUE, "the dense Gram backward supports up to 32768 terms (got %lld)", (long long)chi);
-    DevBuf G((size_t)chi * chi * 16), y((size_t)chi * 16), dw((size_t)npauli * 8);
+    DevBuf dw((size_t)npauli * 8), dl(8), dg((size_t)chi * 16);
     h2d(dw.p, weights, (size_t)npauli * 8);
-    // G_ji = sum_k w_k <phi_j | P_k phi_i> (j >= i), and the forward values
-    const std::vector<double> ev = exp_values_var(s, ps, G.as<double2>(), dw.as<double>());
-    CoefDev c = coef_of(s);
-    {
-      ProfScope pg("k_gramc");
-      hip_check(launch_gramc(G.as<double2>(), chi, chi, c.c, y.as<double2>(), stream()), "k_gramc");
-      pg.end();
-    }
-    std::vector<double> yh(2 * chi);
-    d2h(yh.data(), y.p, (size_t)chi * 16);
+    grad_enqueue(s, ps, dw.as<double>(), coef_of(s).c, dl.as<double>(), dg.as<double2>());
+    d2h(grad, dg.p, (size_t)chi * 16);
+    d2h(loss, dl.p, 8);
     sync();
-    const double g2 = s->g.real() * s->g.real() + s->g.imag() * s->g.imag();
-    for (int64_t j = 0; j < 2 * chi; ++j) grad[j] = 2.0 * g2 * yh[j];
-    double acc = 0.0;
-    for (int32_t k = 0; k < npauli; ++k) acc += weights[k] * ev[k];
-    *loss = acc;
+  })
+}
+
+int nx_state_grad_exp_values_device(const nx_state* s, int32_t npauli, const int32_t* ops, const double* weights,
+                                    const double* d_coef, double* d_loss, double* d_grad, void* stream_) {
+  NX_TRY({
+    const int64_t chi = s->chi();
+    PauliSet ps;
+    add_dense_paulis(s, npauli, ops, ps);
+    StreamOrder order(stream_);
+    if (chi == 0 || npauli == 0) {
+      hip_check(hipMemsetAsync(d_loss, 0, 8, stream()), "hipMemsetAsync");
+      if (chi) hip_check(hipMemsetAsync(d_grad, 0, (size_t)chi * 16, stream()), "hipMemsetAsync");
+    } else {
+      DevBuf dw((size_t)npauli * 8);
+      h2d(dw.p, weights, (size_t)npauli * 8);
+      const double2* c = d_coef ? (const double2*)d_coef : coef_of(s).c;
+      grad_enqueue(s, ps, dw.as<double>(), c, d_loss, (double2*)d_grad);
+    }
+    order.done();
   })
 }
 

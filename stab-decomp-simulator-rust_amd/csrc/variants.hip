@@ -14,7 +14,9 @@
 //               are u64 masks with one bit per variant.  Lane k then turns
 //               variant k's exact value into FP64 and accumulates it.
 //  k_reduce_var per job and variant, the ordered sum of the wave partials
-//  k_gramc      y = G c for the weighted FP64 pair matrix (both triangles)
+//  k_grad_var   the coefficient backward: weighted pair values reduced over
+//               the strings and accumulated into y = G c per tile of pairs
+//               (k_grad_reduce, k_grad_vals, k_grad_loss finish it)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -940,9 +942,6 @@ __global__ __launch_bounds__(256, (W == 1 && VW == 1) ? 8 : 1) void k_pairs_var(
     const int lo = jd.nvar - 64 * v;
     invalid[v] = lo >= 64 ? 0ull : (lo <= 0 ? ~0ull : ~((1ull << lo) - 1ull));
   }
-  double wk[VW];
-#pragma unroll
-  for (int v = 0; v < VW; ++v) wk[v] = (jd.gw && 64 * v + lane < jd.nvar) ? jd.wts[64 * v + lane] : 0.0;
   // acc: FP64 sums (complex coefficients), or the dyadic part A of the
   // exact sums (exact Scalar coefficients) with the sqrt(2) part in accB
   const bool exact = jd.cx.scalar && jd.cy.scalar;
@@ -1057,10 +1056,6 @@ __global__ __launch_bounds__(256, (W == 1 && VW == 1) ? 8 : 1) void k_pairs_var(
           accb = diag ? accb + tr : accb + (tr + tr);
         }
       }
-      double gr = 0.0, gi = 0.0;
-      // the FP64 value of zeta^K 2^{-mag/2} is needed by complex
-      // coefficients and by the weighted Gram row, not by exact sums
-      const bool need_f = !exact || jd.gw;
 #pragma unroll
       for (int v = 0; v < VW; ++v) {
         const int zk = (int)((V.z[v] >> lane) & 1ull);
@@ -1068,42 +1063,12 @@ __global__ __launch_bounds__(256, (W == 1 && VW == 1) ? 8 : 1) void k_pairs_var(
           const int D = (int)((V.d0[v] >> lane) & 1ull) | ((int)((V.d1[v] >> lane) & 1ull) << 1) |
                         ((int)((V.d2[v] >> lane) & 1ull) << 2);
           const int K = (P.c0 + V.dv + D) & 7;
-          if (exact && !czero) add_exact(kc + K, rc + P.mag, w2, acc[v], accB[v]);
-          if (need_f) {
-            const double ir = tabs[TAB_COS + K] * nrm, ii = tabs[TAB_SIN + K] * nrm;
-            if (!exact) {
-              const double tr = cr_ * ir - ci_ * ii;
-              acc[v] = diag ? acc[v] + tr : acc[v] + (tr + tr);
-            }
-            gr = gr + wk[v] * ir;
-            gi = gi + wk[v] * ii;
-          }
-        }
-      }
-      if (jd.gw) {
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-          gr = gr + __shfl_xor(gr, off);
-          gi = gi + __shfl_xor(gi, off);
-        }
-        if (lane == 0) {
-          // G[o][f] and, off the diagonal, G[f][o] = conj(G[o][f])
-          double2* dst = jd.gw + (size_t)o * jd.gw_ld + fx;
-          double2* dsT = jd.gw + (size_t)fx * jd.gw_ld + o;
-          if (jd.accumulate) {
-            double2 old = *dst;
-            old.x = old.x + gr;
-            old.y = old.y + gi;
-            *dst = old;
-            if (o != fx) {
-              double2 oT = *dsT;
-              oT.x = oT.x + gr;
-              oT.y = oT.y - gi;
-              *dsT = oT;
-            }
+          if (exact) {
+            if (!czero) add_exact(kc + K, rc + P.mag, w2, acc[v], accB[v]);
           } else {
-            *dst = make_double2(gr, gi);
-            if (o != fx) *dsT = make_double2(gr, -gi);
+            const double ir = tabs[TAB_COS + K] * nrm, ii = tabs[TAB_SIN + K] * nrm;
+            const double tr = cr_ * ir - ci_ * ii;
+            acc[v] = diag ? acc[v] + tr : acc[v] + (tr + tr);
           }
         }
       }
@@ -1195,30 +1160,225 @@ __global__ __launch_bounds__(256) void k_reduce_var(const PairJobDev* __restrict
   }
 }
 
-// y_j = sum_i G_ji c_i over the full (both triangles written) matrix: one
-// block per row j, threads stride over i, tree sum in LDS
-__global__ __launch_bounds__(256) void k_gramc(const double2* __restrict__ G, int64_t ld, int64_t chi,
-                                               const double2* __restrict__ c, double2* __restrict__ y) {
-  __shared__ double sr[256], si[256];
-  const int64_t j = blockIdx.x;
-  double ar = 0.0, ai = 0.0;
-  for (int64_t i = threadIdx.x; i < chi; i += blockDim.x) {
-    const double2 g = G[(size_t)j * ld + i];
-    const double2 ci = c[i];
-    ar = ar + (g.x * ci.x - g.y * ci.y);
-    ai = ai + (g.x * ci.y + g.y * ci.x);
+// ---------------------------------------------------------------------------
+// Coefficient backward.  For L = sum_k w_k exp_value(P_k) = |g|^2 c^H G c
+// with G = sum_k w_k G_k, (G_k)_{ji} = <phi_j|P_k phi_i> (Hermitian), the
+// gradient in PyTorch's convention is 2 |g|^2 y, y = G c.  The pairs (ket f,
+// bra o >= f) run in tiles of GT kets x GT bras (block row bf, block column
+// bo >= bf); per pair the elimination serves all strings (pair_eval_v), the
+// lanes' string values reduce to g = G_{of} = sum_k w_k <phi_o|P_k phi_f>,
+// and the tile's lanes i < GT accumulate y_o += g c_f (column o = o0 + i)
+// and y_f += conj(g) c_o (row f = f0 + i, o != f).  Each tile writes its GT
+// row and GT column partials; k_grad_reduce sums, for each index j, the
+// partials of the tiles in its block row and block column in a fixed order.
+// The same pass accumulates, per string, the Hermitian sums of
+// exp_value.rs:38-58 (per-wave partials, summed by k_grad_vals).
+constexpr int GT = 16;
+
+// block row bf and column bo of tile t in the row-major order of the block
+// triangle {bo >= bf} of nb x nb blocks: row bf starts at
+// bf nb - bf (bf - 1) / 2
+__device__ __forceinline__ int64_t gtile_start(int64_t bf, int64_t nb) { return bf * nb - bf * (bf - 1) / 2; }
+__device__ __forceinline__ void gtile_of(int64_t t, int64_t nb, int64_t* bf, int64_t* bo) {
+  // rows from the end: m = nb - bf is the smallest m with m (m + 1) / 2 >= T - t
+  const int64_t T = nb * (nb + 1) / 2, r = T - t;
+  int64_t m = (int64_t)ceil(0.5 * (sqrt(1.0 + 8.0 * (double)r) - 1.0));
+  m = (int64_t)uni64((uint64_t)m);
+  if (m > nb) m = nb;
+  if (m < 1) m = 1;
+  while (m > 1 && (m - 1) * m / 2 >= r) --m;
+  while (m * (m + 1) / 2 < r) ++m;
+  *bf = nb - m;
+  *bo = *bf + (t - gtile_start(*bf, nb));
+}
+
+template <int W, int VW>
+__global__ __launch_bounds__(256) void k_grad_var(QFormDev X, VarDev VT, const double2* __restrict__ c, int nvar,
+                                                  const double* __restrict__ wts, const double* __restrict__ tabs,
+                                                  int64_t ntiles, double2* __restrict__ tile_row,
+                                                  double2* __restrict__ tile_col, double* __restrict__ partial) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t)blockIdx.x * 4 + uni((int)(threadIdx.x >> 6));  // uniform in the wave
+  const int64_t nwaves = (int64_t)gridDim.x * 4;
+  const int64_t chi = X.chi, nb = (chi + GT - 1) / GT;
+  const int n = X.n, R = 64 * W;
+  u64 valid[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const int hi = n - 64 * w;
+    valid[w] = hi >= 64 ? ~0ull : (hi <= 0 ? 0ull : ((1ull << hi) - 1ull));
   }
-  sr[threadIdx.x] = ar;
-  si[threadIdx.x] = ai;
-  __syncthreads();
-  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) {
-      sr[threadIdx.x] += sr[threadIdx.x + st];
-      si[threadIdx.x] += si[threadIdx.x + st];
+  u64 invalid[VW];
+  double wk[VW], acc[VW];
+#pragma unroll
+  for (int v = 0; v < VW; ++v) {
+    const int lo = nvar - 64 * v;
+    invalid[v] = lo >= 64 ? 0ull : (lo <= 0 ? ~0ull : ~((1ull << lo) - 1ull));
+    wk[v] = (64 * v + lane < nvar) ? wts[64 * v + lane] : 0.0;
+    acc[v] = 0.0;
+  }
+  for (int64_t t = wave; t < ntiles; t += nwaves) {
+    int64_t bf, bo;
+    gtile_of(t, nb, &bf, &bo);
+    const int64_t f0 = bf * GT, o0 = bo * GT;
+    const int64_t f1 = (f0 + GT < chi) ? f0 + GT : chi, o1 = (o0 + GT < chi) ? o0 + GT : chi;
+    double yr_re = 0.0, yr_im = 0.0, yc_re = 0.0, yc_im = 0.0;
+    for (int64_t fx = f0; fx < f1; ++fx) {
+      const int ncf = uni(X.nc[fx]), kf = uni(X.k[fx]), pf = uni(X.p0[fx]);
+      u64 vp0[VW], vp1[VW], vp2[VW];
+#pragma unroll
+      for (int v = 0; v < VW; ++v) {
+        vp0[v] = uni64(VT.ph[((size_t)fx * 3 + 0) * VW + v]);
+        vp1[v] = uni64(VT.ph[((size_t)fx * 3 + 1) * VW + v]);
+        vp2[v] = uni64(VT.ph[((size_t)fx * 3 + 2) * VW + v]);
+      }
+      const double2 cf = c[fx];
+      for (int64_t o = (o0 > fx ? o0 : fx); o < o1; ++o) {
+        u64 Bf[W][W], Cf[W][W], rf[W];
+        int af[W];
+        load_qrows<W>(X, fx, lane, Bf, af, Cf, rf);
+        u64 Bo[W][W], Co[W][W], ro[W];
+        int ao[W];
+        load_qrows<W>(X, o, lane, Bo, ao, Co, ro);
+        const int nco = uni(X.nc[o]), ko = uni(X.k[o]), po = uni(X.p0[o]);
+        PairRegs<W> P;
+        VarRegs<W, VW> V;
+        P.lane = lane;
+        P.nodiag = ~(1ull << lane);
+#pragma unroll
+        for (int sl = 0; sl < W; ++sl) {
+#pragma unroll
+          for (int w = 0; w < W; ++w) {
+            P.B[sl][w] = Bf[sl][w] ^ Bo[sl][w];
+            P.ca[sl][w] = Co[sl][w];
+            P.cb[sl][w] = Cf[sl][w];
+          }
+          P.a[sl] = (af[sl] - ao[sl]) & 3;
+          const size_t r = (size_t)fx * R + 64 * sl + lane;
+#pragma unroll
+          for (int v = 0; v < VW; ++v) {
+            V.hi[sl][v] = VT.hi[r * VW + v];
+            V.rb[sl][v] = VT.rhs[r * VW + v];
+            V.ra[sl][v] = 0ull;
+          }
+        }
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          P.rhsa[w] = ro[w];
+          P.rhsb[w] = rf[w];
+          P.rem[w] = valid[w];
+        }
+#pragma unroll
+        for (int v = 0; v < VW; ++v) {
+          V.d0[v] = vp0[v];
+          V.d1[v] = vp1[v];
+          V.d2[v] = vp2[v];
+          V.z[v] = invalid[v];
+        }
+        V.bz = 1;  // the identity string is not wanted here
+        V.dv = 0;
+        P.c0 = (pf - po) & 7;
+        P.mag = kf + ko;
+        pair_eval_v<W, VW, false>(P, V, nco, ncf, 0, nullptr, nullptr);
+        if (all_zero<W, VW>(V)) continue;  // every string's pair value vanishes
+        // (c_bra.conj() * c_ket).into(), shared by the strings
+        const double2 co = c[o];
+        const double cr_ = co.x * cf.x + co.y * cf.y, ci_ = co.x * cf.y - co.y * cf.x;
+        const bool diag = (o == fx);
+        const double nrm = tabs[TAB_POW + P.mag];
+        double gr = 0.0, gi = 0.0;
+#pragma unroll
+        for (int v = 0; v < VW; ++v) {
+          if (!((V.z[v] >> lane) & 1ull)) {
+            const int D = (int)((V.d0[v] >> lane) & 1ull) | ((int)((V.d1[v] >> lane) & 1ull) << 1) |
+                          ((int)((V.d2[v] >> lane) & 1ull) << 2);
+            const int K = (P.c0 + V.dv + D) & 7;
+            const double ir = tabs[TAB_COS + K] * nrm, ii = tabs[TAB_SIN + K] * nrm;
+            const double tr = cr_ * ir - ci_ * ii;
+            acc[v] = diag ? acc[v] + tr : acc[v] + (tr + tr);
+            gr = gr + wk[v] * ir;
+            gi = gi + wk[v] * ii;
+          }
+        }
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          gr = gr + __shfl_xor(gr, off);
+          gi = gi + __shfl_xor(gi, off);
+        }
+        // y_o += g c_f ; y_f += conj(g) c_o off the diagonal
+        if (lane == (int)(o - o0)) {
+          yc_re = yc_re + (gr * cf.x - gi * cf.y);
+          yc_im = yc_im + (gr * cf.y + gi * cf.x);
+        }
+        if (!diag && lane == (int)(fx - f0)) {
+          yr_re = yr_re + (gr * co.x + gi * co.y);
+          yr_im = yr_im + (gr * co.y - gi * co.x);
+        }
+      }
     }
+    if (lane < GT) {
+      tile_row[(size_t)t * GT + lane] = make_double2(yr_re, yr_im);
+      tile_col[(size_t)t * GT + lane] = make_double2(yc_re, yc_im);
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < VW; ++v) partial[(size_t)wave * 64 * VW + 64 * v + lane] = acc[v];
+}
+
+// y_j * scale for each index j: the column partials of the tiles (bf, j/GT)
+// for bf = 0 .. j/GT, then the row partials of the tiles (j/GT, bo) for
+// bo = j/GT .. nb - 1, in that order; accumulate != 0 adds to y
+__global__ void k_grad_reduce(const double2* __restrict__ tile_row, const double2* __restrict__ tile_col, int64_t chi,
+                              double scale, int accumulate, double2* __restrict__ y) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= chi) return;
+  const int64_t nb = (chi + GT - 1) / GT, b = j / GT, i = j % GT;
+  double re = 0.0, im = 0.0;
+  for (int64_t bf = 0; bf <= b; ++bf) {
+    const double2 v = tile_col[(size_t)(gtile_start(bf, nb) + (b - bf)) * GT + i];
+    re = re + v.x;
+    im = im + v.y;
+  }
+  const int64_t rb = gtile_start(b, nb);
+  for (int64_t bo = b; bo < nb; ++bo) {
+    const double2 v = tile_row[(size_t)(rb + (bo - b)) * GT + i];
+    re = re + v.x;
+    im = im + v.y;
+  }
+  re = re * scale;
+  im = im * scale;
+  if (accumulate) {
+    const double2 old = y[j];
+    re = old.x + re;
+    im = old.y + im;
+  }
+  y[j] = make_double2(re, im);
+}
+
+// per string k < nvar: sum of the per-wave partials in wave order, times
+// scale, to vals[k]; one block per string
+__global__ __launch_bounds__(256) void k_grad_vals(const double* __restrict__ partial, int64_t nwaves, int stride,
+                                                   double scale, double* __restrict__ vals) {
+  __shared__ double sh[256];
+  const int k = blockIdx.x;
+  double a = 0.0;
+  for (int64_t w = threadIdx.x; w < nwaves; w += 256) a = a + partial[(size_t)w * stride + k];
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x == 0) y[j] = make_double2(sr[0], si[0]);
+  if (threadIdx.x == 0) vals[k] = sh[0] * scale;
+}
+
+// loss = sum_k w_k vals[k] in string order (one thread)
+__global__ void k_grad_loss(const double* __restrict__ vals, const double* __restrict__ wts, int n,
+                            double* __restrict__ loss) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double l = 0.0;
+  for (int k = 0; k < n; ++k) l = l + wts[k] * vals[k];
+  *loss = l;
 }
 
 // ---------------------------------------------------------------------------
@@ -1334,9 +1494,40 @@ hipError_t launch_reduce_var(const PairJobDev* jobs, int njobs, int VW, const do
   return hipGetLastError();
 }
 
-hipError_t launch_gramc(const double2* Gw, int64_t ld, int64_t chi, const double2* c, double2* y, hipStream_t st) {
+int64_t grad_tiles(int64_t chi) {
+  const int64_t nb = (chi + GT - 1) / GT;
+  return nb * (nb + 1) / 2;
+}
+
+int grad_tile_size() { return GT; }
+
+hipError_t launch_grad_var(const QFormDev& X, const VarDev& V, int VW, const double2* c, int nvar, const double* wts,
+                           const double* tabs, int nblocks, double2* tile_row, double2* tile_col, double* partial,
+                           hipStream_t st) {
+  if (X.chi == 0) return hipSuccess;
+  const int64_t nt = grad_tiles(X.chi);
+  NX_DISPATCH_WV(X.W, VW,
+                 k_grad_var<WW, VV><<<dim3(nblocks), dim3(256), 0, st>>>(X, V, c, nvar, wts, tabs, nt, tile_row,
+                                                                         tile_col, partial));
+  return hipGetLastError();
+}
+
+hipError_t launch_grad_reduce(const double2* tile_row, const double2* tile_col, int64_t chi, double scale,
+                              int accumulate, double2* y, hipStream_t st) {
   if (chi == 0) return hipSuccess;
-  k_gramc<<<dim3((unsigned)chi), dim3(256), 0, st>>>(Gw, ld, chi, c, y);
+  k_grad_reduce<<<dim3(nblk(chi, 256)), dim3(256), 0, st>>>(tile_row, tile_col, chi, scale, accumulate, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_grad_vals(const double* partial, int64_t nwaves, int stride, int nvar, double scale, double* vals,
+                            hipStream_t st) {
+  if (nvar == 0) return hipSuccess;
+  k_grad_vals<<<dim3(nvar), dim3(256), 0, st>>>(partial, nwaves, stride, scale, vals);
+  return hipGetLastError();
+}
+
+hipError_t launch_grad_loss(const double* vals, const double* wts, int n, double* loss, hipStream_t st) {
+  k_grad_loss<<<dim3(1), dim3(64), 0, st>>>(vals, wts, n, loss);
   return hipGetLastError();
 }
 

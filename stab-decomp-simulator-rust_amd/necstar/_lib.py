@@ -47,6 +47,7 @@ SIGNATURES = {
     "nx_state_pair_table": (C.c_int, [_P, _U32P]),
     "nx_state_cross_pair_table": (C.c_int, [_P, _P, _U32P]),
     "nx_state_grad_exp_values": (C.c_int, [_P, _I32, _I32P, _DP, _DP, _DP]),
+    "nx_state_grad_exp_values_device": (C.c_int, [_P, _I32, _I32P, _DP, _DP, _DP, _DP, _P]),
     "nx_state_exp_values": (C.c_int, [_P, _I32, _I32P, _DP]),
     "nx_profile_enable": (C.c_int, [C.c_int]),
     "nx_profile_get": (C.c_int, [C.c_char_p, _DP, _I64P]),
