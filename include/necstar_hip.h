@@ -141,9 +141,26 @@ int nx_state_measure(nx_state *s, int32_t nq, const int32_t *qargs, const uint8_
  * DFS order, outcome k as bits little-endian in 2 u64 words (bit i <->
  * qargs[i]) and its count.  Capacity max_out; *nout receives the count,
  * also when the call fails because max_out is too small (call again with
- * that capacity and the same seed). */
+ * that capacity and the same seed).  Every node probability is the
+ * reference's FP64 value: norm_squared() of the projected state summed over
+ * its surviving terms in the reference's pair order (norm.rs:8-30,
+ * projection.rs:35-51), so a seed gives the reference's outcomes.  Holds
+ * one 40-byte record per term pair: at most 23,169 terms. */
 int nx_state_sample(const nx_state *s, int32_t nq, const int32_t *qargs, uint64_t shots, const uint8_t *seed32,
                     int64_t max_out, uint64_t *out_bits, uint64_t *out_counts, int64_t *nout);
+
+/* sample with exact node probabilities (arguments and results as
+ * nx_state_sample): exact sums of the pair values, leading levels whose
+ * marginal is certified uniform drawn at p = 1/2 without norms, and the
+ * last <= 14 levels from exact amplitudes when every qubit is sampled.  The
+ * same distribution; a seed gives the reference's outcomes wherever its
+ * FP64 sums are exact (e.g. dyadic probabilities), and may differ where
+ * they round (a tie p = 1/2 the reference computes as 1/2 +- 1 ulp takes
+ * the other branch of Binomial).  No reference counterpart; no term
+ * limit. */
+int nx_state_sample_exact(const nx_state *s, int32_t nq, const int32_t *qargs, uint64_t shots,
+                          const uint8_t *seed32, int64_t max_out, uint64_t *out_bits, uint64_t *out_counts,
+                          int64_t *nout);
 
 /* QuantumCircuit::random_clifford (python/src/circuit.rs:73-79,
  * crates/necstar/src/circuit/random_clifford.rs:20-27,

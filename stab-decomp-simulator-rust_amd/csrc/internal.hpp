@@ -115,7 +115,17 @@ struct PairJobDev {
   // reduction segments seg0 .. seg0 + nseg - 1 (RED_SEG waves each)
   int64_t seg0;
   int32_t nseg;
+  // non-null (termvar windows, VW = 1): every pair (ket f, bra o >= f) also
+  // writes its exact per-variant values as a record of FOLD_REC words at
+  // rec[FOLD_REC * (f chi - f (f - 1) / 2 + o - f)]: the vanishing-variant
+  // mask, the three Z8 phase planes, then (c0 + dv) & 7 | mag << 8, so that
+  // variant k's value is zeta^((c0 + dv + D_k) & 7) 2^(-mag/2)
+  uint64_t* rec;
 };
+
+// words per pair record (see PairJobDev::rec) and node slots per window in
+// the reference-order fold (nodes of relative depth 1..6: 2 + 4 + .. + 64)
+enum : int { FOLD_REC = 5, FOLD_NODES = 128 };
 
 // waves per first-stage reduction segment
 enum : int { RED_SEG = 128 };
@@ -234,5 +244,19 @@ hipError_t launch_grad_vals(const double* partial, int64_t nwaves, int stride, i
                             hipStream_t st);
 // *loss = sum_k wts[k] vals[k], k < n, in order
 hipError_t launch_grad_loss(const double* vals, const double* wts, int n, double* loss, hipStream_t st);
+
+// Reference-order norms of the nodes of a sampling window (fold.hip).
+// k_fold_diag: D[f][u] = number of non-deterministic projections of term f
+// on the path to node u (255: the term does not survive), from the records
+// of the diagonal pairs.  k_fold_terms: the FP64 value the reference adds
+// for pair p (reference order) at node u, for pairs [p0, p0 + np).
+// t is node-major (t[u * ld + q], ld >= np).  k_fold_sum: sums[u] += those
+// values in pair order, one wave per node.  err (device int) is set to 1
+// when a node value is not of the form zeta^K 2^(-R/2) or 0.
+hipError_t launch_fold_diag(const uint64_t* rec, int64_t chi, int d, uint8_t* D, int* err, hipStream_t st);
+hipError_t launch_fold_terms(const uint64_t* rec, int64_t chi, int64_t p0, int64_t np, int64_t ld, int d,
+                             const uint8_t* D, const int32_t* cs, const double* tabs, double* t, int* err,
+                             hipStream_t st);
+hipError_t launch_fold_sum(const double* t, int64_t np, int64_t ld, int nnodes, double* sums, hipStream_t st);
 
 }  // namespace nx

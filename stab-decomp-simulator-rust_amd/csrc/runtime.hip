@@ -1233,6 +1233,7 @@ static VarPlan make_plan(const std::vector<VarJob>& in, int VW, int J, int64_t t
     d.ub = nullptr;
     d.termvar = 0;
     d.split_ab = j.split_ab;
+    d.rec = nullptr;
     for (int w = 0; w < 4; ++w) d.removed[w] = 0ull;
     d.wave0 = wave;
     d.nwaves = nw;
@@ -1490,7 +1491,10 @@ struct ProjPlan {
   }
   // norm_squared / |g|^2 of Pi psi for the 2^d outcomes of qubits look[0..d),
   // as exact parts: value = A[k] + sqrt(2) B[k] (A, B = the returned halves)
-  std::vector<double> run(const std::vector<int>& fq, const std::vector<int>& fb, const std::vector<int>& look) {
+  // rec != null: the pass also writes every pair's exact per-outcome values
+  // (PairJobDev::rec)
+  std::vector<double> run(const std::vector<int>& fq, const std::vector<int>& fb, const std::vector<int>& look,
+                          uint64_t* rec = nullptr) {
     const int d = (int)look.size(), nv = 1 << d;
     if ((int)(fq.size() + look.size()) > maxunit) fail(NX_E_INTERNAL, "too many projected qubits");
     if (nv > 64 * VW) fail(NX_E_INTERNAL, "too many look-ahead qubits");
@@ -1528,6 +1532,7 @@ struct ProjPlan {
     job.X = job.Y = projv.desc();
     job.V = vt;
     job.termvar = 1;
+    job.rec = rec;
     for (int i = 0; i < d; ++i) job.removed[look[i] >> 6] |= 1ull << (look[i] & 63);
     std::memcpy(stage_h.p, &job, sizeof(PairJobDev));
     h2d(stage_d.p, stage_h.p, stage_bytes);
@@ -1557,7 +1562,18 @@ struct ProjPlan {
 // its look-ahead leaves.
 struct Window {
   int root = 0, d = 0;
+  bool ref = false;                          // norms in nr (reference order), not la/lb
   std::vector<std::vector<double>> la, lb;  // exact parts A, B of node [j][k]
+  std::vector<std::vector<double>> nr;      // norm_squared of node [j][k] as the reference sums it
+};
+
+// Buffers of the reference-order fold (fold.hip), kept across the windows of
+// one sample call: pair records, per-(term, node) projection counts, one
+// chunk of per-(pair, node) terms, the node sums and the error flag.
+struct RefFold {
+  DevBuf rec, D, t, sums;
+  PinnedBuf host;
+  int64_t chunk = 0;
 };
 
 // The largest D such that the Z-basis marginal of the qubits order[0..D)
@@ -1661,6 +1677,8 @@ struct SampleCtx {
   ProjPlan pp;
   int look = 6;
   int64_t passes = 0, tails = 0;
+  bool ref = false;  // node norms in the reference's FP64 summation order
+  RefFold rf;
 };
 
 static Window open_window(SampleCtx& cx, int root) {
@@ -1685,6 +1703,61 @@ static Window open_window(SampleCtx& cx, int root) {
       w.la[j - 1][k] = w.la[j][k] + w.la[j][k | m];
       w.lb[j - 1][k] = w.lb[j][k] + w.lb[j][k | m];
     }
+  }
+  return w;
+}
+
+// A window whose node norms are the reference's FP64 values: norm_squared()
+// of the state projected on the node's path (norm.rs:8-30 over the terms
+// that projection.rs:35-51 keeps), summed in its order.  The pass writes the
+// exact per-leaf values of every pair; k_fold_diag / k_fold_terms /
+// k_fold_sum turn them into each node's sequential sum (fold.hip), and the
+// host multiplies by |g|^2 as norm.rs:29 does.
+static Window open_window_ref(SampleCtx& cx, int root) {
+  Window w;
+  w.root = root;
+  w.ref = true;
+  w.d = std::min<int>(cx.look, (int)cx.q.size() - root);
+  std::vector<int> fq(cx.q.begin(), cx.q.begin() + root), fb(cx.path.begin(), cx.path.begin() + root);
+  std::vector<int> lq(cx.q.begin() + root, cx.q.begin() + root + w.d);
+  const int64_t chi = cx.s->chi();
+  const int64_t np = chi * (chi + 1) / 2;
+  const int nn = (2 << w.d) - 2;
+  RefFold& rf = cx.rf;
+  if (!rf.rec.p) {
+    rf.rec.alloc((size_t)np * FOLD_REC * 8);
+    rf.D.alloc((size_t)chi * FOLD_NODES);
+    rf.chunk = std::min<int64_t>(np, (int64_t)1 << 20);
+    rf.t.alloc((size_t)rf.chunk * FOLD_NODES * 8);
+    rf.sums.alloc(FOLD_NODES * 8 + 256);
+    rf.host.alloc(FOLD_NODES * 8 + 256);
+  }
+  double* sums = rf.sums.as<double>();
+  int* err = (int*)(rf.sums.as<char>() + FOLD_NODES * 8);
+  cx.pp.run(fq, fb, lq, rf.rec.as<uint64_t>());
+  ++cx.passes;
+  {
+    ProfScope ps("k_fold");
+    hip_check(hipMemsetAsync(rf.sums.p, 0, FOLD_NODES * 8 + 4, stream()), "hipMemsetAsync");
+    hip_check(launch_fold_diag(rf.rec.as<uint64_t>(), chi, w.d, rf.D.as<uint8_t>(), err, stream()), "k_fold_diag");
+    for (int64_t p0 = 0; p0 < np; p0 += rf.chunk) {
+      const int64_t m = std::min<int64_t>(rf.chunk, np - p0);
+      hip_check(launch_fold_terms(rf.rec.as<uint64_t>(), chi, p0, m, rf.chunk, w.d, rf.D.as<uint8_t>(),
+                                  coef_of(cx.s).cs, ctx()->tabs, rf.t.as<double>(), err, stream()),
+                "k_fold_terms");
+      hip_check(launch_fold_sum(rf.t.as<double>(), m, rf.chunk, nn, sums, stream()), "k_fold_sum");
+    }
+    ps.end();
+  }
+  d2h(rf.host.p, rf.sums.p, FOLD_NODES * 8 + 4);
+  sync();
+  if (*(const int*)(rf.host.as<char>() + FOLD_NODES * 8))
+    fail(NX_E_INTERNAL, "a projected pair value is not of the form zeta^k 2^(-r/2)");
+  const double* hs = rf.host.as<double>();
+  w.nr.resize(w.d + 1);
+  for (int j = 1; j <= w.d; ++j) {
+    w.nr[j].resize((size_t)1 << j);
+    for (int k = 0; k < (1 << j); ++k) w.nr[j][k] = hs[(1 << j) - 2 + k] * cx.pp.g2;
   }
   return w;
 }
@@ -1738,13 +1811,16 @@ static void recursive_sample(SampleCtx& cx, size_t idx, uint64_t shots, uint64_t
   int j = 0;        // children's relative depth in the window
   if ((int)idx >= cx.uniform) {
     if (!w || (int)idx + 1 > w->root + w->d) {
-      own = ((int)cx.q.size() - (int)idx <= cx.tail_m) ? open_tail(cx, (int)idx) : open_window(cx, (int)idx);
+      own = cx.ref ? open_window_ref(cx, (int)idx)
+                   : (((int)cx.q.size() - (int)idx <= cx.tail_m) ? open_tail(cx, (int)idx)
+                                                                  : open_window(cx, (int)idx));
       w = &own;
       rel = 0;
     }
     j = (int)idx - w->root + 1;
     const uint64_t r1 = rel | ((uint64_t)1 << (j - 1));
-    const double nz = cx.pp.value(w->la[j][rel], w->lb[j][rel]), no = cx.pp.value(w->la[j][r1], w->lb[j][r1]);
+    const double nz = w->ref ? w->nr[j][rel] : cx.pp.value(w->la[j][rel], w->lb[j][rel]);
+    const double no = w->ref ? w->nr[j][r1] : cx.pp.value(w->la[j][r1], w->lb[j][r1]);
     pz = nz / (nz + no);
     if (!std::isnan(pz)) pz = pz < 0.0 ? 0.0 : (pz > 1.0 ? 1.0 : pz);
     if (!(pz >= 0.0)) fail(NX_E_VALUE, "p < 0 or is NaN in binomial distribution");
@@ -2047,8 +2123,11 @@ int nx_state_measure(nx_state* s, int32_t nq, const int32_t* qargs, const uint8_
   })
 }
 
-int nx_state_sample(const nx_state* s, int32_t nq, const int32_t* qargs, uint64_t shots, const uint8_t* seed32,
-                    int64_t max_out, uint64_t* out_bits, uint64_t* out_counts, int64_t* nout) {
+// sampling.rs:13-189.  ref: every node norm as the reference's FP64 sum
+// (Scalar coefficients; seeded outcomes equal the reference's); otherwise
+// exact node norms, certified uniform levels and exact tails.
+static int sample_impl(const nx_state* s, int32_t nq, const int32_t* qargs, uint64_t shots, const uint8_t* seed32,
+                       int64_t max_out, uint64_t* out_bits, uint64_t* out_counts, int64_t* nout, bool ref) {
   NX_TRY({
     validate_qargs(s, nq, qargs);
     uint8_t seed[32];
@@ -2067,13 +2146,21 @@ int nx_state_sample(const nx_state* s, int32_t nq, const int32_t* qargs, uint64_
     if (shots > 0 && nq > 0 && s->chi() == 0)  // every norm is 0: p = 0/0
       fail(NX_E_VALUE, "p < 0 or is NaN in binomial distribution");
     if (shots > 0 && nq > 0) {
-      // at least the last level keeps its window: it also reports a zero state
-      cx.uniform = std::min(uniform_prefix(s, cx.q), nq - 1);
       cx.s = s;
-      // exact tails need every qubit sampled (leaf = one amplitude) and
-      // Scalar coefficients; 2^14 completions cost chi 2^14 amplitude
-      // evaluations against ceil(14/6) windows of chi^2 / 2 pairs
-      cx.tail_m = (!s->is_complex && nq == s->n) ? 14 : 0;
+      cx.ref = ref && !s->is_complex;
+      if (cx.ref) {
+        const int64_t chi = s->chi();
+        if (chi * (chi + 1) / 2 > ((int64_t)1 << 28))
+          fail(NX_E_VALUE, "reference-order sampling holds one record per term pair: at most 23,169 terms, got %lld",
+               (long long)chi);
+      } else {
+        // at least the last level keeps its window: it also reports a zero state
+        cx.uniform = std::min(uniform_prefix(s, cx.q), nq - 1);
+        // exact tails need every qubit sampled (leaf = one amplitude) and
+        // Scalar coefficients; 2^14 completions cost chi 2^14 amplitude
+        // evaluations against ceil(14/6) windows of chi^2 / 2 pairs
+        cx.tail_m = (!s->is_complex && nq == s->n) ? 14 : 0;
+      }
       cx.pp.init(s, nq, cx.look);
       recursive_sample(cx, 0, shots, 0, 0, nullptr, 0);
     } else if (shots > 0) {
@@ -2089,6 +2176,16 @@ int nx_state_sample(const nx_state* s, int32_t nq, const int32_t* qargs, uint64_
     std::memcpy(out_counts, cx.counts.data(), k * 8);
     *nout = k;
   })
+}
+
+int nx_state_sample(const nx_state* s, int32_t nq, const int32_t* qargs, uint64_t shots, const uint8_t* seed32,
+                    int64_t max_out, uint64_t* out_bits, uint64_t* out_counts, int64_t* nout) {
+  return sample_impl(s, nq, qargs, shots, seed32, max_out, out_bits, out_counts, nout, true);
+}
+
+int nx_state_sample_exact(const nx_state* s, int32_t nq, const int32_t* qargs, uint64_t shots, const uint8_t* seed32,
+                          int64_t max_out, uint64_t* out_bits, uint64_t* out_counts, int64_t* nout) {
+  return sample_impl(s, nq, qargs, shots, seed32, max_out, out_bits, out_counts, nout, false);
 }
 
 int nx_state_uniform_prefix(const nx_state* s, int32_t nq, const int32_t* qargs, int32_t* out) {

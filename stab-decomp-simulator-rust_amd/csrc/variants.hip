@@ -1027,6 +1027,18 @@ __global__ __launch_bounds__(256, (W == 1 && VW == 1) ? 8 : 1) void k_pairs_var(
       P.c0 = (pf - po) & 7;
       P.mag = kf + ko;
       pair_eval_v<W, VW, UNITS>(P, V, nco, ncf, tv ? 0 : jd.nunit, jd.uq, jd.ub);
+      if constexpr (UNITS && VW == 1) {
+        if (jd.rec != nullptr && lane == 0) {
+          // exact per-variant values, at the pair's position in the
+          // reference's summation order (row f, then o = f .. chi - 1)
+          uint64_t* rp = jd.rec + (size_t)FOLD_REC * (size_t)(fx * nother - fx * (fx - 1) / 2 + (o - fx));
+          rp[0] = V.z[0];
+          rp[1] = V.d0[0];
+          rp[2] = V.d1[0];
+          rp[3] = V.d2[0];
+          rp[4] = (uint64_t)((P.c0 + V.dv) & 7) | ((uint64_t)(uint32_t)P.mag << 8);
+        }
+      }
       // (c_bra.conj() * c_ket).into(), shared by the variants
       double cr_, ci_;
       int kc = 0, rc = 0, czero = 0;  // exact coefficient product zeta^kc 2^{-rc/2}

@@ -39,6 +39,7 @@ SIGNATURES = {
     "nx_state_discard": (C.c_int, [_P, _I32]),
     "nx_state_measure": (C.c_int, [_P, _I32, _I32P, _U8P, _U8P]),
     "nx_state_sample": (C.c_int, [_P, _I32, _I32P, _U64, _U8P, _I64, _U64P, _U64P, _I64P]),
+    "nx_state_sample_exact": (C.c_int, [_P, _I32, _I32P, _U64, _U8P, _I64, _U64P, _U64P, _I64P]),
     "nx_state_export_term": (C.c_int, [_P, _I64, _U8P, _U8P, _U8P, _U8P, _U8P, _U8P, _U8P]),
     "nx_state_get_scalar_coefficients": (C.c_int, [_P, _U8P, _U8P, _I64P]),
     "nx_state_get_coefficients": (C.c_int, [_P, _DP]),

@@ -247,7 +247,12 @@ class QuantumState:
     def measure_all(self, seed=None):
         return self.measure(list(range(self.num_qubits)), seed)
 
-    def sample(self, qargs, shots, seed=None):
+    def sample(self, qargs, shots, seed=None, exact=False):
+        """sampling.rs:13-189.  exact=False: node probabilities as the
+        reference's FP64 sums (a seed gives the reference's outcomes);
+        exact=True: exact probabilities, certified uniform levels and exact
+        tails (same distribution, much faster on large states; seeded
+        outcomes can differ where the reference's sums round)."""
         qargs = [_usize(q) for q in qargs]
         shots = _usize(shots)
         arr, p = _i32_array(qargs)
@@ -263,7 +268,8 @@ class QuantumState:
             bits = np.zeros(2 * cap, np.uint64)
             counts = np.zeros(cap, np.uint64)
             nout.value = 0
-            code = lib().nx_state_sample(self._h, len(qargs), p, shots, sp, cap,
+            fn = lib().nx_state_sample_exact if exact else lib().nx_state_sample
+            code = fn(self._h, len(qargs), p, shots, sp, cap,
                                          bits.ctypes.data_as(C.POINTER(C.c_uint64)),
                                          counts.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(nout))
             if code == 0 or nout.value <= cap:

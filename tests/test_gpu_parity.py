@@ -273,38 +273,41 @@ def test_measure_same_stream_as_oracle(nx):
         assert m_g == m_o
 
 
-def _dfs_probabilities_dyadic(o, qargs):
-    """True when every conditional probability sample() can meet is a
-    dyadic rational (then the oracle's FP64 sums are exact too)."""
-    sv = np.asarray(o.statevector())
-    p = np.abs(sv) ** 2
-    return bool(np.all(np.abs(p - np.round(p * 2.0 ** 24) / 2.0 ** 24) <= 1e-12))
+def _oracle_counts(o, qargs, shots, seed):
+    return {"".join("1" if b else "0" for b in bits): c for bits, c in o.sample(qargs, shots, seed)}
 
 
-def test_sample_same_stream_as_oracle(nx):
-    """sample() evaluates the probabilities of up to 64 DFS nodes per device
-    pass, exactly (sums of powers of two and sqrt 2).  Where the reference's
-    FP64 sums are exact as well (dyadic outcome probabilities), the same
-    seed gives the oracle's outcomes; at mathematically exact ties of
-    irrational probabilities the reference's own rounding picks a branch of
-    rand_distr's binomial, which no reordering of its sums reproduces (the
-    RNG stream is parity unpinned), so those states are checked
-    statistically below."""
-    rng = np.random.default_rng(505)
-    checked = 0
-    for trial in range(4):
-        n = int(rng.integers(3, 8))
-        gates = WL.random_circuit(rng, n, 50, 4, gate_set=tuple(range(11)))
-        g, o = compile_both(n, gates)
-        qargs = [int(q) for q in rng.permutation(n)]
-        if not _dfs_probabilities_dyadic(o, qargs):
-            continue
-        say(f"sample trial={trial} n={n} chi={o.chi}")
-        res_g = g.sample(qargs, 500, seed=trial)
-        res_o = {"".join("1" if b else "0" for b in bits): c for bits, c in o.sample(qargs, 500, trial)}
-        assert res_g == res_o
-        checked += 1
-    assert checked >= 1
+@pytest.mark.parametrize("seed,n,nt,nq,shots", [
+    (501, 6, 12, 6, 500),     # one window, every qubit (chi 27)
+    (502, 8, 16, 8, 800),     # two windows, every qubit (chi 36)
+    (503, 10, 14, 7, 1000),   # subset of qubits, unsorted, two windows (chi 45)
+    (504, 9, 18, 9, 300),     # two windows, chi 96
+    (505, 7, 12, 4, 2000),    # subset, one window (chi 24)
+    (506, 12, 14, 12, 200),   # every qubit, two windows (chi 16)
+])
+def test_sample_same_stream_as_oracle(nx, seed, n, nt, nq, shots):
+    """sample() on Clifford+T states with generic (irrational, tied)
+    probabilities: every node probability is the reference's FP64 value
+    (norm.rs:8-30 summed over the projected terms in its order), so a seed
+    gives the oracle's outcomes exactly (sampling.rs:80-189).  Single-qubit
+    H and S after the circuit make the pair values complex, so the order of
+    each pair's operands matters to the rounding.  The exact mode
+    (exact=True) may differ on these states; its outcomes are reported."""
+    rng = np.random.default_rng(seed)
+    gates = WL.random_circuit(rng, n, 12 * n, nt, gate_set=tuple(range(11)))
+    for q in range(n):
+        if rng.integers(2):
+            gates += [(WL.H, q), (WL.S, q)]
+    g, o = compile_both(n, gates)
+    qargs = [int(q) for q in rng.permutation(n)[:nq]]
+    say(f"sample seed={seed} n={n} chi={o.chi} qargs={qargs} shots={shots}")
+    res_o = _oracle_counts(o, qargs, shots, seed)
+    res_g = g.sample(qargs, shots, seed=seed)
+    assert res_g == res_o
+    assert res_g == g.sample(qargs, shots, seed=seed)
+    res_x = g.sample(qargs, shots, seed=seed, exact=True)
+    assert sum(res_x.values()) == shots
+    say(f"  exact mode {'equal' if res_x == res_o else 'differs'}")
 
 
 @pytest.mark.parametrize("seed,n,nq,shots", [
@@ -317,15 +320,36 @@ def test_sample_same_stream_as_oracle(nx):
 ])
 def test_sample_clifford_windows_same_stream(nx, seed, n, nq, shots):
     """Clifford states (probabilities 0, 1/2^k, 1): seeded outcomes equal
-    the oracle's across several look-ahead windows."""
+    the oracle's across several look-ahead windows, in both modes (the
+    reference's sums are exact here, so the exact mode agrees too)."""
     rng = np.random.default_rng(seed)
     gates = WL.random_circuit(rng, n, 10 * n, 0, gate_set=tuple(range(11)))
     g, o = compile_both(n, gates)
     qargs = [int(q) for q in rng.choice(n, nq, replace=False)]
     say(f"clifford sample seed={seed} n={n} qargs={qargs}")
-    res_g = g.sample(qargs, shots, seed=seed)
-    res_o = {"".join("1" if b else "0" for b in bits): c for bits, c in o.sample(qargs, shots, seed)}
-    assert res_g == res_o
+    res_o = _oracle_counts(o, qargs, shots, seed)
+    assert g.sample(qargs, shots, seed=seed) == res_o
+    assert g.sample(qargs, shots, seed=seed, exact=True) == res_o
+
+
+def test_sample_dyadic_clifford_t_exact_mode(nx):
+    """The exact mode on Clifford+T states whose outcome probabilities are
+    dyadic (the reference's FP64 sums are exact): the same seed gives the
+    oracle's outcomes."""
+    rng = np.random.default_rng(505)
+    checked = 0
+    for trial in range(8):
+        n = int(rng.integers(3, 8))
+        gates = WL.random_circuit(rng, n, 50, 4, gate_set=tuple(range(11)))
+        g, o = compile_both(n, gates)
+        qargs = [int(q) for q in rng.permutation(n)]
+        p = np.abs(np.asarray(o.statevector())) ** 2
+        if not np.all(np.abs(p - np.round(p * 2.0 ** 24) / 2.0 ** 24) <= 1e-12):
+            continue
+        say(f"dyadic sample trial={trial} n={n} chi={o.chi}")
+        assert g.sample(qargs, 500, seed=trial, exact=True) == _oracle_counts(o, qargs, 500, trial)
+        checked += 1
+    assert checked >= 1
 
 
 def test_projection_norms_vs_oracle(nx):
@@ -363,7 +387,7 @@ def test_projection_norms_vs_oracle(nx):
     (602, 12, 120, 5, 8, 20000),   # subset of qubits, unsorted
 ])
 def test_sample_distribution_clifford_t(nx, seed, n, nc, nt, nq, shots):
-    """Clifford+T states: the sampled distribution matches the exact one
+    """Clifford+T states, exact mode: the sampled distribution matches the exact one
     (from the oracle's state vector; total variation within three times its
     expected sampling noise) and a seed reproduces its outcomes."""
     rng = np.random.default_rng(seed)
@@ -371,8 +395,8 @@ def test_sample_distribution_clifford_t(nx, seed, n, nc, nt, nq, shots):
     g, o = compile_both(n, gates)
     qargs = [int(q) for q in rng.choice(n, nq, replace=False)]
     say(f"sample distribution seed={seed} n={n} chi={o.chi} qargs={qargs}")
-    res = g.sample(qargs, shots, seed=seed)
-    assert res == g.sample(qargs, shots, seed=seed)
+    res = g.sample(qargs, shots, seed=seed, exact=True)
+    assert res == g.sample(qargs, shots, seed=seed, exact=True)
     assert sum(res.values()) == shots
     p = np.abs(np.asarray(o.statevector())) ** 2
     exact = {}
