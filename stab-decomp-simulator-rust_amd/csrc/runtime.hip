@@ -1022,12 +1022,21 @@ static void seed_or_entropy(const uint8_t* seed32, uint8_t out[32]) {
   }
 }
 
+// norm_squared() of the state projected onto qubit q = 0 and = 1, summed as
+// the reference sums it (Scalar coefficients; defined with the sampling
+// windows below)
+static bool ref_outcome_norms(const nx_state* s, int q, double* p0, double* p1);
+
 // measurement.rs:34-103
 static int measure_single(nx_state* s, int q, ChaCha12Rng& rng) {
   StateGuard zero(clone_state(s)), one(clone_state(s));
   project_unnormalized(zero.p, q, 0);
   project_unnormalized(one.p, q, 1);
-  const double p0 = norm_squared(zero.p), p1 = norm_squared(one.p);
+  double p0 = 0.0, p1 = 0.0;
+  if (!ref_outcome_norms(s, q, &p0, &p1)) {
+    p0 = norm_squared(zero.p);
+    p1 = norm_squared(one.p);
+  }
   const double total = p0 + p1;
   if (std::fabs(total) < 1e-12) fail(NX_E_VALUE, "Not implemented: Measurement on a zero-norm state is not possible.");
   const double pz = p0 / total;
@@ -1707,23 +1716,22 @@ static Window open_window(SampleCtx& cx, int root) {
   return w;
 }
 
-// A window whose node norms are the reference's FP64 values: norm_squared()
-// of the state projected on the node's path (norm.rs:8-30 over the terms
-// that projection.rs:35-51 keeps), summed in its order.  The pass writes the
-// exact per-leaf values of every pair; k_fold_diag / k_fold_terms /
-// k_fold_sum turn them into each node's sequential sum (fold.hip), and the
-// host multiplies by |g|^2 as norm.rs:29 does.
-static Window open_window_ref(SampleCtx& cx, int root) {
-  Window w;
-  w.root = root;
-  w.ref = true;
-  w.d = std::min<int>(cx.look, (int)cx.q.size() - root);
-  std::vector<int> fq(cx.q.begin(), cx.q.begin() + root), fb(cx.path.begin(), cx.path.begin() + root);
-  std::vector<int> lq(cx.q.begin() + root, cx.q.begin() + root + w.d);
-  const int64_t chi = cx.s->chi();
+// Node norms in the reference's FP64 summation order: for the window of
+// look-ahead qubits lq below the fixed outcomes (fq, fb), norm_squared() of
+// the state projected on each node's path (norm.rs:8-30 over the terms that
+// projection.rs:35-51 keeps), summed in its order, times |g|^2 as norm.rs:29
+// does; node u = 2^j - 2 + k (relative depth j, outcome index k).  The pass
+// writes the exact per-leaf values of every pair; k_fold_diag /
+// k_fold_terms / k_fold_sum turn them into each node's sequential sum.
+static int64_t ref_fold_max_pairs() { return (int64_t)1 << 28; }
+
+static std::vector<double> ref_node_norms(const nx_state* s, ProjPlan& pp, RefFold& rf, const std::vector<int>& fq,
+                                          const std::vector<int>& fb, const std::vector<int>& lq) {
+  const int d = (int)lq.size();
+  const int64_t chi = s->chi();
   const int64_t np = chi * (chi + 1) / 2;
-  const int nn = (2 << w.d) - 2;
-  RefFold& rf = cx.rf;
+  const int nn = (2 << d) - 2;
+  if (np > ref_fold_max_pairs()) fail(NX_E_INTERNAL, "too many term pairs for the reference-order fold");
   if (!rf.rec.p) {
     rf.rec.alloc((size_t)np * FOLD_REC * 8);
     rf.D.alloc((size_t)chi * FOLD_NODES);
@@ -1734,16 +1742,15 @@ static Window open_window_ref(SampleCtx& cx, int root) {
   }
   double* sums = rf.sums.as<double>();
   int* err = (int*)(rf.sums.as<char>() + FOLD_NODES * 8);
-  cx.pp.run(fq, fb, lq, rf.rec.as<uint64_t>());
-  ++cx.passes;
+  pp.run(fq, fb, lq, rf.rec.as<uint64_t>());
   {
     ProfScope ps("k_fold");
     hip_check(hipMemsetAsync(rf.sums.p, 0, FOLD_NODES * 8 + 4, stream()), "hipMemsetAsync");
-    hip_check(launch_fold_diag(rf.rec.as<uint64_t>(), chi, w.d, rf.D.as<uint8_t>(), err, stream()), "k_fold_diag");
+    hip_check(launch_fold_diag(rf.rec.as<uint64_t>(), chi, d, rf.D.as<uint8_t>(), err, stream()), "k_fold_diag");
     for (int64_t p0 = 0; p0 < np; p0 += rf.chunk) {
       const int64_t m = std::min<int64_t>(rf.chunk, np - p0);
-      hip_check(launch_fold_terms(rf.rec.as<uint64_t>(), chi, p0, m, rf.chunk, w.d, rf.D.as<uint8_t>(),
-                                  coef_of(cx.s).cs, ctx()->tabs, rf.t.as<double>(), err, stream()),
+      hip_check(launch_fold_terms(rf.rec.as<uint64_t>(), chi, p0, m, rf.chunk, d, rf.D.as<uint8_t>(),
+                                  coef_of(s).cs, ctx()->tabs, rf.t.as<double>(), err, stream()),
                 "k_fold_terms");
       hip_check(launch_fold_sum(rf.t.as<double>(), m, rf.chunk, nn, sums, stream()), "k_fold_sum");
     }
@@ -1753,12 +1760,34 @@ static Window open_window_ref(SampleCtx& cx, int root) {
   sync();
   if (*(const int*)(rf.host.as<char>() + FOLD_NODES * 8))
     fail(NX_E_INTERNAL, "a projected pair value is not of the form zeta^k 2^(-r/2)");
-  const double* hs = rf.host.as<double>();
+  std::vector<double> out(nn);
+  for (int u = 0; u < nn; ++u) out[u] = rf.host.as<double>()[u] * pp.g2;
+  return out;
+}
+
+static bool ref_outcome_norms(const nx_state* s, int q, double* p0, double* p1) {
+  const int64_t chi = s->chi();
+  if (s->is_complex || chi == 0 || chi * (chi + 1) / 2 > ref_fold_max_pairs()) return false;
+  ProjPlan pp;
+  pp.init(s, 1, 1);
+  RefFold rf;
+  const std::vector<double> nv = ref_node_norms(s, pp, rf, {}, {}, {q});
+  *p0 = nv[0];
+  *p1 = nv[1];
+  return true;
+}
+
+static Window open_window_ref(SampleCtx& cx, int root) {
+  Window w;
+  w.root = root;
+  w.ref = true;
+  w.d = std::min<int>(cx.look, (int)cx.q.size() - root);
+  std::vector<int> fq(cx.q.begin(), cx.q.begin() + root), fb(cx.path.begin(), cx.path.begin() + root);
+  std::vector<int> lq(cx.q.begin() + root, cx.q.begin() + root + w.d);
+  const std::vector<double> nv = ref_node_norms(cx.s, cx.pp, cx.rf, fq, fb, lq);
+  ++cx.passes;
   w.nr.resize(w.d + 1);
-  for (int j = 1; j <= w.d; ++j) {
-    w.nr[j].resize((size_t)1 << j);
-    for (int k = 0; k < (1 << j); ++k) w.nr[j][k] = hs[(1 << j) - 2 + k] * cx.pp.g2;
-  }
+  for (int j = 1; j <= w.d; ++j) w.nr[j].assign(nv.begin() + ((1 << j) - 2), nv.begin() + ((2 << j) - 2));
   return w;
 }
 
@@ -2150,7 +2179,7 @@ static int sample_impl(const nx_state* s, int32_t nq, const int32_t* qargs, uint
       cx.ref = ref && !s->is_complex;
       if (cx.ref) {
         const int64_t chi = s->chi();
-        if (chi * (chi + 1) / 2 > ((int64_t)1 << 28))
+        if (chi * (chi + 1) / 2 > ref_fold_max_pairs())
           fail(NX_E_VALUE, "reference-order sampling holds one record per term pair: at most 23,169 terms, got %lld",
                (long long)chi);
       } else {

@@ -260,17 +260,25 @@ def test_projection_discard_gates_parity(nx):
 def test_measure_same_stream_as_oracle(nx):
     """Same seed -> same measure() outcomes as the oracle (both restate rand
     0.8's StdRng; measure projects and renormalises per qubit as
-    measurement.rs:34-103 does)."""
+    measurement.rs:34-103 does), and the same global factor bit for bit
+    afterwards: the outcome norms are the reference's FP64 sums, so the
+    renormalisation 1/sqrt(p) is too.  H and S after the circuit make the
+    pair values complex."""
     rng = np.random.default_rng(505)
-    for trial in range(4):
-        n = int(rng.integers(3, 8))
-        gates = WL.random_circuit(rng, n, 50, 4, gate_set=tuple(range(11)))
+    for trial in range(6):
+        n = int(rng.integers(3, 9))
+        gates = WL.random_circuit(rng, n, 10 * n, 6 + trial, gate_set=tuple(range(11)))
+        for q in range(n):
+            if rng.integers(2):
+                gates += [(WL.H, q), (WL.S, q)]
         g, o = compile_both(n, gates)
         say(f"measure trial={trial} n={n} chi={o.chi}")
         qargs = [int(q) for q in rng.permutation(n)]
-        m_g = g.clone().measure(qargs, seed=trial + 7)
-        m_o = o.clone().measure(qargs, trial + 7)
+        gc, oc = g.clone(), o.clone()
+        m_g = gc.measure(qargs, seed=trial + 7)
+        m_o = oc.measure(qargs, trial + 7)
         assert m_g == m_o
+        assert gc.global_factor == oc.global_factor, (gc.global_factor, oc.global_factor)
 
 
 def _oracle_counts(o, qargs, shots, seed):
