@@ -14,7 +14,8 @@ memory, nx_state_norm_async) are timed as well.
 value = (amplitude evaluations + inner products) per second of step time.
 
 The same run also measures, by default:
-  * cfg1 sampling: sample(range(64), 1024, seed=1001);
+  * cfg1 sampling: sample(range(64), 1024, seed=1001), in the reference's
+    FP64 order (the default) and with exact probabilities (exact=True);
   * cfg2 batched vs plain: 256 circuits (n ~ U[16,128], 30 n Cliffords,
     t ~ U[4,32]), norm + 64 Pauli expectations each, as one necstar.Batch
     (autotuned launch shape, captured HIP graph) against the plain per-call
@@ -346,27 +347,45 @@ def main():
 
     sampling = None
     if args.sample_shots > 0:
+        def kprof(name):
+            ms, cnt = C.c_double(), C.c_int64()
+            _lib.check(L.nx_profile_get(name.encode(), C.byref(ms), C.byref(cnt)))
+            return ms.value, cnt.value
+        qs = list(range(n))
+        # exact mode: exact node probabilities, certified uniform levels,
+        # exact tails (timed, then again with per-launch timing)
         ts0 = time.perf_counter()
-        counts = st.sample(list(range(n)), args.sample_shots, seed=cfg["sample_seed"])
+        counts = st.sample(qs, args.sample_shots, seed=cfg["sample_seed"], exact=True)
         te = time.perf_counter() - ts0
-        # again with per-launch timing for the pass counts
         _lib.check(L.nx_profile_enable(1))
-        counts2 = st.sample(list(range(n)), args.sample_shots, seed=cfg["sample_seed"])
-        assert counts2 == counts
-        pms, pcnt = C.c_double(), C.c_int64()
-        _lib.check(L.nx_profile_get(b"k_pairs_var", C.byref(pms), C.byref(pcnt)))
-        sms, scnt = C.c_double(), C.c_int64()
-        _lib.check(L.nx_profile_get(b"k_pairs_sym", C.byref(sms), C.byref(scnt)))
+        p0, s0 = kprof("k_pairs_var"), kprof("k_pairs_sym")
+        assert st.sample(qs, args.sample_shots, seed=cfg["sample_seed"], exact=True) == counts
+        p1, s1 = kprof("k_pairs_var"), kprof("k_pairs_sym")
+        _lib.check(L.nx_profile_enable(0))
+        # reference order (the default): every node probability as the
+        # reference's FP64 sum, one window pass + fold per 6 levels
+        log(f"sampling {args.sample_shots} shots in reference order")
+        _lib.check(L.nx_profile_enable(1))
+        p2, f2 = kprof("k_pairs_var"), kprof("k_fold")
+        ts0 = time.perf_counter()
+        counts_ref = st.sample(qs, args.sample_shots, seed=cfg["sample_seed"])
+        tr = time.perf_counter() - ts0
+        p3, f3 = kprof("k_pairs_var"), kprof("k_fold")
         _lib.check(L.nx_profile_enable(0))
         # DFS nodes with shots (distinct outcome prefixes in the processing
         # order, qubit n-1 first): the reference evaluates 3 norms of the
         # projected state at each (sampling.rs:101-153)
-        keys = [k[::-1] for k in counts]
+        keys = [k[::-1] for k in counts_ref]
         nodes = sum(len({k[:d] for k in keys}) for d in range(n))
-        sampling = {"shots": args.sample_shots, "qubits": n, "seconds": te, "distinct_outcomes": len(counts),
-                    "uniform_levels": st.uniform_prefix(list(range(n - 1, -1, -1))),
-                    "uniform_pass_ms": sms.value, "projection_passes": pcnt.value, "pass_kernel_ms": pms.value,
-                    "dfs_nodes": nodes}
+        sampling = {"shots": args.sample_shots, "qubits": n, "dfs_nodes": nodes,
+                    "reference_order": {"seconds": tr, "distinct_outcomes": len(counts_ref),
+                                        "windows": p3[1] - p2[1], "pass_kernel_ms": p3[0] - p2[0],
+                                        "fold_kernel_ms": f3[0] - f2[0]},
+                    "exact": {"seconds": te, "distinct_outcomes": len(counts),
+                              "uniform_levels": st.uniform_prefix(list(range(n - 1, -1, -1))),
+                              "uniform_pass_ms": s1[0] - s0[0], "projection_passes": p1[1] - p0[1],
+                              "pass_kernel_ms": p1[0] - p0[0]},
+                    "same_outcomes": counts == counts_ref}
 
     cfg2 = cfg2_compare(args.circuits, args.cfg2_steps, log) if args.circuits > 0 else None
 

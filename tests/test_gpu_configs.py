@@ -1,8 +1,9 @@
 """The synthetic workloads at their real shapes on the GPU.
 
-  cfg1  sample(range(64), 1024, seed=1001) of the 64-qubit, 24-T state:
-        every outcome has a nonzero amplitude (oracle), and every qubit's
-        sampled marginal agrees with exp_value(Z_q) within 4 sigma;
+  cfg1  sample(range(64), 1024, seed=1001) of the 64-qubit, 24-T state in
+        both modes (reference order, exact): sampled outcomes have a nonzero
+        amplitude (oracle), and every qubit's sampled marginal agrees with
+        exp_value(Z_q) within 4 sigma;
   cfg2  all 256 circuits through necstar.Batch (autotuned, HIP graph) against
         the plain per-call norm() / exp_value() and, on the smallest
         circuits, the oracle, within 1e-12 relative;
@@ -50,29 +51,34 @@ def test_cfg1_sampling_support_and_marginals(nx):
     n = cfg["n"]
     st = nx.QuantumState.from_circuit(WL.to_circuit(n, cfg["gates"]))
     say(f"cfg1 sampling chi={st.stabilizer_rank}")
-    counts = st.sample(list(range(n)), cfg["shots"], seed=cfg["sample_seed"])
-    assert sum(counts.values()) == cfg["shots"]
-    assert counts == st.sample(list(range(n)), cfg["shots"], seed=cfg["sample_seed"])
-    # (a) sampled outcomes lie in the support: nonzero amplitude by the oracle
+    qs = list(range(n))
+    counts_x = st.sample(qs, cfg["shots"], seed=cfg["sample_seed"], exact=True)
+    assert counts_x == st.sample(qs, cfg["shots"], seed=cfg["sample_seed"], exact=True)
+    say("cfg1 reference-order sampling")
+    counts_r = st.sample(qs, cfg["shots"], seed=cfg["sample_seed"])
     o = O.State.compile(n, cfg["gates"])
-    keys = sorted(counts)
-    rng = np.random.default_rng(17)
-    for key in [keys[int(i)] for i in rng.choice(len(keys), min(48, len(keys)), replace=False)]:
-        x = np.array([1 if ch == "1" else 0 for ch in key], np.uint8)  # char q <-> qubit q
-        assert abs(o.amplitude(x)) > 0.0, key
-    # (b) single-qubit marginals against <Z_q> (the state has norm 1)
     zs = [nx.PauliString.from_str(f"Z{q}") for q in range(n)]
     ez = st.exp_values(zs)
-    shots = cfg["shots"]
-    ones = np.zeros(n)
-    for key, c in counts.items():
-        ones += c * np.array([ch == "1" for ch in key])
-    p1 = (1.0 - ez) / 2.0
-    sigma = np.sqrt(p1 * (1.0 - p1) / shots)
-    dev = np.abs(ones / shots - p1)
-    say(f"cfg1 marginals: largest deviation {np.max(dev / np.maximum(sigma, 1e-300)):.2f} sigma, "
-        f"{len(counts)} distinct outcomes")
-    assert np.all(dev <= 4.0 * sigma + 1e-12), [(q, ones[q] / shots, p1[q]) for q in np.nonzero(dev > 4 * sigma)[0]]
+    rng = np.random.default_rng(17)
+    for mode, counts in (("exact", counts_x), ("reference order", counts_r)):
+        assert sum(counts.values()) == cfg["shots"]
+        # (a) sampled outcomes lie in the support: nonzero amplitude by the oracle
+        keys = sorted(counts)
+        for key in [keys[int(i)] for i in rng.choice(len(keys), min(32, len(keys)), replace=False)]:
+            x = np.array([1 if ch == "1" else 0 for ch in key], np.uint8)  # char q <-> qubit q
+            assert abs(o.amplitude(x)) > 0.0, (mode, key)
+        # (b) single-qubit marginals against <Z_q> (the state has norm 1)
+        shots = cfg["shots"]
+        ones = np.zeros(n)
+        for key, c in counts.items():
+            ones += c * np.array([ch == "1" for ch in key])
+        p1 = (1.0 - ez) / 2.0
+        sigma = np.sqrt(p1 * (1.0 - p1) / shots)
+        dev = np.abs(ones / shots - p1)
+        say(f"cfg1 {mode}: largest marginal deviation {np.max(dev / np.maximum(sigma, 1e-300)):.2f} sigma, "
+            f"{len(counts)} distinct outcomes")
+        assert np.all(dev <= 4.0 * sigma + 1e-12), [(mode, q, ones[q] / shots, p1[q])
+                                                     for q in np.nonzero(dev > 4 * sigma)[0]]
 
 
 def test_sample_reports_needed_capacity(nx):
