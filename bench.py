@@ -483,6 +483,10 @@ def bench_cfg4(args):
     evals = float(chi) * B
     kms = ms.value / max(cnt.value, 1)
     amp = d_out.view(torch.complex128).cpu().numpy()
+    if args.copy_probe:  # a 1 GiB device copy: calibrates the HBM byte counters
+        src = torch.empty(1 << 27, dtype=torch.float64, device=dev)
+        torch.empty_like(src).copy_(src)
+        torch.cuda.synchronize()
     print(json.dumps({
         "metric": "cfg4 CH-form amplitude evals/s, n=128", "value": evals / step, "unit": "amplitude evals/s",
         "higher_is_better": True, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

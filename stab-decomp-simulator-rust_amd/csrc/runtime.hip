@@ -640,7 +640,11 @@ static cplx run_pairs(const QForms& X, const QForms& Y, int64_t nfixed, int64_t 
                          partial.as<double4>(), nblocks, d_exact, ld, stream()),
             "k_pairs");
   ps.end();
-  hip_check(launch_reduce(partial.as<double4>(), nblocks, t_res.as<double4>(), stream()), "k_reduce");
+  {
+    ProfScope pr("k_reduce");
+    hip_check(launch_reduce(partial.as<double4>(), nblocks, t_res.as<double4>(), stream()), "k_reduce");
+    pr.end();
+  }
   sync();
   return ab_value(t_res.as<double>());
 }
