@@ -194,7 +194,8 @@ hipError_t launch_pauli_var(const FormsDev& in, const int32_t* voff, const int32
 // word width W; wave_job[w] = job of wave w; partial holds 2 (64 * VW + 1)
 // doubles per wave of the plan: the real Hermitian sums per variant and the
 // base, then their sqrt(2) parts (exact sums of Scalar-coefficient states
-// are A + sqrt(2) B with A, B sums of powers of two)
+// are A + sqrt(2) B with A, B sums of powers of two).  units: 0 Pauli
+// variants, 1 unit constraints, 2 unit constraints and pair records (VW = 1)
 hipError_t launch_pairs_var(int W, int VW, int units, const PairJobDev* jobs, const int32_t* wave_job,
                             int64_t wave_base, int64_t nwaves, int J, const double* tabs, double* partial,
                             hipStream_t st);

@@ -1170,11 +1170,13 @@ struct VarPlan {
   // jobs_dev: the job table to read (default: this plan's own)
   // out_override: where the reduction writes instead of out (page-locked
   // host memory for results the host reads right after)
-  void enqueue(hipStream_t st, const PairJobDev* jobs_dev = nullptr, double* out_override = nullptr) const {
+  // rec: the jobs carry pair records (PairJobDev::rec; unit plans, VW = 1)
+  void enqueue(hipStream_t st, const PairJobDev* jobs_dev = nullptr, double* out_override = nullptr,
+               bool rec = false) const {
     const PairJobDev* jd = jobs_dev ? jobs_dev : jobs.as<PairJobDev>();
     for (const Launch& L : launches)
-      hip_check(launch_pairs_var(L.W, VW, units, jd, wave_job.as<int32_t>(), L.wave_base, L.nwaves, J, ctx()->tabs,
-                                 partial.as<double>(), st),
+      hip_check(launch_pairs_var(L.W, VW, rec ? 2 : units, jd, wave_job.as<int32_t>(), L.wave_base, L.nwaves, J,
+                                 ctx()->tabs, partial.as<double>(), st),
                 "k_pairs_var");
     hip_check(launch_reduce_var(jd, njobs, VW, partial.as<double>(), seg_job.as<int32_t>(), nseg,
                                 seg_part.as<double>(), out_override ? out_override : out.as<double>(), st),
@@ -1559,7 +1561,8 @@ struct ProjPlan {
     }
     {
       ProfScope ps("k_pairs_var");
-      plan.enqueue(stream(), (const PairJobDev*)dbase, res_h.as<double>());
+      if (rec && VW != 1) fail(NX_E_INTERNAL, "pair records need one word of variants");
+      plan.enqueue(stream(), (const PairJobDev*)dbase, res_h.as<double>(), rec != nullptr);
       ps.end();
     }
     sync();

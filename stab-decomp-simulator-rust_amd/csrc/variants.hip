@@ -911,8 +911,10 @@ __global__ __launch_bounds__(256) void k_pauli_var(FormsDev in, const int32_t* _
 //   sum_f sum_{o >= f} (o == f ? 1 : 2) Re(conj(c_o) c_f <X_o | Y_f^(k)>)
 // which is exp_value.rs:38-58 for Y_f^(k) = P_k phi_f (norm.rs:16-27 for
 // the base).  Per wave, partial holds 64*VW variant sums then the base.
-template <int W, int VW, bool UNITS>
-__global__ __launch_bounds__(256, (W == 1 && VW == 1) ? 8 : 1) void k_pairs_var(const PairJobDev* __restrict__ jobs,
+// REC: also write the pair records of jd.rec (window passes whose node
+// probabilities are folded in the reference's order; VW = 1)
+template <int W, int VW, bool UNITS, bool REC = false>
+__global__ __launch_bounds__(256, (W == 1 && VW == 1 && !REC) ? 8 : 1) void k_pairs_var(const PairJobDev* __restrict__ jobs,
                                                    const int32_t* __restrict__ wave_job, int64_t wave_base,
                                                    int64_t nwaves_launch, int J, const double* __restrict__ tabs,
                                                    double* __restrict__ partial) {
@@ -1027,8 +1029,8 @@ __global__ __launch_bounds__(256, (W == 1 && VW == 1) ? 8 : 1) void k_pairs_var(
       P.c0 = (pf - po) & 7;
       P.mag = kf + ko;
       pair_eval_v<W, VW, UNITS>(P, V, nco, ncf, tv ? 0 : jd.nunit, jd.uq, jd.ub);
-      if constexpr (UNITS && VW == 1) {
-        if (jd.rec != nullptr && lane == 0) {
+      if constexpr (REC && UNITS && VW == 1) {
+        if (lane == 0) {
           // exact per-variant values, at the pair's position in the
           // reference's summation order (row f, then o = f .. chi - 1)
           uint64_t* rp = jd.rec + (size_t)FOLD_REC * (size_t)(fx * nother - fx * (fx - 1) / 2 + (o - fx));
@@ -1413,6 +1415,10 @@ static inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1
                  k_pairs_var<WW, VV, UU><<<dim3(nblk(nwaves, 4)), dim3(256), 0, st>>>(jobs, wave_job, wave_base, \
                                                                                       nwaves, J, tabs, partial))
 
+#define NX_LAUNCH_PAIRS_VAR_REC(WW)                                                          \
+  k_pairs_var<WW, 1, true, true><<<dim3(nblk(nwaves, 4)), dim3(256), 0, st>>>(jobs, wave_job, wave_base, \
+                                                                             nwaves, J, tabs, partial)
+
 hipError_t launch_pauli_var(const FormsDev& in, const int32_t* voff, const int32_t* vops, int nvar, int VW,
                             uint64_t* vhi, uint64_t* vrhs, uint64_t* vph, hipStream_t st) {
   if (in.chi == 0) return hipSuccess;
@@ -1442,8 +1448,20 @@ hipError_t launch_pairs_var(int W, int VW, int units, const PairJobDev* jobs, co
                             int64_t wave_base, int64_t nwaves, int J, const double* tabs, double* partial,
                             hipStream_t st) {
   if (nwaves == 0) return hipSuccess;
-  if (units) NX_LAUNCH_PAIRS_VAR(true)
-  else NX_LAUNCH_PAIRS_VAR(false)
+  if (units == 2) {  // with pair records: one word of variants
+    if (VW != 1) return hipErrorInvalidValue;
+    switch (W) {
+      case 1: NX_LAUNCH_PAIRS_VAR_REC(1); break;
+      case 2: NX_LAUNCH_PAIRS_VAR_REC(2); break;
+      case 3: NX_LAUNCH_PAIRS_VAR_REC(3); break;
+      case 4: NX_LAUNCH_PAIRS_VAR_REC(4); break;
+      default: return hipErrorInvalidValue;
+    }
+  } else if (units) {
+    NX_LAUNCH_PAIRS_VAR(true)
+  } else {
+    NX_LAUNCH_PAIRS_VAR(false)
+  }
   return hipGetLastError();
 }
 
