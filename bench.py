@@ -246,6 +246,8 @@ def main():
     ap.add_argument("--cfg2-steps", type=int, default=2, help="timed batched runs of the cfg2 leg")
     ap.add_argument("--sample-shots", type=int, default=1024,
                     help="also time QuantumState.sample over all 64 qubits with this many shots (cfg1; 0: skip)")
+    ap.add_argument("--cfg4-shots", type=int, default=0,
+                    help="with --config 4: also sample all 128 qubits with this many shots (exact mode)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 1 GiB device copy (PMC calibration)")
     args = ap.parse_args()
@@ -487,13 +489,34 @@ def bench_cfg4(args):
         src = torch.empty(1 << 27, dtype=torch.float64, device=dev)
         torch.empty_like(src).copy_(src)
         torch.cuda.synchronize()
+    sampling = "not run (--cfg4-shots 0)"
+    if args.cfg4_shots > 0:
+        # a stated reduced-shot run of the 128-qubit sampling (exact mode: the
+        # reference-order records would need 124 GB); per-window progress on stderr
+        os.environ["NX_SAMPLE_PROGRESS"] = "1"
+        _lib.check(L.nx_profile_enable(1))
+        p0 = C.c_double(), C.c_int64()
+        _lib.check(L.nx_profile_get(b"k_pairs_var", C.byref(p0[0]), C.byref(p0[1])))
+        ts = time.perf_counter()
+        counts = st.sample(list(range(n)), args.cfg4_shots, seed=1004, exact=True)
+        tsamp = time.perf_counter() - ts
+        p1 = C.c_double(), C.c_int64()
+        _lib.check(L.nx_profile_get(b"k_pairs_var", C.byref(p1[0]), C.byref(p1[1])))
+        _lib.check(L.nx_profile_enable(0))
+        del os.environ["NX_SAMPLE_PROGRESS"]
+        keys = [k[::-1] for k in counts]
+        sampling = {"shots": args.cfg4_shots, "qubits": n, "mode": "exact", "seconds": tsamp,
+                    "distinct_outcomes": len(counts), "window_passes": p1[1].value - p0[1].value,
+                    "window_kernel_ms": p1[0].value - p0[0].value,
+                    "uniform_levels": st.uniform_prefix(list(range(n - 1, -1, -1))),
+                    "dfs_nodes": sum(len({k[:d] for k in keys}) for d in range(n))}
     print(json.dumps({
         "metric": "cfg4 CH-form amplitude evals/s, n=128", "value": evals / step, "unit": "amplitude evals/s",
         "higher_is_better": True, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": step * 1e3, "k_amplitude_ms": kms, "kernel_evals_per_s": evals / (kms / 1e3),
         "compile_s": compile_s, "sum_abs2": float(np.sum(np.abs(amp) ** 2)),
         "config": {"workload": "cfg4: n=128, 6000 Clifford + 40 T, 2^16 strings of 128 bits", "chi": chi,
-                   "bitstrings": B, "sampling": "not run (each chi^2 norm is ~3e9 pair evaluations)"},
+                   "bitstrings": B, "sampling": sampling},
     }), flush=True)
 
 

@@ -8,10 +8,12 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -1695,7 +1697,18 @@ struct SampleCtx {
   int64_t passes = 0, tails = 0;
   bool ref = false;  // node norms in the reference's FP64 summation order
   RefFold rf;
+  // NX_SAMPLE_PROGRESS set: one line on stderr per window (long runs)
+  bool progress = false;
+  std::chrono::steady_clock::time_point t0;
 };
+
+static void sample_progress(const SampleCtx& cx, const char* what, int level) {
+  if (!cx.progress) return;
+  const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - cx.t0).count();
+  std::fprintf(stderr, "[necstar sample] %s at level %d: %lld windows, %lld tails, %.2f s\n", what, level,
+               (long long)cx.passes, (long long)cx.tails, s);
+  std::fflush(stderr);
+}
 
 static Window open_window(SampleCtx& cx, int root) {
   Window w;
@@ -1850,6 +1863,7 @@ static void recursive_sample(SampleCtx& cx, size_t idx, uint64_t shots, uint64_t
       own = cx.ref ? open_window_ref(cx, (int)idx)
                    : (((int)cx.q.size() - (int)idx <= cx.tail_m) ? open_tail(cx, (int)idx)
                                                                   : open_window(cx, (int)idx));
+      sample_progress(cx, "window", (int)idx);
       w = &own;
       rel = 0;
     }
@@ -2184,6 +2198,8 @@ static int sample_impl(const nx_state* s, int32_t nq, const int32_t* qargs, uint
     if (shots > 0 && nq > 0) {
       cx.s = s;
       cx.ref = ref && !s->is_complex;
+      cx.progress = std::getenv("NX_SAMPLE_PROGRESS") != nullptr;
+      cx.t0 = std::chrono::steady_clock::now();
       if (cx.ref) {
         const int64_t chi = s->chi();
         if (chi * (chi + 1) / 2 > ref_fold_max_pairs())
@@ -2192,6 +2208,7 @@ static int sample_impl(const nx_state* s, int32_t nq, const int32_t* qargs, uint
       } else {
         // at least the last level keeps its window: it also reports a zero state
         cx.uniform = std::min(uniform_prefix(s, cx.q), nq - 1);
+        sample_progress(cx, "certified uniform levels", cx.uniform);
         // exact tails need every qubit sampled (leaf = one amplitude) and
         // Scalar coefficients; 2^14 completions cost chi 2^14 amplitude
         // evaluations against ceil(14/6) windows of chi^2 / 2 pairs
