@@ -12,6 +12,9 @@ norm (nx_state_norm, the reference's synchronous call), the amplitudes stay
 in device memory.  The same steps fully asynchronous (norm into device
 memory, nx_state_norm_async) are timed as well.
 value = (amplitude evaluations + inner products) per second of step time.
+The timed region records HIP events around the dominant kernel only (its
+launch duration gives the roofline figure); a separate loop with events
+around every launch gives the per-kernel breakdown.
 
 The same run also measures, by default:
   * cfg1 sampling: sample(range(64), 1024, seed=1001), in the reference's
@@ -21,6 +24,9 @@ The same run also measures, by default:
     (autotuned launch shape, captured HIP graph) against the plain per-call
     API (norm() and exp_value(P) for every string of every circuit, all of
     them timed);
+  * cfg2 sampling, batched vs plain: the cfg2 circuits with at most 200
+    terms, 1024 samples on all qubits each (exact mode), in one
+    necstar.sample_many call against one sample() call per circuit;
   * the CPU baseline: the oracle (a C restatement of the reference
     algorithm) on a bounded sample of the cfg1 units, on one core and on an
     8-process pool.
@@ -230,6 +236,44 @@ def cfg2_compare(ncirc, steps, log):
     }
 
 
+def cfg2_sampling(chimax, shots, log):
+    """cfg2 sampling on the circuits with at most `chimax` terms: `shots`
+    Z-basis samples on all qubits per circuit (exact mode), all circuits in
+    one necstar.sample_many call against one QuantumState.sample call per
+    circuit (both timed in full); the outcomes must be identical."""
+    import numpy as np
+    import necstar
+    import workloads as WL
+    from necstar import QuantumState
+    cfg = WL.config(2)
+    sts, qargs, seeds = [], [], []
+    for i, c in enumerate(cfg["circuits"]):
+        st = QuantumState.from_circuit(WL.to_circuit(c["n"], c["gates"]))
+        if st.stabilizer_rank <= chimax:
+            sts.append(st)
+            qargs.append(list(range(c["n"])))
+            seeds.append(20000 + i)
+    chis = np.array([s.stabilizer_rank for s in sts])
+    log(f"cfg2 sampling: {len(sts)} circuits with chi <= {chimax}, {shots} shots each; batched leg")
+    stats = {}
+    t0 = time.perf_counter()
+    got = necstar.sample_many(sts, qargs, [shots] * len(sts), seeds, stats=stats)
+    batched_s = time.perf_counter() - t0
+    log(f"cfg2 sampling: batched {batched_s:.2f} s ({stats['rounds']} rounds); plain leg next")
+    t0 = time.perf_counter()
+    same = True
+    for k, st in enumerate(sts):
+        same = same and st.sample(qargs[k], shots, seed=seeds[k], exact=True) == got[k]
+    plain_s = time.perf_counter() - t0
+    return {"workload": f"cfg2 circuits with chi <= {chimax} ({len(sts)} of 256, n 16-128), {shots} Z-basis "
+                        "samples on all qubits each, exact mode",
+            "circuits": len(sts), "chi_max": int(chis.max()), "chi_sum": int(chis.sum()), "shots": shots,
+            "batched_s": batched_s, "plain_s": plain_s, "batched_vs_plain": plain_s / batched_s,
+            "identical_outcomes": bool(same), "rounds": stats["rounds"], "window_passes": stats["windows"],
+            "exact_tails": stats["tails"], "round_s": stats["round_s"], "gpu_wait_s": stats["gpu_wait_s"],
+            "dfs_wait_s": stats["wait_s"]}
+
+
 # ---------------------------------------------------------------------------
 
 def main():
@@ -244,6 +288,9 @@ def main():
                          "3: Pauli-loss forward vs backward; 4: n=128 amplitude sweep")
     ap.add_argument("--circuits", type=int, default=256, help="cfg2 circuits (the first N; 0: skip the cfg2 leg)")
     ap.add_argument("--cfg2-steps", type=int, default=2, help="timed batched runs of the cfg2 leg")
+    ap.add_argument("--cfg2-sample-chimax", type=int, default=200,
+                    help="cfg2 sampling leg: the circuits with at most this many terms (0: skip the leg)")
+    ap.add_argument("--cfg2-sample-shots", type=int, default=1024, help="cfg2 sampling leg: shots per circuit")
     ap.add_argument("--sample-shots", type=int, default=1024,
                     help="also time QuantumState.sample over all 64 qubits with this many shots (cfg1; 0: skip)")
     ap.add_argument("--cfg4-shots", type=int, default=0,
@@ -282,7 +329,10 @@ def main():
     if args.config == 3:
         return bench_cfg3(args)
     if args.config == 2:
-        print(json.dumps(cfg2_compare(args.circuits or 256, args.cfg2_steps, log)), flush=True)
+        res = cfg2_compare(args.circuits or 256, args.cfg2_steps, log)
+        if args.cfg2_sample_chimax > 0:
+            res["sampling"] = cfg2_sampling(args.cfg2_sample_chimax, args.cfg2_sample_shots, log)
+        print(json.dumps(res), flush=True)
         return
     if args.config == 4:
         return bench_cfg4(args)
@@ -317,16 +367,38 @@ def main():
         _lib.check(L.nx_state_amplitudes_async(st._h, B, bits_p, 1, out_p, ts))
         _lib.check(L.nx_state_norm(st._h, C.byref(norm)))
 
+    def kernel_avg(k):
+        ms, cnt = C.c_double(), C.c_int64()
+        _lib.check(L.nx_profile_get(k.encode(), C.byref(ms), C.byref(cnt)))
+        return ms.value / max(cnt.value, 1), cnt.value
+
     for _ in range(args.warmup):
         step()
         step_sync()
     torch.cuda.synchronize()
     assert float(d_norm.item()) == norm.value, (float(d_norm.item()), norm.value)
+    # per-kernel breakdown: HIP events around every launch of the step (the
+    # event records themselves lengthen the step; this loop is not `value`)
+    _lib.check(L.nx_profile_only(None))
+    _lib.check(L.nx_profile_enable(1))
+    torch.cuda.synchronize()
+    tb0 = time.perf_counter()
+    for _ in range(args.steps):
+        step_sync()
+    torch.cuda.synchronize()
+    ms_step_profiled = (time.perf_counter() - tb0) * 1e3 / args.steps
+    prof = {k: kernel_avg(k) for k in ("k_amplitude", "k_pairs", "k_qform", "k_amp_reduce", "k_reduce")}
+    _lib.check(L.nx_profile_enable(0))
+    dom = "k_pairs" if prof["k_pairs"][0] >= prof["k_amplitude"][0] else "k_amplitude"
+    # fully asynchronous steps (norm into device memory), no events
     ts0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     ms_step_async = (time.perf_counter() - ts0) * 1e3 / args.steps
+    # the timed region (`value`): synchronous steps, HIP events around the
+    # dominant kernel only (its launch duration for the roofline)
+    _lib.check(L.nx_profile_only(dom.encode()))
     _lib.check(L.nx_profile_enable(1))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -334,12 +406,10 @@ def main():
         step_sync()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    prof = {}
-    for k in ("k_amplitude", "k_pairs", "k_qform", "k_amp_reduce", "k_reduce"):
-        ms, cnt = C.c_double(), C.c_int64()
-        _lib.check(L.nx_profile_get(k.encode(), C.byref(ms), C.byref(cnt)))
-        prof[k] = (ms.value / max(cnt.value, 1), cnt.value)
+    dom_ms, dom_cnt = kernel_avg(dom)
     _lib.check(L.nx_profile_enable(0))
+    _lib.check(L.nx_profile_only(None))
+    assert dom_cnt == args.steps, (dom, dom_cnt)
     amp = d_out.view(torch.complex128).cpu().numpy()
     if args.copy_probe:
         src = torch.empty(1 << 27, dtype=torch.float64, device=dev)
@@ -390,18 +460,21 @@ def main():
                     "same_outcomes": counts == counts_ref}
 
     cfg2 = cfg2_compare(args.circuits, args.cfg2_steps, log) if args.circuits > 0 else None
+    cfg2_samp = (cfg2_sampling(args.cfg2_sample_chimax, args.cfg2_sample_shots, log)
+                 if args.cfg2_sample_chimax > 0 else None)
 
     ms_step = (t1 - t0) * 1e3 / args.steps
     amp_units = chi * B
     pair_units = chi * (chi + 1) // 2
     value = (amp_units + pair_units) / (ms_step / 1e3)
-    amp_ms, pair_ms, qf_ms = prof["k_amplitude"][0], prof["k_pairs"][0], prof["k_qform"][0]
+    amp_ms, pair_ms = prof["k_amplitude"][0], prof["k_pairs"][0]
     amp_rate = amp_units / (amp_ms / 1e3) if amp_ms > 0 else None
     pair_rate = pair_units / (pair_ms / 1e3) if pair_ms > 0 else None
-    dom = "k_pairs" if pair_ms >= amp_ms else "k_amplitude"
-    roofline = roofline_for(dom, pair_units if dom == "k_pairs" else amp_units, max(pair_ms, amp_ms))
+    # the dominant kernel at its duration in the timed region, the other one
+    # from the breakdown loop
+    roofline = roofline_for(dom, pair_units if dom == "k_pairs" else amp_units, dom_ms)
     other = roofline_for("k_amplitude" if dom == "k_pairs" else "k_pairs",
-                         amp_units if dom == "k_pairs" else pair_units, min(pair_ms, amp_ms))
+                         amp_units if dom == "k_pairs" else pair_units, amp_ms if dom == "k_pairs" else pair_ms)
 
     cpu = None
     if cpu_one is not None:
@@ -434,10 +507,12 @@ def main():
                    "n": n, "chi": chi, "bitstrings": B, "pairs": pair_units, "cliffords": 2000, "t": 24},
         "amplitude_evals_per_s": amp_rate, "inner_products_per_s": pair_rate,
         "kernel_ms": {k: v[0] for k, v in prof.items()}, "kernel_sum_ms": kernel_sum,
-        "step_overhead_frac": (ms_step - kernel_sum) / ms_step,
+        "ms_per_step_profiled": ms_step_profiled,
+        "step_overhead_frac": (ms_step_profiled - kernel_sum) / ms_step_profiled,
         "ms_per_step_async": ms_step_async,
         "norm": norm.value, "sum_abs2_amplitudes": float(np.sum(np.abs(amp) ** 2)), "compile_s": compile_s,
         "batched_vs_plain": cfg2["batched_vs_plain"] if cfg2 else None, "cfg2": cfg2,
+        "cfg2_sampling": cfg2_samp,
         "sampling": sampling, "roofline": roofline, "roofline_other": other, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -260,12 +260,35 @@ int nx_batch_shape(const nx_batch *b, int32_t *J, int64_t *waves, int32_t *varia
 int nx_batch_run(nx_batch *b, double *out);
 void nx_batch_free(nx_batch *b);
 
-/* Kernel timing on the library's own stream (HIP events around each
- * amplitude / pair / qform / program launch; callers' events on other
- * streams cannot see these launches).  kernel: "k_amplitude", "k_pairs",
- * "k_pairs_var", "k_gramc", "k_qform", "k_program", "batch_pairs" (a whole
- * nx_batch_run launch plan).  No reference counterpart (measurement aid). */
+/* Sampling many states at once: for state b, the outcomes of
+ * nx_state_sample_exact(states[b], nq[b], qargs_b, shots[b], seed_b) -- the
+ * same draws, state by state -- where qargs concatenates the nq[b] qubit
+ * lists and seeds the 32-byte seeds (NULL: fresh entropy per state).  Each
+ * state's DFS keeps its own RNG order; the windows that all states wait for
+ * at one time are evaluated together (one projection launch and one pair
+ * launch plan per round).  nout[b] receives state b's number of distinct
+ * outcomes; its outcomes follow those of states 0..b-1 in out_bits (two
+ * words each, as nx_state_sample) and out_counts.  When the sum exceeds
+ * max_out, nout is still filled and the call fails (retry with the sum).
+ * stats (6 int64, or NULL): rounds, window passes, exact tails, then the
+ * microseconds spent evaluating rounds, waiting for the DFS threads, and
+ * (of the first) waiting for the rounds' launches to finish.  The
+ * reference has no batch API: this is the batched form of calling
+ * QuantumState::sample (sampling.rs:80-189) per circuit. */
+int nx_batch_sample_exact(int32_t nstates, const nx_state *const *states, const int32_t *nq, const int32_t *qargs,
+                          const uint64_t *shots, const uint8_t *seeds, int64_t max_out, uint64_t *out_bits,
+                          uint64_t *out_counts, int64_t *nout, int64_t *stats);
+
+/* Kernel timing with HIP events recorded around each launch on the stream
+ * the library enqueues on.  kernel: "k_amplitude", "k_amp_reduce",
+ * "k_amplitude_exact", "k_pairs", "k_reduce", "k_pairs_var", "k_pairs_sym",
+ * "k_grad_var", "k_qform", "k_qproject_var", "k_fold", "k_program",
+ * "batch_pairs" (a whole nx_batch_run launch plan).  Enabling clears the totals.  No
+ * reference counterpart (measurement aid). */
 int nx_profile_enable(int on);
+/* Restricts the timing to the comma-separated kernel names (NULL or "":
+ * every kernel): each timed launch adds two event records to the stream. */
+int nx_profile_only(const char *kernels);
 /* kernel "allocations": *count = device-buffer requests to the library's
  * allocator plus page-locked host allocations so far (*total_ms = 0),
  * whether or not profiling is enabled. */

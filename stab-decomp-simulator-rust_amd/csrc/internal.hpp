@@ -163,6 +163,24 @@ hipError_t launch_tail_strings(const uint64_t* base, const int32_t* qs, int d, i
 hipError_t launch_tail_leaves(const double* al, int64_t nv, double* ab, hipStream_t st);
 hipError_t launch_amplitude_exact(const QFormDev& q, const int32_t* cs, const uint64_t* bits, int64_t nb, int nchunks,
                                   const double* tabs, double* xout, hipStream_t st);
+// Several sampling tails at once (all of row word width W): job j's 2^d
+// completions of base on qubits qs into strings, their exact amplitudes
+// over the terms of q in nchunks term chunks (launch_amplitude_exact's
+// blocks, from block0 on) into al, and the leaf norms A, B into ab[0..nv),
+// ab[nv..2nv); strings str0 .. str0 + nv - 1 of the launch are job j's.
+struct TailJobDev {
+  QFormDev q;
+  const int32_t* cs;
+  const uint64_t* base;
+  const int32_t* qs;
+  uint64_t* strings;
+  double* al;
+  double* ab;
+  int32_t d, nchunks;
+  int64_t nv, str0, block0;
+};
+hipError_t launch_tails(int W, const TailJobDev* jobs, int njobs, int64_t total_strings, int64_t total_blocks,
+                        const double* tabs, hipStream_t st);
 // Pair sums over units of J = 8 consecutive other indices o of a fixed
 // index f (triangle: o in [f, nother), nfixed == nother, rows from the
 // closed form below; else o in [0, nother)), four waves per block, nblocks
@@ -223,6 +241,22 @@ hipError_t launch_pairs_sym(const QFormDev& X, const int64_t* offsets, int J, in
 hipError_t launch_qproject_var(const QFormDev& in, const QFormDev& out, const VarDev& vout, int VW,
                                const int32_t* fq, const int32_t* fb, int nfix, const int32_t* uq,
                                const uint64_t* ub, int nunit, hipStream_t st);
+// k_qproject_var for several states in one launch (one row word width W and
+// VW per launch): job j projects the terms of jobs[j].in as launch_qproject_var
+// does with its own fixed and look-ahead outcomes, in blocks jobs[j].block0
+// .. jobs[j + 1].block0 - 1 (four terms per block); nblocks blocks in all
+struct QProjJobDev {
+  QFormDev in, out;
+  VarDev vout;
+  const int32_t* fq;
+  const int32_t* fb;
+  const int32_t* uq;
+  const uint64_t* ub;
+  int32_t nfix, nunit;
+  int64_t block0;
+};
+hipError_t launch_qproject_var_jobs(int W, int VW, const QProjJobDev* jobs, int njobs, int64_t nblocks,
+                                    hipStream_t st);
 hipError_t launch_qproject(const QFormDev& in, const QFormDev& out, const int32_t* uq, const int32_t* ub, int nunit,
                            hipStream_t st);
 // Coefficient backward (k_grad_var): for the Pauli variants V of the terms

@@ -546,13 +546,14 @@ struct IConst {
   static constexpr int value = V;
 };
 
+// one block of an amplitude launch: string column bx, term chunk by
 template <int W, int XB, bool EX>
-__global__ __launch_bounds__(AMP_NT) void k_amplitude(QFormDev q, const double2* __restrict__ coef,
-                                                   const u64* __restrict__ bits, int64_t nb, int64_t nchunks,
-                                                   const double* __restrict__ tabs, double2* __restrict__ out,
-                                                   uint8_t* __restrict__ per_term, int enumerate, int final_g,
-                                                   double gre, double gim, const int32_t* __restrict__ cs,
-                                                   double* __restrict__ xout) {
+__device__ __forceinline__ void amp_block(const QFormDev& q, const double2* __restrict__ coef,
+                                          const u64* __restrict__ bits, int64_t nb, int64_t nchunks,
+                                          const double* __restrict__ tabs, double2* __restrict__ out,
+                                          uint8_t* __restrict__ per_term, int enumerate, int final_g, double gre,
+                                          double gim, const int32_t* __restrict__ cs, double* __restrict__ xout,
+                                          int64_t bx, int64_t by) {
   using L = AmpLds<W>;
   using Y = QLay<W>;
   constexpr int NB = L::NB;
@@ -565,14 +566,14 @@ __global__ __launch_bounds__(AMP_NT) void k_amplitude(QFormDev q, const double2*
     s_cos[threadIdx.x] = tabs[TAB_COS + threadIdx.x];
     s_sin[threadIdx.x] = tabs[TAB_SIN + threadIdx.x];
   }
-  const int64_t t0 = (int64_t)blockIdx.y * q.chi / nchunks;
-  const int64_t t1 = ((int64_t)blockIdx.y + 1) * q.chi / nchunks;
+  const int64_t t0 = by * q.chi / nchunks;
+  const int64_t t1 = (by + 1) * q.chi / nchunks;
   int64_t bidx[XB];
   bool active[XB];
   u64 x[XB][W];
 #pragma unroll
   for (int j = 0; j < XB; ++j) {
-    bidx[j] = ((int64_t)blockIdx.x * XB + j) * blockDim.x + threadIdx.x;
+    bidx[j] = (bx * XB + j) * blockDim.x + threadIdx.x;
     active[j] = bidx[j] < nb;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
@@ -787,13 +788,47 @@ __global__ __launch_bounds__(AMP_NT) void k_amplitude(QFormDev q, const double2*
       // four components per string, per chunk (the global factor is applied
       // by the caller)
 #pragma unroll
-      for (int m = 0; m < 4; ++m) xout[((size_t)blockIdx.y * nb + bidx[j]) * 4 + m] = al[j][m];
+      for (int m = 0; m < 4; ++m) xout[((size_t)by * nb + bidx[j]) * 4 + m] = al[j][m];
     } else if (final_g) {  // * global_factor (statevector.rs:23)
       out[bidx[j]] = make_double2(are[j] * gre - aim[j] * gim, are[j] * gim + aim[j] * gre);
     } else {
-      out[(size_t)blockIdx.y * nb + bidx[j]] = make_double2(are[j], aim[j]);
+      out[(size_t)by * nb + bidx[j]] = make_double2(are[j], aim[j]);
     }
   }
+}
+
+template <int W, int XB, bool EX>
+__global__ __launch_bounds__(AMP_NT) void k_amplitude(QFormDev q, const double2* __restrict__ coef,
+                                                   const u64* __restrict__ bits, int64_t nb, int64_t nchunks,
+                                                   const double* __restrict__ tabs, double2* __restrict__ out,
+                                                   uint8_t* __restrict__ per_term, int enumerate, int final_g,
+                                                   double gre, double gim, const int32_t* __restrict__ cs,
+                                                   double* __restrict__ xout) {
+  amp_block<W, XB, EX>(q, coef, bits, nb, nchunks, tabs, out, per_term, enumerate, final_g, gre, gim, cs, xout,
+                       blockIdx.x, blockIdx.y);
+}
+
+// Exact amplitudes of several sampling tails in one launch (one row word
+// width): block b belongs to the job j with jobs[j].block0 <= b <
+// jobs[j + 1].block0; within it, string column (b - block0) % cols and term
+// chunk (b - block0) / cols, as k_amplitude<W, XB, true> over that job's
+// strings.
+template <int W, int XB>
+__global__ __launch_bounds__(AMP_NT) void k_amplitude_tails(const TailJobDev* __restrict__ jobs, int njobs,
+                                                            const double* __restrict__ tabs) {
+  const int64_t b = blockIdx.x;
+  int lo = 0, hi = njobs;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (uni64((u64)jobs[mid].block0) <= (u64)b) lo = mid;
+    else hi = mid;
+  }
+  const TailJobDev& j = jobs[lo];
+  const int64_t per = (int64_t)AMP_NT * XB;
+  const int64_t cols = ((int64_t)uni64((u64)j.nv) + per - 1) / per;
+  const int64_t local = b - (int64_t)uni64((u64)j.block0);
+  amp_block<W, XB, true>(j.q, nullptr, j.strings, j.nv, j.nchunks, tabs, nullptr, nullptr, 0, 0, 0.0, 0.0, j.cs,
+                         j.al, local % cols, local / cols);
 }
 
 // The 2^d completions of a sampling path: string k = base with bit i of k on
@@ -841,6 +876,91 @@ __global__ void k_tail_leaves(const double* __restrict__ al, int64_t nv, double*
   B.add_prod(-a0, a3);
   ab[k] = A.value();
   ab[nv + k] = B.value();
+}
+
+// the job of global string g (jobs[j].str0 <= g < jobs[j + 1].str0)
+__device__ __forceinline__ int tail_job_of(const TailJobDev* jobs, int njobs, int64_t g) {
+  int lo = 0, hi = njobs;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (jobs[mid].str0 <= g) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void k_tail_strings_jobs(const TailJobDev* __restrict__ jobs, int njobs, int64_t total) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const TailJobDev& j = jobs[tail_job_of(jobs, njobs, g)];
+  const int64_t k = g - j.str0;
+  if (k >= j.nv) return;
+  const int W = j.q.W;
+  for (int w = 0; w < W; ++w) {
+    u64 x = j.base[w];
+    for (int i = 0; i < j.d; ++i)
+      if (((k >> i) & 1) && (j.qs[i] >> 6) == w) x |= 1ull << (j.qs[i] & 63);
+    j.strings[(size_t)k * W + w] = x;
+  }
+}
+
+// the chunks of each string's components summed in chunk order (as
+// k_amp_reduce_exact), then its leaf norm (as k_tail_leaves)
+__global__ void k_tail_leaves_jobs(const TailJobDev* __restrict__ jobs, int njobs, int64_t total) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const TailJobDev& j = jobs[tail_job_of(jobs, njobs, g)];
+  const int64_t k = g - j.str0;
+  if (k >= j.nv) return;
+  double a[4];
+  for (int m = 0; m < 4; ++m) {
+    double v = j.al[k * 4 + m];
+    for (int c = 1; c < j.nchunks; ++c) v = v + j.al[((size_t)c * j.nv + k) * 4 + m];
+    a[m] = v;
+  }
+  DDev A, B;
+  A.add_prod(a[0], a[0]);
+  A.add_prod(a[1], a[1]);
+  A.add_prod(a[2], a[2]);
+  A.add_prod(a[3], a[3]);
+  B.add_prod(a[0], a[1]);
+  B.add_prod(a[1], a[2]);
+  B.add_prod(a[2], a[3]);
+  B.add_prod(-a[0], a[3]);
+  j.ab[k] = A.value();
+  j.ab[j.nv + k] = B.value();
+}
+
+hipError_t launch_tails(int W, const TailJobDev* jobs, int njobs, int64_t total_strings, int64_t total_blocks,
+                        const double* tabs, hipStream_t st) {
+  if (njobs <= 0 || total_strings <= 0) return hipSuccess;
+  k_tail_strings_jobs<<<dim3((unsigned)((total_strings + 255) / 256)), dim3(256), 0, st>>>(jobs, njobs,
+                                                                                        total_strings);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+#define NX_AMPT_LAUNCH(WW, XBV)                                                                                  \
+  {                                                                                                               \
+    const size_t sh = AmpLds<WW>::BYTES;                                                                          \
+    if (sh > 64 * 1024) {                                                                                         \
+      e = hipFuncSetAttribute((const void*)k_amplitude_tails<WW, XBV>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                              (int)sh);                                                                           \
+      if (e != hipSuccess) return e;                                                                              \
+    }                                                                                                             \
+    k_amplitude_tails<WW, XBV><<<dim3((unsigned)total_blocks), dim3(AMP_NT), sh, st>>>(jobs, njobs, tabs);        \
+  }
+  switch (W) {
+    case 1: NX_AMPT_LAUNCH(1, 4) break;
+    case 2: NX_AMPT_LAUNCH(2, 4) break;
+    case 3: NX_AMPT_LAUNCH(3, 2) break;
+    case 4: NX_AMPT_LAUNCH(4, 2) break;
+    default: return hipErrorInvalidValue;
+  }
+#undef NX_AMPT_LAUNCH
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_tail_leaves_jobs<<<dim3((unsigned)((total_strings + 255) / 256)), dim3(256), 0, st>>>(jobs, njobs,
+                                                                                       total_strings);
+  return hipGetLastError();
 }
 
 hipError_t launch_tail_strings(const uint64_t* base, const int32_t* qs, int d, int W, int64_t nv, uint64_t* out,

@@ -10,6 +10,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <complex>
 #include <cstdarg>
 #include <cstdio>
@@ -22,6 +23,7 @@
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -263,6 +265,7 @@ static void d2d(void* d, const void* s, size_t b) {
 // totals are asked for, so profiled runs keep the stream's own pacing.
 struct Prof {
   bool on = false;
+  std::string only;  // ",name,name," : only these scopes are timed (empty: all)
   struct Rec {
     std::string name;
     hipEvent_t e0, e1;
@@ -294,6 +297,7 @@ struct ProfScope {
   explicit ProfScope(const char* nm) : name(nm), on(false) {
     std::lock_guard<std::mutex> lk(g_prof_mu);
     if (!g_prof.on) return;
+    if (!g_prof.only.empty() && g_prof.only.find("," + std::string(nm) + ",") == std::string::npos) return;
     on = true;
     if (g_prof.pool.empty()) {
       hipEvent_t a, b;
@@ -1169,6 +1173,7 @@ struct VarPlan {
   std::vector<Launch> launches;
   int64_t nseg = 0;
   DevBuf jobs, wave_job, row_off, partial, out, seg_job, seg_part;
+  std::vector<PairJobDev> hjobs;  // host copy of the job table
   // jobs_dev: the job table to read (default: this plan's own)
   // out_override: where the reduction writes instead of out (page-locked
   // host memory for results the host reads right after)
@@ -1280,6 +1285,7 @@ static VarPlan make_plan(const std::vector<VarJob>& in, int VW, int J, int64_t t
   for (size_t i = 0; i < in.size(); ++i) hj[i].row_off = pl.row_off.as<int64_t>() + roff_base[i];
   pl.jobs.alloc(std::max<size_t>(hj.size(), 1) * sizeof(PairJobDev));
   h2d(pl.jobs.p, hj.data(), hj.size() * sizeof(PairJobDev));
+  pl.hjobs = hj;
   pl.wave_job.alloc(std::max<size_t>(wj.size(), 1) * 4);
   h2d(pl.wave_job.p, wj.data(), wj.size() * 4);
   pl.partial.alloc(std::max<int64_t>(wave, 1) * 2 * (64 * VW + 1) * 8);
@@ -1454,50 +1460,63 @@ struct ProjPlan {
   int VW = 1, maxunit = 0;
   double g2 = 1.0;
   // one upload per pass: [job | fixed qubits | fixed outcomes | look-ahead
-  // qubits | look-ahead outcome masks], staged in page-locked memory
-  DevBuf stage_d;
-  PinnedBuf stage_h, res_h;
+  // qubits | look-ahead outcome masks], staged in page-locked memory: this
+  // plan's own buffers, or slices of a sampling batch's (one upload for the
+  // windows of all its states)
+  DevBuf stage_d_own;
+  PinnedBuf stage_h_own, res_h;
+  char* sh = nullptr;  // host staging
+  char* sd = nullptr;  // device staging
   size_t o_fq = 0, o_fb = 0, o_uq = 0, o_ub = 0, stage_bytes = 0;
-  void init(const nx_state* st, int max_units, int max_look) {
-    s = st;
-    VW = (1 << max_look) <= 64 ? 1 : 2;
-    maxunit = max_units;
-    VarJob j;
-    j.X = &qform_of(st);
-    j.Y = j.X;
-    j.cx = coef_of(st);
-    j.cy = j.cx;
-    j.nvar = 1 << max_look;
-    j.split_ab = 1;
-    // chunks of 8 pairs and three rounds of resident waves: one window pass
-    // of a single state is short, and fine work units keep its waves even
-    // (1,024 shots of a 64-qubit, 729-term state on all qubits: 1.26 s with
-    // 16-pair chunks in 16k waves, 1.19 s so)
-    static std::mutex res_mu;
-    static std::map<std::tuple<int, int, int>, int64_t> resident;  // (device, W, VW) -> waves
-    int64_t rw = 0;
-    {
-      int dev = 0;
-      hip_check(hipGetDevice(&dev), "hipGetDevice");
-      const auto key = std::make_tuple(dev, j.X->W, VW);
-      std::lock_guard<std::mutex> lk(res_mu);
-      auto it = resident.find(key);
-      if (it == resident.end()) it = resident.emplace(key, pairs_var_resident_waves(j.X->W, VW, 1)).first;
-      rw = it->second;
-    }
-    plan = make_plan({j}, VW, 8, std::max<int64_t>(3 * rw, 1024));
-    hj.resize(1);
-    d2h(hj.data(), plan.jobs.p, sizeof(PairJobDev));
+  int nfix_staged = 0;  // fixed outcomes of the staged pass
+  static int vw_for(int max_look) { return (1 << max_look) <= 64 ? 1 : 2; }
+  static size_t stage_size(int max_units, int max_look, size_t* ofq = nullptr, size_t* ofb = nullptr,
+                           size_t* ouq = nullptr, size_t* oub = nullptr) {
     const size_t mu = (size_t)std::max(max_units, 1);
+    const int vw = vw_for(max_look);
     auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    o_fq = up(sizeof(PairJobDev));
-    o_fb = o_fq + up(mu * 4);
-    o_uq = o_fb + up(mu * 4);
-    o_ub = o_uq + up(mu * 4);
-    stage_bytes = o_ub + up(mu * (VW + 1) * 8);
-    stage_d.alloc(stage_bytes);
-    stage_h.alloc(stage_bytes);
-    res_h.alloc((size_t)2 * 64 * VW * 8);
+    const size_t a = up(sizeof(PairJobDev)), b = a + up(mu * 4), c = b + up(mu * 4), d = c + up(mu * 4);
+    if (ofq) *ofq = a;
+    if (ofb) *ofb = b;
+    if (ouq) *ouq = c;
+    if (oub) *oub = d;
+    return d + up(mu * (vw + 1) * 8);
+  }
+  // ext_h / ext_d: stage_size() bytes of page-locked host and device memory
+  // owned by a sampling batch, whose own launch plans run the passes (null:
+  // this plan allocates its staging and builds its own one-job launch plan)
+  void init(const nx_state* st, int max_units, int max_look, char* ext_h = nullptr, char* ext_d = nullptr) {
+    s = st;
+    VW = vw_for(max_look);
+    maxunit = max_units;
+    if (ext_h && ext_d) {
+      hj.assign(1, PairJobDev{});
+    } else {
+      VarJob j;
+      j.X = &qform_of(st);
+      j.Y = j.X;
+      j.cx = coef_of(st);
+      j.cy = j.cx;
+      j.nvar = 1 << max_look;
+      j.split_ab = 1;
+      // chunks of 8 pairs and three rounds of resident waves: one window pass
+      // of a single state is short, and fine work units keep its waves even
+      // (1,024 shots of a 64-qubit, 729-term state on all qubits: 1.26 s with
+      // 16-pair chunks in 16k waves, 1.19 s so)
+      plan = make_plan({j}, VW, 8, std::max<int64_t>(3 * resident_waves(j.X->W, VW), 1024));
+      hj = plan.hjobs;
+    }
+    stage_bytes = stage_size(max_units, max_look, &o_fq, &o_fb, &o_uq, &o_ub);
+    if (ext_h && ext_d) {
+      sh = ext_h;
+      sd = ext_d;
+    } else {
+      stage_d_own.alloc(stage_bytes);
+      stage_h_own.alloc(stage_bytes);
+      sh = stage_h_own.as<char>();
+      sd = stage_d_own.as<char>();
+      res_h.alloc((size_t)2 * 64 * VW * 8);
+    }
     projv.alloc_pair(st->chi(), st->n);
     const size_t rows = (size_t)st->chi() * 64 * projv.W;
     vhi.alloc(std::max<size_t>(rows * VW * 8, 8));
@@ -1506,28 +1525,38 @@ struct ProjPlan {
     sync();
     g2 = st->g.real() * st->g.real() + st->g.imag() * st->g.imag();
   }
-  // norm_squared / |g|^2 of Pi psi for the 2^d outcomes of qubits look[0..d),
-  // as exact parts: value = A[k] + sqrt(2) B[k] (A, B = the returned halves)
-  // rec != null: the pass also writes every pair's exact per-outcome values
-  // (PairJobDev::rec)
-  std::vector<double> run(const std::vector<int>& fq, const std::vector<int>& fb, const std::vector<int>& look,
-                          uint64_t* rec = nullptr) {
+  // waves of k_pairs_var<W, VW, units> resident on this device (cached)
+  static int64_t resident_waves(int W, int VW) {
+    static std::mutex res_mu;
+    static std::map<std::tuple<int, int, int>, int64_t> resident;  // (device, W, VW) -> waves
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    const auto key = std::make_tuple(dev, W, VW);
+    std::lock_guard<std::mutex> lk(res_mu);
+    auto it = resident.find(key);
+    if (it == resident.end()) it = resident.emplace(key, pairs_var_resident_waves(W, VW, 1)).first;
+    return it->second;
+  }
+  VarDev tables() const { return VarDev{vhi.as<uint64_t>(), vrhs.as<uint64_t>(), vph.as<uint64_t>()}; }
+  // Host side of a pass: the fixed outcomes (fq, fb) and the look-ahead
+  // qubits into the staging block, and the pass's job (hj[0], also staged:
+  // its pointers refer to the device staging block).  Returns 2^d.
+  int stage(const std::vector<int>& fq, const std::vector<int>& fb, const std::vector<int>& look, uint64_t* rec) {
     const int d = (int)look.size(), nv = 1 << d;
     if ((int)(fq.size() + look.size()) > maxunit) fail(NX_E_INTERNAL, "too many projected qubits");
     if (nv > 64 * VW) fail(NX_E_INTERNAL, "too many look-ahead qubits");
-    // the fixed outcomes and then the look-ahead outcomes, one per variant,
-    // are substituted once per term (k_qproject_var)
-    const QForms& q0 = qform_of(s);
-    char* dbase = (char*)stage_d.p;
     PairJobDev& job = hj[0];
     for (int w = 0; w < 4; ++w) job.removed[w] = 0ull;
+    int32_t* hfq = (int32_t*)(sh + o_fq);
+    int32_t* hfb = (int32_t*)(sh + o_fb);
     for (size_t i = 0; i < fq.size(); ++i) {
-      stage_h.as<int32_t>(o_fq)[i] = fq[i];
-      stage_h.as<int32_t>(o_fb)[i] = fb[i];
+      hfq[i] = fq[i];
+      hfb[i] = fb[i];
       job.removed[fq[i] >> 6] |= 1ull << (fq[i] & 63);
     }
-    int32_t* uq = stage_h.as<int32_t>(o_uq);
-    uint64_t* ub = stage_h.as<uint64_t>(o_ub);
+    nfix_staged = (int)fq.size();
+    int32_t* uq = (int32_t*)(sh + o_uq);
+    uint64_t* ub = (uint64_t*)(sh + o_ub);
     for (int i = 0; i < d; ++i) {
       uq[i] = look[i];
       uint64_t* e = ub + (size_t)i * (VW + 1);
@@ -1540,31 +1569,53 @@ struct ProjPlan {
       }
     }
     job.nunit = (int32_t)d;
-    job.uq = (const int32_t*)(dbase + o_uq);
-    job.ub = (const uint64_t*)(dbase + o_ub);
+    job.uq = (const int32_t*)(sd + o_uq);
+    job.ub = (const uint64_t*)(sd + o_ub);
     job.nvar = nv;
     // the look-ahead outcomes are substituted once per term as well
     // (k_qproject_var): the pairs start from the projected forms and tables
-    const VarDev vt{vhi.as<uint64_t>(), vrhs.as<uint64_t>(), vph.as<uint64_t>()};
     job.X = job.Y = projv.desc();
-    job.V = vt;
+    job.V = tables();
     job.termvar = 1;
     job.rec = rec;
     for (int i = 0; i < d; ++i) job.removed[look[i] >> 6] |= 1ull << (look[i] & 63);
-    std::memcpy(stage_h.p, &job, sizeof(PairJobDev));
-    h2d(stage_d.p, stage_h.p, stage_bytes);
+    std::memcpy(sh, &job, sizeof(PairJobDev));
+    return nv;
+  }
+  // the projection job of the staged pass (for launch_qproject_var_jobs)
+  QProjJobDev qproj_job() const {
+    QProjJobDev q;
+    q.in = qform_of(s).desc();
+    q.out = projv.desc();
+    q.vout = tables();
+    q.fq = (const int32_t*)(sd + o_fq);
+    q.fb = (const int32_t*)(sd + o_fb);
+    q.nfix = nfix_staged;
+    q.uq = hj[0].uq;
+    q.ub = hj[0].ub;
+    q.nunit = hj[0].nunit;
+    q.block0 = 0;
+    return q;
+  }
+  // norm_squared / |g|^2 of Pi psi for the 2^d outcomes of qubits look[0..d),
+  // as exact parts: value = A[k] + sqrt(2) B[k] (A, B = the returned halves)
+  // rec != null: the pass also writes every pair's exact per-outcome values
+  // (PairJobDev::rec)
+  std::vector<double> run(const std::vector<int>& fq, const std::vector<int>& fb, const std::vector<int>& look,
+                          uint64_t* rec = nullptr) {
+    if (rec && VW != 1) fail(NX_E_INTERNAL, "pair records need one word of variants");
+    const int nv = stage(fq, fb, look, rec);
+    h2d(sd, sh, stage_bytes);
     {
       ProfScope ps("k_qproject_var");
-      hip_check(launch_qproject_var(q0.desc(), projv.desc(), vt, VW, (const int32_t*)(dbase + o_fq),
-                                    (const int32_t*)(dbase + o_fb), (int)fq.size(),
-                                    (const int32_t*)(dbase + o_uq), (const uint64_t*)(dbase + o_ub), d, stream()),
+      const QProjJobDev q = qproj_job();
+      hip_check(launch_qproject_var(q.in, q.out, q.vout, VW, q.fq, q.fb, q.nfix, q.uq, q.ub, q.nunit, stream()),
                 "k_qproject_var");
       ps.end();
     }
     {
       ProfScope ps("k_pairs_var");
-      if (rec && VW != 1) fail(NX_E_INTERNAL, "pair records need one word of variants");
-      plan.enqueue(stream(), (const PairJobDev*)dbase, res_h.as<double>(), rec != nullptr);
+      plan.enqueue(stream(), (const PairJobDev*)sd, res_h.as<double>(), rec != nullptr);
       ps.end();
     }
     sync();
@@ -1683,7 +1734,10 @@ static void tail_leaves(const nx_state* s, TailBufs& tb, const std::vector<uint6
   B.assign(tb.host.as<double>() + nv, tb.host.as<double>() + 2 * nv);
 }
 
+struct SampleBatch;
+
 struct SampleCtx {
+  SampleBatch* batch = nullptr;  // non-null: windows come from a sampling batch's dispatcher
   int uniform = 0;  // the first `uniform` qubits of q have a uniform marginal
   int tail_m = 0;   // windows of at most this many levels come from exact amplitudes (0: never)
   const nx_state* s = nullptr;
@@ -1710,20 +1764,25 @@ static void sample_progress(const SampleCtx& cx, const char* what, int level) {
   std::fflush(stderr);
 }
 
-static Window open_window(SampleCtx& cx, int root) {
-  Window w;
+// The window rooted at depth `root` of the path cx.path: its look-ahead
+// qubits and the fixed outcomes above it
+static void window_args(const SampleCtx& cx, int root, Window& w, std::vector<int>& fq, std::vector<int>& fb,
+                        std::vector<int>& lq) {
   w.root = root;
   w.d = std::min<int>(cx.look, (int)cx.q.size() - root);
-  std::vector<int> fq(cx.q.begin(), cx.q.begin() + root), fb(cx.path.begin(), cx.path.begin() + root);
-  std::vector<int> lq(cx.q.begin() + root, cx.q.begin() + root + w.d);
+  fq.assign(cx.q.begin(), cx.q.begin() + root);
+  fb.assign(cx.path.begin(), cx.path.begin() + root);
+  lq.assign(cx.q.begin() + root, cx.q.begin() + root + w.d);
+}
+
+// leaves A, B (2^d each) into the window, then its inner nodes as exact
+// sums of their leaves (powers of two in A and B)
+static void window_fill(Window& w, const double* A, const double* B) {
+  const size_t nv = (size_t)1 << w.d;
   w.la.resize(w.d + 1);
   w.lb.resize(w.d + 1);
-  const std::vector<double> ab = cx.pp.run(fq, fb, lq);
-  const size_t nv = (size_t)1 << w.d;
-  w.la[w.d].assign(ab.begin(), ab.begin() + nv);
-  w.lb[w.d].assign(ab.begin() + nv, ab.begin() + 2 * nv);
-  ++cx.passes;
-  // inner nodes: exact sums of their leaves (powers of two in A and B)
+  w.la[w.d].assign(A, A + nv);
+  w.lb[w.d].assign(B, B + nv);
   for (int j = w.d; j >= 2; --j) {
     const size_t m = (size_t)1 << (j - 1);
     w.la[j - 1].assign(m, 0.0);
@@ -1733,6 +1792,16 @@ static Window open_window(SampleCtx& cx, int root) {
       w.lb[j - 1][k] = w.lb[j][k] + w.lb[j][k | m];
     }
   }
+}
+
+static Window open_window(SampleCtx& cx, int root) {
+  Window w;
+  std::vector<int> fq, fb, lq;
+  window_args(cx, root, w, fq, fb, lq);
+  const std::vector<double> ab = cx.pp.run(fq, fb, lq);
+  const size_t nv = (size_t)1 << w.d;
+  window_fill(w, ab.data(), ab.data() + nv);
+  ++cx.passes;
   return w;
 }
 
@@ -1841,6 +1910,278 @@ static Window open_tail(SampleCtx& cx, int root) {
   return w;
 }
 
+// ---------------------------------------------------------------------------
+// Sampling many states at once (exact mode).  Each state's DFS runs on a host
+// thread of its own, in its own RNG order, and stops at every window it
+// needs; once every running DFS waits, the dispatcher (the calling thread,
+// the only one that touches the GPU) evaluates all waiting windows -- the
+// projections of all states in one launch per row width
+// (k_qproject_var_jobs), all window passes in one launch plan (one
+// k_pairs_var launch per row width) -- and resumes them.  Each state draws
+// exactly what sample_exact draws for it alone.
+struct SampleBatch {
+  struct Req {
+    SampleCtx* cx;
+    int root;
+    bool tail;
+    bool done = false;
+    Window* out;
+  };
+  std::mutex mu;
+  std::condition_variable cv_disp, cv_work;
+  int running = 0;  // DFS threads not finished
+  int waiting = 0;  // of which waiting for a window
+  bool abort = false;
+  std::vector<Req*> pending;
+  // staging of all states' window passes (one upload per round) and the
+  // job tables after it
+  PinnedBuf stage_h, res_h, tab_h;
+  DevBuf stage_d;
+  size_t stage_bytes = 0, jobs_off = 0, tab_cap = 0;
+  int64_t rounds = 0, windows = 0, tails = 0;
+  double round_s = 0.0, wait_s = 0.0;  // in batch_round / waiting for the DFS threads
+  double sync_s = 0.0;                 // of round_s: waiting for the round's launches
+};
+
+struct BatchAbort {};
+
+static Window batch_window(SampleCtx& cx, int root, bool tail) {
+  SampleBatch& b = *cx.batch;
+  Window w;
+  SampleBatch::Req r{&cx, root, tail, false, &w};
+  std::unique_lock<std::mutex> lk(b.mu);
+  if (b.abort) throw BatchAbort{};
+  b.pending.push_back(&r);
+  ++b.waiting;
+  b.cv_disp.notify_one();
+  b.cv_work.wait(lk, [&] { return r.done; });
+  if (b.abort) throw BatchAbort{};
+  return w;
+}
+
+// one round: every waiting state's window or exact tail, all enqueued
+// before one wait
+static void batch_round(SampleBatch& b, std::vector<SampleBatch::Req*>& reqs) {
+  std::vector<SampleBatch::Req*> win, tl;
+  for (SampleBatch::Req* r : reqs) (r->tail ? tl : win).push_back(r);
+  size_t jo = b.jobs_off;  // staging cursor after the states' own slices
+  auto take = [&](size_t bytes) {
+    const size_t o = jo;
+    jo += (bytes + 255) & ~(size_t)255;
+    if (jo > b.stage_bytes) fail(NX_E_INTERNAL, "sampling batch staging too small");
+    return o;
+  };
+  // windows: host staging of every pass, projection jobs grouped by (W, VW)
+  std::vector<int> nvs;
+  std::vector<VarJob> vj(win.size());
+  std::map<std::pair<int, int>, std::vector<QProjJobDev>> groups;
+  for (size_t i = 0; i < win.size(); ++i) {
+    SampleBatch::Req* r = win[i];
+    Window& w = *r->out;
+    std::vector<int> fq, fb, lq;
+    window_args(*r->cx, r->root, w, fq, fb, lq);
+    ProjPlan& pp = r->cx->pp;
+    nvs.push_back(pp.stage(fq, fb, lq, nullptr));
+    groups[{pp.projv.W, pp.VW}].push_back(pp.qproj_job());
+    vj[i].X = vj[i].Y = &pp.projv;
+    vj[i].cx = vj[i].cy = coef_of(pp.s);
+    vj[i].nvar = nvs[i];
+    vj[i].split_ab = 1;
+    vj[i].out_base = (int64_t)i * 2 * 64 * pp.VW;
+  }
+  std::vector<std::tuple<int, int, size_t, int64_t, int>> qlaunch;  // W, VW, table offset, blocks, jobs
+  for (auto& g : groups) {
+    int64_t blocks = 0;
+    for (QProjJobDev& q : g.second) {
+      q.block0 = blocks;
+      blocks += (q.in.chi + 3) / 4;
+    }
+    const size_t o = take(g.second.size() * sizeof(QProjJobDev));
+    std::memcpy(b.stage_h.as<char>(o), g.second.data(), g.second.size() * sizeof(QProjJobDev));
+    qlaunch.emplace_back(g.first.first, g.first.second, o, blocks, (int)g.second.size());
+  }
+  // tails: the 2^d completions of each path, grouped by row words
+  struct TailAt {
+    int64_t nv;
+    size_t ab;  // offset (doubles) of its leaf norms in the round's results
+  };
+  std::vector<TailAt> tat(tl.size());
+  std::map<int, std::vector<TailJobDev>> tgroups;
+  size_t nstr_words = 0, nal = 0, nab = 0;
+  for (size_t i = 0; i < tl.size(); ++i) {
+    const SampleCtx& cx = *tl[i]->cx;
+    const QForms& q = qform_of(cx.s);
+    const int d = (int)cx.q.size() - tl[i]->root, W = q.W;
+    const int64_t nv = (int64_t)1 << d;
+    TailJobDev t{};
+    t.q = q.desc();
+    t.cs = coef_of(cx.s).cs;
+    t.d = d;
+    t.nv = nv;
+    t.nchunks = amp_chunks(W, 1, nv, cx.s->chi());
+    const size_t ob = take(32 + 4 * (size_t)d);
+    uint64_t* base = b.stage_h.as<uint64_t>(ob);
+    for (int w = 0; w < 4; ++w) base[w] = 0ull;
+    for (int k = 0; k < tl[i]->root; ++k)
+      if (cx.path[k]) base[cx.q[k] >> 6] |= 1ull << (cx.q[k] & 63);
+    std::memcpy(b.stage_h.as<char>(ob + 32), cx.q.data() + tl[i]->root, 4 * (size_t)d);
+    t.base = (const uint64_t*)(b.stage_d.as<char>() + ob);
+    t.qs = (const int32_t*)(b.stage_d.as<char>() + ob + 32);
+    // device offsets for now (words / doubles), pointers once allocated
+    t.strings = (uint64_t*)(uintptr_t)nstr_words;
+    t.al = (double*)(uintptr_t)nal;
+    t.ab = (double*)(uintptr_t)nab;
+    tat[i] = {nv, nab};
+    nstr_words += (size_t)nv * W;
+    nal += (size_t)t.nchunks * nv * 4;
+    nab += 2 * (size_t)nv;
+    tgroups[W].push_back(t);
+  }
+  DevBuf tstr, tal, tab;
+  std::vector<std::tuple<int, size_t, int, int64_t, int64_t>> tlaunch;  // W, table offset, jobs, strings, blocks
+  if (!tl.empty()) {
+    tstr.alloc(nstr_words * 8);
+    tal.alloc(nal * 8);
+    tab.alloc(nab * 8);
+    if (b.tab_cap < nab) {
+      b.tab_h.alloc(nab * 8);
+      b.tab_cap = nab;
+    }
+    for (auto& g : tgroups) {
+      int64_t str0 = 0, blocks = 0;
+      const int64_t per = 256 * (int64_t)amp_strings_per_thread(g.first);
+      for (TailJobDev& t : g.second) {
+        t.strings = tstr.as<uint64_t>() + (uintptr_t)t.strings;
+        t.al = tal.as<double>() + (uintptr_t)t.al;
+        t.ab = tab.as<double>() + (uintptr_t)t.ab;
+        t.str0 = str0;
+        t.block0 = blocks;
+        str0 += t.nv;
+        blocks += ((t.nv + per - 1) / per) * t.nchunks;
+      }
+      const size_t o = take(g.second.size() * sizeof(TailJobDev));
+      std::memcpy(b.stage_h.as<char>(o), g.second.data(), g.second.size() * sizeof(TailJobDev));
+      tlaunch.emplace_back(g.first, o, (int)g.second.size(), str0, blocks);
+    }
+  }
+  // one upload of the staged block, then the launches
+  h2d(b.stage_d.p, b.stage_h.p, jo);
+  {
+    ProfScope ps("k_qproject_var");
+    for (auto& L : qlaunch)
+      hip_check(launch_qproject_var_jobs(std::get<0>(L), std::get<1>(L),
+                                         (const QProjJobDev*)(b.stage_d.as<char>() + std::get<2>(L)), std::get<4>(L),
+                                         std::get<3>(L), stream()),
+                "k_qproject_var_jobs");
+    ps.end();
+  }
+  if (!tl.empty()) {
+    ProfScope ps("k_amplitude_exact");
+    for (auto& L : tlaunch)
+      hip_check(launch_tails(std::get<0>(L), (const TailJobDev*)(b.stage_d.as<char>() + std::get<1>(L)),
+                             std::get<2>(L), std::get<3>(L), std::get<4>(L), ctx()->tabs, stream()),
+                "launch_tails");
+    ps.end();
+    d2h(b.tab_h.p, tab.p, nab * 8);
+  }
+  VarPlan plan;
+  if (!win.empty()) {
+    int VW = 1;
+    int64_t target = 0;
+    for (size_t i = 0; i < win.size(); ++i) {
+      VW = std::max(VW, win[i]->cx->pp.VW);
+      target = std::max<int64_t>(target, 3 * ProjPlan::resident_waves(win[i]->cx->pp.projv.W, win[i]->cx->pp.VW));
+    }
+    plan = make_plan(vj, VW, 8, std::max<int64_t>(target, 1024));
+    // the plan's jobs carry each window's unit constraints, as a lone pass does
+    std::vector<PairJobDev>& hj = plan.hjobs;
+    for (size_t i = 0; i < win.size(); ++i) {
+      const PairJobDev& src = win[i]->cx->pp.hj[0];
+      PairJobDev& d = hj[i];
+      d.X = src.X;
+      d.Y = src.Y;
+      d.V = src.V;
+      d.nvar = src.nvar;
+      d.nunit = src.nunit;
+      d.uq = src.uq;
+      d.ub = src.ub;
+      d.termvar = src.termvar;
+      d.rec = nullptr;
+      for (int w = 0; w < 4; ++w) d.removed[w] = src.removed[w];
+    }
+    h2d(plan.jobs.p, hj.data(), win.size() * sizeof(PairJobDev));
+    ProfScope ps("k_pairs_var");
+    plan.enqueue(stream(), nullptr, b.res_h.as<double>(), false);
+    ps.end();
+  }
+  const auto ts = std::chrono::steady_clock::now();
+  sync();
+  b.sync_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
+  for (size_t i = 0; i < win.size(); ++i) {
+    const double* r = b.res_h.as<double>() + vj[i].out_base;
+    window_fill(*win[i]->out, r, r + nvs[i]);
+    ++win[i]->cx->passes;
+  }
+  for (size_t i = 0; i < tl.size(); ++i) {
+    Window& w = *tl[i]->out;
+    w.root = tl[i]->root;
+    w.d = (int)tl[i]->cx->q.size() - w.root;
+    const double* A = b.tab_h.as<double>() + tat[i].ab;
+    window_fill(w, A, A + tat[i].nv);
+    ++tl[i]->cx->tails;
+  }
+  b.windows += (int64_t)win.size();
+  b.tails += (int64_t)tl.size();
+}
+
+// dispatcher: until every DFS has finished, wait for all running ones to
+// stop at a window, then evaluate the round
+static void batch_dispatch(SampleBatch& b) {
+  for (;;) {
+    std::vector<SampleBatch::Req*> reqs;
+    const auto tw = std::chrono::steady_clock::now();
+    {
+      std::unique_lock<std::mutex> lk(b.mu);
+      b.cv_disp.wait(lk, [&] { return b.waiting == b.running; });
+      if (b.running == 0) return;
+      reqs.swap(b.pending);
+    }
+    const auto tr = std::chrono::steady_clock::now();
+    b.wait_s += std::chrono::duration<double>(tr - tw).count();
+    bool ok = true;
+    std::string err;
+    int code = NX_OK;
+    try {
+      batch_round(b, reqs);
+      ++b.rounds;
+      b.round_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - tr).count();
+    } catch (const NxError& e) {
+      ok = false;
+      code = e.code;
+      err = g_err;
+    } catch (const std::exception& e) {
+      ok = false;
+      code = NX_E_INTERNAL;
+      err = e.what();
+    }
+    {
+      std::lock_guard<std::mutex> lk(b.mu);
+      if (!ok) b.abort = true;
+      for (SampleBatch::Req* r : reqs) r->done = true;
+      b.waiting -= (int)reqs.size();
+    }
+    b.cv_work.notify_all();
+    if (!ok) {
+      // every DFS thread was waiting in this round: they wake, see `abort`,
+      // unwind (BatchAbort) and finish; a later request throws at once
+      std::unique_lock<std::mutex> lk(b.mu);
+      b.cv_disp.wait(lk, [&] { return b.running == 0; });
+      g_err = err;
+      throw NxError{code};
+    }
+  }
+}
+
 // sampling.rs:80-189.  The norm of a child is ||Pi psi||^2 for the projector
 // onto the outcomes chosen so far (the reference projects, then discards
 // the measured qubit, which leaves the norm unchanged); each window pass
@@ -1860,9 +2201,10 @@ static void recursive_sample(SampleCtx& cx, size_t idx, uint64_t shots, uint64_t
   int j = 0;        // children's relative depth in the window
   if ((int)idx >= cx.uniform) {
     if (!w || (int)idx + 1 > w->root + w->d) {
-      own = cx.ref ? open_window_ref(cx, (int)idx)
-                   : (((int)cx.q.size() - (int)idx <= cx.tail_m) ? open_tail(cx, (int)idx)
-                                                                  : open_window(cx, (int)idx));
+      const bool tail = !cx.ref && (int)cx.q.size() - (int)idx <= cx.tail_m;
+      own = cx.batch ? batch_window(cx, (int)idx, tail)
+                     : (cx.ref ? open_window_ref(cx, (int)idx) : (tail ? open_tail(cx, (int)idx)
+                                                                       : open_window(cx, (int)idx)));
       sample_progress(cx, "window", (int)idx);
       w = &own;
       rel = 0;
@@ -2241,6 +2583,137 @@ int nx_state_sample_exact(const nx_state* s, int32_t nq, const int32_t* qargs, u
   return sample_impl(s, nq, qargs, shots, seed32, max_out, out_bits, out_counts, nout, false);
 }
 
+int nx_batch_sample_exact(int32_t nstates, const nx_state* const* states, const int32_t* nq, const int32_t* qargs,
+                          const uint64_t* shots, const uint8_t* seeds, int64_t max_out, uint64_t* out_bits,
+                          uint64_t* out_counts, int64_t* nout, int64_t* stats) {
+  NX_TRY({
+    if (nstates < 0) fail(NX_E_VALUE, "negative state count");
+    std::vector<std::unique_ptr<SampleCtx>> cxs;
+    std::vector<std::unique_ptr<ChaCha12Rng>> rngs;
+    std::vector<uint64_t> nshots(nstates);
+    SampleBatch b;
+    size_t qoff = 0, soff = 0;
+    std::vector<size_t> stage_at(nstates, 0);
+    for (int32_t i = 0; i < nstates; ++i) {
+      const nx_state* s = states[i];
+      if (!s) fail(NX_E_VALUE, "state %d is null", i);
+      if (nq[i] < 0) fail(NX_E_VALUE, "state %d: negative qubit count", i);
+      validate_qargs(s, nq[i], qargs + qoff);
+      if (nq[i] > 128) fail(NX_E_VALUE, "Sampling more than 128 qubits is not supported.");
+      uint8_t seed[32];
+      seed_or_entropy(seeds ? seeds + (size_t)32 * i : nullptr, seed);
+      rngs.emplace_back(new ChaCha12Rng(seed));
+      cxs.emplace_back(new SampleCtx());
+      SampleCtx& cx = *cxs.back();
+      cx.rng = rngs.back().get();
+      std::vector<std::pair<int, int>> sorted;
+      for (int k = 0; k < nq[i]; ++k) sorted.push_back({qargs[qoff + k], k});
+      std::stable_sort(sorted.begin(), sorted.end(), [](auto& a, auto& c) { return a.first > c.first; });
+      for (auto& pq : sorted) {
+        cx.q.push_back(pq.first);
+        cx.bit.push_back(pq.second);
+      }
+      qoff += (size_t)nq[i];
+      nshots[i] = shots[i];
+      if (shots[i] > 0 && nq[i] > 0 && s->chi() == 0)  // every norm is 0: p = 0/0
+        fail(NX_E_VALUE, "p < 0 or is NaN in binomial distribution");
+      if (shots[i] > 0 && nq[i] > 0) {
+        cx.s = s;
+        stage_at[i] = soff;
+        soff += ProjPlan::stage_size(nq[i], cx.look);
+      }
+    }
+    // one staging block for every state's window passes, the projection job
+    // tables after it
+    b.jobs_off = soff;
+    b.stage_bytes = soff + (size_t)std::max(nstates, 1) * (sizeof(QProjJobDev) + sizeof(TailJobDev) + 1024) +
+                    32 * 256;
+    b.stage_h.alloc(b.stage_bytes);
+    b.stage_d.alloc(b.stage_bytes);
+    b.res_h.alloc((size_t)std::max(nstates, 1) * 2 * 64 * 2 * 8);
+    std::vector<int32_t> live;
+    for (int32_t i = 0; i < nstates; ++i) {
+      SampleCtx& cx = *cxs[i];
+      if (!cx.s) {
+        if (nshots[i] > 0) {  // no qubits: one empty outcome
+          cx.bits.push_back(0);
+          cx.bits.push_back(0);
+          cx.counts.push_back(nshots[i]);
+        }
+        continue;
+      }
+      const nx_state* s = cx.s;
+      const int n = (int)cx.q.size();
+      cx.batch = &b;
+      cx.uniform = std::min(uniform_prefix(s, cx.q), n - 1);
+      cx.tail_m = (!s->is_complex && n == s->n) ? 14 : 0;
+      cx.pp.init(s, n, cx.look, b.stage_h.as<char>(stage_at[i]), b.stage_d.as<char>() + stage_at[i]);
+      live.push_back(i);
+    }
+    // the DFS threads; this thread dispatches their windows
+    std::vector<int> codes(nstates, NX_OK);
+    std::vector<std::string> msgs(nstates);
+    b.running = (int)live.size();
+    std::vector<std::thread> th;
+    for (int32_t i : live) {
+      th.emplace_back([&, i] {
+        SampleCtx& cx = *cxs[i];
+        try {
+          recursive_sample(cx, 0, nshots[i], 0, 0, nullptr, 0);
+        } catch (const BatchAbort&) {
+        } catch (const NxError& e) {
+          codes[i] = e.code;
+          msgs[i] = g_err;
+        } catch (const std::exception& e) {
+          codes[i] = NX_E_INTERNAL;
+          msgs[i] = e.what();
+        }
+        {
+          std::lock_guard<std::mutex> lk(b.mu);
+          --b.running;
+        }
+        b.cv_disp.notify_one();
+      });
+    }
+    int dcode = NX_OK;
+    std::string dmsg;
+    try {
+      batch_dispatch(b);
+    } catch (const NxError& e) {
+      dcode = e.code;
+      dmsg = g_err;
+    }
+    for (std::thread& t : th) t.join();
+    if (dcode != NX_OK) fail(dcode, "%s", dmsg.c_str());
+    for (int32_t i = 0; i < nstates; ++i)
+      if (codes[i] != NX_OK) fail(codes[i], "%s", msgs[i].c_str());
+    int64_t total = 0;
+    for (int32_t i = 0; i < nstates; ++i) {
+      nout[i] = (int64_t)cxs[i]->counts.size();
+      total += nout[i];
+    }
+    if (stats) {
+      stats[0] = b.rounds;
+      stats[1] = b.windows;
+      stats[2] = b.tails;
+      stats[3] = (int64_t)(b.round_s * 1e6);
+      stats[4] = (int64_t)(b.wait_s * 1e6);
+      stats[5] = (int64_t)(b.sync_s * 1e6);
+    }
+    // the caller learns the capacity it needs (sum of nout) even when this fails
+    if (total > max_out) fail(NX_E_VALUE, "sample output buffer too small (%lld outcomes)", (long long)total);
+    int64_t o = 0;
+    for (int32_t i = 0; i < nstates; ++i) {
+      const int64_t k = nout[i];
+      if (k) {
+        std::memcpy(out_bits + 2 * o, cxs[i]->bits.data(), (size_t)k * 16);
+        std::memcpy(out_counts + o, cxs[i]->counts.data(), (size_t)k * 8);
+      }
+      o += k;
+    }
+  })
+}
+
 int nx_state_uniform_prefix(const nx_state* s, int32_t nq, const int32_t* qargs, int32_t* out) {
   NX_TRY({
     validate_qargs(s, nq, qargs);
@@ -2567,6 +3040,13 @@ int nx_profile_enable(int on) {
     g_prof.collect();
     g_prof.on = on != 0;
     g_prof.acc.clear();
+  })
+}
+
+int nx_profile_only(const char* kernels) {
+  NX_TRY({
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof.only = (kernels && *kernels) ? "," + std::string(kernels) + "," : std::string();
   })
 }
 

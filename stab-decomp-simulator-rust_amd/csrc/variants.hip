@@ -437,15 +437,12 @@ __global__ __launch_bounds__(256) void k_qproject(QFormDev in, QFormDev out, con
 // with variant differences hi_f ^ hi_o, the ket's and the bra's rhs flips and
 // phase d_f - d_o: each pair of the window pass starts after the unit
 // substitutions (Pi_k is a projector, so this is <phi_o| Pi_k |phi_f>).
+// term t of one state (one wavefront)
 template <int W, int VW>
-__global__ __launch_bounds__(256) void k_qproject_var(QFormDev in, QFormDev out, VarDev vout,
-                                                      const int32_t* __restrict__ fq,
-                                                      const int32_t* __restrict__ fb, int nfix,
-                                                      const int32_t* __restrict__ uq,
-                                                      const uint64_t* __restrict__ ub, int nunit) {
-  const int64_t t = (int64_t)blockIdx.x * 4 + uni((int)(threadIdx.x >> 6));  // uniform in the wave
-  const int lane = threadIdx.x & 63;
-  if (t >= in.chi) return;
+__device__ __forceinline__ void qproject_var_term(const QFormDev& in, const QFormDev& out, const VarDev& vout,
+                                                  const int32_t* __restrict__ fq, const int32_t* __restrict__ fb,
+                                                  int nfix, const int32_t* __restrict__ uq,
+                                                  const uint64_t* __restrict__ ub, int nunit, int64_t t, int lane) {
   const int R = 64 * W;
   PairRegs<W> P;
   VarRegs<W, VW> V;
@@ -520,6 +517,35 @@ __global__ __launch_bounds__(256) void k_qproject_var(QFormDev in, QFormDev out,
       ((u64*)vout.ph)[((size_t)t * 3 + 2) * VW + v] = V.d2[v];
     }
   }
+}
+
+template <int W, int VW>
+__global__ __launch_bounds__(256) void k_qproject_var(QFormDev in, QFormDev out, VarDev vout,
+                                                      const int32_t* __restrict__ fq,
+                                                      const int32_t* __restrict__ fb, int nfix,
+                                                      const int32_t* __restrict__ uq,
+                                                      const uint64_t* __restrict__ ub, int nunit) {
+  const int64_t t = (int64_t)blockIdx.x * 4 + uni((int)(threadIdx.x >> 6));  // uniform in the wave
+  if (t >= in.chi) return;
+  qproject_var_term<W, VW>(in, out, vout, fq, fb, nfix, uq, ub, nunit, t, threadIdx.x & 63);
+}
+
+// The same for several states at once (sampling windows of many circuits):
+// block b belongs to the job j with jobs[j].block0 <= b < jobs[j + 1].block0
+// (found by a scalar binary search over the njobs jobs), four terms per block.
+template <int W, int VW>
+__global__ __launch_bounds__(256) void k_qproject_var_jobs(const QProjJobDev* __restrict__ jobs, int njobs) {
+  const int64_t b = blockIdx.x;
+  int lo = 0, hi = njobs;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (uni64((u64)jobs[mid].block0) <= (u64)b) lo = mid;
+    else hi = mid;
+  }
+  const QProjJobDev& j = jobs[lo];
+  const int64_t t = (b - (int64_t)uni64((u64)j.block0)) * 4 + uni((int)(threadIdx.x >> 6));
+  if (t >= j.in.chi) return;
+  qproject_var_term<W, VW>(j.in, j.out, j.vout, j.fq, j.fb, j.nfix, j.uq, j.ub, j.nunit, t, threadIdx.x & 63);
 }
 
 // ---------------------------------------------------------------------------
@@ -1483,6 +1509,26 @@ hipError_t launch_qproject_var(const QFormDev& in, const QFormDev& out, const Va
     default: return hipErrorInvalidValue;
   }
 #undef NX_QPV
+  return hipGetLastError();
+}
+
+hipError_t launch_qproject_var_jobs(int W, int VW, const QProjJobDev* jobs, int njobs, int64_t nblocks,
+                                    hipStream_t st) {
+  if (njobs <= 0 || nblocks <= 0) return hipSuccess;
+  const dim3 g((unsigned)nblocks), b(256);
+#define NX_QPVJ(WW, VV) k_qproject_var_jobs<WW, VV><<<g, b, 0, st>>>(jobs, njobs)
+  switch (W * 4 + VW) {
+    case 1 * 4 + 1: NX_QPVJ(1, 1); break;
+    case 1 * 4 + 2: NX_QPVJ(1, 2); break;
+    case 2 * 4 + 1: NX_QPVJ(2, 1); break;
+    case 2 * 4 + 2: NX_QPVJ(2, 2); break;
+    case 3 * 4 + 1: NX_QPVJ(3, 1); break;
+    case 3 * 4 + 2: NX_QPVJ(3, 2); break;
+    case 4 * 4 + 1: NX_QPVJ(4, 1); break;
+    case 4 * 4 + 2: NX_QPVJ(4, 2); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef NX_QPVJ
   return hipGetLastError();
 }
 

@@ -9,6 +9,6 @@ from .gate import QuantumGate
 from .circuit import QuantumCircuit
 from .pauli_string import PauliString
 from .state import QuantumState
-from .batch import Batch
+from .batch import Batch, sample_many
 
-__all__ = ["QuantumGate", "QuantumCircuit", "QuantumState", "PauliString", "Batch"]
+__all__ = ["QuantumGate", "QuantumCircuit", "QuantumState", "PauliString", "Batch", "sample_many"]

@@ -51,6 +51,7 @@ SIGNATURES = {
     "nx_state_grad_exp_values_device": (C.c_int, [_P, _I32, _I32P, _DP, _DP, _DP, _DP, _P]),
     "nx_state_exp_values": (C.c_int, [_P, _I32, _I32P, _DP]),
     "nx_profile_enable": (C.c_int, [C.c_int]),
+    "nx_profile_only": (C.c_int, [C.c_char_p]),
     "nx_profile_get": (C.c_int, [C.c_char_p, _DP, _I64P]),
     "nx_state_drop_cache": (C.c_int, [_P]),
     "nx_state_projection_norms": (C.c_int, [_P, _I32, _I32P, _I32P, _I32, _I32P, _DP]),
@@ -61,6 +62,8 @@ SIGNATURES = {
     "nx_batch_shape": (C.c_int, [_P, _I32P, _I64P, _I32P]),
     "nx_batch_run": (C.c_int, [_P, _DP]),
     "nx_batch_free": (None, [_P]),
+    "nx_batch_sample_exact": (C.c_int, [_I32, C.POINTER(_P), _I32P, _I32P, _U64P, _U8P, _I64, _U64P, _U64P,
+                                        _I64P, _I64P]),
 }
 
 _lib = None
