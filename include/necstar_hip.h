@@ -207,6 +207,15 @@ int nx_state_uniform_prefix(const nx_state *s, int32_t nq, const int32_t *qargs,
 int nx_state_uniform_prefix_given(const nx_state *s, int32_t nfixed, const int32_t *fixed_q, const int32_t *fixed_b,
                                   int32_t nq, const int32_t *qargs, int32_t *out);
 
+/* |<x|psi>|^2 / |g|^2 = A[y] + sqrt(2) B[y] for the 2^d completions x =
+ * base | y of `base` (W words, zero on the qs qubits) on qubits qs[0..d)
+ * (bit i of y -> qubit qs[i]), d <= 16, n <= 128, Scalar coefficients:
+ * the exact leaf norms of a sampling tail (sampling with exact=True).
+ * Parity aid; the reference evaluates amplitudes one at a time
+ * (amplitude.rs:28-44, statevector.rs:12-24). */
+int nx_state_completion_norms(const nx_state *s, const uint64_t *base, int32_t d, const int32_t *qs, double *A,
+                              double *B);
+
 /* Exact pair scalars <phi_i|phi_j> for all i, j < chi (chi*chi uint32:
  * 0xFFFFFFFF zero, else k | r << 8).  Parity aid; the reference computes
  * the same values in StabilizerCHForm::inner_product (inner_product.rs:14-48). */
@@ -281,7 +290,7 @@ int nx_batch_sample_exact(int32_t nstates, const nx_state *const *states, const 
 
 /* Kernel timing with HIP events recorded around each launch on the stream
  * the library enqueues on.  kernel: "k_amplitude", "k_amp_reduce",
- * "k_amplitude_exact", "k_pairs", "k_reduce", "k_pairs_var", "k_pairs_sym",
+ * "k_tail", "k_pairs", "k_reduce", "k_pairs_var", "k_pairs_sym",
  * "k_grad_var", "k_qform", "k_qproject_var", "k_fold", "k_program",
  * "batch_pairs" (a whole nx_batch_run launch plan).  Enabling clears the totals.  No
  * reference counterpart (measurement aid). */

@@ -7,7 +7,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 OUT = os.path.join(PKG, "necstar", "libnecstar_hip.so")
-SOURCES = ["kernels.hip", "variants.hip", "fold.hip", "runtime.hip"]
+SOURCES = ["kernels.hip", "variants.hip", "fold.hip", "tails.hip", "runtime.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

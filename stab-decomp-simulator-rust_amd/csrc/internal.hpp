@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 namespace nx {
 
 // Program op codes.  0..10 are the Clifford gates in the numbering of the
@@ -145,7 +147,7 @@ inline int amp_strings_per_thread(int W) { return W <= 2 ? 4 : 2; }
 // number of term chunks (blockIdx.y) of an amplitude launch over nb strings
 // and chi terms: the launch fills whole rounds of the resident blocks, with
 // at least AMP_MIN_TERMS terms per chunk and at most 64 chunks
-int amp_chunks(int W, int exact, int64_t nb, int64_t chi);
+int amp_chunks(int W, int64_t nb, int64_t chi);
 // chunk c covers terms [c chi / nchunks, (c + 1) chi / nchunks); nchunks > 1:
 // partial must hold nchunks * nb values
 hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const uint64_t* bits, int64_t nb, int nchunks,
@@ -153,34 +155,32 @@ hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const uint64
                             double gre, double gim, hipStream_t st);
 hipError_t launch_amp_reduce(const double2* partial, int64_t nb, int nchunks, double2* out, double gre, double gim,
                              hipStream_t st);
-// exact amplitudes with Scalar coefficients cs: xout[b * 4 + m] = component
-// m (basis 1, zeta, zeta^2, zeta^3) of sum_t c_t <x_b|phi_t>; xout holds
-// nchunks * nb * 4 doubles (chunks are summed into the first)
-// sampling tails: the 2^d completions of base on qubits qs (W words each),
-// and the leaf norms A[k], B[k] = ab[k], ab[nv + k] of exact components al
-hipError_t launch_tail_strings(const uint64_t* base, const int32_t* qs, int d, int W, int64_t nv, uint64_t* out,
-                               hipStream_t st);
-hipError_t launch_tail_leaves(const double* al, int64_t nv, double* ab, hipStream_t st);
-hipError_t launch_amplitude_exact(const QFormDev& q, const int32_t* cs, const uint64_t* bits, int64_t nb, int nchunks,
-                                  const double* tabs, double* xout, hipStream_t st);
-// Several sampling tails at once (all of row word width W): job j's 2^d
-// completions of base on qubits qs into strings, their exact amplitudes
-// over the terms of q in nchunks term chunks (launch_amplitude_exact's
-// blocks, from block0 on) into al, and the leaf norms A, B into ab[0..nv),
-// ab[nv..2nv); strings str0 .. str0 + nv - 1 of the launch are job j's.
+// Sampling tails (tails.hip): job j holds the completions x = base | y of
+// a path on its last d <= 16 qubits qs (bit i of y -> qubit qs[i]); for
+// each term t of q, k_tail_restrict writes a record of TAIL_REC words to
+// recs (the term restricted to the d tail variables), k_tail_eval sums the
+// terms' exact values into the components al[(c nv + y) 4 + m] of term
+// chunk c < nchunks (blocks block0 .. block0 + cols * nchunks - 1, cols =
+// ceil(nv / 1024)), k_tail_norms the chunks in order into the leaf norms
+// ab[y] = A, ab[nv + y] = B (|psi(x)|^2 / |g|^2 = A + sqrt2 B).  term0 and
+// str0: the job's first term (restrict grid) and completion (norms grid)
+// in the launch.  All jobs of one launch have row word width W <= 2.
+constexpr int TAIL_REC = 68;
 struct TailJobDev {
   QFormDev q;
   const int32_t* cs;
   const uint64_t* base;
   const int32_t* qs;
-  uint64_t* strings;
+  uint32_t* recs;
   double* al;
   double* ab;
   int32_t d, nchunks;
-  int64_t nv, str0, block0;
+  int64_t nv, term0, block0, str0;
 };
-hipError_t launch_tails(int W, const TailJobDev* jobs, int njobs, int64_t total_strings, int64_t total_blocks,
-                        const double* tabs, hipStream_t st);
+// term chunks of a tail over chi terms (at least 256 terms per chunk)
+inline int tail_chunks(int64_t chi) { return (int)std::max<int64_t>(1, std::min<int64_t>(64, chi / 256)); }
+hipError_t launch_tails(int W, const TailJobDev* jobs, int njobs, int64_t total_terms, int64_t total_blocks,
+                        int64_t total_strings, hipStream_t st);
 // Pair sums over units of J = 8 consecutive other indices o of a fixed
 // index f (triangle: o in [f, nother), nfixed == nother, rows from the
 // closed form below; else o in [0, nother)), four waves per block, nblocks

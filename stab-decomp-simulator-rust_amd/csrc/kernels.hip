@@ -450,7 +450,7 @@ __device__ __forceinline__ void amp_store_raw(u64* raw, const u64 (&pre)[AmpLds<
 }
 
 // tables of one term from its staged rows; c and nrm = 2^{-k/2} of the term
-template <int W, bool EX>
+template <int W>
 __device__ __forceinline__ void amp_build(const u64* raw, char* buf, bool bigc, double2 c, double nrm,
                                           const double* s_cos, const double* s_sin) {
   using L = AmpLds<W>;
@@ -485,7 +485,7 @@ __device__ __forceinline__ void amp_build(const u64* raw, char* buf, bool bigc, 
     }
   }
   if (threadIdx.x < L::NCD * W) ((u64*)(buf + L::B_C))[threadIdx.x] = raw[L::R_C + threadIdx.x];
-  if (!EX && threadIdx.x < 9) {
+  if (threadIdx.x < 9) {
     double tr = 0.0, ti = 0.0;
     if (threadIdx.x < 8) {
       const double zr = s_cos[threadIdx.x] * nrm, zi = s_sin[threadIdx.x] * nrm;
@@ -515,45 +515,18 @@ __device__ __forceinline__ void ld_entry(const char* p, uint32_t (&e)[2 * 4]) {
   }
 }
 
-// Exact amplitudes (EX): with Scalar coefficients every term adds
-// zeta^K 2^{-R/2} to the amplitude.  For even R that is +-2^{-R/2} on one of
-// the components of the basis 1, zeta, zeta^2, zeta^3 (zeta^{j+4} = -zeta^j);
-// for odd R, sqrt2 = zeta + zeta^-1 splits it into 2^{-(R+1)/2} on zeta^{K+1}
-// and zeta^{K-1}.  Each addend is a power of two, so the four components are
-// exact sums while their exponent span fits a double.
-__device__ __forceinline__ void add_zeta(int K, int R, double (&al)[4]) {
-  int i1, i2;
-  double v1, v2;
-  if (!(R & 1)) {
-    i1 = K & 3;
-    v1 = ldexp((K & 4) ? -1.0 : 1.0, -(R >> 1));
-    i2 = 0;
-    v2 = 0.0;
-  } else {
-    const int e = -((R + 1) >> 1);
-    const int ka = (K + 1) & 7, kb = (K + 7) & 7;
-    i1 = ka & 3;
-    v1 = ldexp((ka & 4) ? -1.0 : 1.0, e);
-    i2 = kb & 3;
-    v2 = ldexp((kb & 4) ? -1.0 : 1.0, e);
-  }
-#pragma unroll
-  for (int j = 0; j < 4; ++j) al[j] = al[j] + ((j == i1) ? v1 : 0.0) + ((j == i2) ? v2 : 0.0);
-}
-
 template <int V>
 struct IConst {
   static constexpr int value = V;
 };
 
 // one block of an amplitude launch: string column bx, term chunk by
-template <int W, int XB, bool EX>
+template <int W, int XB>
 __device__ __forceinline__ void amp_block(const QFormDev& q, const double2* __restrict__ coef,
                                           const u64* __restrict__ bits, int64_t nb, int64_t nchunks,
                                           const double* __restrict__ tabs, double2* __restrict__ out,
                                           uint8_t* __restrict__ per_term, int enumerate, int final_g, double gre,
-                                          double gim, const int32_t* __restrict__ cs, double* __restrict__ xout,
-                                          int64_t bx, int64_t by) {
+                                          double gim, int64_t bx, int64_t by) {
   using L = AmpLds<W>;
   using Y = QLay<W>;
   constexpr int NB = L::NB;
@@ -599,13 +572,8 @@ __device__ __forceinline__ void amp_block(const QFormDev& q, const double2* __re
       for (int b = 0; b < NB; ++b) off[j][b] = nib(j, b) * (uint32_t)(4 * Y::nd(b));
   }
   double are[XB], aim[XB];
-  double al[EX ? XB : 1][4];
 #pragma unroll
   for (int j = 0; j < XB; ++j) { are[j] = 0.0; aim[j] = 0.0; }
-#pragma unroll
-  for (int j = 0; j < (EX ? XB : 1); ++j)
-#pragma unroll
-    for (int m = 0; m < 4; ++m) al[j][m] = 0.0;
   u64 pre[L::NPRE];
   if (t0 < t1) {
     amp_load_raw<W>(q, t0, pre);
@@ -613,8 +581,7 @@ __device__ __forceinline__ void amp_block(const QFormDev& q, const double2* __re
   }
   __syncthreads();
   if (t0 < t1) {
-    const double2 c = EX ? make_double2(0.0, 0.0) : coef[t0];
-    amp_build<W, EX>(raw, bufs, uni(q.nc[t0]) > L::NCD, c, tabs[TAB_POW + uni(q.k[t0])], s_cos, s_sin);
+    amp_build<W>(raw, bufs, uni(q.nc[t0]) > L::NCD, coef[t0], tabs[TAB_POW + uni(q.k[t0])], s_cos, s_sin);
   }
   __syncthreads();
 
@@ -679,14 +646,12 @@ __device__ __forceinline__ void amp_block(const QFormDev& q, const double2* __re
   // wave-uniform values of a term (scalar registers), read one term ahead
   struct TermU {
     int k, p0, nc;
-    int32_t cst;
     u64 alo[W], ahi[W], rhs[W];
   };
   auto load_u = [&](int64_t t, TermU& u) {
     u.k = uni(q.k[t]);
     u.p0 = uni(q.p0[t]);
     u.nc = uni(q.nc[t]);
-    u.cst = EX ? uni(cs[t]) : 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
       u.alo[w] = uni64(q.alo[(size_t)t * W + w]);
@@ -705,12 +670,11 @@ __device__ __forceinline__ void amp_block(const QFormDev& q, const double2* __re
     TermU nxt = cur;
     if (more) {  // in flight during the evaluation below
       amp_load_raw<W>(q, t + 1, pre);
-      if (!EX) cn = coef[t + 1];
+      cn = coef[t + 1];
       load_u(t + 1, nxt);
     }
     const char* buf = bufs + BI * L::BUF;
-    const int k = cur.k, p0 = cur.p0, nc = cur.nc;
-    const int32_t cst = cur.cst;
+    const int p0 = cur.p0, nc = cur.nc;
     const bool bigc = nc > L::NCD;
     const u64 z = (u64)(t >> 62);
     // support test: bad[j] & 1 is set outside the support
@@ -762,17 +726,13 @@ __device__ __forceinline__ void amp_block(const QFormDev& q, const double2* __re
       const int ph = (int)(((uint32_t)p0 + 2u * pa + 4u * pq) & 7u);
       const bool outside = (bad[j] & 1u) != 0u;
       if (per_term && active[j]) per_term[(size_t)t * nb + bidx[j]] = outside ? (uint8_t)0xFF : (uint8_t)ph;
-      if constexpr (EX) {
-        if (!outside && cst >= 0) add_zeta((ph + cs_k(cst)) & 7, k + cs_r(cst), al[j]);
-      } else {
-        const double2 d = ((const double2*)(buf + L::B_PH))[outside ? 8 : ph];
-        are[j] = are[j] + d.x;
-        aim[j] = aim[j] + d.y;
-      }
+      const double2 d = ((const double2*)(buf + L::B_PH))[outside ? 8 : ph];
+      are[j] = are[j] + d.x;
+      aim[j] = aim[j] + d.y;
     }
     if (more) amp_store_raw<W>(raw, pre);
     __syncthreads();
-    if (more) amp_build<W, EX>(raw, bufs + (1 - BI) * L::BUF, nxt.nc > L::NCD, cn, tabs[TAB_POW + nxt.k], s_cos, s_sin);
+    if (more) amp_build<W>(raw, bufs + (1 - BI) * L::BUF, nxt.nc > L::NCD, cn, tabs[TAB_POW + nxt.k], s_cos, s_sin);
     __syncthreads();
     cur = nxt;
   };
@@ -784,12 +744,7 @@ __device__ __forceinline__ void amp_block(const QFormDev& q, const double2* __re
 #pragma unroll
   for (int j = 0; j < XB; ++j) {
     if (!active[j]) continue;
-    if constexpr (EX) {
-      // four components per string, per chunk (the global factor is applied
-      // by the caller)
-#pragma unroll
-      for (int m = 0; m < 4; ++m) xout[((size_t)by * nb + bidx[j]) * 4 + m] = al[j][m];
-    } else if (final_g) {  // * global_factor (statevector.rs:23)
+    if (final_g) {  // * global_factor (statevector.rs:23)
       out[bidx[j]] = make_double2(are[j] * gre - aim[j] * gim, are[j] * gim + aim[j] * gre);
     } else {
       out[(size_t)by * nb + bidx[j]] = make_double2(are[j], aim[j]);
@@ -797,190 +752,14 @@ __device__ __forceinline__ void amp_block(const QFormDev& q, const double2* __re
   }
 }
 
-template <int W, int XB, bool EX>
+template <int W, int XB>
 __global__ __launch_bounds__(AMP_NT) void k_amplitude(QFormDev q, const double2* __restrict__ coef,
                                                    const u64* __restrict__ bits, int64_t nb, int64_t nchunks,
                                                    const double* __restrict__ tabs, double2* __restrict__ out,
                                                    uint8_t* __restrict__ per_term, int enumerate, int final_g,
-                                                   double gre, double gim, const int32_t* __restrict__ cs,
-                                                   double* __restrict__ xout) {
-  amp_block<W, XB, EX>(q, coef, bits, nb, nchunks, tabs, out, per_term, enumerate, final_g, gre, gim, cs, xout,
-                       blockIdx.x, blockIdx.y);
-}
-
-// Exact amplitudes of several sampling tails in one launch (one row word
-// width): block b belongs to the job j with jobs[j].block0 <= b <
-// jobs[j + 1].block0; within it, string column (b - block0) % cols and term
-// chunk (b - block0) / cols, as k_amplitude<W, XB, true> over that job's
-// strings.
-template <int W, int XB>
-__global__ __launch_bounds__(AMP_NT) void k_amplitude_tails(const TailJobDev* __restrict__ jobs, int njobs,
-                                                            const double* __restrict__ tabs) {
-  const int64_t b = blockIdx.x;
-  int lo = 0, hi = njobs;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (uni64((u64)jobs[mid].block0) <= (u64)b) lo = mid;
-    else hi = mid;
-  }
-  const TailJobDev& j = jobs[lo];
-  const int64_t per = (int64_t)AMP_NT * XB;
-  const int64_t cols = ((int64_t)uni64((u64)j.nv) + per - 1) / per;
-  const int64_t local = b - (int64_t)uni64((u64)j.block0);
-  amp_block<W, XB, true>(j.q, nullptr, j.strings, j.nv, j.nchunks, tabs, nullptr, nullptr, 0, 0, 0.0, 0.0, j.cs,
-                         j.al, local % cols, local / cols);
-}
-
-// The 2^d completions of a sampling path: string k = base with bit i of k on
-// qubit qs[i] (W words per string)
-__global__ void k_tail_strings(const u64* __restrict__ base, const int32_t* __restrict__ qs, int d, int W, int64_t nv,
-                               u64* __restrict__ out) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nv) return;
-  for (int w = 0; w < W; ++w) {
-    u64 x = base[w];
-    for (int i = 0; i < d; ++i)
-      if (((k >> i) & 1) && (qs[i] >> 6) == w) x |= 1ull << (qs[i] & 63);
-    out[(size_t)k * W + w] = x;
-  }
-}
-
-// |psi|^2 = A + sqrt2 B from the exact components a_0..a_3 (basis 1, zeta,
-// zeta^2, zeta^3): A = sum a_j^2, B = a0 a1 + a1 a2 + a2 a3 - a0 a3, each
-// accumulated in double-double from exact (FMA) products and rounded once
-struct DDev {
-  double hi = 0.0, lo = 0.0;
-  __device__ void add_prod(double a, double b) {
-    const double p = a * b, pe = fma(a, b, -p);
-    const double s = hi + p, bb = s - hi;
-    double e = (hi - (s - bb)) + (p - bb);
-    e = e + (lo + pe);
-    hi = s + e;
-    lo = e - (hi - s);
-  }
-  __device__ double value() const { return hi + lo; }
-};
-
-__global__ void k_tail_leaves(const double* __restrict__ al, int64_t nv, double* __restrict__ ab) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nv) return;
-  const double a0 = al[k * 4], a1 = al[k * 4 + 1], a2 = al[k * 4 + 2], a3 = al[k * 4 + 3];
-  DDev A, B;
-  A.add_prod(a0, a0);
-  A.add_prod(a1, a1);
-  A.add_prod(a2, a2);
-  A.add_prod(a3, a3);
-  B.add_prod(a0, a1);
-  B.add_prod(a1, a2);
-  B.add_prod(a2, a3);
-  B.add_prod(-a0, a3);
-  ab[k] = A.value();
-  ab[nv + k] = B.value();
-}
-
-// the job of global string g (jobs[j].str0 <= g < jobs[j + 1].str0)
-__device__ __forceinline__ int tail_job_of(const TailJobDev* jobs, int njobs, int64_t g) {
-  int lo = 0, hi = njobs;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (jobs[mid].str0 <= g) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
-
-__global__ void k_tail_strings_jobs(const TailJobDev* __restrict__ jobs, int njobs, int64_t total) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= total) return;
-  const TailJobDev& j = jobs[tail_job_of(jobs, njobs, g)];
-  const int64_t k = g - j.str0;
-  if (k >= j.nv) return;
-  const int W = j.q.W;
-  for (int w = 0; w < W; ++w) {
-    u64 x = j.base[w];
-    for (int i = 0; i < j.d; ++i)
-      if (((k >> i) & 1) && (j.qs[i] >> 6) == w) x |= 1ull << (j.qs[i] & 63);
-    j.strings[(size_t)k * W + w] = x;
-  }
-}
-
-// the chunks of each string's components summed in chunk order (as
-// k_amp_reduce_exact), then its leaf norm (as k_tail_leaves)
-__global__ void k_tail_leaves_jobs(const TailJobDev* __restrict__ jobs, int njobs, int64_t total) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= total) return;
-  const TailJobDev& j = jobs[tail_job_of(jobs, njobs, g)];
-  const int64_t k = g - j.str0;
-  if (k >= j.nv) return;
-  double a[4];
-  for (int m = 0; m < 4; ++m) {
-    double v = j.al[k * 4 + m];
-    for (int c = 1; c < j.nchunks; ++c) v = v + j.al[((size_t)c * j.nv + k) * 4 + m];
-    a[m] = v;
-  }
-  DDev A, B;
-  A.add_prod(a[0], a[0]);
-  A.add_prod(a[1], a[1]);
-  A.add_prod(a[2], a[2]);
-  A.add_prod(a[3], a[3]);
-  B.add_prod(a[0], a[1]);
-  B.add_prod(a[1], a[2]);
-  B.add_prod(a[2], a[3]);
-  B.add_prod(-a[0], a[3]);
-  j.ab[k] = A.value();
-  j.ab[j.nv + k] = B.value();
-}
-
-hipError_t launch_tails(int W, const TailJobDev* jobs, int njobs, int64_t total_strings, int64_t total_blocks,
-                        const double* tabs, hipStream_t st) {
-  if (njobs <= 0 || total_strings <= 0) return hipSuccess;
-  k_tail_strings_jobs<<<dim3((unsigned)((total_strings + 255) / 256)), dim3(256), 0, st>>>(jobs, njobs,
-                                                                                        total_strings);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-#define NX_AMPT_LAUNCH(WW, XBV)                                                                                  \
-  {                                                                                                               \
-    const size_t sh = AmpLds<WW>::BYTES;                                                                          \
-    if (sh > 64 * 1024) {                                                                                         \
-      e = hipFuncSetAttribute((const void*)k_amplitude_tails<WW, XBV>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                              (int)sh);                                                                           \
-      if (e != hipSuccess) return e;                                                                              \
-    }                                                                                                             \
-    k_amplitude_tails<WW, XBV><<<dim3((unsigned)total_blocks), dim3(AMP_NT), sh, st>>>(jobs, njobs, tabs);        \
-  }
-  switch (W) {
-    case 1: NX_AMPT_LAUNCH(1, 4) break;
-    case 2: NX_AMPT_LAUNCH(2, 4) break;
-    case 3: NX_AMPT_LAUNCH(3, 2) break;
-    case 4: NX_AMPT_LAUNCH(4, 2) break;
-    default: return hipErrorInvalidValue;
-  }
-#undef NX_AMPT_LAUNCH
-  e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  k_tail_leaves_jobs<<<dim3((unsigned)((total_strings + 255) / 256)), dim3(256), 0, st>>>(jobs, njobs,
-                                                                                       total_strings);
-  return hipGetLastError();
-}
-
-hipError_t launch_tail_strings(const uint64_t* base, const int32_t* qs, int d, int W, int64_t nv, uint64_t* out,
-                               hipStream_t st) {
-  k_tail_strings<<<dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st>>>(base, qs, d, W, nv, out);
-  return hipGetLastError();
-}
-
-hipError_t launch_tail_leaves(const double* al, int64_t nv, double* ab, hipStream_t st) {
-  k_tail_leaves<<<dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st>>>(al, nv, ab);
-  return hipGetLastError();
-}
-
-// exact components: sum of the chunks in chunk order into chunk 0's slots
-__global__ void k_amp_reduce_exact(double* __restrict__ xout, int64_t nb, int nchunks) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb * 4) return;
-  double a = xout[i];
-  for (int c = 1; c < nchunks; ++c) a = a + xout[(size_t)c * nb * 4 + i];
-  xout[i] = a;
+                                                   double gre, double gim) {
+  amp_block<W, XB>(q, coef, bits, nb, nchunks, tabs, out, per_term, enumerate, final_g, gre, gim, blockIdx.x,
+                   blockIdx.y);
 }
 
 // sum of the per-chunk partial amplitudes in chunk order, then * global_factor
@@ -1302,14 +1081,14 @@ hipError_t launch_qform(const FormsDev& in, const QFormDev& q, hipStream_t st) {
   return hipGetLastError();
 }
 
-// resident blocks of k_amplitude<W, XB, EX> on the device (occupancy x CUs),
+// resident blocks of k_amplitude<W, XB> on the device (occupancy x CUs),
 // queried once per instance
-template <int W, int XB, bool EX>
+template <int W, int XB>
 static int amp_resident() {
   static std::atomic<int> cached{0};
   int v = cached.load(std::memory_order_relaxed);
   if (v > 0) return v;
-  const void* f = (const void*)k_amplitude<W, XB, EX>;
+  const void* f = (const void*)k_amplitude<W, XB>;
   const size_t sh = AmpLds<W>::BYTES;
   if (sh > 64 * 1024) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
   int dev = 0, cus = 0, per = 0;
@@ -1325,10 +1104,10 @@ static int amp_resident() {
 constexpr int AMP_ROUNDS = 2;      // rounds of resident blocks a chunked launch fills
 constexpr int AMP_MIN_TERMS = 8;   // fewest terms per chunk
 
-int amp_chunks(int W, int exact, int64_t nb, int64_t chi) {
+int amp_chunks(int W, int64_t nb, int64_t chi) {
   if (nb <= 0 || chi <= 1) return 1;
   int cap = 1;
-#define NX_AMP_CAP(WW, XBV) cap = exact ? amp_resident<WW, XBV, true>() : amp_resident<WW, XBV, false>();
+#define NX_AMP_CAP(WW, XBV) cap = amp_resident<WW, XBV>();
   switch (W) {
     case 1: NX_AMP_CAP(1, 4) break;
     case 2: NX_AMP_CAP(2, 4) break;
@@ -1357,12 +1136,12 @@ hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const u64* b
   {                                                                                                               \
     const size_t sh = AmpLds<WW>::BYTES;                                                                          \
     if (sh > 64 * 1024) {                                                                                         \
-      hipError_t e = hipFuncSetAttribute((const void*)k_amplitude<WW, XBV, false>,                               \
+      hipError_t e = hipFuncSetAttribute((const void*)k_amplitude<WW, XBV>,                                      \
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);                    \
       if (e != hipSuccess) return e;                                                                              \
     }                                                                                                             \
-    k_amplitude<WW, XBV, false><<<grid, dim3(AMP_NT), sh, st>>>(q, coef, bits, nb, nchunks, tabs, dst, per_term,  \
-                                                                 enumerate, final_g, gre, gim, nullptr, nullptr); \
+    k_amplitude<WW, XBV><<<grid, dim3(AMP_NT), sh, st>>>(q, coef, bits, nb, nchunks, tabs, dst, per_term,         \
+                                                         enumerate, final_g, gre, gim);                           \
   }
   switch (q.W) {
     case 1: NX_AMP_LAUNCH(1, 4) break;
@@ -1372,36 +1151,6 @@ hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const u64* b
     default: return hipErrorInvalidValue;
   }
 #undef NX_AMP_LAUNCH
-  return hipGetLastError();
-}
-
-hipError_t launch_amplitude_exact(const QFormDev& q, const int32_t* cs, const u64* bits, int64_t nb, int nchunks,
-                                  const double* tabs, double* xout, hipStream_t st) {
-  if (nb == 0) return hipSuccess;
-  const int xb = amp_strings_per_thread(q.W);
-  const dim3 grid(blocks_for(nb, AMP_NT * xb), nchunks);
-#define NX_AMPX_LAUNCH(WW, XBV)                                                                                  \
-  {                                                                                                               \
-    const size_t sh = AmpLds<WW>::BYTES;                                                                          \
-    if (sh > 64 * 1024) {                                                                                         \
-      hipError_t e = hipFuncSetAttribute((const void*)k_amplitude<WW, XBV, true>,                                \
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);                    \
-      if (e != hipSuccess) return e;                                                                              \
-    }                                                                                                             \
-    k_amplitude<WW, XBV, true><<<grid, dim3(AMP_NT), sh, st>>>(q, nullptr, bits, nb, nchunks, tabs, nullptr,       \
-                                                                nullptr, 0, 0, 0.0, 0.0, cs, xout);               \
-  }
-  switch (q.W) {
-    case 1: NX_AMPX_LAUNCH(1, 4) break;
-    case 2: NX_AMPX_LAUNCH(2, 4) break;
-    case 3: NX_AMPX_LAUNCH(3, 2) break;
-    case 4: NX_AMPX_LAUNCH(4, 2) break;
-    default: return hipErrorInvalidValue;
-  }
-#undef NX_AMPX_LAUNCH
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess || nchunks <= 1) return e;
-  k_amp_reduce_exact<<<dim3(blocks_for(nb * 4, 256)), dim3(256), 0, st>>>(xout, nb, nchunks);
   return hipGetLastError();
 }
 

@@ -1690,42 +1690,59 @@ static int uniform_prefix(const nx_state* s, const std::vector<int>& order) {
 
 // Leaf norms of a sampling tail (Scalar coefficients): for the 2^d
 // completions of the path `base` on qubits qs, |psi(x)|^2 / |g|^2 = A + sqrt2 B
-// from exact amplitudes (components in the basis 1, zeta, zeta^2, zeta^3).
-// Device buffers and the page-locked result block are kept per state size.
+// from exact amplitudes (components in the basis 1, zeta, zeta^2, zeta^3):
+// every term restricted to the d tail variables (tails.hip).  Device buffers
+// and the page-locked staging / result block are kept per state size.
 struct TailBufs {
-  int64_t nv = 0;
-  DevBuf strings, al, ab, meta;
+  int64_t nv = 0, chi = 0;
+  DevBuf recs, al, ab, meta;
   PinnedBuf host;
 };
 
+// job fields of one tail (pointers into the caller's buffers); d <= 16
+static TailJobDev tail_job(const nx_state* s, int d) {
+  if (d < 1 || d > 16) fail(NX_E_INTERNAL, "sampling tail of %d qubits", d);
+  if (words_for(s->n) > 2) fail(NX_E_INTERNAL, "sampling tails need at most 128 qubits");
+  TailJobDev t{};
+  t.q = qform_of(s).desc();
+  t.cs = coef_of(s).cs;
+  t.d = d;
+  t.nv = (int64_t)1 << d;
+  t.nchunks = tail_chunks(s->chi());
+  return t;
+}
+
 static void tail_leaves(const nx_state* s, TailBufs& tb, const std::vector<uint64_t>& base,
                         const std::vector<int32_t>& qs, std::vector<double>& A, std::vector<double>& B) {
-  const QForms& q = qform_of(s);
-  const CoefDev c = coef_of(s);
-  const int d = (int)qs.size(), W = q.W;
-  const int64_t nv = (int64_t)1 << d, chi = s->chi();
-  const int nchunks = amp_chunks(W, 1, nv, chi);
-  if (tb.nv < nv) {
-    tb.nv = nv;
-    tb.strings.alloc((size_t)nv * W * 8);
-    tb.al.alloc((size_t)64 * nv * 4 * 8);
-    tb.ab.alloc((size_t)nv * 2 * 8);
-    tb.meta.alloc(64 * 4 + 32);
-    tb.host.alloc(std::max<size_t>((size_t)nv * 2 * 8, 512));
+  const int d = (int)qs.size();
+  TailJobDev t = tail_job(s, d);
+  const int64_t nv = t.nv, chi = s->chi();
+  if (tb.nv < nv || tb.chi < chi) {
+    tb.nv = std::max(tb.nv, nv);
+    tb.chi = std::max(tb.chi, chi);
+    tb.recs.alloc((size_t)tb.chi * TAIL_REC * 4);
+    tb.al.alloc((size_t)64 * tb.nv * 4 * 8);
+    tb.ab.alloc((size_t)tb.nv * 2 * 8);
+    tb.meta.alloc(512 + sizeof(TailJobDev));
+    tb.host.alloc(std::max<size_t>((size_t)tb.nv * 2 * 8, 512 + sizeof(TailJobDev)));
   }
-  // base words then qubit list through one staged upload
-  std::memcpy(tb.host.p, base.data(), (size_t)W * 8);
-  std::memcpy(tb.host.as<char>(32), qs.data(), (size_t)d * 4);
-  h2d(tb.meta.p, tb.host.p, 32 + (size_t)d * 4);
-  const uint64_t* d_base = tb.meta.as<uint64_t>();
-  const int32_t* d_qs = (const int32_t*)((const char*)tb.meta.p + 32);
+  // [job | base words | qubit list] through one staged upload
+  const size_t ob = 256 * ((sizeof(TailJobDev) + 255) / 256), oq = ob + 32;
+  std::memset(tb.host.as<char>(ob), 0, 32);
+  std::memcpy(tb.host.as<char>(ob), base.data(), base.size() * 8);
+  std::memcpy(tb.host.as<char>(oq), qs.data(), (size_t)d * 4);
+  t.base = (const uint64_t*)(tb.meta.as<char>() + ob);
+  t.qs = (const int32_t*)(tb.meta.as<char>() + oq);
+  t.recs = tb.recs.as<uint32_t>();
+  t.al = tb.al.as<double>();
+  t.ab = tb.ab.as<double>();
+  t.term0 = t.block0 = t.str0 = 0;
+  std::memcpy(tb.host.p, &t, sizeof t);
+  h2d(tb.meta.p, tb.host.p, oq + (size_t)d * 4);
   {
-    ProfScope ps("k_amplitude_exact");
-    hip_check(launch_tail_strings(d_base, d_qs, d, W, nv, tb.strings.as<uint64_t>(), stream()), "k_tail_strings");
-    hip_check(launch_amplitude_exact(q.desc(), c.cs, tb.strings.as<uint64_t>(), nv, nchunks, ctx()->tabs,
-                                     tb.al.as<double>(), stream()),
-              "k_amplitude_exact");
-    hip_check(launch_tail_leaves(tb.al.as<double>(), nv, tb.ab.as<double>(), stream()), "k_tail_leaves");
+    ProfScope ps("k_tail");
+    const int64_t blocks = ((nv + 1023) / 1024) * t.nchunks;
+    hip_check(launch_tails(t.q.W, tb.meta.as<TailJobDev>(), 1, chi, blocks, nv, stream()), "launch_tails");
     ps.end();
   }
   d2h(tb.host.p, tb.ab.p, (size_t)nv * 16);
@@ -2007,18 +2024,11 @@ static void batch_round(SampleBatch& b, std::vector<SampleBatch::Req*>& reqs) {
   };
   std::vector<TailAt> tat(tl.size());
   std::map<int, std::vector<TailJobDev>> tgroups;
-  size_t nstr_words = 0, nal = 0, nab = 0;
+  size_t nrec = 0, nal = 0, nab = 0;
   for (size_t i = 0; i < tl.size(); ++i) {
     const SampleCtx& cx = *tl[i]->cx;
-    const QForms& q = qform_of(cx.s);
-    const int d = (int)cx.q.size() - tl[i]->root, W = q.W;
-    const int64_t nv = (int64_t)1 << d;
-    TailJobDev t{};
-    t.q = q.desc();
-    t.cs = coef_of(cx.s).cs;
-    t.d = d;
-    t.nv = nv;
-    t.nchunks = amp_chunks(W, 1, nv, cx.s->chi());
+    const int d = (int)cx.q.size() - tl[i]->root;
+    TailJobDev t = tail_job(cx.s, d);
     const size_t ob = take(32 + 4 * (size_t)d);
     uint64_t* base = b.stage_h.as<uint64_t>(ob);
     for (int w = 0; w < 4; ++w) base[w] = 0ull;
@@ -2027,20 +2037,20 @@ static void batch_round(SampleBatch& b, std::vector<SampleBatch::Req*>& reqs) {
     std::memcpy(b.stage_h.as<char>(ob + 32), cx.q.data() + tl[i]->root, 4 * (size_t)d);
     t.base = (const uint64_t*)(b.stage_d.as<char>() + ob);
     t.qs = (const int32_t*)(b.stage_d.as<char>() + ob + 32);
-    // device offsets for now (words / doubles), pointers once allocated
-    t.strings = (uint64_t*)(uintptr_t)nstr_words;
+    // offsets into the round's buffers for now, pointers once allocated
+    t.recs = (uint32_t*)(uintptr_t)nrec;
     t.al = (double*)(uintptr_t)nal;
     t.ab = (double*)(uintptr_t)nab;
-    tat[i] = {nv, nab};
-    nstr_words += (size_t)nv * W;
-    nal += (size_t)t.nchunks * nv * 4;
-    nab += 2 * (size_t)nv;
-    tgroups[W].push_back(t);
+    tat[i] = {t.nv, nab};
+    nrec += (size_t)cx.s->chi() * TAIL_REC;
+    nal += (size_t)t.nchunks * t.nv * 4;
+    nab += 2 * (size_t)t.nv;
+    tgroups[t.q.W].push_back(t);
   }
-  DevBuf tstr, tal, tab;
-  std::vector<std::tuple<int, size_t, int, int64_t, int64_t>> tlaunch;  // W, table offset, jobs, strings, blocks
+  DevBuf trec, tal, tab;
+  std::vector<std::tuple<int, size_t, int, int64_t, int64_t, int64_t>> tlaunch;  // W, table, jobs, terms, blocks, strings
   if (!tl.empty()) {
-    tstr.alloc(nstr_words * 8);
+    trec.alloc(nrec * 4);
     tal.alloc(nal * 8);
     tab.alloc(nab * 8);
     if (b.tab_cap < nab) {
@@ -2048,20 +2058,21 @@ static void batch_round(SampleBatch& b, std::vector<SampleBatch::Req*>& reqs) {
       b.tab_cap = nab;
     }
     for (auto& g : tgroups) {
-      int64_t str0 = 0, blocks = 0;
-      const int64_t per = 256 * (int64_t)amp_strings_per_thread(g.first);
+      int64_t term0 = 0, str0 = 0, blocks = 0;
       for (TailJobDev& t : g.second) {
-        t.strings = tstr.as<uint64_t>() + (uintptr_t)t.strings;
+        t.recs = trec.as<uint32_t>() + (uintptr_t)t.recs;
         t.al = tal.as<double>() + (uintptr_t)t.al;
         t.ab = tab.as<double>() + (uintptr_t)t.ab;
+        t.term0 = term0;
         t.str0 = str0;
         t.block0 = blocks;
+        term0 += t.q.chi;
         str0 += t.nv;
-        blocks += ((t.nv + per - 1) / per) * t.nchunks;
+        blocks += ((t.nv + 1023) / 1024) * t.nchunks;
       }
       const size_t o = take(g.second.size() * sizeof(TailJobDev));
       std::memcpy(b.stage_h.as<char>(o), g.second.data(), g.second.size() * sizeof(TailJobDev));
-      tlaunch.emplace_back(g.first, o, (int)g.second.size(), str0, blocks);
+      tlaunch.emplace_back(g.first, o, (int)g.second.size(), term0, blocks, str0);
     }
   }
   // one upload of the staged block, then the launches
@@ -2076,10 +2087,10 @@ static void batch_round(SampleBatch& b, std::vector<SampleBatch::Req*>& reqs) {
     ps.end();
   }
   if (!tl.empty()) {
-    ProfScope ps("k_amplitude_exact");
+    ProfScope ps("k_tail");
     for (auto& L : tlaunch)
       hip_check(launch_tails(std::get<0>(L), (const TailJobDev*)(b.stage_d.as<char>() + std::get<1>(L)),
-                             std::get<2>(L), std::get<3>(L), std::get<4>(L), ctx()->tabs, stream()),
+                             std::get<2>(L), std::get<3>(L), std::get<4>(L), std::get<5>(L), stream()),
                 "launch_tails");
     ps.end();
     d2h(b.tab_h.p, tab.p, nab * 8);
@@ -2386,7 +2397,7 @@ static void amplitudes_enqueue(const nx_state* s, int64_t nbits, const uint64_t*
   CoefDev c = coef_of(s);
   // the term loop is split into chunks when the strings alone give too
   // few blocks to fill the GPU
-  const int nchunks = amp_chunks(q.W, 0, nbits, s->chi());
+  const int nchunks = amp_chunks(q.W, nbits, s->chi());
   DevBuf partial(nchunks > 1 ? (size_t)nchunks * nbits * 16 : 0);
   ProfScope ps("k_amplitude");
   hip_check(launch_amplitude(q.desc(), c.c, d_bits, nbits, nchunks, ctx()->tabs, (double2*)d_out,
@@ -2686,7 +2697,7 @@ int nx_batch_sample_exact(int32_t nstates, const nx_state* const* states, const 
     for (std::thread& t : th) t.join();
     if (dcode != NX_OK) fail(dcode, "%s", dmsg.c_str());
     for (int32_t i = 0; i < nstates; ++i)
-      if (codes[i] != NX_OK) fail(codes[i], "%s", msgs[i].c_str());
+      if (codes[i] != NX_OK) fail(codes[i], "state %d: %s", i, msgs[i].c_str());
     int64_t total = 0;
     for (int32_t i = 0; i < nstates; ++i) {
       nout[i] = (int64_t)cxs[i]->counts.size();
@@ -2711,6 +2722,31 @@ int nx_batch_sample_exact(int32_t nstates, const nx_state* const* states, const 
       }
       o += k;
     }
+  })
+}
+
+int nx_state_completion_norms(const nx_state* s, const uint64_t* base, int32_t d, const int32_t* qs, double* A,
+                              double* B) {
+  NX_TRY({
+    if (s->is_complex) fail(NX_E_VALUE, "completion norms need Scalar coefficients");
+    if (s->n > 128) fail(NX_E_VALUE, "completion norms are limited to 128 qubits");
+    if (d < 1 || d > 16) fail(NX_E_VALUE, "between 1 and 16 completion qubits are supported, got %d", d);
+    std::vector<int32_t> q(qs, qs + d);
+    validate_qargs(s, d, qs);
+    const int W = words_for(s->n);
+    std::vector<uint64_t> b(base, base + W);
+    for (int i = 0; i < d; ++i)
+      if ((b[q[i] >> 6] >> (q[i] & 63)) & 1ull) fail(NX_E_VALUE, "completion qubit %d is set in the base", q[i]);
+    if (s->chi() == 0) {
+      std::fill(A, A + ((int64_t)1 << d), 0.0);
+      std::fill(B, B + ((int64_t)1 << d), 0.0);
+      return NX_OK;
+    }
+    TailBufs tb;
+    std::vector<double> va, vb;
+    tail_leaves(s, tb, b, q, va, vb);
+    std::copy(va.begin(), va.end(), A);
+    std::copy(vb.begin(), vb.end(), B);
   })
 }
 

@@ -62,6 +62,7 @@ SIGNATURES = {
     "nx_batch_shape": (C.c_int, [_P, _I32P, _I64P, _I32P]),
     "nx_batch_run": (C.c_int, [_P, _DP]),
     "nx_batch_free": (None, [_P]),
+    "nx_state_completion_norms": (C.c_int, [_P, _U64P, _I32, _I32P, _DP, _DP]),
     "nx_batch_sample_exact": (C.c_int, [_I32, C.POINTER(_P), _I32P, _I32P, _U64P, _U8P, _I64, _U64P, _U64P,
                                         _I64P, _I64P]),
 }

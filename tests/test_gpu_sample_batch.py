@@ -69,6 +69,24 @@ def test_batch_equals_single_calls(nx):
     assert nx.sample_many(states, qargs, shots, seeds) == got
 
 
+def test_batch_many_small_tails(nx):
+    """Many states of 3..9 qubits (every qubit sampled): their exact tails
+    have fewer than 64 completions, so one launch's wavefronts straddle
+    several tails; with larger states beside them."""
+    rng = np.random.default_rng(8404)
+    states, qargs, shots, seeds = [], [], [], []
+    for i in range(40):
+        n = int(rng.integers(3, 10)) if i % 8 else int(rng.integers(20, 70))
+        gates = WL.random_circuit(rng, n, 10 * n, int(rng.integers(0, 9)), gate_set=tuple(range(11)))
+        states.append(_state(nx, n, gates))
+        qargs.append(list(range(n)))
+        shots.append(int(rng.integers(1, 400)))
+        seeds.append(6100 + i)
+    got = nx.sample_many(states, qargs, shots, seeds)
+    for i, st in enumerate(states):
+        assert got[i] == st.sample(qargs[i], shots[i], seed=seeds[i], exact=True), i
+
+
 def test_batch_clifford_states_equal_oracle(nx):
     """Clifford states: the batched outcomes equal the oracle's seeded
     outcomes (the reference's FP64 sums are exact on dyadic probabilities)."""
