@@ -236,7 +236,7 @@ def cfg2_compare(ncirc, steps, log):
     }
 
 
-def cfg2_sampling(chimax, shots, log):
+def cfg2_sampling(chimax, shots, log, plain=True):
     """cfg2 sampling on the circuits with at most `chimax` terms: `shots`
     Z-basis samples on all qubits per circuit (exact mode), all circuits in
     one necstar.sample_many call against one QuantumState.sample call per
@@ -259,17 +259,19 @@ def cfg2_sampling(chimax, shots, log):
     t0 = time.perf_counter()
     got = necstar.sample_many(sts, qargs, [shots] * len(sts), seeds, stats=stats)
     batched_s = time.perf_counter() - t0
-    log(f"cfg2 sampling: batched {batched_s:.2f} s ({stats['rounds']} rounds); plain leg next")
-    t0 = time.perf_counter()
-    same = True
-    for k, st in enumerate(sts):
-        same = same and st.sample(qargs[k], shots, seed=seeds[k], exact=True) == got[k]
-    plain_s = time.perf_counter() - t0
+    log(f"cfg2 sampling: batched {batched_s:.2f} s ({stats['rounds']} rounds)")
+    same, plain_s = None, None
+    if plain:
+        t0 = time.perf_counter()
+        same = True
+        for k, st in enumerate(sts):
+            same = same and st.sample(qargs[k], shots, seed=seeds[k], exact=True) == got[k]
+        plain_s = time.perf_counter() - t0
     return {"workload": f"cfg2 circuits with chi <= {chimax} ({len(sts)} of 256, n 16-128), {shots} Z-basis "
                         "samples on all qubits each, exact mode",
             "circuits": len(sts), "chi_max": int(chis.max()), "chi_sum": int(chis.sum()), "shots": shots,
-            "batched_s": batched_s, "plain_s": plain_s, "batched_vs_plain": plain_s / batched_s,
-            "identical_outcomes": bool(same), "rounds": stats["rounds"], "window_passes": stats["windows"],
+            "batched_s": batched_s, "plain_s": plain_s,
+            "batched_vs_plain": plain_s / batched_s if plain_s is not None else None, "identical_outcomes": same, "rounds": stats["rounds"], "window_passes": stats["windows"],
             "exact_tails": stats["tails"], "round_s": stats["round_s"], "gpu_wait_s": stats["gpu_wait_s"],
             "dfs_wait_s": stats["wait_s"]}
 
@@ -291,6 +293,7 @@ def main():
     ap.add_argument("--cfg2-sample-chimax", type=int, default=200,
                     help="cfg2 sampling leg: the circuits with at most this many terms (0: skip the leg)")
     ap.add_argument("--cfg2-sample-shots", type=int, default=1024, help="cfg2 sampling leg: shots per circuit")
+    ap.add_argument("--cfg2-sample-plain", type=int, default=1, help="cfg2 sampling leg: also time the plain calls")
     ap.add_argument("--sample-shots", type=int, default=1024,
                     help="also time QuantumState.sample over all 64 qubits with this many shots (cfg1; 0: skip)")
     ap.add_argument("--cfg4-shots", type=int, default=0,
@@ -329,9 +332,10 @@ def main():
     if args.config == 3:
         return bench_cfg3(args)
     if args.config == 2:
-        res = cfg2_compare(args.circuits or 256, args.cfg2_steps, log)
+        res = cfg2_compare(args.circuits, args.cfg2_steps, log) if args.circuits > 0 else {}
         if args.cfg2_sample_chimax > 0:
-            res["sampling"] = cfg2_sampling(args.cfg2_sample_chimax, args.cfg2_sample_shots, log)
+            res["sampling"] = cfg2_sampling(args.cfg2_sample_chimax, args.cfg2_sample_shots, log,
+                                            plain=args.cfg2_sample_plain)
         print(json.dumps(res), flush=True)
         return
     if args.config == 4:

@@ -177,8 +177,14 @@ struct TailJobDev {
   int32_t d, nchunks;
   int64_t nv, term0, block0, str0;
 };
-// term chunks of a tail over chi terms (at least 256 terms per chunk)
-inline int tail_chunks(int64_t chi) { return (int)std::max<int64_t>(1, std::min<int64_t>(64, chi / 256)); }
+// term chunks of a tail of nv completions over chi terms: enough blocks
+// (completion columns of 1024 x chunks) for about 512 per tail, at least 16
+// terms per chunk, at most 64 chunks
+inline int tail_chunks(int64_t chi, int64_t nv) {
+  const int64_t cols = (nv + 1023) / 1024;
+  const int64_t want = (512 + cols - 1) / cols;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(want, chi / 16), 64));
+}
 hipError_t launch_tails(int W, const TailJobDev* jobs, int njobs, int64_t total_terms, int64_t total_blocks,
                         int64_t total_strings, hipStream_t st);
 // Pair sums over units of J = 8 consecutive other indices o of a fixed

@@ -1708,7 +1708,7 @@ static TailJobDev tail_job(const nx_state* s, int d) {
   t.cs = coef_of(s).cs;
   t.d = d;
   t.nv = (int64_t)1 << d;
-  t.nchunks = tail_chunks(s->chi());
+  t.nchunks = tail_chunks(s->chi(), t.nv);
   return t;
 }
 

@@ -232,23 +232,18 @@ __global__ __launch_bounds__(256) void k_tail_restrict(const TailJobDev* __restr
   }
 }
 
-// exact zeta^K 2^{-R/2} into the four components (as add_zeta in kernels.hip)
-__device__ __forceinline__ void tail_add(int K, int R, double (&al)[4]) {
-  int i1, i2;
-  double v1, v2;
-  if (!(R & 1)) {
-    i1 = K & 3;
-    v1 = ldexp((K & 4) ? -1.0 : 1.0, -(R >> 1));
-    i2 = 0;
-    v2 = 0.0;
-  } else {
-    const int e = -((R + 1) >> 1);
-    const int ka = (K + 1) & 7, kb = (K + 7) & 7;
-    i1 = ka & 3;
-    v1 = ldexp((ka & 4) ? -1.0 : 1.0, e);
-    i2 = kb & 3;
-    v2 = ldexp((kb & 4) ? -1.0 : 1.0, e);
-  }
+// Exact zeta^K 2^{-R/2} into the four components of the basis 1, zeta,
+// zeta^2, zeta^3 (zeta^{j+4} = -zeta^j).  For even R it is +-2^{-R/2} on
+// component K mod 4; for odd R, sqrt2 = zeta + zeta^-1 splits it into
+// 2^{-(R+1)/2} on zeta^{K+1} and zeta^{K-1}.  mag = 2^{-R/2} (even R) or
+// 2^{-(R+1)/2} (odd R), the same for every completion of a term.  Each
+// addend is a power of two, so the components are exact sums while their
+// exponent span fits a double.
+__device__ __forceinline__ void tail_add(int K, bool odd, double mag, double (&al)[4]) {
+  const int ka = odd ? ((K + 1) & 7) : K, kb = (K + 7) & 7;
+  const double v1 = (ka & 4) ? -mag : mag;
+  const double v2 = odd ? ((kb & 4) ? -mag : mag) : 0.0;
+  const int i1 = ka & 3, i2 = kb & 3;
 #pragma unroll
   for (int j = 0; j < 4; ++j) al[j] = al[j] + ((j == i1) ? v1 : 0.0) + ((j == i2) ? v2 : 0.0);
 }
@@ -296,13 +291,15 @@ __global__ __launch_bounds__(256) void k_tail_eval(const TailJobDev* __restrict_
       const u32 w0 = r[0];
       const u32 alo = w0 & 0xFFFFu, ahi = w0 >> 16, target = w1 >> 16;
       const int p0 = (int)(w1 & 7u), Rr = (int)r[TR_K] + cs_r(cs), Kc = cs_k(cs);
+      const bool odd = Rr & 1;
+      const double mag = ldexp(1.0, -((Rr + (odd ? 1 : 0)) >> 1));
 #pragma unroll
       for (int x = 0; x < TAIL_XB; ++x) {
         const u32 acc = xor3(r[off[x][0]], r[off[x][1]], r[off[x][2]]) ^ r[off[x][3]];
         const bool supp = (acc >> 16) == target;
         const int ph = (p0 + 2 * __builtin_popcount(y[x] & alo) +
                         4 * (__builtin_popcount(y[x] & ((ahi ^ acc) & 0xFFFFu)) & 1)) & 7;
-        if (supp && active[x]) tail_add((ph + Kc) & 7, Rr, al[x]);
+        if (supp && active[x]) tail_add((ph + Kc) & 7, odd, mag, al[x]);
       }
     }
   }
