@@ -27,6 +27,8 @@ The same run also measures, by default:
   * cfg2 sampling, batched vs plain: the cfg2 circuits with at most 200
     terms, 1024 samples on all qubits each (exact mode), in one
     necstar.sample_many call against one sample() call per circuit;
+  * cfg3: the coefficient backward against its forward (bench_cfg3) and
+    cfg4: the n=128 amplitude sweep over 2^16 strings (bench_cfg4);
   * the CPU baseline: the oracle (a C restatement of the reference
     algorithm) on a bounded sample of the cfg1 units, on one core and on an
     8-process pool.
@@ -298,6 +300,8 @@ def main():
                     help="also time QuantumState.sample over all 64 qubits with this many shots (cfg1; 0: skip)")
     ap.add_argument("--cfg4-shots", type=int, default=0,
                     help="with --config 4: also sample all 128 qubits with this many shots (exact mode)")
+    ap.add_argument("--other-configs", type=int, default=1,
+                    help="with --config 1: also run the cfg3 backward/forward and cfg4 amplitude-sweep legs")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 1 GiB device copy (PMC calibration)")
     args = ap.parse_args()
@@ -330,7 +334,8 @@ def main():
     L = _lib.lib()
     _lib.check(L.nx_init(0))
     if args.config == 3:
-        return bench_cfg3(args)
+        print(json.dumps(bench_cfg3(args.steps, args.warmup)), flush=True)
+        return
     if args.config == 2:
         res = cfg2_compare(args.circuits, args.cfg2_steps, log) if args.circuits > 0 else {}
         if args.cfg2_sample_chimax > 0:
@@ -339,7 +344,8 @@ def main():
         print(json.dumps(res), flush=True)
         return
     if args.config == 4:
-        return bench_cfg4(args)
+        print(json.dumps(bench_cfg4(args.steps, args.warmup, args.cfg4_shots, args.copy_probe)), flush=True)
+        return
     cfg = WL.config(1)
     n = cfg["n"]
     t0 = time.perf_counter()
@@ -466,6 +472,12 @@ def main():
     cfg2 = cfg2_compare(args.circuits, args.cfg2_steps, log) if args.circuits > 0 else None
     cfg2_samp = (cfg2_sampling(args.cfg2_sample_chimax, args.cfg2_sample_shots, log)
                  if args.cfg2_sample_chimax > 0 else None)
+    cfg3 = cfg4 = None
+    if args.other_configs:
+        log("cfg3: coefficient backward vs forward")
+        cfg3 = bench_cfg3(5, 1)
+        log("cfg4: n=128 amplitude sweep")
+        cfg4 = bench_cfg4(3, 1)
 
     ms_step = (t1 - t0) * 1e3 / args.steps
     amp_units = chi * B
@@ -516,13 +528,13 @@ def main():
         "ms_per_step_async": ms_step_async,
         "norm": norm.value, "sum_abs2_amplitudes": float(np.sum(np.abs(amp) ** 2)), "compile_s": compile_s,
         "batched_vs_plain": cfg2["batched_vs_plain"] if cfg2 else None, "cfg2": cfg2,
-        "cfg2_sampling": cfg2_samp,
+        "cfg2_sampling": cfg2_samp, "cfg3": cfg3, "cfg4": cfg4,
         "sampling": sampling, "roofline": roofline, "roofline_other": other, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 
 
-def bench_cfg4(args):
+def bench_cfg4(steps, warmup, shots=0, copy_probe=False):
     """Large amplitude sweep: n=128 (two words per row), 6000 Clifford + 40 T
     gates, <x|psi> for 2^16 random 128-bit strings with inputs and outputs
     resident in HBM.  One step is one pass over all chi x 2^16 (term, string)
@@ -548,28 +560,28 @@ def bench_cfg4(args):
     torch.cuda.synchronize()
     bits_p = C.cast(d_bits.data_ptr(), C.POINTER(C.c_uint64))
     out_p = C.cast(d_out.data_ptr(), C.POINTER(C.c_double))
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         _lib.check(L.nx_state_amplitudes_device(st._h, B, bits_p, 2, out_p))
     torch.cuda.synchronize()
     _lib.check(L.nx_profile_enable(1))
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         _lib.check(L.nx_state_amplitudes_device(st._h, B, bits_p, 2, out_p))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms, cnt = C.c_double(), C.c_int64()
     _lib.check(L.nx_profile_get(b"k_amplitude", C.byref(ms), C.byref(cnt)))
     _lib.check(L.nx_profile_enable(0))
-    step = (t1 - t0) / args.steps
+    step = (t1 - t0) / steps
     evals = float(chi) * B
     kms = ms.value / max(cnt.value, 1)
     amp = d_out.view(torch.complex128).cpu().numpy()
-    if args.copy_probe:  # a 1 GiB device copy: calibrates the HBM byte counters
+    if copy_probe:  # a 1 GiB device copy: calibrates the HBM byte counters
         src = torch.empty(1 << 27, dtype=torch.float64, device=dev)
         torch.empty_like(src).copy_(src)
         torch.cuda.synchronize()
     sampling = "not run (--cfg4-shots 0)"
-    if args.cfg4_shots > 0:
+    if shots > 0:
         # a stated reduced-shot run of the 128-qubit sampling (exact mode: the
         # reference-order records would need 124 GB); per-window progress on stderr
         os.environ["NX_SAMPLE_PROGRESS"] = "1"
@@ -577,29 +589,29 @@ def bench_cfg4(args):
         p0 = C.c_double(), C.c_int64()
         _lib.check(L.nx_profile_get(b"k_pairs_var", C.byref(p0[0]), C.byref(p0[1])))
         ts = time.perf_counter()
-        counts = st.sample(list(range(n)), args.cfg4_shots, seed=1004, exact=True)
+        counts = st.sample(list(range(n)), shots, seed=1004, exact=True)
         tsamp = time.perf_counter() - ts
         p1 = C.c_double(), C.c_int64()
         _lib.check(L.nx_profile_get(b"k_pairs_var", C.byref(p1[0]), C.byref(p1[1])))
         _lib.check(L.nx_profile_enable(0))
         del os.environ["NX_SAMPLE_PROGRESS"]
         keys = [k[::-1] for k in counts]
-        sampling = {"shots": args.cfg4_shots, "qubits": n, "mode": "exact", "seconds": tsamp,
+        sampling = {"shots": shots, "qubits": n, "mode": "exact", "seconds": tsamp,
                     "distinct_outcomes": len(counts), "window_passes": p1[1].value - p0[1].value,
                     "window_kernel_ms": p1[0].value - p0[0].value,
                     "uniform_levels": st.uniform_prefix(list(range(n - 1, -1, -1))),
                     "dfs_nodes": sum(len({k[:d] for k in keys}) for d in range(n))}
-    print(json.dumps({
+    return {
         "metric": "cfg4 CH-form amplitude evals/s, n=128", "value": evals / step, "unit": "amplitude evals/s",
-        "higher_is_better": True, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "higher_is_better": True, "n_gpus": 1, "steps": steps, "warmup": warmup,
         "ms_per_step": step * 1e3, "k_amplitude_ms": kms, "kernel_evals_per_s": evals / (kms / 1e3),
         "compile_s": compile_s, "sum_abs2": float(np.sum(np.abs(amp) ** 2)),
         "config": {"workload": "cfg4: n=128, 6000 Clifford + 40 T, 2^16 strings of 128 bits", "chi": chi,
                    "bitstrings": B, "sampling": sampling},
-    }), flush=True)
+    }
 
 
-def bench_cfg3(args):
+def bench_cfg3(steps, warmup):
     """Differentiation workload: n=48, 1500 Clifford + 30 T, 128 Paulis of
     weight 2-6 with N(0,1) weights, coefficients perturbed by N(0, 1e-3).
     Forward: L = sum_k w_k exp_value(P_k) (exp_values: one pass over the
@@ -617,17 +629,17 @@ def bench_cfg3(args):
         return "".join("IXYZ"[x] for x in reversed(p))
 
     def measure(st, paulis, w):
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             st.exp_values(paulis)
             st.grad_exp_values(paulis, w)
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             ev = st.exp_values(paulis)
         t1 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             loss, grad = st.grad_exp_values(paulis, w)
         t2 = time.perf_counter()
-        fwd, bwd = (t1 - t0) / args.steps, (t2 - t1) / args.steps
+        fwd, bwd = (t1 - t0) / steps, (t2 - t1) / steps
         chi = st.stabilizer_rank
         return {"chi": chi, "forward_s": fwd, "backward_s": bwd, "backward_over_forward": bwd / fwd,
                 "pairs_per_pass": chi * (chi + 1) // 2, "loss": float(np.dot(w, ev)), "loss_bwd": loss,
@@ -650,13 +662,13 @@ def bench_cfg3(args):
     good = [p for p, e in zip(cands, ev) if abs(e) > 1e-3]
     ops = (good + WL.random_paulis(rng, n, 128, 2, 6))[:128]
     nondeg = measure(st2, [PauliString.from_str(ptxt(p)) for p in ops], rng.standard_normal(128))
-    print(json.dumps({
+    return {
         "metric": "coefficient backward / forward cost, cfg3", "value": nondeg["backward_over_forward"],
-        "unit": "ratio", "higher_is_better": False, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "unit": "ratio", "higher_is_better": False, "n_gpus": 1, "steps": steps, "warmup": warmup,
         "nondegenerate": nondeg, "cfg3": real,
         "config": {"workload": "cfg3 shape: n=48, 30 T, 128 Paulis; value on the non-degenerate circuit "
                                "(150 Cliffords), cfg3 itself (1500 Cliffords) beside it"},
-    }), flush=True)
+    }
 
 
 if __name__ == "__main__":
