@@ -236,6 +236,40 @@ __device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int*
       const int add = (ap == 1) ? 3 : 1;
       P.mag -= 1;
       P.c0 = (P.c0 + (ap == 1 ? 1 : 7)) & 7;
+      if constexpr (W == 1) {
+        // Further odd variables q with no coupling to p nor to each other are
+        // summed out in the same step: no such sum changes another's row or
+        // coefficient (B_pq = 0), so the updates commute, every row and
+        // coefficient they read is the one before the step, and they apply at
+        // once (up to eight variables per step).
+        u64 cand = uni64(__ballot(P.a[0] & 1)) & nz[0] & ~Bp[0] & ~(1ull << pb);
+        if (cand) {
+          const u64 B0 = P.B[0][0];
+          const int a0 = P.a[0];
+          const u32 mbp = bitmask(B0, pb);
+          u64 Bn = xor_and(B0, Bp[0], mbp), clr = 1ull << pb;
+          int an = a0 + (int)((u32)add & mbp);
+#pragma unroll
+          for (int k = 0; k < 7; ++k) {
+            if (!cand) break;
+            const int q = ffs64(cand);
+            const u64 Bq = rl64(B0, q);
+            const int aq = rl32(a0, q);
+            const u32 mbq = bitmask(B0, q);
+            P.mag -= 1;
+            P.c0 = (P.c0 + (aq == 1 ? 1 : 7)) & 7;
+            an += (int)((u32)((aq == 1) ? 3 : 1) & mbq);
+            Bn = xor_and(Bn, Bq, mbq);
+            clr |= 1ull << q;
+            cand &= ~Bq & ~(1ull << q);
+          }
+          P.rem[0] &= ~clr;
+          P.a[0] = an & 3;
+          P.B[0][0] = Bn & P.nodiag & ~clr;
+          uniform_state<W>(P);
+          continue;
+        }
+      }
 #pragma unroll
       for (int sl = 0; sl < W; ++sl) {
         // mb = all ones where B_pl (bit p of this lane's row)
