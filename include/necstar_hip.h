@@ -289,7 +289,7 @@ int nx_batch_sample_exact(int32_t nstates, const nx_state *const *states, const 
                           uint64_t *out_counts, int64_t *nout, int64_t *stats);
 
 /* Kernel timing with HIP events recorded around each launch on the stream
- * the library enqueues on.  kernel: "k_amplitude", "k_amp_reduce",
+ * the library enqueues on.  kernel: "k_amplitude", "k_amp_tables", "k_amp_reduce",
  * "k_tail", "k_pairs", "k_reduce", "k_pairs_var", "k_pairs_sym",
  * "k_grad_var", "k_qform", "k_qproject_var", "k_fold", "k_program",
  * "batch_pairs" (a whole nx_batch_run launch plan).  Enabling clears the totals.  No

@@ -150,9 +150,14 @@ inline int amp_strings_per_thread(int W) { return W <= 2 ? 4 : 2; }
 int amp_chunks(int W, int64_t nb, int64_t chi);
 // chunk c covers terms [c chi / nchunks, (c + 1) chi / nchunks); nchunks > 1:
 // partial must hold nchunks * nb values
-hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const uint64_t* bits, int64_t nb, int nchunks,
-                            const double* tabs, double2* out, double2* partial, uint8_t* per_term, int enumerate,
-                            double gre, double gim, hipStream_t st);
+// per-term nibble tables of the amplitude kernel: amp_table_bytes(W) bytes
+// per term, written for all chi terms of q by launch_amp_tables and read by
+// launch_amplitude
+size_t amp_table_bytes(int W);
+hipError_t launch_amp_tables(const QFormDev& q, void* tab, hipStream_t st);
+hipError_t launch_amplitude(const QFormDev& q, const void* tab, const double2* coef, const uint64_t* bits, int64_t nb,
+                            int nchunks, const double* tabs, double2* out, double2* partial, uint8_t* per_term,
+                            int enumerate, double gre, double gim, hipStream_t st);
 hipError_t launch_amp_reduce(const double2* partial, int64_t nb, int nchunks, double2* out, double gre, double gim,
                              hipStream_t st);
 // Sampling tails (tails.hip): job j holds the completions x = base | y of

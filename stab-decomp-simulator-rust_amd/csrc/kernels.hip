@@ -377,8 +377,10 @@ __global__ __launch_bounds__(256) void k_qform(FormsDev in, QFormDev q) {
 // position never conflict.  The nibble offsets of a thread's strings do not
 // change over the term loop: for W = 1 they are computed once and the two
 // table buffers are addressed with immediate offsets (the term loop is
-// unrolled by two).  The block stages the next term's rows while it
-// evaluates the current one.
+// unrolled by two).  The tables of every term are built once per state by
+// k_amp_tables into device memory; an evaluating block copies the next
+// term's tables into its other LDS buffer while it evaluates the current
+// one (one barrier per term).
 // FP64: per term the table P[ph] = Scalar(ph, k).to_complex() * coefficient
 // (the reference's to_statevector arithmetic: (cos, sin) * 2^{-k/2}, then the
 // complex product) is formed once; P[8] = 0 is selected outside the support.
@@ -415,14 +417,26 @@ struct AmpLds {
   static constexpr int NB = 16 * W;  // nibble positions
   static constexpr int NE = NB * 16; // entries of one table
   static constexpr int NCD = AMP_NCD;
-  // staging of one term (words): Q rows, CT rows ([64W][W] each), C rows [NCD][W]
+  // staging of one term for the table build (words): Q rows, CT rows
+  // ([64W][W] each), C rows [NCD][W]
   static constexpr int R_CT = 64 * W * W, R_C = 2 * 64 * W * W, RAW = R_C + NCD * W;
-  // one table buffer (bytes): TQ (packed), TC [NE][W] words, C rows [NCD][W]
-  // words, P[0..8] (double2)
-  static constexpr int B_CT = (QLay<W>::BYTES + 15) & ~15, B_C = B_CT + NE * W * 8, B_PH = B_C + NCD * W * 8,
-                       BUF = B_PH + 9 * 16;
   static constexpr int NPRE = (RAW + AMP_NT - 1) / AMP_NT;
-  static constexpr size_t BYTES = (size_t)RAW * 8 + 2 * (size_t)BUF;
+  // one term's tables (bytes; the same image in device memory and in an LDS
+  // buffer): TQ (packed), C rows [NCD][W] words, P[0..8] (double2, formed in
+  // LDS from the coefficient), TC [NE][W] words (terms with more than NCD
+  // constraints only)
+  static constexpr int B_C = (QLay<W>::BYTES + 15) & ~15, B_PH = B_C + NCD * W * 8, B_CT = B_PH + 9 * 16,
+                       BUF = B_CT + NE * W * 8;
+  // 16-byte pieces a copy moves: V_HEAD before P (every term), then V_CT of
+  // TC from piece V_CT0 on; at most NPF per thread, the first NPF_REG of
+  // them held in registers while the current term is evaluated
+  static constexpr int V_HEAD = B_PH / 16, V_CT = NE * W * 8 / 16, V_CT0 = B_CT / 16;
+  static constexpr int NPF = (V_HEAD + V_CT + AMP_NT - 1) / AMP_NT;
+  static constexpr int NPF_REG = NPF < 4 ? NPF : 4;
+  static constexpr size_t BYTES = 2 * (size_t)BUF;         // evaluation: two buffers
+  static constexpr size_t BUILD_BYTES = (size_t)RAW * 8 + BUF;  // table build: staging + one buffer
+  // piece e of a copy -> its 16-byte index in the table image
+  static __device__ __forceinline__ int piece(int e) { return e < V_HEAD ? e : e - V_HEAD + V_CT0; }
 };
 
 template <int W>
@@ -449,10 +463,9 @@ __device__ __forceinline__ void amp_store_raw(u64* raw, const u64 (&pre)[AmpLds<
   }
 }
 
-// tables of one term from its staged rows; c and nrm = 2^{-k/2} of the term
+// tables of one term from its staged rows (all but P)
 template <int W>
-__device__ __forceinline__ void amp_build(const u64* raw, char* buf, bool bigc, double2 c, double nrm,
-                                          const double* s_cos, const double* s_sin) {
+__device__ __forceinline__ void amp_build(const u64* raw, char* buf, bool bigc) {
   using L = AmpLds<W>;
   using Y = QLay<W>;
 #pragma unroll
@@ -485,14 +498,47 @@ __device__ __forceinline__ void amp_build(const u64* raw, char* buf, bool bigc, 
     }
   }
   if (threadIdx.x < L::NCD * W) ((u64*)(buf + L::B_C))[threadIdx.x] = raw[L::R_C + threadIdx.x];
+}
+
+// P[ph] = (cos, sin)(ph pi / 4) * nrm * c for ph < 8, P[8] = 0; c and nrm =
+// 2^{-k/2} of the term (threads 0..8)
+__device__ __forceinline__ void amp_phase_table(char* buf_ph, double2 c, double nrm, const double* __restrict__ tabs) {
   if (threadIdx.x < 9) {
     double tr = 0.0, ti = 0.0;
     if (threadIdx.x < 8) {
-      const double zr = s_cos[threadIdx.x] * nrm, zi = s_sin[threadIdx.x] * nrm;
+      const double zr = tabs[TAB_COS + threadIdx.x] * nrm, zi = tabs[TAB_SIN + threadIdx.x] * nrm;
       tr = zr * c.x - zi * c.y;
       ti = zr * c.y + zi * c.x;
     }
-    ((double2*)(buf + L::B_PH))[threadIdx.x] = make_double2(tr, ti);
+    ((double2*)buf_ph)[threadIdx.x] = make_double2(tr, ti);
+  }
+}
+
+// the tables of every term (one block per term, grid-strided) into tab:
+// chi images of AmpLds<W>::BUF bytes (P and, for terms with at most NCD
+// constraints, TC left unwritten)
+template <int W>
+__global__ __launch_bounds__(AMP_NT) void k_amp_tables(QFormDev q, uint4* __restrict__ tab) {
+  using L = AmpLds<W>;
+  extern __shared__ __align__(16) u64 amp_lds[];
+  u64* raw = amp_lds;
+  char* buf = (char*)(amp_lds + L::RAW);
+  for (int64_t t = blockIdx.x; t < q.chi; t += gridDim.x) {
+    u64 pre[L::NPRE];
+    amp_load_raw<W>(q, t, pre);
+    amp_store_raw<W>(raw, pre);
+    const bool bigc = uni(q.nc[t]) > L::NCD;
+    __syncthreads();
+    amp_build<W>(raw, buf, bigc);
+    __syncthreads();
+    const int ncp = L::V_HEAD + (bigc ? L::V_CT : 0);
+    uint4* dst = tab + (size_t)t * (L::BUF / 16);
+#pragma unroll
+    for (int i = 0; i < L::NPF; ++i) {
+      const int e = threadIdx.x + AMP_NT * i;
+      if (e < ncp) dst[L::piece(e)] = ((const uint4*)buf)[L::piece(e)];
+    }
+    __syncthreads();
   }
 }
 
@@ -515,6 +561,10 @@ __device__ __forceinline__ void ld_entry(const char* p, uint32_t (&e)[2 * 4]) {
   }
 }
 
+// 16-byte copies of the amplitude tables (a plain vector type, kept in
+// registers across the term evaluation)
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 template <int V>
 struct IConst {
   static constexpr int value = V;
@@ -522,7 +572,8 @@ struct IConst {
 
 // one block of an amplitude launch: string column bx, term chunk by
 template <int W, int XB>
-__device__ __forceinline__ void amp_block(const QFormDev& q, const double2* __restrict__ coef,
+__device__ __forceinline__ void amp_block(const QFormDev& q, const uint4* __restrict__ tab,
+                                          const double2* __restrict__ coef,
                                           const u64* __restrict__ bits, int64_t nb, int64_t nchunks,
                                           const double* __restrict__ tabs, double2* __restrict__ out,
                                           uint8_t* __restrict__ per_term, int enumerate, int final_g, double gre,
@@ -532,13 +583,7 @@ __device__ __forceinline__ void amp_block(const QFormDev& q, const double2* __re
   constexpr int NB = L::NB;
   constexpr bool PREADDR = (W == 1);
   extern __shared__ __align__(16) u64 amp_lds[];
-  u64* raw = amp_lds;
-  char* bufs = (char*)(amp_lds + L::RAW);  // two table buffers of L::BUF bytes
-  __shared__ double s_cos[8], s_sin[8];
-  if (threadIdx.x < 8) {
-    s_cos[threadIdx.x] = tabs[TAB_COS + threadIdx.x];
-    s_sin[threadIdx.x] = tabs[TAB_SIN + threadIdx.x];
-  }
+  char* bufs = (char*)amp_lds;  // two table buffers of L::BUF bytes
   const int64_t t0 = by * q.chi / nchunks;
   const int64_t t1 = (by + 1) * q.chi / nchunks;
   int64_t bidx[XB];
@@ -574,14 +619,38 @@ __device__ __forceinline__ void amp_block(const QFormDev& q, const double2* __re
   double are[XB], aim[XB];
 #pragma unroll
   for (int j = 0; j < XB; ++j) { are[j] = 0.0; aim[j] = 0.0; }
-  u64 pre[L::NPRE];
+  // table pieces of term t: fetched into registers (pieces past ncp load
+  // piece 0 and are not stored), then stored into an LDS buffer, the
+  // pieces past NPF_REG straight from device memory
+  auto tab_fetch = [&](int64_t t, int ncp, u32x4(&pf)[L::NPF_REG]) {
+    const u32x4* src = (const u32x4*)tab + (size_t)t * (L::BUF / 16);
+#pragma unroll
+    for (int i = 0; i < L::NPF_REG; ++i) {
+      const int e = threadIdx.x + AMP_NT * i;
+      pf[i] = src[e < ncp ? L::piece(e) : 0];
+    }
+  };
+  auto tab_put = [&](char* buf, int64_t t, int ncp, const u32x4(&pf)[L::NPF_REG]) {
+    const u32x4* src = (const u32x4*)tab + (size_t)t * (L::BUF / 16);
+    u32x4* dst = (u32x4*)buf;
+#pragma unroll
+    for (int i = 0; i < L::NPF_REG; ++i) {
+      const int e = threadIdx.x + AMP_NT * i;
+      if (e < ncp) dst[L::piece(e)] = pf[i];
+    }
+#pragma unroll
+    for (int i = L::NPF_REG; i < L::NPF; ++i) {
+      const int e = threadIdx.x + AMP_NT * i;
+      if (e < ncp) dst[L::piece(e)] = src[L::piece(e)];
+    }
+  };
+  auto tab_pieces = [&](int nc) { return L::V_HEAD + (nc > L::NCD ? L::V_CT : 0); };
   if (t0 < t1) {
-    amp_load_raw<W>(q, t0, pre);
-    amp_store_raw<W>(raw, pre);
-  }
-  __syncthreads();
-  if (t0 < t1) {
-    amp_build<W>(raw, bufs, uni(q.nc[t0]) > L::NCD, coef[t0], tabs[TAB_POW + uni(q.k[t0])], s_cos, s_sin);
+    const int ncp0 = tab_pieces(uni(q.nc[t0]));
+    u32x4 pf0[L::NPF_REG];
+    tab_fetch(t0, ncp0, pf0);
+    tab_put(bufs, t0, ncp0, pf0);
+    amp_phase_table(bufs + L::B_PH, coef[t0], tabs[TAB_POW + uni(q.k[t0])], tabs);
   }
   __syncthreads();
 
@@ -662,16 +731,19 @@ __device__ __forceinline__ void amp_block(const QFormDev& q, const double2* __re
   TermU cur;
   if (t0 < t1) load_u(t0, cur);
 
-  // evaluate term t from buffer BI, stage term t + 1 into the other buffer
+  // evaluate term t from buffer BI, copy term t + 1 into the other buffer
   auto run = [&](auto bic, int64_t t) {
     constexpr int BI = decltype(bic)::value;
     const bool more = t + 1 < t1;
     double2 cn = make_double2(0.0, 0.0);
     TermU nxt = cur;
+    u32x4 pf[L::NPF_REG];
+    int ncp = 0;
     if (more) {  // in flight during the evaluation below
-      amp_load_raw<W>(q, t + 1, pre);
-      cn = coef[t + 1];
       load_u(t + 1, nxt);
+      cn = coef[t + 1];
+      ncp = tab_pieces(nxt.nc);
+      tab_fetch(t + 1, ncp, pf);
     }
     const char* buf = bufs + BI * L::BUF;
     const int p0 = cur.p0, nc = cur.nc;
@@ -730,9 +802,11 @@ __device__ __forceinline__ void amp_block(const QFormDev& q, const double2* __re
       are[j] = are[j] + d.x;
       aim[j] = aim[j] + d.y;
     }
-    if (more) amp_store_raw<W>(raw, pre);
-    __syncthreads();
-    if (more) amp_build<W>(raw, bufs + (1 - BI) * L::BUF, nxt.nc > L::NCD, cn, tabs[TAB_POW + nxt.k], s_cos, s_sin);
+    if (more) {
+      char* nbuf = bufs + (1 - BI) * L::BUF;
+      tab_put(nbuf, t + 1, ncp, pf);
+      amp_phase_table(nbuf + L::B_PH, cn, tabs[TAB_POW + nxt.k], tabs);
+    }
     __syncthreads();
     cur = nxt;
   };
@@ -753,12 +827,13 @@ __device__ __forceinline__ void amp_block(const QFormDev& q, const double2* __re
 }
 
 template <int W, int XB>
-__global__ __launch_bounds__(AMP_NT) void k_amplitude(QFormDev q, const double2* __restrict__ coef,
+__global__ __launch_bounds__(AMP_NT) void k_amplitude(QFormDev q, const uint4* __restrict__ tab,
+                                                   const double2* __restrict__ coef,
                                                    const u64* __restrict__ bits, int64_t nb, int64_t nchunks,
                                                    const double* __restrict__ tabs, double2* __restrict__ out,
                                                    uint8_t* __restrict__ per_term, int enumerate, int final_g,
                                                    double gre, double gim) {
-  amp_block<W, XB>(q, coef, bits, nb, nchunks, tabs, out, per_term, enumerate, final_g, gre, gim, blockIdx.x,
+  amp_block<W, XB>(q, tab, coef, bits, nb, nchunks, tabs, out, per_term, enumerate, final_g, gre, gim, blockIdx.x,
                    blockIdx.y);
 }
 
@@ -1124,9 +1199,42 @@ int amp_chunks(int W, int64_t nb, int64_t chi) {
   return 1;
 }
 
-hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const u64* bits, int64_t nb, int nchunks,
-                            const double* tabs, double2* out, double2* partial, uint8_t* per_term, int enumerate,
-                            double gre, double gim, hipStream_t st) {
+size_t amp_table_bytes(int W) {
+  switch (W) {
+    case 1: return AmpLds<1>::BUF;
+    case 2: return AmpLds<2>::BUF;
+    case 3: return AmpLds<3>::BUF;
+    default: return AmpLds<4>::BUF;
+  }
+}
+
+hipError_t launch_amp_tables(const QFormDev& q, void* tab, hipStream_t st) {
+  if (q.chi <= 0) return hipSuccess;
+  const unsigned nblk = (unsigned)std::min<int64_t>(q.chi, 16384);
+#define NX_TAB_LAUNCH(WW)                                                                                       \
+  {                                                                                                              \
+    const size_t sh = AmpLds<WW>::BUILD_BYTES;                                                                   \
+    if (sh > 64 * 1024) {                                                                                        \
+      hipError_t e = hipFuncSetAttribute((const void*)k_amp_tables<WW>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                         (int)sh);                                                               \
+      if (e != hipSuccess) return e;                                                                             \
+    }                                                                                                            \
+    k_amp_tables<WW><<<dim3(nblk), dim3(AMP_NT), sh, st>>>(q, (uint4*)tab);                                      \
+  }
+  switch (q.W) {
+    case 1: NX_TAB_LAUNCH(1) break;
+    case 2: NX_TAB_LAUNCH(2) break;
+    case 3: NX_TAB_LAUNCH(3) break;
+    case 4: NX_TAB_LAUNCH(4) break;
+    default: return hipErrorInvalidValue;
+  }
+#undef NX_TAB_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t launch_amplitude(const QFormDev& q, const void* tab, const double2* coef, const u64* bits, int64_t nb,
+                            int nchunks, const double* tabs, double2* out, double2* partial, uint8_t* per_term,
+                            int enumerate, double gre, double gim, hipStream_t st) {
   if (nb == 0) return hipSuccess;
   const int final_g = nchunks == 1;
   double2* dst = final_g ? out : partial;
@@ -1140,8 +1248,8 @@ hipError_t launch_amplitude(const QFormDev& q, const double2* coef, const u64* b
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);                    \
       if (e != hipSuccess) return e;                                                                              \
     }                                                                                                             \
-    k_amplitude<WW, XBV><<<grid, dim3(AMP_NT), sh, st>>>(q, coef, bits, nb, nchunks, tabs, dst, per_term,         \
-                                                         enumerate, final_g, gre, gim);                           \
+    k_amplitude<WW, XBV><<<grid, dim3(AMP_NT), sh, st>>>(q, (const uint4*)tab, coef, bits, nb, nchunks, tabs, dst, \
+                                                         per_term, enumerate, final_g, gre, gim);                 \
   }
   switch (q.W) {
     case 1: NX_AMP_LAUNCH(1, 4) break;

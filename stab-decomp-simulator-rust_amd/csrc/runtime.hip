@@ -358,6 +358,10 @@ struct Forms {
 
 struct QForms {
   DevBuf Q, B, C, CT, a, alo, ahi, rhs, nc, k, p0, meta;
+  // the amplitude kernel's per-term tables, built on first use after the
+  // forms were (re)computed
+  DevBuf T;
+  bool t_ok = false;
   int64_t chi = 0;
   int n = 0, W = 1;
   // the pair kernels index rows with 32-bit word offsets
@@ -507,6 +511,7 @@ static Forms kron(const Forms& A, const Forms& B) {
 // quadratic forms of f into q (its buffers are reused when the shape matches)
 static void fill_qform(const Forms& f, QForms& q) {
   if (q.chi != f.chi || q.n != f.n || !q.meta.p) q.alloc(f.chi, f.n);
+  q.t_ok = false;
   ProfScope ps("k_qform");
   hip_check(launch_qform(f.desc(), q.desc(), stream()), "k_qform");
   ps.end();
@@ -577,6 +582,20 @@ static const QForms& qform_of(const nx_state* s) {
     s->q_ok = true;
   }
   return s->q;
+}
+
+// the amplitude tables of the state's forms (k_amp_tables once per forms)
+static const void* amp_tables_of(const nx_state* s) {
+  QForms& q = const_cast<QForms&>(qform_of(s));
+  if (!q.t_ok) {
+    const size_t bytes = (size_t)q.chi * amp_table_bytes(q.W);
+    if (!q.T.p || q.T.cap < bytes) q.T.alloc(bytes);
+    ProfScope ps("k_amp_tables");
+    hip_check(launch_amp_tables(q.desc(), q.T.p, stream()), "k_amp_tables");
+    ps.end();
+    q.t_ok = true;
+  }
+  return q.T.p;
 }
 
 static CoefDev coef_of(const nx_state* s) {
@@ -2394,13 +2413,14 @@ static void amplitudes_enqueue(const nx_state* s, int64_t nbits, const uint64_t*
                                uint8_t* d_per_term, int enumerate) {
   if (nbits <= 0) return;  // no strings: no launch (a grid of zero blocks is not a valid launch)
   const QForms& q = qform_of(s);
+  const void* tab = amp_tables_of(s);
   CoefDev c = coef_of(s);
   // the term loop is split into chunks when the strings alone give too
   // few blocks to fill the GPU
   const int nchunks = amp_chunks(q.W, nbits, s->chi());
   DevBuf partial(nchunks > 1 ? (size_t)nchunks * nbits * 16 : 0);
   ProfScope ps("k_amplitude");
-  hip_check(launch_amplitude(q.desc(), c.c, d_bits, nbits, nchunks, ctx()->tabs, (double2*)d_out,
+  hip_check(launch_amplitude(q.desc(), tab, c.c, d_bits, nbits, nchunks, ctx()->tabs, (double2*)d_out,
                              partial.as<double2>(), d_per_term, enumerate, s->g.real(), s->g.imag(), stream()),
             "k_amplitude");
   ps.end();
