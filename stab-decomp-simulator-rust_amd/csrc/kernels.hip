@@ -916,12 +916,38 @@ __device__ __forceinline__ void add_re_exact(int K, int R, int x2, double& A, do
   else B = B + v;
 }
 
+// The same, evaluated in vector registers (every lane alike) and without
+// branches: a term that is off (a zero coefficient) or has cos = 0 adds 0.0,
+// which leaves the exact sums unchanged.
+__device__ __forceinline__ void add_re_exact_v(int K, int R, int x2, bool live, double& A, double& B) {
+  K &= 7;
+  const bool on = live && (K & 3) != 2;
+  const bool neg = K >= 3 && K <= 5;
+  const int kodd = K & 1, rodd = R & 1;
+  const int e = -((R + 1) >> 1) - (kodd & (rodd ^ 1)) + x2;
+  const double v = on ? ldexp(neg ? -1.0 : 1.0, e) : 0.0;
+  const bool toA = kodd == rodd;
+  A = A + (toA ? v : 0.0);
+  B = B + (toA ? 0.0 : v);
+}
+
+// a wave-uniform value copied into a vector register, so that what is
+// computed from it issues on the vector unit (the pair kernel is bound by
+// scalar issue)
+__device__ __forceinline__ int vreg(int x) {
+  int r;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
+  return r;
+}
+
 // SAME: X and Y (and cx and cy) are the same state's forms and coefficients
 // (norm, exp_value): one set of table pointers is kept in scalar registers,
 // which leaves room for 8 waves per SIMD at W = 1.
 // MODE: the mode argument when it is 0..3 at compile time (no per-pair role
-// selects), or -1 to read it at run time.
-template <int W, bool SAME, int MODE>
+// selects), or -1 to read it at run time.  CK: 1 (exact Scalar
+// coefficients) or 0 (complex128) when known at compile time, with no
+// exact_out; -1 to read both at run time.
+template <int W, bool SAME, int MODE, int CK>
 __global__ __launch_bounds__(256, (SAME && W == 1) ? 8 : 1) void k_pairs(QFormDev X, QFormDev Y,
                                                int64_t nfixed, int64_t nother, int triangle, int J, int mode_rt,
                                                CoefDev cx, CoefDev cy, const double* __restrict__ tabs,
@@ -942,7 +968,8 @@ __global__ __launch_bounds__(256, (SAME && W == 1) ? 8 : 1) void k_pairs(QFormDe
   const int mode = MODE >= 0 ? MODE : mode_rt;
   const bool fixed_is_ket = mode & 1;
   const bool herm = (mode >> 1) & 1;
-  const bool exact = cx.scalar != 0;
+  const bool exact = CK >= 0 ? CK == 1 : cx.scalar != 0;
+  if (CK >= 0) exact_out = nullptr;
   // cos, sin of k pi / 4 (the host tables) in LDS, for complex128
   // coefficients: read with a lane address instead of a scalar load
   __shared__ double s_cs[16];
@@ -966,17 +993,29 @@ __global__ __launch_bounds__(256, (SAME && W == 1) ? 8 : 1) void k_pairs(QFormDe
     const int ncf = uni(mf.x), kf = uni(mf.y), pf = uni(mf.z);
     const CoefDev& cf = (SAME || !fixed_is_ket) ? cx : cy;
     const CoefDev& co = (SAME || fixed_is_ket) ? cx : cy;
-    const int32_t csf = cf.scalar ? uni(cf.cs[fx]) : 0;
-    const double2 cvf = cf.scalar ? make_double2(0.0, 0.0) : cf.c[fx];
+    const bool scf = CK >= 0 ? CK == 1 : cf.scalar != 0, sco = CK >= 0 ? CK == 1 : co.scalar != 0;
+    const int32_t csf = scf ? uni(cf.cs[fx]) : 0;
+    const double2 cvf = scf ? make_double2(0.0, 0.0) : cf.c[fx];
+    // the unit's other terms: lane j holds the record, right-hand sides and
+    // packed coefficient of term o_lo + j (one vector load each per unit
+    // instead of scalar loads and address arithmetic per pair)
+    const int jl = lane < o_hi - o_lo ? lane : 0;
+    const int4 mo_l = qo.meta[o_lo + jl];
+    u64 ro_l[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) ro_l[w] = qo.rhs[(uint32_t)(o_lo + jl) * W + w];
+    const int32_t cso_l = sco ? co.cs[o_lo + jl] : 0;
     for (int o = o_lo; o < o_hi; ++o) {
+      const int jo = o - o_lo;
       u64 Bo[W][W], Co[W][W], ro[W];
       int ao[W];
-      load_qrows<W>(qo, o, lane, Bo, ao, Co, ro);
-      const int4 mo = qo.meta[o];
-      const int nco = uni(mo.x), ko = uni(mo.y), po = uni(mo.z);
+      load_qrows<W, false>(qo, o, lane, Bo, ao, Co, ro);
+#pragma unroll
+      for (int w = 0; w < W; ++w) ro[w] = rl64(ro_l[w], jo);
+      const int nco = rl32(mo_l.x, jo), ko = rl32(mo_l.y, jo), po = rl32(mo_l.z, jo);
       // the other term's coefficient, read now and used after the elimination
-      const int32_t cso = co.scalar ? uni(co.cs[o]) : 0;
-      const double2 cvo = co.scalar ? make_double2(0.0, 0.0) : co.c[o];
+      const int32_t cso = sco ? rl32(cso_l, jo) : 0;
+      const double2 cvo = sco ? make_double2(0.0, 0.0) : co.c[o];
       const int bra = fixed_is_ket ? o : fx;
       const int ket = fixed_is_ket ? fx : o;
       PairRegs<W> P;
@@ -1013,16 +1052,17 @@ __global__ __launch_bounds__(256, (SAME && W == 1) ? 8 : 1) void k_pairs(QFormDe
         exact_out[(size_t)bra * exact_ld + ket] = zero ? 0xFFFFFFFFu : ((uint32_t)K | ((uint32_t)Rr << 8));
       if (zero) continue;
       if (exact) {
-        // c_bra* c_ket <bra|ket> = zeta^{K'} 2^{-R'/2}, summed exactly
-        const int32_t sb = fixed_is_ket ? cso : csf, sk = fixed_is_ket ? csf : cso;
-        if ((sb | sk) < 0) continue;  // a Scalar::Zero coefficient
-        const int Kp = K + cs_k(sk) - cs_k(sb);
-        const int Rp = Rr + cs_r(sb) + cs_r(sk);
+        // c_bra* c_ket <bra|ket> = zeta^{K'} 2^{-R'/2}, summed exactly (a
+        // Scalar::Zero coefficient, cs < 0, adds nothing)
+        const int32_t sb = vreg(fixed_is_ket ? cso : csf), sk = vreg(fixed_is_ket ? csf : cso);
+        const bool live = (sb | sk) >= 0;
+        const int Kp = vreg(K) + cs_k(sk) - cs_k(sb);
+        const int Rp = vreg(Rr) + cs_r(sb) + cs_r(sk);
         if (herm) {
-          add_re_exact(Kp, Rp, bra == ket ? 0 : 1, a_re, b_re);
+          add_re_exact_v(Kp, Rp, vreg(bra) != ket ? 1 : 0, live, a_re, b_re);
         } else {
-          add_re_exact(Kp, Rp, 0, a_re, b_re);
-          add_re_exact(Kp - 2, Rp, 0, a_im, b_im);  // Im zeta^K = Re zeta^{K-2}
+          add_re_exact_v(Kp, Rp, 0, live, a_re, b_re);
+          add_re_exact_v(Kp - 2, Rp, 0, live, a_im, b_im);  // Im zeta^K = Re zeta^{K-2}
         }
       } else {
         // (c_bra.conj() * c_ket).into() * <bra|ket> as num-complex forms it
@@ -1278,11 +1318,17 @@ hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, int64_t nfixed, in
   // the combinations the runtime uses: norm (same, fixed bra, Hermitian),
   // pair tables (same, complex), exp_value (fixed ket, Hermitian), inner
   // products (complex); any other mode is read at run time
-#define NX_PAIRS(SM, MD)                                                                                      \
-  NX_DISPATCH_W(X.W, k_pairs<WW, SM, MD><<<dim3(nblocks), dim3(256), 0, st>>>(                                \
+#define NX_PAIRS_CK(SM, MD, CKV)                                                                              \
+  NX_DISPATCH_W(X.W, k_pairs<WW, SM, MD, CKV><<<dim3(nblocks), dim3(256), 0, st>>>(                           \
                          X, SM ? X : Y, nfixed, nother, triangle, J, mode, cx, SM ? cx : cy, tabs, partial,    \
                          exact_out, exact_ld))
-  if (same && mode == 2) {
+#define NX_PAIRS(SM, MD) NX_PAIRS_CK(SM, MD, -1)
+  // the norm: coefficient kind fixed at compile time
+  if (same && mode == 2 && !exact_out && cx.scalar) {
+    NX_PAIRS_CK(true, 2, 1);
+  } else if (same && mode == 2 && !exact_out) {
+    NX_PAIRS_CK(true, 2, 0);
+  } else if (same && mode == 2) {
     NX_PAIRS(true, 2);
   } else if (same && mode == 0) {
     NX_PAIRS(true, 0);
@@ -1296,6 +1342,7 @@ hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, int64_t nfixed, in
     NX_PAIRS(false, -1);
   }
 #undef NX_PAIRS
+#undef NX_PAIRS_CK
   return hipGetLastError();
 }
 

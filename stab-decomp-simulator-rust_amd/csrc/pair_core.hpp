@@ -114,9 +114,10 @@ __device__ __forceinline__ void subst(PairRegs<W>& P, int p, const u64 (&r)[W], 
   const int ap = bcast_int<W>(P.a, p);
   u64 Bp[W];
   bcast<W>(P.B, p, Bp);
-  P.c0 = (P.c0 + 2 * ap * c) & 7;
+  const int t2 = 2 * ap * c;
+  P.c0 = (P.c0 + t2) & 7;
   // a_l += ap (1 - 2c) r_l + 2 (c + r_l) B_pl ;  B_l ^= r_l ((ap odd) r ^ Bp) ^ B_pl r
-  const int lin = c ? (4 - ap) & 3 : ap;
+  const int lin = ap - t2;  // ap (1 - 2c); only its value mod 4 is used
   const int pw = wordof<W>(p), pb = p & 63;
   const u32 cm = c ? ~0u : 0u;
 #pragma unroll
@@ -139,13 +140,13 @@ __device__ __forceinline__ void subst(PairRegs<W>& P, int p, const u64 (&r)[W], 
       const u32 ma = bitmask(wsel<W>(P.ca[sl], pw), pb);
 #pragma unroll
       for (int w = 0; w < W; ++w) P.ca[sl][w] = xor_and(P.ca[sl][w], r[w], ma) & ((w == pw) ? ~(1ull << pb) : ~0ull);
-      if (c) P.rhsa[sl] ^= __ballot(ma != 0u);
+      P.rhsa[sl] ^= __ballot((ma & cm) != 0u);
     }
     if (CONS != CONS_NONE) {
       const u32 mb2 = bitmask(wsel<W>(P.cb[sl], pw), pb);
 #pragma unroll
       for (int w = 0; w < W; ++w) P.cb[sl][w] = xor_and(P.cb[sl][w], r[w], mb2) & ((w == pw) ? ~(1ull << pb) : ~0ull);
-      if (c) P.rhsb[sl] ^= __ballot(mb2 != 0u);
+      P.rhsb[sl] ^= __ballot((mb2 & cm) != 0u);
     }
   }
   clearbit<W>(P.rem, pw, pb);
@@ -233,43 +234,44 @@ __device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int*
     const int ap = bcast_int<W>(P.a, p);
     if (ap & 1) {
       // sum over x_p: sqrt2 zeta^s i^{-s lambda}, lambda = <B_p, x>, s = +-1
-      const int add = (ap == 1) ? 3 : 1;
-      P.mag -= 1;
-      P.c0 = (P.c0 + (ap == 1 ? 1 : 7)) & 7;
       if constexpr (W == 1) {
         // Further odd variables q with no coupling to p nor to each other are
         // summed out in the same step: no such sum changes another's row or
         // coefficient (B_pq = 0), so the updates commute, every row and
         // coefficient they read is the one before the step, and they apply at
-        // once (up to eight variables per step).
+        // once (up to eight variables per step, the set S).  Per variable q:
+        // mag -1, c0 + 1 (a_q = 1) or 7 (a_q = 3); per row l coupled to q:
+        // a_l + 3 (a_q = 1) or 1 (a_q = 3), B_l ^= B_q.
         u64 cand = uni64(__ballot(P.a[0] & 1)) & nz[0] & ~Bp[0] & ~(1ull << pb);
         if (cand) {
           const u64 B0 = P.B[0][0];
           const int a0 = P.a[0];
-          const u32 mbp = bitmask(B0, pb);
-          u64 Bn = xor_and(B0, Bp[0], mbp), clr = 1ull << pb;
-          int an = a0 + (int)((u32)add & mbp);
+          u64 Bn = xor_and(B0, Bp[0], bitmask(B0, pb)), S = 1ull << pb;
 #pragma unroll
           for (int k = 0; k < 7; ++k) {
-            if (!cand) break;
             const int q = ffs64(cand);
             const u64 Bq = rl64(B0, q);
-            const int aq = rl32(a0, q);
-            const u32 mbq = bitmask(B0, q);
-            P.mag -= 1;
-            P.c0 = (P.c0 + (aq == 1 ? 1 : 7)) & 7;
-            an += (int)((u32)((aq == 1) ? 3 : 1) & mbq);
-            Bn = xor_and(Bn, Bq, mbq);
-            clr |= 1ull << q;
-            cand &= ~Bq & ~(1ull << q);
+            Bn = xor_and(Bn, Bq, bitmask(B0, q));
+            const u64 bq = 1ull << q;
+            S |= bq;
+            cand &= ~(Bq | bq);
+            if (!cand) break;
           }
-          P.rem[0] &= ~clr;
-          P.a[0] = an & 3;
-          P.B[0][0] = Bn & P.nodiag & ~clr;
+          const u64 S1 = S & uni64(__ballot(a0 == 1));
+          const int ns = __builtin_popcountll(S), n1 = __builtin_popcountll(S1);
+          P.mag -= ns;
+          P.c0 = (P.c0 + 7 * ns - 6 * n1) & 7;
+          const u64 nb = B0 & S;
+          P.a[0] = (a0 + 2 * __builtin_popcountll(nb & S1) + __builtin_popcountll(nb)) & 3;
+          P.rem[0] &= ~S;
+          P.B[0][0] = Bn & P.nodiag & ~S;
           uniform_state<W>(P);
           continue;
         }
       }
+      const int add = (ap == 1) ? 3 : 1;
+      P.mag -= 1;
+      P.c0 = (P.c0 + (ap == 1 ? 1 : 7)) & 7;
 #pragma unroll
       for (int sl = 0; sl < W; ++sl) {
         // mb = all ones where B_pl (bit p of this lane's row)
@@ -300,8 +302,8 @@ __device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int*
 }
 
 // rows of term t (word offsets fit 32 bits: chi * 64W * W < 2^32 is checked
-// where the forms are allocated)
-template <int W>
+// where the forms are allocated); RHS: also the constraint right-hand sides
+template <int W, bool RHS = true>
 __device__ __forceinline__ void load_qrows(const QFormDev& q, int64_t t, int lane, u64 (&B)[W][W], int (&a)[W],
                                            u64 (&C)[W][W], u64 (&rhs)[W]) {
   const uint32_t R = 64 * W;
@@ -316,8 +318,10 @@ __device__ __forceinline__ void load_qrows(const QFormDev& q, int64_t t, int lan
     }
     a[sl] = q.a[(uint32_t)t * R + r];
   }
+  if constexpr (RHS) {
 #pragma unroll
-  for (int w = 0; w < W; ++w) rhs[w] = uni64(q.rhs[(uint32_t)t * W + w]);
+    for (int w = 0; w < W; ++w) rhs[w] = uni64(q.rhs[(uint32_t)t * W + w]);
+  }
 }
 
 }  // namespace nx
