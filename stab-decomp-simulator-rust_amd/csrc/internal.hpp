@@ -143,7 +143,7 @@ hipError_t launch_kron(const FormsDev& A, const FormsDev& B, const FormsDev& O, 
 hipError_t launch_gather(const FormsDev& in, const FormsDev& O, const int64_t* idx, hipStream_t st);
 hipError_t launch_qform(const FormsDev& in, const QFormDev& q, hipStream_t st);
 // bitstrings per thread of the amplitude kernel
-inline int amp_strings_per_thread(int W) { return W <= 2 ? 4 : 2; }
+inline int amp_strings_per_thread(int W) { return W == 1 ? 3 : (W == 2 ? 4 : 2); }
 // number of term chunks (blockIdx.y) of an amplitude launch over nb strings
 // and chi terms: the launch fills whole rounds of the resident blocks, with
 // at least AMP_MIN_TERMS terms per chunk and at most 64 chunks

@@ -1224,7 +1224,7 @@ int amp_chunks(int W, int64_t nb, int64_t chi) {
   int cap = 1;
 #define NX_AMP_CAP(WW, XBV) cap = amp_resident<WW, XBV>();
   switch (W) {
-    case 1: NX_AMP_CAP(1, 4) break;
+    case 1: NX_AMP_CAP(1, 3) break;
     case 2: NX_AMP_CAP(2, 4) break;
     case 3: NX_AMP_CAP(3, 2) break;
     default: NX_AMP_CAP(4, 2) break;
@@ -1292,7 +1292,7 @@ hipError_t launch_amplitude(const QFormDev& q, const void* tab, const double2* c
                                                          per_term, enumerate, final_g, gre, gim);                 \
   }
   switch (q.W) {
-    case 1: NX_AMP_LAUNCH(1, 4) break;
+    case 1: NX_AMP_LAUNCH(1, 3) break;
     case 2: NX_AMP_LAUNCH(2, 4) break;
     case 3: NX_AMP_LAUNCH(3, 2) break;
     case 4: NX_AMP_LAUNCH(4, 2) break;
