@@ -141,7 +141,9 @@ hipError_t launch_program(const FormsDev& in, const FormsDev& out, const ProgOp*
                           hipStream_t st);
 hipError_t launch_kron(const FormsDev& A, const FormsDev& B, const FormsDev& O, hipStream_t st);
 hipError_t launch_gather(const FormsDev& in, const FormsDev& O, const int64_t* idx, hipStream_t st);
-hipError_t launch_qform(const FormsDev& in, const QFormDev& q, hipStream_t st);
+// tab (one word per row only): also the amplitude tables of every term,
+// amp_table_bytes(1) bytes each
+hipError_t launch_qform(const FormsDev& in, const QFormDev& q, hipStream_t st, void* tab = nullptr);
 // bitstrings per thread of the amplitude kernel
 inline int amp_strings_per_thread(int W) { return W == 1 ? 3 : (W == 2 ? 4 : 2); }
 // number of term chunks (blockIdx.y) of an amplitude launch over nb strings

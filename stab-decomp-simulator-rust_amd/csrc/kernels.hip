@@ -173,6 +173,62 @@ __global__ void k_gather(FormsDev in, FormsDev O, const int64_t* __restrict__ id
 }
 
 // ---------------------------------------------------------------------------
+// Layout of the amplitude kernel's per-term tables (built with the
+// quadratic forms at one word per row, else by k_amp_tables).
+constexpr int AMP_NCD = 8;  // constraints tested row by row
+constexpr int AMP_NT = 256;  // threads per block
+
+// Q-table layout.  Q_l holds only bits m > l, so the entries of nibble
+// position b (variables 4b .. 4b+3) are zero below bit 4b + 1, which lies in
+// dword b >> 3.  An entry keeps the dwords from there on, the first rounded
+// down to an even dword (8-byte aligned entries) except in the last word,
+// whose upper half alone is kept for its last 8 positions.  For W = 1 that
+// is 8 bytes for positions 0..7 and 4 bytes for 8..15; the 16 entries of a
+// position then span 32 or 16 LDS banks once and a wave's lookups at one
+// position never conflict.
+template <int W>
+struct QLay {
+  static constexpr __host__ __device__ int d0(int b) {
+    return ((b >> 3) == 2 * W - 1) ? (b >> 3) : ((b >> 3) & ~1);
+  }
+  static constexpr __host__ __device__ int nd(int b) { return 2 * W - d0(b); }
+  // byte offset of the 16 entries of position b
+  static constexpr __host__ __device__ int off(int b) {
+    int o = 0;
+    for (int g = 0; g < (b >> 3); ++g) o += 8 * 16 * 4 * nd(8 * g);
+    return o + (b & 7) * 16 * 4 * nd(b);
+  }
+  static constexpr int BYTES = off(16 * W);
+};
+
+template <int W>
+struct AmpLds {
+  static constexpr int NB = 16 * W;  // nibble positions
+  static constexpr int NE = NB * 16; // entries of one table
+  static constexpr int NCD = AMP_NCD;
+  // staging of one term for the table build (words): Q rows, CT rows
+  // ([64W][W] each), C rows [NCD][W]
+  static constexpr int R_CT = 64 * W * W, R_C = 2 * 64 * W * W, RAW = R_C + NCD * W;
+  static constexpr int NPRE = (RAW + AMP_NT - 1) / AMP_NT;
+  // one term's tables (bytes; the same image in device memory and in an LDS
+  // buffer): TQ (packed), C rows [NCD][W] words, P[0..8] (double2, formed in
+  // LDS from the coefficient), TC [NE][W] words (terms with more than NCD
+  // constraints only)
+  static constexpr int B_C = (QLay<W>::BYTES + 15) & ~15, B_PH = B_C + NCD * W * 8, B_CT = B_PH + 9 * 16,
+                       BUF = B_CT + NE * W * 8;
+  // 16-byte pieces a copy moves: V_HEAD before P (every term), then V_CT of
+  // TC from piece V_CT0 on; at most NPF per thread, the first NPF_REG of
+  // them held in registers while the current term is evaluated
+  static constexpr int V_HEAD = B_PH / 16, V_CT = NE * W * 8 / 16, V_CT0 = B_CT / 16;
+  static constexpr int NPF = (V_HEAD + V_CT + AMP_NT - 1) / AMP_NT;
+  static constexpr int NPF_REG = NPF < 4 ? NPF : 4;
+  static constexpr size_t BYTES = 2 * (size_t)BUF;         // evaluation: two buffers
+  static constexpr size_t BUILD_BYTES = (size_t)RAW * 8 + BUF;  // table build: staging + one buffer
+  // piece e of a copy -> its 16-byte index in the table image
+  static __device__ __forceinline__ int piece(int e) { return e < V_HEAD ? e : e - V_HEAD + V_CT0; }
+};
+
+// ---------------------------------------------------------------------------
 // Quadratic-form rewrite of each term.  With u_j = (F_j & ~v) ^ (M_j & v),
 // w_j = (M_j & ~v) ^ (F_j & v), c_j = par(F_j & M_j & v), applying X to the
 // set bits of x in ascending order (amplitude.rs:36-43) leaves
@@ -207,8 +263,10 @@ __device__ __forceinline__ u64 transpose64(u64 x, int lane) {
   return x;
 }
 
+// tab != nullptr (W = 1 only): also the term's amplitude tables, the image
+// k_amp_tables writes, at tab + t AmpLds<1>::BUF
 template <int W>
-__global__ __launch_bounds__(256) void k_qform(FormsDev in, QFormDev q) {
+__global__ __launch_bounds__(256) void k_qform(FormsDev in, QFormDev q, char* __restrict__ tab) {
   const int64_t t = (int64_t)blockIdx.x * 4 + uni((int)(threadIdx.x >> 6));  // uniform in the wave
   const int lane = threadIdx.x & 63;
   if (t >= in.chi) return;
@@ -232,12 +290,13 @@ __global__ __launch_bounds__(256) void k_qform(FormsDev in, QFormDev q) {
     a[sl] = (f.g[sl] + 2 * L) & 3;
   }
   u64 Bm[W][W];
+  // W = 1: per-wave rows and nibble tables in LDS
+  __shared__ u64 s_wt[4][64], s_tab[4][16 * 16];
+  const int wv = threadIdx.x >> 6;
   if constexpr (W == 1) {
     // Q_l = (XOR_{k in u_l} WT_k) restricted to m > l, WT = W transposed:
     // WT rows in LDS, combined four at a time into per-wave nibble tables
     // (16 lookups per lane instead of 64 parities)
-    __shared__ u64 s_wt[4][64], s_tab[4][16 * 16];
-    const int wv = threadIdx.x >> 6;
     s_wt[wv][lane] = transpose64(Wm[0][0], lane);  // row k: bit m = bit k of w_m
     wave_lds_sync();
     // s_tab[16 b + v] = XOR of WT_{4b + i} for the set bits i of v
@@ -327,6 +386,42 @@ __global__ __launch_bounds__(256) void k_qform(FormsDev in, QFormDev q) {
       }
     }
   }
+  if constexpr (W == 1) {
+    if (tab) {
+      // the amplitude tables (as amp_build forms them from the stored rows):
+      // nibble tables of the Q rows, of the C^T rows when the term has more
+      // than AMP_NCD constraints, and the first AMP_NCD constraint rows
+      using L = AmpLds<1>;
+      using Y = QLay<1>;
+      wave_lds_sync();
+      s_wt[wv][lane] = Q[0][0];
+      s_tab[wv][lane] = CT[0][0];
+      wave_lds_sync();
+      char* img = tab + (size_t)t * L::BUF;
+      const bool bigc = nc > L::NCD;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int e = lane + 64 * i, b = e >> 4, v = e & 15;
+        u64 qv = 0ull, cv = 0ull;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if ((v >> k) & 1) {
+            qv ^= s_wt[wv][4 * b + k];
+            cv ^= s_tab[wv][4 * b + k];
+          }
+        // positions 0-7 keep both dwords, 8-15 the upper one
+        uint32_t* dq = (uint32_t*)(img + Y::off(b)) + v * Y::nd(b);
+        if (Y::d0(b) == 0) {
+          dq[0] = lo32(qv);
+          dq[1] = hi32(qv);
+        } else {
+          dq[0] = hi32(qv);
+        }
+        if (bigc) ((u64*)(img + L::B_CT))[e] = cv;
+      }
+      if (lane < L::NCD) ((u64*)(img + L::B_C))[lane] = C[0][0];
+    }
+  }
   // store
 #pragma unroll
   for (int sl = 0; sl < W; ++sl) {
@@ -386,58 +481,6 @@ __global__ __launch_bounds__(256) void k_qform(FormsDev in, QFormDev q) {
 // complex product) is formed once; P[8] = 0 is selected outside the support.
 // Each string adds P[ph] in term order within the chunk; chunks are summed
 // in chunk order by k_amp_reduce, which also applies the global factor.
-constexpr int AMP_NCD = 8;  // constraints tested row by row
-constexpr int AMP_NT = 256;  // threads per block
-
-// Q-table layout.  Q_l holds only bits m > l, so the entries of nibble
-// position b (variables 4b .. 4b+3) are zero below bit 4b + 1, which lies in
-// dword b >> 3.  An entry keeps the dwords from there on, the first rounded
-// down to an even dword (8-byte aligned entries) except in the last word,
-// whose upper half alone is kept for its last 8 positions.  For W = 1 that
-// is 8 bytes for positions 0..7 and 4 bytes for 8..15; the 16 entries of a
-// position then span 32 or 16 LDS banks once and a wave's lookups at one
-// position never conflict.
-template <int W>
-struct QLay {
-  static constexpr __host__ __device__ int d0(int b) {
-    return ((b >> 3) == 2 * W - 1) ? (b >> 3) : ((b >> 3) & ~1);
-  }
-  static constexpr __host__ __device__ int nd(int b) { return 2 * W - d0(b); }
-  // byte offset of the 16 entries of position b
-  static constexpr __host__ __device__ int off(int b) {
-    int o = 0;
-    for (int g = 0; g < (b >> 3); ++g) o += 8 * 16 * 4 * nd(8 * g);
-    return o + (b & 7) * 16 * 4 * nd(b);
-  }
-  static constexpr int BYTES = off(16 * W);
-};
-
-template <int W>
-struct AmpLds {
-  static constexpr int NB = 16 * W;  // nibble positions
-  static constexpr int NE = NB * 16; // entries of one table
-  static constexpr int NCD = AMP_NCD;
-  // staging of one term for the table build (words): Q rows, CT rows
-  // ([64W][W] each), C rows [NCD][W]
-  static constexpr int R_CT = 64 * W * W, R_C = 2 * 64 * W * W, RAW = R_C + NCD * W;
-  static constexpr int NPRE = (RAW + AMP_NT - 1) / AMP_NT;
-  // one term's tables (bytes; the same image in device memory and in an LDS
-  // buffer): TQ (packed), C rows [NCD][W] words, P[0..8] (double2, formed in
-  // LDS from the coefficient), TC [NE][W] words (terms with more than NCD
-  // constraints only)
-  static constexpr int B_C = (QLay<W>::BYTES + 15) & ~15, B_PH = B_C + NCD * W * 8, B_CT = B_PH + 9 * 16,
-                       BUF = B_CT + NE * W * 8;
-  // 16-byte pieces a copy moves: V_HEAD before P (every term), then V_CT of
-  // TC from piece V_CT0 on; at most NPF per thread, the first NPF_REG of
-  // them held in registers while the current term is evaluated
-  static constexpr int V_HEAD = B_PH / 16, V_CT = NE * W * 8 / 16, V_CT0 = B_CT / 16;
-  static constexpr int NPF = (V_HEAD + V_CT + AMP_NT - 1) / AMP_NT;
-  static constexpr int NPF_REG = NPF < 4 ? NPF : 4;
-  static constexpr size_t BYTES = 2 * (size_t)BUF;         // evaluation: two buffers
-  static constexpr size_t BUILD_BYTES = (size_t)RAW * 8 + BUF;  // table build: staging + one buffer
-  // piece e of a copy -> its 16-byte index in the table image
-  static __device__ __forceinline__ int piece(int e) { return e < V_HEAD ? e : e - V_HEAD + V_CT0; }
-};
 
 template <int W>
 __device__ __forceinline__ void amp_load_raw(const QFormDev& q, int64_t t, u64 (&pre)[AmpLds<W>::NPRE]) {
@@ -1190,9 +1233,10 @@ hipError_t launch_gather(const FormsDev& in, const FormsDev& O, const int64_t* i
   return hipGetLastError();
 }
 
-hipError_t launch_qform(const FormsDev& in, const QFormDev& q, hipStream_t st) {
+hipError_t launch_qform(const FormsDev& in, const QFormDev& q, hipStream_t st, void* tab) {
+  if (tab && in.W != 1) return hipErrorInvalidValue;
   if (in.chi == 0) return hipSuccess;
-  NX_DISPATCH_W(in.W, k_qform<WW><<<dim3(blocks_for(in.chi, 4)), dim3(256), 0, st>>>(in, q));
+  NX_DISPATCH_W(in.W, k_qform<WW><<<dim3(blocks_for(in.chi, 4)), dim3(256), 0, st>>>(in, q, (char*)tab));
   return hipGetLastError();
 }
 

@@ -508,13 +508,22 @@ static Forms kron(const Forms& A, const Forms& B) {
   return out;
 }
 
-// quadratic forms of f into q (its buffers are reused when the shape matches)
-static void fill_qform(const Forms& f, QForms& q) {
+// quadratic forms of f into q (its buffers are reused when the shape
+// matches); with_tables (one word per row): the amplitude tables as well, in
+// the same launch
+static void fill_qform(const Forms& f, QForms& q, bool with_tables = false) {
   if (q.chi != f.chi || q.n != f.n || !q.meta.p) q.alloc(f.chi, f.n);
   q.t_ok = false;
+  void* tab = nullptr;
+  if (with_tables && q.W == 1) {
+    const size_t bytes = (size_t)q.chi * amp_table_bytes(1);
+    if (!q.T.p || q.T.cap < bytes) q.T.alloc(bytes);
+    tab = q.T.p;
+  }
   ProfScope ps("k_qform");
-  hip_check(launch_qform(f.desc(), q.desc(), stream()), "k_qform");
+  hip_check(launch_qform(f.desc(), q.desc(), stream(), tab), "k_qform");
   ps.end();
+  q.t_ok = tab != nullptr;
 }
 
 static QForms make_qform(const Forms& f) {
@@ -584,9 +593,14 @@ static const QForms& qform_of(const nx_state* s) {
   return s->q;
 }
 
-// the amplitude tables of the state's forms (k_amp_tables once per forms)
+// the amplitude tables of the state's forms: with the forms when they are
+// computed here at one word per row, else k_amp_tables once per forms
 static const void* amp_tables_of(const nx_state* s) {
-  QForms& q = const_cast<QForms&>(qform_of(s));
+  if (!s->q_ok) {
+    fill_qform(s->forms, s->q, true);
+    s->q_ok = true;
+  }
+  QForms& q = s->q;
   if (!q.t_ok) {
     const size_t bytes = (size_t)q.chi * amp_table_bytes(q.W);
     if (!q.T.p || q.T.cap < bytes) q.T.alloc(bytes);
@@ -2412,8 +2426,8 @@ int nx_state_inner_product(const nx_state* a, const nx_state* b, double* re, dou
 static void amplitudes_enqueue(const nx_state* s, int64_t nbits, const uint64_t* d_bits, double* d_out,
                                uint8_t* d_per_term, int enumerate) {
   if (nbits <= 0) return;  // no strings: no launch (a grid of zero blocks is not a valid launch)
-  const QForms& q = qform_of(s);
   const void* tab = amp_tables_of(s);
+  const QForms& q = qform_of(s);
   CoefDev c = coef_of(s);
   // the term loop is split into chunks when the strings alone give too
   // few blocks to fill the GPU
