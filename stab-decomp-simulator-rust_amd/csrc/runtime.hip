@@ -1206,6 +1206,11 @@ struct VarPlan {
   std::vector<Launch> launches;
   int64_t nseg = 0;
   DevBuf jobs, wave_job, row_off, partial, out, seg_job, seg_part;
+  // a plan built into a PlanStage: its tables live there (these pointers,
+  // not the buffers above)
+  const PairJobDev* jobs_p = nullptr;
+  const int32_t *wave_job_p = nullptr, *seg_job_p = nullptr;
+  double *partial_p = nullptr, *seg_part_p = nullptr;
   std::vector<PairJobDev> hjobs;  // host copy of the job table
   // jobs_dev: the job table to read (default: this plan's own)
   // out_override: where the reduction writes instead of out (page-locked
@@ -1213,23 +1218,39 @@ struct VarPlan {
   // rec: the jobs carry pair records (PairJobDev::rec; unit plans, VW = 1)
   void enqueue(hipStream_t st, const PairJobDev* jobs_dev = nullptr, double* out_override = nullptr,
                bool rec = false) const {
-    const PairJobDev* jd = jobs_dev ? jobs_dev : jobs.as<PairJobDev>();
+    const PairJobDev* jd = jobs_dev ? jobs_dev : (jobs_p ? jobs_p : jobs.as<PairJobDev>());
+    const int32_t* wj = wave_job_p ? wave_job_p : wave_job.as<int32_t>();
+    double* part = partial_p ? partial_p : partial.as<double>();
     for (const Launch& L : launches)
-      hip_check(launch_pairs_var(L.W, VW, rec ? 2 : units, jd, wave_job.as<int32_t>(), L.wave_base, L.nwaves, J,
-                                 ctx()->tabs, partial.as<double>(), st),
+      hip_check(launch_pairs_var(L.W, VW, rec ? 2 : units, jd, wj, L.wave_base, L.nwaves, J, ctx()->tabs, part, st),
                 "k_pairs_var");
-    hip_check(launch_reduce_var(jd, njobs, VW, partial.as<double>(), seg_job.as<int32_t>(), nseg,
-                                seg_part.as<double>(), out_override ? out_override : out.as<double>(), st),
+    hip_check(launch_reduce_var(jd, njobs, VW, part, seg_job_p ? seg_job_p : seg_job.as<int32_t>(), nseg,
+                                seg_part_p ? seg_part_p : seg_part.as<double>(),
+                                out_override ? out_override : out.as<double>(), st),
               "k_reduce_var");
   }
+};
+
+// Page-locked staging of a plan's tables and the device block they are
+// uploaded to in one copy, plus the plan's partial-sum buffers; kept across
+// the plans of a sampling batch (grown between rounds, never while a round's
+// launches may still read them)
+struct PlanStage {
+  PinnedBuf h;
+  DevBuf d, partial, seg_part;
+  size_t cap = 0, used = 0, pcap = 0, scap = 0;
+  PairJobDev* jobs_h = nullptr;  // the staged job table (host side)
 };
 
 // Builds the device job table.  Jobs are grouped by word width (one launch
 // each); within a launch the waves are shared out in proportion to each
 // job's work (units x n^2), at least one wave per job and at most one per
 // unit.  row_frac < 1 keeps only that fraction of each job's ket rows (a
-// shortened plan of the same shape, for timing launch shapes).
-static VarPlan make_plan(const std::vector<VarJob>& in, int VW, int J, int64_t target_waves, double row_frac = 1.0) {
+// shortened plan of the same shape, for timing launch shapes).  With ps the
+// tables go into ps (uploaded by the caller: ps->used bytes of ps->h to
+// ps->d) and the plan has no result buffer (enqueue with out_override).
+static VarPlan make_plan(const std::vector<VarJob>& in, int VW, int J, int64_t target_waves, double row_frac = 1.0,
+                         PlanStage* ps = nullptr) {
   VarPlan pl;
   pl.VW = VW;
   pl.J = J;
@@ -1309,10 +1330,48 @@ static VarPlan make_plan(const std::vector<VarJob>& in, int VW, int J, int64_t t
     for (int32_t k = 0; k < hj[i].nseg; ++k) sj.push_back((int32_t)i);
   }
   pl.nseg = (int64_t)sj.size();
+  pl.units = (!in.empty() && in[0].V == nullptr) ? 1 : 0;
+  if (ps) {
+    const auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t need = al(sj.size() * 4) + al(roff.size() * 8) + al(hj.size() * sizeof(PairJobDev)) +
+                        al(wj.size() * 4);
+    if (ps->cap < need) {
+      ps->cap = 2 * need;
+      ps->h.alloc(ps->cap);
+      ps->d.alloc(ps->cap);
+    }
+    size_t o = 0;
+    const auto put = [&](const void* src, size_t bytes) {
+      std::memcpy(ps->h.as<char>(o), src, bytes);
+      char* dp = ps->d.as<char>() + o;
+      o += al(bytes);
+      return dp;
+    };
+    pl.seg_job_p = (const int32_t*)put(sj.data(), sj.size() * 4);
+    const int64_t* roff_d = (const int64_t*)put(roff.data(), roff.size() * 8);
+    for (size_t i = 0; i < in.size(); ++i) hj[i].row_off = roff_d + roff_base[i];
+    ps->jobs_h = ps->h.as<PairJobDev>(o);
+    pl.jobs_p = (const PairJobDev*)put(hj.data(), hj.size() * sizeof(PairJobDev));
+    pl.wave_job_p = (const int32_t*)put(wj.data(), wj.size() * 4);
+    ps->used = o;
+    const size_t pbytes = (size_t)std::max<int64_t>(wave, 1) * 2 * (64 * VW + 1) * 8;
+    const size_t sbytes = (size_t)std::max<int64_t>(pl.nseg, 1) * 2 * (64 * VW + 1) * 8;
+    if (ps->pcap < pbytes) {
+      ps->pcap = 2 * pbytes;
+      ps->partial.alloc(ps->pcap);
+    }
+    if (ps->scap < sbytes) {
+      ps->scap = 2 * sbytes;
+      ps->seg_part.alloc(ps->scap);
+    }
+    pl.partial_p = ps->partial.as<double>();
+    pl.seg_part_p = ps->seg_part.as<double>();
+    pl.hjobs = hj;
+    return pl;
+  }
   pl.seg_job.alloc(std::max<size_t>(sj.size(), 1) * 4);
   h2d(pl.seg_job.p, sj.data(), sj.size() * 4);
   pl.seg_part.alloc(std::max<int64_t>(pl.nseg, 1) * 2 * (64 * VW + 1) * 8);
-  pl.units = (!in.empty() && in[0].V == nullptr) ? 1 : 0;
   pl.row_off.alloc(std::max<size_t>(roff.size(), 1) * 8);
   h2d(pl.row_off.p, roff.data(), roff.size() * 8);
   for (size_t i = 0; i < in.size(); ++i) hj[i].row_off = pl.row_off.as<int64_t>() + roff_base[i];
@@ -1988,6 +2047,11 @@ struct SampleBatch {
   PinnedBuf stage_h, res_h, tab_h;
   DevBuf stage_d;
   size_t stage_bytes = 0, jobs_off = 0, tab_cap = 0;
+  // the round's pair plan tables and the tails' work buffers, kept across
+  // rounds (grown when a round needs more)
+  PlanStage plan_stage;
+  DevBuf trec, tal, tab;
+  size_t trec_cap = 0, tal_cap = 0, tabd_cap = 0;
   int64_t rounds = 0, windows = 0, tails = 0;
   double round_s = 0.0, wait_s = 0.0;  // in batch_round / waiting for the DFS threads
   double sync_s = 0.0;                 // of round_s: waiting for the round's launches
@@ -2080,12 +2144,21 @@ static void batch_round(SampleBatch& b, std::vector<SampleBatch::Req*>& reqs) {
     nab += 2 * (size_t)t.nv;
     tgroups[t.q.W].push_back(t);
   }
-  DevBuf trec, tal, tab;
+  DevBuf &trec = b.trec, &tal = b.tal, &tab = b.tab;
   std::vector<std::tuple<int, size_t, int, int64_t, int64_t, int64_t>> tlaunch;  // W, table, jobs, terms, blocks, strings
   if (!tl.empty()) {
-    trec.alloc(nrec * 4);
-    tal.alloc(nal * 8);
-    tab.alloc(nab * 8);
+    if (b.trec_cap < nrec) {
+      b.trec_cap = 2 * nrec;
+      trec.alloc(b.trec_cap * 4);
+    }
+    if (b.tal_cap < nal) {
+      b.tal_cap = 2 * nal;
+      tal.alloc(b.tal_cap * 8);
+    }
+    if (b.tabd_cap < nab) {
+      b.tabd_cap = 2 * nab;
+      tab.alloc(b.tabd_cap * 8);
+    }
     if (b.tab_cap < nab) {
       b.tab_h.alloc(nab * 8);
       b.tab_cap = nab;
@@ -2108,8 +2181,35 @@ static void batch_round(SampleBatch& b, std::vector<SampleBatch::Req*>& reqs) {
       tlaunch.emplace_back(g.first, o, (int)g.second.size(), term0, blocks, str0);
     }
   }
-  // one upload of the staged block, then the launches
+  // the windows' pair plan, staged (its tables uploaded with one copy)
+  VarPlan plan;
+  if (!win.empty()) {
+    int VW = 1;
+    int64_t target = 0;
+    for (size_t i = 0; i < win.size(); ++i) {
+      VW = std::max(VW, win[i]->cx->pp.VW);
+      target = std::max<int64_t>(target, 3 * ProjPlan::resident_waves(win[i]->cx->pp.projv.W, win[i]->cx->pp.VW));
+    }
+    plan = make_plan(vj, VW, 8, std::max<int64_t>(target, 1024), 1.0, &b.plan_stage);
+    // the plan's jobs carry each window's unit constraints, as a lone pass does
+    for (size_t i = 0; i < win.size(); ++i) {
+      const PairJobDev& src = win[i]->cx->pp.hj[0];
+      PairJobDev& d = b.plan_stage.jobs_h[i];
+      d.X = src.X;
+      d.Y = src.Y;
+      d.V = src.V;
+      d.nvar = src.nvar;
+      d.nunit = src.nunit;
+      d.uq = src.uq;
+      d.ub = src.ub;
+      d.termvar = src.termvar;
+      d.rec = nullptr;
+      for (int w = 0; w < 4; ++w) d.removed[w] = src.removed[w];
+    }
+  }
+  // one upload of the staged block (and of the plan's tables), then the launches
   h2d(b.stage_d.p, b.stage_h.p, jo);
+  if (!win.empty()) h2d(b.plan_stage.d.p, b.plan_stage.h.p, b.plan_stage.used);
   {
     ProfScope ps("k_qproject_var");
     for (auto& L : qlaunch)
@@ -2128,32 +2228,7 @@ static void batch_round(SampleBatch& b, std::vector<SampleBatch::Req*>& reqs) {
     ps.end();
     d2h(b.tab_h.p, tab.p, nab * 8);
   }
-  VarPlan plan;
   if (!win.empty()) {
-    int VW = 1;
-    int64_t target = 0;
-    for (size_t i = 0; i < win.size(); ++i) {
-      VW = std::max(VW, win[i]->cx->pp.VW);
-      target = std::max<int64_t>(target, 3 * ProjPlan::resident_waves(win[i]->cx->pp.projv.W, win[i]->cx->pp.VW));
-    }
-    plan = make_plan(vj, VW, 8, std::max<int64_t>(target, 1024));
-    // the plan's jobs carry each window's unit constraints, as a lone pass does
-    std::vector<PairJobDev>& hj = plan.hjobs;
-    for (size_t i = 0; i < win.size(); ++i) {
-      const PairJobDev& src = win[i]->cx->pp.hj[0];
-      PairJobDev& d = hj[i];
-      d.X = src.X;
-      d.Y = src.Y;
-      d.V = src.V;
-      d.nvar = src.nvar;
-      d.nunit = src.nunit;
-      d.uq = src.uq;
-      d.ub = src.ub;
-      d.termvar = src.termvar;
-      d.rec = nullptr;
-      for (int w = 0; w < 4; ++w) d.removed[w] = src.removed[w];
-    }
-    h2d(plan.jobs.p, hj.data(), win.size() * sizeof(PairJobDev));
     ProfScope ps("k_pairs_var");
     plan.enqueue(stream(), nullptr, b.res_h.as<double>(), false);
     ps.end();
