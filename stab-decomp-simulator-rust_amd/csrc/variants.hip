@@ -149,7 +149,8 @@ __device__ __forceinline__ void subst_v(PairRegs<W>& P, VarRegs<W, VW>& V, int p
   bcast_v<W, VW>(V.hi, p, H);
   u64 Bp[W];
   bcast<W>(P.B, p, Bp);
-  P.c0 = (P.c0 + 2 * ap * c) & 7;
+  const int t2 = 2 * ap * c;
+  P.c0 = (P.c0 + t2) & 7;
   u64 L[VW];
 #pragma unroll
   for (int v = 0; v < VW; ++v) {
@@ -163,8 +164,8 @@ __device__ __forceinline__ void subst_v(PairRegs<W>& P, VarRegs<W, VW>& V, int p
     }
     L[v] = H[v] ^ ((ap & 1) ? E[v] : 0ull);
   }
-  if (PH) V.dv = (V.dv + 8 - 2 * ap * c) & 7;
-  const int lin = c ? (4 - ap) & 3 : ap;
+  if (PH) V.dv = (V.dv + 8 - t2) & 7;
+  const int lin = ap - t2;  // ap (1 - 2c); only its value mod 4 is used
   const int pw = wordof<W>(p), pb = p & 63;
   const u32 cm = c ? ~0u : 0u;
 #pragma unroll
@@ -187,7 +188,7 @@ __device__ __forceinline__ void subst_v(PairRegs<W>& P, VarRegs<W, VW>& V, int p
       const u32 ma = (u32)-ha;
 #pragma unroll
       for (int w = 0; w < W; ++w) P.ca[sl][w] = xor_and(P.ca[sl][w], r[w], ma) & ((w == pw) ? ~(1ull << pb) : ~0ull);
-      if (c) P.rhsa[sl] ^= __ballot(ha);
+      P.rhsa[sl] ^= __ballot(ha & c);
       if (UNITS) {
 #pragma unroll
         for (int v = 0; v < VW; ++v) V.ra[sl][v] = xor_and(V.ra[sl][v], E[v], ma);
@@ -198,7 +199,7 @@ __device__ __forceinline__ void subst_v(PairRegs<W>& P, VarRegs<W, VW>& V, int p
       const u32 mb2 = (u32)-hb;
 #pragma unroll
       for (int w = 0; w < W; ++w) P.cb[sl][w] = xor_and(P.cb[sl][w], r[w], mb2) & ((w == pw) ? ~(1ull << pb) : ~0ull);
-      if (c) P.rhsb[sl] ^= __ballot(hb);
+      P.rhsb[sl] ^= __ballot(hb & c);
       if (CONS == CONS_B || UNITS) {
 #pragma unroll
         for (int v = 0; v < VW; ++v) V.rb[sl][v] = xor_and(V.rb[sl][v], E[v], mb2);
