@@ -126,26 +126,14 @@ __global__ __launch_bounds__(256) void k_fold_diag(const uint64_t* __restrict__ 
   D[(size_t)f * FOLD_NODES + u] = out;
 }
 
-// t is node-major: t[u * ld + q] for pair p0 + q; threads run along the
-// pairs of one node (coalesced stores), a block within one node
-__global__ __launch_bounds__(256) void k_fold_terms(const uint64_t* __restrict__ rec, int64_t chi, int64_t p0,
-                                                    int64_t np, int64_t ld, int d, const uint8_t* __restrict__ D,
-                                                    const int32_t* __restrict__ cs, const double* __restrict__ tabs,
-                                                    double* __restrict__ t, int* __restrict__ err) {
-  const int64_t per = (np + 255) & ~(int64_t)255;  // threads per node
-  const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int u = (int)(gid / per);
-  const int64_t q = gid - (int64_t)u * per;
-  if (q >= np) return;
-  const int64_t p = p0 + q;
+// The term of pair p at node u (0.0 when it vanishes): the node value as the
+// exact sum of its leaves, then the FP64 product as the reference forms it
+__device__ __forceinline__ double fold_term(const uint64_t* __restrict__ rec, int64_t chi, int64_t p, int u,
+                                            uint64_t leaves, const uint8_t* __restrict__ D,
+                                            const int32_t* __restrict__ cs, const double* __restrict__ tabs,
+                                            int* __restrict__ err) {
   const uint64_t* r = rec + (size_t)FOLD_REC * (size_t)p;
-  int j, k;
-  node_of(u, &j, &k);
-  const uint64_t leaves = leaves_of(d, j, k);
-  if ((leaves & ~r[0]) == 0ull) {  // every leaf below the node vanishes
-    t[(size_t)u * ld + q] = 0.0;
-    return;
-  }
+  if ((leaves & ~r[0]) == 0ull) return 0.0;  // every leaf below the node vanishes
   // row f of pair p: the largest f with row_start(f) <= p
   const double b = 2.0 * (double)chi + 1.0;
   int64_t f = (int64_t)((b - sqrt(b * b - 8.0 * (double)p)) * 0.5);
@@ -154,92 +142,107 @@ __global__ __launch_bounds__(256) void k_fold_terms(const uint64_t* __restrict__
   while (f > 0 && row_start(f, chi) > p) --f;
   while (f + 1 < chi && row_start(f + 1, chi) <= p) ++f;
   const int64_t o = f + (p - row_start(f, chi));
-  double val = 0.0;
   const int32_t sf = cs[f], so = cs[o];
-  if ((sf | so) >= 0) {
-    int a[4], zero, K, h;
-    node_components(r, leaves, a);
-    if (!z8_scalar(a, &zero, &K, &h)) {
-      *err = 1;
-    } else if (!zero) {
-      const int Df = D[(size_t)f * FOLD_NODES + u], Do = D[(size_t)o * FOLD_NODES + u];
-      const int mag = (int)(r[4] >> 8);
-      // reference pair (i, j) = (f, o): conj(c_f') c_o' and <phi_f'|phi_o'> =
-      // conj(<phi_o|Pi|phi_f>) 2^((Df + Do)/2)
-      const int kc = (cs_k(so) - cs_k(sf)) & 7;
-      const int rc = cs_r(sf) + cs_r(so) + Df + Do;
-      const int Kr = (8 - K) & 7;
-      const int Rr = mag - h - Df - Do;
-      if (Df == 255 || Do == 255 || rc < -TAB_RMAX || rc > TAB_RMAX || Rr < -TAB_RMAX || Rr > TAB_RMAX) {
-        *err = 1;
-      } else {
-        const double nc = tabs[TAB_POW + rc], ni = tabs[TAB_POW + Rr];
-        const double cr = tabs[TAB_COS + kc] * nc, ci = tabs[TAB_SIN + kc] * nc;
-        const double pr = tabs[TAB_COS + Kr] * ni, pi = tabs[TAB_SIN + Kr] * ni;
-        const double tr = cr * pr - ci * pi;
-        val = (f == o) ? tr : tr + tr;
-      }
-    }
+  if ((sf | so) < 0) return 0.0;
+  int a[4], zero, K, h;
+  node_components(r, leaves, a);
+  if (!z8_scalar(a, &zero, &K, &h)) {
+    *err = 1;
+    return 0.0;
   }
-  t[(size_t)u * ld + q] = val;
+  if (zero) return 0.0;
+  const int Df = D[(size_t)f * FOLD_NODES + u], Do = D[(size_t)o * FOLD_NODES + u];
+  const int mag = (int)(r[4] >> 8);
+  // reference pair (i, j) = (f, o): conj(c_f') c_o' and <phi_f'|phi_o'> =
+  // conj(<phi_o|Pi|phi_f>) 2^((Df + Do)/2)
+  const int kc = (cs_k(so) - cs_k(sf)) & 7;
+  const int rc = cs_r(sf) + cs_r(so) + Df + Do;
+  const int Kr = (8 - K) & 7;
+  const int Rr = mag - h - Df - Do;
+  if (Df == 255 || Do == 255 || rc < -TAB_RMAX || rc > TAB_RMAX || Rr < -TAB_RMAX || Rr > TAB_RMAX) {
+    *err = 1;
+    return 0.0;
+  }
+  const double nc = tabs[TAB_POW + rc], ni = tabs[TAB_POW + Rr];
+  const double cr = tabs[TAB_COS + kc] * nc, ci = tabs[TAB_SIN + kc] * nc;
+  const double pr = tabs[TAB_COS + Kr] * ni, pi = tabs[TAB_SIN + Kr] * ni;
+  const double tr = cr * pr - ci * pi;
+  return (f == o) ? tr : tr + tr;
 }
 
-// One wave per node.  Batches of 512 of the node's terms (pair order) are
-// loaded 64 per instruction; the nonzero ones are packed in order into LDS
-// (ballot + mbcnt) and added one by one from there (every lane reads the
-// same address), while the next batch is in flight.  A zero term leaves the
-// sum unchanged (the sum starts at +0 and is never -0), so skipping it keeps
-// the reference's rounding.
-constexpr int FOLD_CH = 8;  // 64-term loads per batch
-__global__ __launch_bounds__(64) void k_fold_sum(const double* __restrict__ t, int64_t np, int64_t ld,
-                                                 double* __restrict__ sums) {
-  __shared__ double buf[64 * FOLD_CH];
+// Block = 256 consecutive pairs p0 + 256 c .. of one node u.  The nonzero
+// terms are packed in pair order into t[u * ld + 256 c ...] and their count
+// goes to cnt[u * nch + c] (nch = chunks per node): the sum then reads only
+// the nonzero terms.  A zero term is an identity of the sum (it starts at +0
+// and is never -0), so dropping it keeps the reference's rounding.
+__global__ __launch_bounds__(256) void k_fold_terms(const uint64_t* __restrict__ rec, int64_t chi, int64_t p0,
+                                                    int64_t np, int64_t ld, int64_t nch, int d,
+                                                    const uint8_t* __restrict__ D, const int32_t* __restrict__ cs,
+                                                    const double* __restrict__ tabs, double* __restrict__ t,
+                                                    int32_t* __restrict__ cnt, int* __restrict__ err) {
+  const int u = (int)(blockIdx.x / nch);
+  const int64_t c = blockIdx.x - (int64_t)u * nch;
+  const int64_t q = 256 * c + threadIdx.x;
+  int j, k;
+  node_of(u, &j, &k);
+  const double val = q < np ? fold_term(rec, chi, p0 + q, u, leaves_of(d, j, k), D, cs, tabs, err) : 0.0;
+  __shared__ int wc[4];
+  const uint64_t m = __ballot(val != 0.0);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) wc[wv] = __popcll(m);
+  __syncthreads();
+  int off = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) off += w < wv ? wc[w] : 0;
+  const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+  if (val != 0.0) t[(size_t)u * ld + 256 * c + off + below] = val;
+  if (threadIdx.x == 0) cnt[(size_t)u * nch + c] = wc[0] + wc[1] + wc[2] + wc[3];
+}
+
+// One wave per node: the node's packed terms, chunk by chunk in pair order,
+// added one by one from LDS (every lane reads the same address).  A chunk's
+// 256 slots are loaded whole (the adds read only its count of them), the next
+// chunk's slots and count while the current chunk's dependent adds run, and
+// the LDS reads of each group of 16 terms (the last one included) are issued
+// before the adds of the group before it.
+__global__ __launch_bounds__(64) void k_fold_sum(const double* __restrict__ t, const int32_t* __restrict__ cnt,
+                                                 int64_t ld, int64_t nch, double* __restrict__ sums) {
+  __shared__ double buf[256 + 16];  // 16 slots of padding: a last group may read past the chunk
   const int u = (int)blockIdx.x;
   const int lane = threadIdx.x;
   const double* col = t + (size_t)u * ld;
+  const int32_t* cu = cnt + (size_t)u * nch;
   double s = sums[u];
-  double v[FOLD_CH];
+  double v[4];
 #pragma unroll
-  for (int c = 0; c < FOLD_CH; ++c) {
-    const int64_t q = 64 * c + lane;
-    v[c] = q < np ? col[q] : 0.0;
-  }
-  for (int64_t q0 = 0; q0 < np; q0 += 64 * FOLD_CH) {
-    int cnt = 0;
+  for (int i = 0; i < 4; ++i) v[i] = col[64 * i + lane];
+  int next = cu[0];
+  for (int64_t c = 0; c < nch; ++c) {
+    const int cur = __builtin_amdgcn_readfirstlane(next);
 #pragma unroll
-    for (int c = 0; c < FOLD_CH; ++c) {
-      const uint64_t m = __ballot(v[c] != 0.0);
-      const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-      if (v[c] != 0.0) buf[cnt + below] = v[c];
-      cnt += __popcll(m);
-    }
-    const int64_t q1 = q0 + 64 * FOLD_CH;
+    for (int i = 0; i < 4; ++i) buf[64 * i + lane] = v[i];
+    if (c + 1 < nch) {
 #pragma unroll
-    for (int c = 0; c < FOLD_CH; ++c) {
-      const int64_t q = q1 + 64 * c + lane;
-      v[c] = q < np ? col[q] : 0.0;
+      for (int i = 0; i < 4; ++i) v[i] = col[256 * (c + 1) + 64 * i + lane];
+      next = cu[c + 1];
     }
     __syncthreads();
-    // groups of 16: the next group's LDS reads are issued before the
-    // current group's dependent adds, so their latency is hidden
+    double a[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) a[i] = buf[i];
     int k = 0;
-    if (cnt >= 16) {
-      double a[16];
+    for (; k + 16 < cur; k += 16) {
+      double b[16];
 #pragma unroll
-      for (int i = 0; i < 16; ++i) a[i] = buf[i];
-      for (k = 16; k + 16 <= cnt; k += 16) {
-        double b[16];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) b[i] = buf[k + i];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) s = s + a[i];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) a[i] = b[i];
-      }
+      for (int i = 0; i < 16; ++i) b[i] = buf[k + 16 + i];
 #pragma unroll
       for (int i = 0; i < 16; ++i) s = s + a[i];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) a[i] = b[i];
     }
-    for (; k < cnt; ++k) s = s + buf[k];
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      if (k + i < cur) s = s + a[i];
     __syncthreads();
   }
   if (lane == 0) sums[u] = s;
@@ -254,20 +257,24 @@ hipError_t launch_fold_diag(const uint64_t* rec, int64_t chi, int d, uint8_t* D,
 }
 
 hipError_t launch_fold_terms(const uint64_t* rec, int64_t chi, int64_t p0, int64_t np, int64_t ld, int d,
-                             const uint8_t* D, const int32_t* cs, const double* tabs, double* t, int* err,
-                             hipStream_t st) {
+                             const uint8_t* D, const int32_t* cs, const double* tabs, double* t, int32_t* cnt,
+                             int* err, hipStream_t st) {
   if (np <= 0) return hipSuccess;
   if (d < 1 || d > 6 || p0 < 0 || p0 + np > chi * (chi + 1) / 2 || np > ld) return hipErrorInvalidValue;
   const int nn = (2 << d) - 2;
-  const int64_t per = (np + 255) & ~(int64_t)255;
-  k_fold_terms<<<dim3((unsigned)(per / 256 * nn)), dim3(256), 0, st>>>(rec, chi, p0, np, ld, d, D, cs, tabs, t, err);
+  const int64_t nch = (np + 255) / 256;
+  if (256 * nch > ld) return hipErrorInvalidValue;
+  k_fold_terms<<<dim3((unsigned)(nch * nn)), dim3(256), 0, st>>>(rec, chi, p0, np, ld, nch, d, D, cs, tabs, t, cnt,
+                                                                err);
   return hipGetLastError();
 }
 
-hipError_t launch_fold_sum(const double* t, int64_t np, int64_t ld, int nnodes, double* sums, hipStream_t st) {
+hipError_t launch_fold_sum(const double* t, const int32_t* cnt, int64_t np, int64_t ld, int nnodes, double* sums,
+                           hipStream_t st) {
   if (np <= 0) return hipSuccess;
-  if (nnodes < 1 || nnodes > FOLD_NODES || np > ld) return hipErrorInvalidValue;
-  k_fold_sum<<<dim3((unsigned)nnodes), dim3(64), 0, st>>>(t, np, ld, sums);
+  const int64_t nch = (np + 255) / 256;
+  if (nnodes < 1 || nnodes > FOLD_NODES || 256 * nch > ld) return hipErrorInvalidValue;
+  k_fold_sum<<<dim3((unsigned)nnodes), dim3(64), 0, st>>>(t, cnt, ld, nch, sums);
   return hipGetLastError();
 }
 

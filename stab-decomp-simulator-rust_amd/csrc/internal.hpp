@@ -297,14 +297,17 @@ hipError_t launch_grad_loss(const double* vals, const double* wts, int n, double
 // k_fold_diag: D[f][u] = number of non-deterministic projections of term f
 // on the path to node u (255: the term does not survive), from the records
 // of the diagonal pairs.  k_fold_terms: the FP64 value the reference adds
-// for pair p (reference order) at node u, for pairs [p0, p0 + np).
-// t is node-major (t[u * ld + q], ld >= np).  k_fold_sum: sums[u] += those
-// values in pair order, one wave per node.  err (device int) is set to 1
-// when a node value is not of the form zeta^K 2^(-R/2) or 0.
+// for pair p (reference order) at node u, for pairs [p0, p0 + np); per
+// 256-pair chunk c the nonzero ones, in pair order, at t[u * ld + 256 c ...]
+// (ld: a multiple of 256, at least np) and their count at cnt[u * nch + c]
+// (nch = ceil(np / 256)).  k_fold_sum: sums[u] += those values in pair
+// order, one wave per node.  err (device int) is set to 1 when a node value
+// is not of the form zeta^K 2^(-R/2) or 0.
 hipError_t launch_fold_diag(const uint64_t* rec, int64_t chi, int d, uint8_t* D, int* err, hipStream_t st);
 hipError_t launch_fold_terms(const uint64_t* rec, int64_t chi, int64_t p0, int64_t np, int64_t ld, int d,
-                             const uint8_t* D, const int32_t* cs, const double* tabs, double* t, int* err,
-                             hipStream_t st);
-hipError_t launch_fold_sum(const double* t, int64_t np, int64_t ld, int nnodes, double* sums, hipStream_t st);
+                             const uint8_t* D, const int32_t* cs, const double* tabs, double* t, int32_t* cnt,
+                             int* err, hipStream_t st);
+hipError_t launch_fold_sum(const double* t, const int32_t* cnt, int64_t np, int64_t ld, int nnodes, double* sums,
+                           hipStream_t st);
 
 }  // namespace nx

@@ -1732,7 +1732,7 @@ struct Window {
 // one sample call: pair records, per-(term, node) projection counts, one
 // chunk of per-(pair, node) terms, the node sums and the error flag.
 struct RefFold {
-  DevBuf rec, D, t, sums;
+  DevBuf rec, D, t, cnt, sums;
   PinnedBuf host;
   int64_t chunk = 0;
 };
@@ -1933,8 +1933,9 @@ static std::vector<double> ref_node_norms(const nx_state* s, ProjPlan& pp, RefFo
   if (!rf.rec.p) {
     rf.rec.alloc((size_t)np * FOLD_REC * 8);
     rf.D.alloc((size_t)chi * FOLD_NODES);
-    rf.chunk = std::min<int64_t>(np, (int64_t)1 << 20);
+    rf.chunk = (std::min<int64_t>(np, (int64_t)1 << 20) + 255) / 256 * 256;
     rf.t.alloc((size_t)rf.chunk * FOLD_NODES * 8);
+    rf.cnt.alloc((size_t)(rf.chunk / 256) * FOLD_NODES * 4);
     rf.sums.alloc(FOLD_NODES * 8 + 256);
     rf.host.alloc(FOLD_NODES * 8 + 256);
   }
@@ -1948,9 +1949,10 @@ static std::vector<double> ref_node_norms(const nx_state* s, ProjPlan& pp, RefFo
     for (int64_t p0 = 0; p0 < np; p0 += rf.chunk) {
       const int64_t m = std::min<int64_t>(rf.chunk, np - p0);
       hip_check(launch_fold_terms(rf.rec.as<uint64_t>(), chi, p0, m, rf.chunk, d, rf.D.as<uint8_t>(),
-                                  coef_of(s).cs, ctx()->tabs, rf.t.as<double>(), err, stream()),
+                                  coef_of(s).cs, ctx()->tabs, rf.t.as<double>(), rf.cnt.as<int32_t>(), err, stream()),
                 "k_fold_terms");
-      hip_check(launch_fold_sum(rf.t.as<double>(), m, rf.chunk, nn, sums, stream()), "k_fold_sum");
+      hip_check(launch_fold_sum(rf.t.as<double>(), rf.cnt.as<int32_t>(), m, rf.chunk, nn, sums, stream()),
+                "k_fold_sum");
     }
     ps.end();
   }
