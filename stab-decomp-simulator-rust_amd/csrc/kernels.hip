@@ -1390,6 +1390,10 @@ hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, int64_t nfixed, in
     NX_PAIRS(true, 0);
   } else if (same) {
     NX_PAIRS(true, -1);
+  } else if (mode == 3 && !exact_out && cx.scalar && cy.scalar) {  // exp_value: exact coefficients
+    NX_PAIRS_CK(false, 3, 1);
+  } else if (mode == 3 && !exact_out && !cx.scalar && !cy.scalar) {
+    NX_PAIRS_CK(false, 3, 0);
   } else if (mode == 3) {
     NX_PAIRS(false, 3);
   } else if (mode == 0) {
