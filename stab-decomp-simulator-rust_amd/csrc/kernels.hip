@@ -881,27 +881,23 @@ __global__ __launch_bounds__(AMP_NT) void k_amplitude(QFormDev q, const uint4* _
 }
 
 // sum of the per-chunk partial amplitudes in chunk order, then * global_factor
-// (eight loads issued before their adds)
+// (the loads of eight chunks, the last group's too, issued before their adds)
 __global__ void k_amp_reduce(const double2* __restrict__ part, int64_t nb, int nchunks, double2* __restrict__ out,
                              double gre, double gim) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   double r = 0.0, i = 0.0;
-  int c = 0;
-  for (; c + 8 <= nchunks; c += 8) {
+  for (int c = 0; c < nchunks; c += 8) {
     double2 v[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = part[(size_t)(c + k) * nb + b];
+    for (int k = 0; k < 8; ++k) v[k] = c + k < nchunks ? part[(size_t)(c + k) * nb + b] : make_double2(0.0, 0.0);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      r = r + v[k].x;
-      i = i + v[k].y;
+      if (c + k < nchunks) {
+        r = r + v[k].x;
+        i = i + v[k].y;
+      }
     }
-  }
-  for (; c < nchunks; ++c) {
-    const double2 v = part[(size_t)c * nb + b];
-    r = r + v.x;
-    i = i + v.y;
   }
   out[b] = make_double2(r * gre - i * gim, r * gim + i * gre);
 }
