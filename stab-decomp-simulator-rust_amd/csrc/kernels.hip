@@ -1152,7 +1152,7 @@ __global__ __launch_bounds__(256, (SAME && W == 1) ? 8 : 1) void k_pairs(QFormDe
 
 // single-block reduction of the per-block partial sums in a fixed order:
 // thread i sums partial[i], partial[i + 1024], ... in index order (eight
-// loads in flight per step), then a fixed tree over the threads.  With
+// loads in flight per group), then a fixed tree over the threads.  With
 // norm_out, also sqrt((A_re + sqrt2 B_re) g2) there (the norm, as the host
 // forms it from the same sums)
 __global__ __launch_bounds__(1024) void k_reduce(const double4* __restrict__ partial, int64_t n,
@@ -1160,25 +1160,20 @@ __global__ __launch_bounds__(1024) void k_reduce(const double4* __restrict__ par
                                                  double g2) {
   __shared__ double4 sh[1024];
   double4 r = make_double4(0.0, 0.0, 0.0, 0.0);
-  int64_t k = threadIdx.x;
-  for (; k + 7 * 1024 < n; k += 8 * 1024) {
+  // groups of eight loads (the last, partial group too) issued before their adds
+  for (int64_t k = threadIdx.x; k < n; k += 8 * 1024) {
     double4 v[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = partial[k + u * 1024];
+    for (int u = 0; u < 8; ++u) v[u] = k + u * 1024 < n ? partial[k + u * 1024] : make_double4(0.0, 0.0, 0.0, 0.0);
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      r.x += v[u].x;
-      r.y += v[u].y;
-      r.z += v[u].z;
-      r.w += v[u].w;
+      if (k + u * 1024 < n) {
+        r.x += v[u].x;
+        r.y += v[u].y;
+        r.z += v[u].z;
+        r.w += v[u].w;
+      }
     }
-  }
-  for (; k < n; k += 1024) {
-    const double4 v = partial[k];
-    r.x += v.x;
-    r.y += v.y;
-    r.z += v.z;
-    r.w += v.w;
   }
   sh[threadIdx.x] = r;
   __syncthreads();
