@@ -367,6 +367,17 @@ __global__ __launch_bounds__(256) void k_qform(FormsDev in, QFormDev q, char* __
     if (getbit<W>(f.s, k)) flipbit<W>(rhs, nc, 1);
     ++nc;
   }
+  // reduced echelon form on the rows' lowest variables (same affine
+  // subspace: the right-hand sides follow the row operations), so that the
+  // pair kernels substitute a bra row without updating the other bra rows
+  rref_rows<W>(C, nc, lane, [&](int c, const bool (&hit)[W]) {
+    const bool rc = getbit<W>(rhs, c) != 0;
+#pragma unroll
+    for (int sl = 0; sl < W; ++sl) {
+      const u64 m = __ballot(hit[sl]);
+      rhs[sl] ^= rc ? m : 0ull;
+    }
+  });
   // C transposed: row j holds the constraints (bits over c) containing x_j
   u64 CT[W][W];
 #pragma unroll
@@ -997,7 +1008,9 @@ __device__ __forceinline__ int vreg(int x) {
 // MODE: the mode argument when it is 0..3 at compile time (no per-pair role
 // selects), or -1 to read it at run time.  CK: 1 (exact Scalar
 // coefficients) or 0 (complex128) when known at compile time, with no
-// exact_out; -1 to read both at run time.
+// exact_out; -1 to read both at run time.  X and Y are written by k_qform
+// (constraint rows in reduced echelon form, and the meta records), not
+// projected forms.
 template <int W, bool SAME, int MODE, int CK>
 __global__ __launch_bounds__(256, (SAME && W == 1) ? 8 : 1) void k_pairs(QFormDev X, QFormDev Y,
                                                int64_t nfixed, int64_t nother, int triangle, int J, int mode_rt,
@@ -1098,7 +1111,7 @@ __global__ __launch_bounds__(256, (SAME && W == 1) ? 8 : 1) void k_pairs(QFormDe
       P.mag = kf + ko;
       const int nca = fixed_is_ket ? nco : ncf, ncb = fixed_is_ket ? ncf : nco;
       int zero = 0, K = 0, Rr = 0;
-      pair_eval<W>(P, nca, ncb, &zero, &K, &Rr);
+      pair_eval<W, true>(P, nca, ncb, &zero, &K, &Rr);
       if (exact_out && lane == 0)
         exact_out[(size_t)bra * exact_ld + ket] = zero ? 0xFFFFFFFFu : ((uint32_t)K | ((uint32_t)Rr << 8));
       if (zero) continue;

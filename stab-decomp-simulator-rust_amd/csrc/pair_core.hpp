@@ -152,12 +152,22 @@ __device__ __forceinline__ void subst(PairRegs<W>& P, int p, const u64 (&r)[W], 
   clearbit<W>(P.rem, pw, pb);
 }
 
-// returns 1 if a constraint is inconsistent (the sum is zero)
-template <int W, bool USEB>
+// returns 1 if a constraint is inconsistent (the sum is zero).  RREF: the
+// bra rows are in reduced echelon form on their lowest variables (as the
+// quadratic-form kernel writes them): no bra row holds another's lowest
+// variable, so a bra substitution leaves the other bra rows as they are and
+// none of them becomes empty.
+template <int W, bool USEB, bool RREF = false>
 __device__ __forceinline__ int take_constraint(PairRegs<W>& P, int c) {
   u64 row[W];
   bcast<W>(USEB ? P.cb : P.ca, c, row);
   const int rhs = USEB ? getbit<W>(P.rhsb, c) : getbit<W>(P.rhsa, c);
+  if (!USEB && RREF) {
+    const int p = ffsv<W>(row);
+    flipbit<W>(row, p, 1);
+    subst<W, CONS_B>(P, p, row, rhs);
+    return 0;
+  }
   if (!anyv<W>(row)) return rhs;
   const int p = ffsv<W>(row);
   flipbit<W>(row, p, 1);
@@ -180,12 +190,18 @@ __device__ __forceinline__ void uniform_state(PairRegs<W>& P) {
   P.mag = uni(P.mag);
 }
 
-// exact <bra|ket>: returns zero flag, K (mod 8), r
-template <int W>
+// exact <bra|ket>: returns zero flag, K (mod 8), r.  RREF: the bra rows
+// are in reduced echelon form (take_constraint).
+template <int W, bool RREF = false>
 __device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int* zero, int* K, int* R) {
   uniform_state<W>(P);
   for (int c = 0; c < nca; ++c) {
-    if (uni(take_constraint<W, false>(P, c))) { *zero = 1; return; }
+    if (RREF) {
+      take_constraint<W, false, true>(P, c);
+    } else if (uni(take_constraint<W, false>(P, c))) {
+      *zero = 1;
+      return;
+    }
     uniform_state<W>(P);
   }
   for (int c = 0; c < ncb; ++c) {
@@ -299,6 +315,31 @@ __device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int*
   *zero = zf;
   *K = P.c0;
   *R = P.mag;
+}
+
+// Gauss-Jordan on a term's nc constraint rows (row c in lane c & 63 of
+// slot c >> 6), each row solved for its lowest variable: afterwards that
+// variable appears in no other row and stays the row's lowest (a row only
+// ever gains variables above a later pivot, which exceeds its own).  The
+// steps depend on C alone, so every kernel that starts from the same rows
+// takes the same steps; also(c, hit) applies step c (row l ^= row c for the
+// lanes with hit[slot of l]) to what the caller keeps per row.
+template <int W, typename Also>
+__device__ __forceinline__ void rref_rows(u64 (&C)[W][W], int nc, int lane, Also&& also) {
+  for (int c = 0; c < nc; ++c) {
+    u64 row[W];
+    bcast<W>(C, c, row);
+    const int p = ffsv<W>(row);  // independent rows: row != 0
+    const int pw = wordof<W>(p), pb = p & 63;
+    bool hit[W];
+#pragma unroll
+    for (int sl = 0; sl < W; ++sl) {
+      hit[sl] = (64 * sl + lane != c) && ((wsel<W>(C[sl], pw) >> pb) & 1ull);
+#pragma unroll
+      for (int w = 0; w < W; ++w) C[sl][w] ^= hit[sl] ? row[w] : 0ull;
+    }
+    also(c, hit);
+  }
 }
 
 // rows of term t (word offsets fit 32 bits: chi * 64W * W < 2^32 is checked

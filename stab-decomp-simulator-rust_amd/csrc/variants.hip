@@ -223,6 +223,16 @@ __device__ __forceinline__ void take_constraint_v(PairRegs<W>& P, VarRegs<W, VW>
 #pragma unroll
     for (int v = 0; v < VW; ++v) E[v] = 0ull;
   }
+  if (!USEB && !UNITS) {
+    // without units the forms are the quadratic-form kernel's, bra rows in
+    // reduced echelon form on their lowest variables: no bra row holds
+    // another's lowest variable, so a bra substitution leaves the other bra
+    // rows as they are and none of them becomes empty
+    const int p = ffsv<W>(row);
+    flipbit<W>(row, p, 1);
+    subst_v<W, VW, CONS_B, UNITS>(P, V, p, row, rhs, E);
+    return;
+  }
   if (!anyv<W>(row)) {
     // 0 = rhs_k: the sum vanishes for the variants with rhs_k = 1
     if (rhs) V.bz = 1;
@@ -817,7 +827,8 @@ __global__ __launch_bounds__(256) void k_pairs_sym(QFormDev X, const int64_t* __
 // the phase, Y_q = Z_q, X_q, then +2.  With ds the total change of s, the
 // quadratic form's a_j = gamma_j/2 + 2 (par(s & w_j) ^ c_j) gains
 // 2 (zy_j ^ par(ds & w_j)), the constraint rhs (s at the positions with
-// v = 0, in order) flips by ds there, and p0 = phase changes by dph.
+// v = 0, in order) flips by ds there -- carried through the row reduction
+// the stored rows went through -- and p0 = phase changes by dph.
 template <int W, int VW>
 __global__ __launch_bounds__(256) void k_pauli_var(FormsDev in, const int32_t* __restrict__ voff,
                                                    const int32_t* __restrict__ vops, int nvar,
@@ -837,12 +848,26 @@ __global__ __launch_bounds__(256) void k_pauli_var(FormsDev in, const int32_t* _
     for (int w = 0; w < W; ++w) Wm[sl][w] = (f.M[sl][w] & ~f.v[w]) ^ (f.F[sl][w] & f.v[w]);
     kpos[sl] = -1;
   }
+  // the constraint rows as the quadratic-form kernel forms them (column k
+  // of F for the c-th k with v_k = 0), to reduce the rhs flips with them
+  u64 C[W][W];
+#pragma unroll
+  for (int sl = 0; sl < W; ++sl)
+#pragma unroll
+    for (int w = 0; w < W; ++w) C[sl][w] = 0ull;
   int nc = 0;
   for (int k = 0; k < n; ++k) {
     if (getbit<W>(f.v, k)) continue;
+    u64 col[W];
+#pragma unroll
+    for (int sw = 0; sw < W; ++sw) col[sw] = __ballot(getbit<W>(f.F[sw], k));
 #pragma unroll
     for (int sl = 0; sl < W; ++sl)
-      if (owns<W>(f, nc, sl)) kpos[sl] = k;
+      if (owns<W>(f, nc, sl)) {
+        kpos[sl] = k;
+#pragma unroll
+        for (int w = 0; w < W; ++w) C[sl][w] = col[w];
+      }
     ++nc;
   }
   u64 hacc[W][VW], racc[W][VW], p0[VW], p1[VW], p2[VW];
@@ -912,6 +937,16 @@ __global__ __launch_bounds__(256) void k_pauli_var(FormsDev in, const int32_t* _
         if (dph & 4) p2[v] |= bit;
       }
   }
+  // the flips of the stored (reduced) rows: each row is a sum of the rows
+  // above, so its flip is the sum of theirs
+  rref_rows<W>(C, nc, lane, [&](int c, const bool (&hit)[W]) {
+    u64 E[VW];
+    bcast_v<W, VW>(racc, c, E);
+#pragma unroll
+    for (int sl = 0; sl < W; ++sl)
+#pragma unroll
+      for (int v = 0; v < VW; ++v) racc[sl][v] ^= hit[sl] ? E[v] : 0ull;
+  });
 #pragma unroll
   for (int sl = 0; sl < W; ++sl) {
     const size_t r = (size_t)t * R + 64 * sl + lane;
