@@ -98,14 +98,14 @@ struct PairJobDev {
   int64_t out_base;  // variant k -> out[out_base + k]
   int64_t base_out;  // >= 0: the base (ket itself, identity) -> out[base_out]
   // unit constraints (sampling): x_{uq[u]} = ub[u*(VW+1)] for the base, the
-  // variants in masks ub[u*(VW+1) + 1 .. VW] take the other value; the ket
-  // then carries no variant tables (V unused)
+  // variants in masks ub[u*(VW+1) + 1 .. VW] take the other value; they are
+  // substituted into every term by launch_qproject_var (the pair kernel only
+  // reads the projected forms, so nunit > 0 requires termvar)
   int32_t nunit;
   const int32_t* uq;
   const uint64_t* ub;
   // termvar != 0 (with units): X = Y are forms projected onto the window
-  // outcomes by launch_qproject_var and V holds their variant tables; the
-  // units are not substituted again
+  // outcomes by launch_qproject_var and V holds their variant tables
   int32_t termvar;
   // split_ab != 0: the variant sums are written as their dyadic part A at
   // out[out_base + k] and sqrt(2) part B at out[out_base + nvar + k]

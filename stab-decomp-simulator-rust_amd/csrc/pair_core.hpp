@@ -318,18 +318,20 @@ __device__ __forceinline__ void pair_eval(PairRegs<W>& P, int nca, int ncb, int*
 }
 
 // Gauss-Jordan on a term's nc constraint rows (row c in lane c & 63 of
-// slot c >> 6), each row solved for its lowest variable: afterwards that
-// variable appears in no other row and stays the row's lowest (a row only
-// ever gains variables above a later pivot, which exceeds its own).  The
-// steps depend on C alone, so every kernel that starts from the same rows
-// takes the same steps; also(c, hit) applies step c (row l ^= row c for the
-// lanes with hit[slot of l]) to what the caller keeps per row.
+// slot c >> 6), each nonzero row solved for its lowest variable: afterwards
+// that variable appears in no other row and stays the row's lowest (a row
+// only ever gains variables above a later pivot, which exceeds its own);
+// empty rows (dependent rows of a projected form) stay empty.  The steps
+// depend on C alone, so every kernel that starts from the same rows takes
+// the same steps; also(c, hit) applies step c (row l ^= row c for the lanes
+// with hit[slot of l]) to what the caller keeps per row.
 template <int W, typename Also>
 __device__ __forceinline__ void rref_rows(u64 (&C)[W][W], int nc, int lane, Also&& also) {
   for (int c = 0; c < nc; ++c) {
     u64 row[W];
     bcast<W>(C, c, row);
-    const int p = ffsv<W>(row);  // independent rows: row != 0
+    if (!anyv<W>(row)) continue;
+    const int p = ffsv<W>(row);
     const int pw = wordof<W>(p), pb = p & 63;
     bool hit[W];
 #pragma unroll

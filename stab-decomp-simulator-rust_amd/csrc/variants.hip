@@ -223,11 +223,18 @@ __device__ __forceinline__ void take_constraint_v(PairRegs<W>& P, VarRegs<W, VW>
 #pragma unroll
     for (int v = 0; v < VW; ++v) E[v] = 0ull;
   }
-  if (!USEB && !UNITS) {
-    // without units the forms are the quadratic-form kernel's, bra rows in
-    // reduced echelon form on their lowest variables: no bra row holds
-    // another's lowest variable, so a bra substitution leaves the other bra
-    // rows as they are and none of them becomes empty
+  if (!USEB) {
+    // the bra rows are in reduced echelon form on their lowest variables
+    // (as k_qform and k_qproject_var store them): no bra row holds another's
+    // lowest variable, so a bra substitution leaves the other bra rows (and
+    // their rhs flips) as they are.  Only projected rows can be empty.
+    if (UNITS && !anyv<W>(row)) {
+      // 0 = rhs_k: the sum vanishes for the variants with rhs_k = 1
+      if (rhs) V.bz = 1;
+#pragma unroll
+      for (int v = 0; v < VW; ++v) V.z[v] |= rhs ? ~E[v] : E[v];
+      return;
+    }
     const int p = ffsv<W>(row);
     flipbit<W>(row, p, 1);
     subst_v<W, VW, CONS_B, UNITS>(P, V, p, row, rhs, E);
@@ -242,33 +249,17 @@ __device__ __forceinline__ void take_constraint_v(PairRegs<W>& P, VarRegs<W, VW>
   }
   const int p = ffsv<W>(row);
   flipbit<W>(row, p, 1);
-  subst_v<W, VW, USEB ? CONS_B : CONS_AB, UNITS>(P, V, p, row, rhs, E);
+  subst_v<W, VW, CONS_B, UNITS>(P, V, p, row, rhs, E);
 }
 
 // pair_eval of pair_core.hpp for every variant and for the base (the ket
 // itself).  P.c0 follows the base; the planes hold each variant's offset.
-// With UNITS, the unit constraints x_{uq[u]} = ub base bit (flipped for the
-// variants in the following VW masks) are substituted first: they restrict
-// the sum to one Z-basis outcome per variant on those qubits, i.e. the pair
-// value <bra| Pi |ket> of the projector Pi = |b><b| on them.
+// With UNITS both terms are forms projected by k_qproject_var onto the
+// window outcomes (one variant per outcome, the unit constraints already
+// substituted into each term), and the bra rows carry rhs flips too.
 template <int W, int VW, bool UNITS>
-__device__ __forceinline__ void pair_eval_v(PairRegs<W>& P, VarRegs<W, VW>& V, int nca, int ncb, int nunit,
-                                            const int32_t* __restrict__ uq, const u64* __restrict__ ub) {
+__device__ __forceinline__ void pair_eval_v(PairRegs<W>& P, VarRegs<W, VW>& V, int nca, int ncb) {
   uniform_state<W, VW>(P, V);
-  if (UNITS) {
-    u64 r0[W];
-#pragma unroll
-    for (int w = 0; w < W; ++w) r0[w] = 0ull;
-    for (int u = 0; u < nunit; ++u) {
-      const int q = uni(uq[u]);
-      const int b = (int)uni64(ub[(size_t)u * (VW + 1)]);
-      u64 E[VW];
-#pragma unroll
-      for (int v = 0; v < VW; ++v) E[v] = uni64(ub[(size_t)u * (VW + 1) + 1 + v]);
-      subst_v<W, VW, CONS_AB, UNITS>(P, V, q, r0, b, E);
-      uniform_state<W, VW>(P, V);
-    }
-  }
   for (int c = 0; c < nca; ++c) {
     take_constraint_v<W, VW, false, UNITS>(P, V, c);
     uniform_state<W, VW>(P, V);
@@ -497,6 +488,23 @@ __device__ __forceinline__ void qproject_var_term(const QFormDev& in, const QFor
     subst_v<W, VW, CONS_AB, true>(P, V, q, r0, b, E);
     uniform_state<W, VW>(P, V);
   }
+  // back to reduced echelon form: a substitution for x_q takes x_q out of
+  // every row, which can empty a row or leave two rows with the same lowest
+  // variable.  The base rhs and the variants' flips follow the row
+  // operations (the window pass substitutes bra rows without updating the
+  // other bra rows).
+  rref_rows<W>(P.ca, uni(in.nc[t]), lane, [&](int c, const bool (&hit)[W]) {
+    const bool rc = getbit<W>(P.rhsa, c) != 0;
+    u64 E[VW];
+    bcast_v<W, VW>(V.ra, c, E);
+#pragma unroll
+    for (int sl = 0; sl < W; ++sl) {
+      const u64 m = __ballot(hit[sl]);
+      P.rhsa[sl] = uni64(P.rhsa[sl] ^ (rc ? m : 0ull));
+#pragma unroll
+      for (int v = 0; v < VW; ++v) V.ra[sl][v] ^= hit[sl] ? E[v] : 0ull;
+    }
+  });
 #pragma unroll
   for (int sl = 0; sl < W; ++sl) {
     const int r = 64 * sl + lane;
@@ -1090,7 +1098,7 @@ __global__ __launch_bounds__(256, (W == 1 && VW == 1 && !REC) ? 8 : 1) void k_pa
       V.dv = tv ? 1 : 0;
       P.c0 = (pf - po) & 7;
       P.mag = kf + ko;
-      pair_eval_v<W, VW, UNITS>(P, V, nco, ncf, tv ? 0 : jd.nunit, jd.uq, jd.ub);
+      pair_eval_v<W, VW, UNITS>(P, V, nco, ncf);
       if constexpr (REC && UNITS && VW == 1) {
         if (lane == 0) {
           // exact per-variant values, at the pair's position in the
@@ -1355,7 +1363,7 @@ __global__ __launch_bounds__(256) void k_grad_var(QFormDev X, VarDev VT, const d
         V.dv = 0;
         P.c0 = (pf - po) & 7;
         P.mag = kf + ko;
-        pair_eval_v<W, VW, false>(P, V, nco, ncf, 0, nullptr, nullptr);
+        pair_eval_v<W, VW, false>(P, V, nco, ncf);
         if (all_zero<W, VW>(V)) continue;  // every string's pair value vanishes
         // (c_bra.conj() * c_ket).into(), shared by the strings
         const double2 co = c[o];
