@@ -405,6 +405,15 @@ __global__ __launch_bounds__(256) void k_qproject(QFormDev in, QFormDev out, con
     subst<W, CONS_AB>(P, uni(uq[u]), r0, uni(ub[u]));
     uniform_state<W>(P);
   }
+  // back to reduced echelon form (as k_qproject_var_term), the rhs following
+  rref_rows<W>(P.ca, uni(in.nc[t]), lane, [&](int c, const bool (&hit)[W]) {
+    const bool rc = getbit<W>(P.rhsa, c) != 0;
+#pragma unroll
+    for (int sl = 0; sl < W; ++sl) {
+      const u64 m = __ballot(hit[sl]);
+      P.rhsa[sl] = uni64(P.rhsa[sl] ^ (rc ? m : 0ull));
+    }
+  });
 #pragma unroll
   for (int sl = 0; sl < W; ++sl) {
     const int r = 64 * sl + lane;
@@ -593,7 +602,9 @@ __device__ __forceinline__ int take_constraint_sym(PairRegs<W>& P, VarRegs<W, VW
   u64 E[VW];
 #pragma unroll
   for (int v = 0; v < VW; ++v) E[v] = 0ull;
-  subst_v<W, VW, USEB ? CONS_B : CONS_AB, false, false>(P, V, p, row, rhs, E);
+  // bra rows in reduced echelon form (every stored form is): a bra
+  // substitution leaves the other bra rows as they are
+  subst_v<W, VW, CONS_B, false, false>(P, V, p, row, rhs, E);
   return 0;
 }
 
