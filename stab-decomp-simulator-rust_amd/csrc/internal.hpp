@@ -199,10 +199,13 @@ hipError_t launch_tails(int W, const TailJobDev* jobs, int njobs, int64_t total_
 // closed form below; else o in [0, nother)), four waves per block, nblocks
 // blocks; per block partial[block] = {A_re, A_im, B_re, B_im}: the sum is
 // A + sqrt(2) B, accumulated exactly (powers of two) for Scalar
-// coefficients, B = 0 for complex128 coefficients
+// coefficients, B = 0 for complex128 coefficients.  With ev0 and ev1 the
+// launch itself carries the two timing events (start and end of the
+// kernel, no separate event records on the stream).
 hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, int64_t nfixed, int64_t nother, int triangle, int J,
                         int mode, const CoefDev& cx, const CoefDev& cy, const double* tabs, double4* partial,
-                        int nblocks, uint32_t* exact_out, int64_t exact_ld, hipStream_t st);
+                        int nblocks, uint32_t* exact_out, int64_t exact_ld, hipStream_t st,
+                        hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 // sum of n double4 partials in a fixed order (one block) into out (if not
 // null); with norm_out, sqrt((A_re + sqrt2 B_re) g2) into norm_out[0]
 hipError_t launch_reduce(const double4* partial, int64_t n, double4* out, hipStream_t st,

@@ -14,6 +14,7 @@
 //              <phi_a|phi_b> evaluated by GF(2) variable elimination (lane =
 //              variable row, pivots broadcast with readlane), FP64 partial
 //              sums per wavefront
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1371,17 +1372,25 @@ hipError_t launch_amp_reduce(const double2* partial, int64_t nb, int nchunks, do
 
 hipError_t launch_pairs(const QFormDev& X, const QFormDev& Y, int64_t nfixed, int64_t nother, int triangle, int J,
                         int mode, const CoefDev& cx, const CoefDev& cy, const double* tabs, double4* partial,
-                        int nblocks, uint32_t* exact_out, int64_t exact_ld, hipStream_t st) {
+                        int nblocks, uint32_t* exact_out, int64_t exact_ld, hipStream_t st, hipEvent_t ev0,
+                        hipEvent_t ev1) {
   if ((triangle && nfixed != nother) || J != PAIR_UNIT) return hipErrorInvalidValue;
+  const bool timed = ev0 && ev1;
   const bool same = X.B == Y.B && X.C == Y.C && X.a == Y.a && cx.scalar == cy.scalar && cx.cs == cy.cs &&
                     cx.c == cy.c;
   // the combinations the runtime uses: norm (same, fixed bra, Hermitian),
   // pair tables (same, complex), exp_value (fixed ket, Hermitian), inner
   // products (complex); any other mode is read at run time
 #define NX_PAIRS_CK(SM, MD, CKV)                                                                              \
-  NX_DISPATCH_W(X.W, k_pairs<WW, SM, MD, CKV><<<dim3(nblocks), dim3(256), 0, st>>>(                           \
-                         X, SM ? X : Y, nfixed, nother, triangle, J, mode, cx, SM ? cx : cy, tabs, partial,    \
-                         exact_out, exact_ld))
+  NX_DISPATCH_W(X.W, if (timed) {                                                                             \
+    hipExtLaunchKernelGGL(k_pairs<WW, SM, MD, CKV>, dim3(nblocks), dim3(256), 0, st, ev0, ev1, 0, X,          \
+                          SM ? X : Y, nfixed, nother, triangle, J, mode, cx, SM ? cx : cy, tabs, partial,     \
+                          exact_out, exact_ld);                                                               \
+  } else {                                                                                                    \
+    k_pairs<WW, SM, MD, CKV><<<dim3(nblocks), dim3(256), 0, st>>>(X, SM ? X : Y, nfixed, nother, triangle, J, \
+                                                                 mode, cx, SM ? cx : cy, tabs, partial,       \
+                                                                 exact_out, exact_ld);                        \
+  })
 #define NX_PAIRS(SM, MD) NX_PAIRS_CK(SM, MD, -1)
   // the norm: coefficient kind fixed at compile time
   if (same && mode == 2 && !exact_out && cx.scalar) {

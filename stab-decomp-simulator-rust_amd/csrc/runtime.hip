@@ -290,11 +290,15 @@ struct Prof {
 static std::mutex g_prof_mu;
 static Prof g_prof;
 
+// ext: the timed launch records the pair itself (start_ev() and end_ev() go
+// to a launch that takes timing events), so the stream carries no event
+// records of its own
 struct ProfScope {
   const char* name;
   bool on;
+  bool ext;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  explicit ProfScope(const char* nm) : name(nm), on(false) {
+  explicit ProfScope(const char* nm, bool ext_ = false) : name(nm), on(false), ext(ext_) {
     std::lock_guard<std::mutex> lk(g_prof_mu);
     if (!g_prof.on) return;
     if (!g_prof.only.empty() && g_prof.only.find("," + std::string(nm) + ",") == std::string::npos) return;
@@ -308,20 +312,27 @@ struct ProfScope {
     e0 = g_prof.pool.back().first;
     e1 = g_prof.pool.back().second;
     g_prof.pool.pop_back();
-    hip_check(hipEventRecord(e0, stream()), "hipEventRecord");
+    if (!ext) hip_check(hipEventRecord(e0, stream()), "hipEventRecord");
   }
+  hipEvent_t start_ev() const { return on && ext ? e0 : nullptr; }
+  hipEvent_t end_ev() const { return on && ext ? e1 : nullptr; }
+  // after the launch (ext: only once it was issued with start_ev / end_ev)
   void end() {
     if (!on) return;
     on = false;
     std::lock_guard<std::mutex> lk(g_prof_mu);
-    hip_check(hipEventRecord(e1, stream()), "hipEventRecord");
+    if (!ext) hip_check(hipEventRecord(e1, stream()), "hipEventRecord");
     g_prof.pending.push_back({name, e0, e1});
     if (g_prof.pending.size() >= 4096) g_prof.collect();
   }
   ~ProfScope() {
-    if (on) {
-      try { end(); } catch (...) {}
+    if (!on) return;
+    if (ext) {  // the launch did not happen: the events were never recorded
+      std::lock_guard<std::mutex> lk(g_prof_mu);
+      g_prof.pool.push_back({e0, e1});
+      return;
     }
+    try { end(); } catch (...) {}
   }
 };
 
@@ -674,9 +685,9 @@ static cplx run_pairs(const QForms& X, const QForms& Y, int64_t nfixed, int64_t 
   // buffer per calling thread): no copy after it
   static thread_local PinnedBuf t_res;
   if (!t_res.p) t_res.alloc(64);
-  ProfScope ps("k_pairs");
+  ProfScope ps("k_pairs", true);
   hip_check(launch_pairs(X.desc(), Y.desc(), nfixed, nother, triangle ? 1 : 0, PAIR_J, mode, cx, cy, ctx()->tabs,
-                         partial.as<double4>(), nblocks, d_exact, ld, stream()),
+                         partial.as<double4>(), nblocks, d_exact, ld, stream(), ps.start_ev(), ps.end_ev()),
             "k_pairs");
   ps.end();
   {
@@ -700,9 +711,9 @@ static void norm_enqueue(const nx_state* s, double* d_out) {
   CoefDev c = coef_of(s);
   const int nblocks = pair_blocks(pair_units(s->chi(), s->chi(), true));
   DevBuf partial((size_t)nblocks * 32);
-  ProfScope ps("k_pairs");
+  ProfScope ps("k_pairs", true);
   hip_check(launch_pairs(q.desc(), q.desc(), s->chi(), s->chi(), 1, PAIR_J, /*fixed bra, herm*/ 2, c, c, ctx()->tabs,
-                         partial.as<double4>(), nblocks, nullptr, 0, stream()),
+                         partial.as<double4>(), nblocks, nullptr, 0, stream(), ps.start_ev(), ps.end_ev()),
             "k_pairs");
   ps.end();
   ProfScope pr("k_reduce");
