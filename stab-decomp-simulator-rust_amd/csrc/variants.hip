@@ -1212,19 +1212,28 @@ __global__ __launch_bounds__(256) void k_reduce_seg(const PairJobDev* __restrict
   }
 }
 
-// Second stage: block j adds job j's segment sums in segment order; the
-// value is A + sqrt(2) B (or A and B apart with split_ab).
+// Second stage: block j adds job j's segment sums, in four runs of
+// consecutive segments (one per quarter of the block, each in segment order)
+// whose totals are then added in run order; the value is A + sqrt(2) B (or A
+// and B apart with split_ab).
+constexpr int RED_VAR_NT = 1024;
 template <int VW>
-__global__ __launch_bounds__(256) void k_reduce_var(const PairJobDev* __restrict__ jobs,
-                                                    const double* __restrict__ seg_part, double* __restrict__ out) {
+__global__ __launch_bounds__(RED_VAR_NT) void k_reduce_var(const PairJobDev* __restrict__ jobs,
+                                                           const double* __restrict__ seg_part,
+                                                           double* __restrict__ out) {
   constexpr int NVS = 64 * VW + 1;
+  constexpr int NG = RED_VAR_NT / 256;
   constexpr double SQRT2 = 1.4142135623730951;  // nearest double to sqrt(2)
+  __shared__ double s_a[NG][NVS], s_b[NG][NVS];
   const PairJobDev& jd = jobs[blockIdx.x];
-  for (int v = threadIdx.x; v < NVS; v += 256) {
+  const int g = threadIdx.x >> 8, tl = threadIdx.x & 255;
+  const int64_t per = (jd.nseg + NG - 1) / NG;
+  const int64_t r0 = jd.seg0 + g * per;
+  const int64_t s1 = (r0 + per < jd.seg0 + jd.nseg) ? r0 + per : jd.seg0 + jd.nseg;
+  for (int v = tl; v < NVS; v += 256) {
     // loads issued 8 segments at a time, added in segment order
     double ta = 0.0, tb = 0.0;
-    const int64_t s1 = jd.seg0 + jd.nseg;
-    int64_t s = jd.seg0;
+    int64_t s = r0;
     for (; s + 8 <= s1; s += 8) {
       double va[8], vb[8];
 #pragma unroll
@@ -1241,6 +1250,18 @@ __global__ __launch_bounds__(256) void k_reduce_var(const PairJobDev* __restrict
     for (; s < s1; ++s) {
       ta = ta + seg_part[(size_t)s * 2 * NVS + v];
       tb = tb + seg_part[(size_t)s * 2 * NVS + NVS + v];
+    }
+    s_a[g][v] = ta;
+    s_b[g][v] = tb;
+  }
+  __syncthreads();
+  if (g != 0) return;
+  for (int v = tl; v < NVS; v += 256) {
+    double ta = s_a[0][v], tb = s_b[0][v];
+#pragma unroll
+    for (int k = 1; k < NG; ++k) {
+      ta = ta + s_a[k][v];
+      tb = tb + s_b[k][v];
     }
     const double t = ta + SQRT2 * tb;
     if (v < jd.nvar) {
@@ -1615,10 +1636,10 @@ hipError_t launch_reduce_var(const PairJobDev* jobs, int njobs, int VW, const do
   if (njobs == 0 || nseg == 0) return hipSuccess;
   if (VW == 1) {
     k_reduce_seg<1><<<dim3((unsigned)nseg), dim3(256), 0, st>>>(jobs, seg_job, partial, seg_part);
-    k_reduce_var<1><<<dim3(njobs), dim3(256), 0, st>>>(jobs, seg_part, out);
+    k_reduce_var<1><<<dim3(njobs), dim3(RED_VAR_NT), 0, st>>>(jobs, seg_part, out);
   } else if (VW == 2) {
     k_reduce_seg<2><<<dim3((unsigned)nseg), dim3(256), 0, st>>>(jobs, seg_job, partial, seg_part);
-    k_reduce_var<2><<<dim3(njobs), dim3(256), 0, st>>>(jobs, seg_part, out);
+    k_reduce_var<2><<<dim3(njobs), dim3(RED_VAR_NT), 0, st>>>(jobs, seg_part, out);
   } else {
     return hipErrorInvalidValue;
   }
