@@ -1183,31 +1183,47 @@ __global__ __launch_bounds__(256, (W == 1 && VW == 1 && !REC) ? 8 : 1) void k_pa
 
 // First stage: block b sums, for each of the 2 (64 VW + 1) values of a wave
 // (the variant and base sums, A parts then sqrt(2) parts), the partials of
-// the waves of segment b in wave order; a thread per value, so each wave's
-// row is one coalesced read.
+// the waves of segment b: four runs of consecutive waves (one per quarter of
+// the block, each in wave order) whose totals are added in run order; a
+// thread per value, so each wave's row is one coalesced read.
+constexpr int RED_SEG_NT = 512;
 template <int VW>
-__global__ __launch_bounds__(256) void k_reduce_seg(const PairJobDev* __restrict__ jobs,
-                                                    const int32_t* __restrict__ seg_job,
-                                                    const double* __restrict__ partial,
-                                                    double* __restrict__ seg_part) {
+__global__ __launch_bounds__(RED_SEG_NT) void k_reduce_seg(const PairJobDev* __restrict__ jobs,
+                                                           const int32_t* __restrict__ seg_job,
+                                                           const double* __restrict__ partial,
+                                                           double* __restrict__ seg_part) {
   constexpr int NV2 = 2 * (64 * VW + 1);
+  constexpr int GT = 128, NG = RED_SEG_NT / GT;
+  __shared__ double s_p[NG][NV2];
   const int64_t b = blockIdx.x;
   const PairJobDev& jd = jobs[seg_job[b]];
   const int64_t w0 = jd.wave0 + (b - jd.seg0) * RED_SEG;
   const int64_t wend = jd.wave0 + jd.nwaves;
   const int64_t w1 = (w0 + RED_SEG < wend) ? w0 + RED_SEG : wend;
-  for (int t = threadIdx.x; t < NV2; t += 256) {
+  const int g = threadIdx.x / GT, tl = threadIdx.x % GT;
+  constexpr int64_t PER = RED_SEG / NG;
+  const int64_t r0 = w0 + g * PER;
+  const int64_t r1 = (r0 + PER < w1) ? r0 + PER : w1;
+  for (int t = tl; t < NV2; t += GT) {
     // loads issued 8 at a time, added in wave order
     double a = 0.0;
-    int64_t w = w0;
-    for (; w + 8 <= w1; w += 8) {
+    int64_t w = r0;
+    for (; w + 8 <= r1; w += 8) {
       double v[8];
 #pragma unroll
       for (int k = 0; k < 8; ++k) v[k] = partial[(size_t)(w + k) * NV2 + t];
 #pragma unroll
       for (int k = 0; k < 8; ++k) a = a + v[k];
     }
-    for (; w < w1; ++w) a = a + partial[(size_t)w * NV2 + t];
+    for (; w < r1; ++w) a = a + partial[(size_t)w * NV2 + t];
+    s_p[g][t] = a;
+  }
+  __syncthreads();
+  if (g != 0) return;
+  for (int t = tl; t < NV2; t += GT) {
+    double a = s_p[0][t];
+#pragma unroll
+    for (int k = 1; k < NG; ++k) a = a + s_p[k][t];
     seg_part[(size_t)b * NV2 + t] = a;
   }
 }
@@ -1635,10 +1651,10 @@ hipError_t launch_reduce_var(const PairJobDev* jobs, int njobs, int VW, const do
                              int64_t nseg, double* seg_part, double* out, hipStream_t st) {
   if (njobs == 0 || nseg == 0) return hipSuccess;
   if (VW == 1) {
-    k_reduce_seg<1><<<dim3((unsigned)nseg), dim3(256), 0, st>>>(jobs, seg_job, partial, seg_part);
+    k_reduce_seg<1><<<dim3((unsigned)nseg), dim3(RED_SEG_NT), 0, st>>>(jobs, seg_job, partial, seg_part);
     k_reduce_var<1><<<dim3(njobs), dim3(RED_VAR_NT), 0, st>>>(jobs, seg_part, out);
   } else if (VW == 2) {
-    k_reduce_seg<2><<<dim3((unsigned)nseg), dim3(256), 0, st>>>(jobs, seg_job, partial, seg_part);
+    k_reduce_seg<2><<<dim3((unsigned)nseg), dim3(RED_SEG_NT), 0, st>>>(jobs, seg_job, partial, seg_part);
     k_reduce_var<2><<<dim3(njobs), dim3(RED_VAR_NT), 0, st>>>(jobs, seg_part, out);
   } else {
     return hipErrorInvalidValue;
