@@ -239,11 +239,14 @@ def cfg2_compare(ncirc, steps, log):
     }
 
 
-def cfg2_sampling(chimax, shots, log, plain=True):
+def cfg2_sampling(chimax, shots, log, plain=True, chunk=0):
     """cfg2 sampling on the circuits with at most `chimax` terms: `shots`
     Z-basis samples on all qubits per circuit (exact mode), all circuits in
     one necstar.sample_many call against one QuantumState.sample call per
-    circuit (both timed in full); the outcomes must be identical."""
+    circuit (both timed in full); the outcomes must be identical.  chunk > 0:
+    one sample_many call per `chunk` consecutive circuits, each logged (long
+    runs over the large circuits; a state's outcomes do not depend on the
+    calls it shares)."""
     import numpy as np
     import necstar
     import workloads as WL
@@ -260,7 +263,18 @@ def cfg2_sampling(chimax, shots, log, plain=True):
     log(f"cfg2 sampling: {len(sts)} circuits with chi <= {chimax}, {shots} shots each; batched leg")
     stats = {}
     t0 = time.perf_counter()
-    got = necstar.sample_many(sts, qargs, [shots] * len(sts), seeds, stats=stats)
+    if chunk > 0:
+        got = []
+        for c0 in range(0, len(sts), chunk):
+            part = {}
+            got += necstar.sample_many(sts[c0:c0 + chunk], qargs[c0:c0 + chunk], [shots] * len(sts[c0:c0 + chunk]),
+                                       seeds[c0:c0 + chunk], stats=part)
+            for k, v in part.items():
+                stats[k] = stats.get(k, 0) + v
+            log(f"cfg2 sampling: circuits {c0}..{min(c0 + chunk, len(sts)) - 1} (chi up to "
+                f"{int(chis[c0:c0 + chunk].max())}) done at {time.perf_counter() - t0:.1f} s")
+    else:
+        got = necstar.sample_many(sts, qargs, [shots] * len(sts), seeds, stats=stats)
     batched_s = time.perf_counter() - t0
     log(f"cfg2 sampling: batched {batched_s:.2f} s ({stats['rounds']} rounds)")
     same, plain_s = None, None
@@ -297,6 +311,8 @@ def main():
                     help="cfg2 sampling leg: the circuits with at most this many terms (0: skip the leg)")
     ap.add_argument("--cfg2-sample-shots", type=int, default=1024, help="cfg2 sampling leg: shots per circuit")
     ap.add_argument("--cfg2-sample-plain", type=int, default=1, help="cfg2 sampling leg: also time the plain calls")
+    ap.add_argument("--cfg2-sample-chunk", type=int, default=0,
+                    help="cfg2 sampling leg: circuits per sample_many call, each logged (0: one call)")
     ap.add_argument("--sample-shots", type=int, default=1024,
                     help="also time QuantumState.sample over all 64 qubits with this many shots (cfg1; 0: skip)")
     ap.add_argument("--cfg4-shots", type=int, default=0,
@@ -341,7 +357,7 @@ def main():
         res = cfg2_compare(args.circuits, args.cfg2_steps, log) if args.circuits > 0 else {}
         if args.cfg2_sample_chimax > 0:
             res["sampling"] = cfg2_sampling(args.cfg2_sample_chimax, args.cfg2_sample_shots, log,
-                                            plain=args.cfg2_sample_plain)
+                                            plain=args.cfg2_sample_plain, chunk=args.cfg2_sample_chunk)
         print(json.dumps(res), flush=True)
         return
     if args.config == 4:
